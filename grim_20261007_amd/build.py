@@ -1,0 +1,64 @@
+"""Build the GRIM-2D HIP library for gfx950 in-tree: grim_20261007_amd/libgrim2d.so.
+
+    python -m grim_20261007_amd.build          (or build() from Python)
+
+The sources are compiled by hipcc with packed FP32 arithmetic disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops) and linked into one shared
+library whose C ABI is declared in include/grim2d.h.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libgrim2d.so"
+SOURCES = ("assembly.hip", "dense.hip", "api.hip")
+ARCH = os.environ.get("GRIM_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    path = Path(rocm) / "bin" / "hipcc"
+    return str(path) if path.exists() else "hipcc"
+
+
+def build(verbose: bool = False, jobs: int = 3) -> Path:
+    objs = []
+    flags = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
+             "-Xclang", "-packed-fp32-ops", "-Wno-unused-result", f"-I{CSRC}"]
+    procs = []
+    for name in SOURCES:
+        obj = CSRC / (Path(name).stem + ".o")
+        cmd = [hipcc(), *flags, "-c", str(CSRC / name), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        objs.append(obj)
+        if len(procs) >= jobs:
+            _wait(procs.pop(0))
+    while procs:
+        _wait(procs.pop(0))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
+    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if out.returncode != 0:
+        raise RuntimeError(f"link failed:\n{out.stdout}")
+    os.replace(tmp, LIB)
+    for obj in objs:
+        obj.unlink(missing_ok=True)
+    return LIB
+
+
+def _wait(item):
+    name, proc = item
+    out, _ = proc.communicate()
+    if proc.returncode != 0:
+        raise RuntimeError(f"hipcc failed on {name}:\n{out.decode(errors='replace')}")
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,777 @@
+// Impedance-matrix fill, plane-wave loads and far-field projection for closed
+// PEC contours (TM: S + gamma (D + M/2), TE: -M/2 + K' - gamma W, gamma = 1/(jk)).
+//
+// Replaces the reference's CPU fill: the far pass of _assemble_multi with its
+// native block quadrature (tools/GHOST/ghost_backend/twod/operators.py:2335-2518,
+// twod/assembly/native/far.c:100-207), the near-pair rules
+// (operators.py:418-501, 549-757, 1172-1292, 1971-2085), the routed scatter
+// (assembly/scatter.py:303-368, formulations/combined_regions.py:149-213), the
+// plane-wave loads (assembly/kernels.py:222-294) and the matched far field
+// (assembly/kernels.py:162-219).
+//
+// Layout: element e runs from node e to node next[e]; every node starts exactly
+// one element and ends exactly one (closed contours).  Matrices are column-major
+// complex128 (interleaved re, im), Z[p + q * ld].
+#include "common.h"
+
+__constant__ double c_gl8_t[8], c_gl8_w[8], c_gl16_t[16], c_gl16_w[16];
+__constant__ double c_touch_x[800], c_touch_y[800], c_touch_w[800];
+
+template <int Q> struct Rule;
+template <> struct Rule<8> {
+    static __device__ __forceinline__ double t(int i) { return c_gl8_t[i]; }
+    static __device__ __forceinline__ double w(int i) { return c_gl8_w[i]; }
+};
+template <> struct Rule<16> {
+    static __device__ __forceinline__ double t(int i) { return c_gl16_t[i]; }
+    static __device__ __forceinline__ double w(int i) { return c_gl16_w[i]; }
+};
+
+struct Blk {   // TM and TE contributions of one element pair, [obs local][src local]
+    cplx tm[2][2], te[2][2];
+};
+
+__device__ __forceinline__ void zero_blk(Blk& b) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            b.tm[a][c] = {0.0, 0.0};
+            b.te[a][c] = {0.0, 0.0};
+        }
+}
+
+// W from S (Maue identity, operators.py:1295-1321, 1943-1967): S already scaled.
+__device__ __forceinline__ void combine(const cplx (&S)[2][2], const cplx (&K)[2][2], const cplx (&D)[2][2],
+                                        const Elem& o, const Elem& s, double k, Blk& out) {
+    const double g = -1.0 / k;   // gamma = 1/(jk) = (0, -1/k)
+    const double nn = o.nx * s.nx + o.ny * s.ny;
+    double denom = o.len * s.len;
+    if (denom < GRIM_EPS * GRIM_EPS) denom = GRIM_EPS * GRIM_EPS;
+    const cplx sum = cadd(cadd(S[0][0], S[0][1]), cadd(S[1][0], S[1][1]));
+    const cplx q = {sum.re / denom, sum.im / denom};
+    const double kk = -k * k * nn;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const double tsign = (a == b) ? 1.0 : -1.0;
+            const cplx w = {kk * S[a][b].re + tsign * q.re, kk * S[a][b].im + tsign * q.im};
+            // gamma * D = (0, g) * D = (-g D.im, g D.re)
+            out.tm[a][b] = {S[a][b].re - g * D[a][b].im, S[a][b].im + g * D[a][b].re};
+            out.te[a][b] = {K[a][b].re + g * w.im, K[a][b].im - g * w.re};
+        }
+}
+
+// Far element-pair block with the Q x Q Gauss rule (far.c:100-191 semantics,
+// kernels evaluated directly instead of through an interpolation table).
+template <int Q>
+__device__ __noinline__ void far_pair(const Elem& o, const Elem& s, double k, Blk& out) {
+    cplx S[2][2], K[2][2], D[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) S[a][b] = K[a][b] = D[a][b] = {0.0, 0.0};
+    for (int qi = 0; qi < Q; ++qi) {
+        const double ti = Rule<Q>::t(qi), wi = Rule<Q>::w(qi);
+        const double xo = fma(ti, o.sx, o.x0), yo = fma(ti, o.sy, o.y0);
+        cplx ps0 = {0, 0}, ps1 = {0, 0}, pk0 = {0, 0}, pk1 = {0, 0}, pd0 = {0, 0}, pd1 = {0, 0};
+        for (int qj = 0; qj < Q; ++qj) {
+            const double tj = Rule<Q>::t(qj), wj = Rule<Q>::w(qj);
+            const double dx = xo - fma(tj, s.sx, s.x0);
+            const double dy = yo - fma(tj, s.sy, s.y0);
+            const double r = sqrt(dx * dx + dy * dy);
+            cplx g, h;
+            green_kernels(k, r, g, h);
+            const double rr = r > GRIM_EPS ? r : GRIM_EPS;
+            const double po = (dx * o.nx + dy * o.ny) / rr;
+            const double pso = (dx * s.nx + dy * s.ny) / rr;
+            const double w0 = wj * (1.0 - tj), w1 = wj * tj;
+            ps0.re = fma(w0, g.re, ps0.re); ps0.im = fma(w0, g.im, ps0.im);
+            ps1.re = fma(w1, g.re, ps1.re); ps1.im = fma(w1, g.im, ps1.im);
+            const double kr_ = -h.re * po, ki_ = -h.im * po;
+            pk0.re = fma(w0, kr_, pk0.re); pk0.im = fma(w0, ki_, pk0.im);
+            pk1.re = fma(w1, kr_, pk1.re); pk1.im = fma(w1, ki_, pk1.im);
+            const double dr_ = h.re * pso, di_ = h.im * pso;
+            pd0.re = fma(w0, dr_, pd0.re); pd0.im = fma(w0, di_, pd0.im);
+            pd1.re = fma(w1, dr_, pd1.re); pd1.im = fma(w1, di_, pd1.im);
+        }
+        const double wo0 = wi * (1.0 - ti), wo1 = wi * ti;
+        S[0][0] = cadd(S[0][0], cscale(ps0, wo0)); S[0][1] = cadd(S[0][1], cscale(ps1, wo0));
+        S[1][0] = cadd(S[1][0], cscale(ps0, wo1)); S[1][1] = cadd(S[1][1], cscale(ps1, wo1));
+        K[0][0] = cadd(K[0][0], cscale(pk0, wo0)); K[0][1] = cadd(K[0][1], cscale(pk1, wo0));
+        K[1][0] = cadd(K[1][0], cscale(pk0, wo1)); K[1][1] = cadd(K[1][1], cscale(pk1, wo1));
+        D[0][0] = cadd(D[0][0], cscale(pd0, wo0)); D[0][1] = cadd(D[0][1], cscale(pd1, wo0));
+        D[1][0] = cadd(D[1][0], cscale(pd0, wo1)); D[1][1] = cadd(D[1][1], cscale(pd1, wo1));
+    }
+    const double scale = o.len * s.len;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            S[a][b] = cscale(S[a][b], scale);
+            K[a][b] = cscale(K[a][b], scale);
+            D[a][b] = cscale(D[a][b], scale);
+        }
+    combine(S, K, D, o, s, k, out);
+}
+
+template <int Q>
+__device__ __forceinline__ void far_or_zero(const Elem& o, int e, int ne, const Elem& s, int f, int nf,
+                                            double k, bool active, Blk& out) {
+    if (active && is_far(o, e, ne, s, f, nf)) far_pair<Q>(o, s, k, out);
+    else zero_blk(out);
+}
+
+// Far pass.  One wavefront = 64 consecutive observation elements (lanes) x a
+// chunk of source columns walked in order.  Column q of node row p collects
+// blk(p, prev q)[0][1] + blk(p, q)[0][0] + blk(prev p, prev q)[1][1] + blk(prev p, q)[1][0];
+// the row neighbour comes from lane - 1 by a shuffle.  Rows whose previous
+// element is not lane - 1 (lane 0, contour starts) are completed by far_fixup.
+template <int Q>
+__global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                 const int* __restrict__ next, const int* __restrict__ prev,
+                                                 double k, int chunk, double* __restrict__ ztm, int64_t ldtm,
+                                                 double* __restrict__ zte, int64_t ldte) {
+    const int lane = threadIdx.x;
+    const int e = blockIdx.x * 64 + lane;
+    const bool active = e < n;
+    const int ee = active ? e : n - 1;
+    const Elem o = load_elem(p0, p1, ee);
+    const int ne = next[ee];
+    const bool rowok = active && lane > 0 && prev[ee] == ee - 1;
+    const int c0 = blockIdx.y * chunk;
+    const int c1 = min(n, c0 + chunk);
+    cplx pend_tm0 = {0, 0}, pend_tm1 = {0, 0}, pend_te0 = {0, 0}, pend_te1 = {0, 0};
+    for (int f = c0; f < c1; ++f) {
+        const int pf = prev[f];
+        cplx ltm0, ltm1, lte0, lte1;
+        if (f == c0 || pf != f - 1) {
+            const Elem sp = load_elem(p0, p1, pf);
+            Blk b;
+            far_or_zero<Q>(o, ee, ne, sp, pf, next[pf], k, active, b);
+            ltm0 = b.tm[0][1]; ltm1 = b.tm[1][1]; lte0 = b.te[0][1]; lte1 = b.te[1][1];
+        } else {
+            ltm0 = pend_tm0; ltm1 = pend_tm1; lte0 = pend_te0; lte1 = pend_te1;
+        }
+        const Elem s = load_elem(p0, p1, f);
+        Blk b;
+        far_or_zero<Q>(o, ee, ne, s, f, next[f], k, active, b);
+        const cplx ctm0 = cadd(ltm0, b.tm[0][0]), ctm1 = cadd(ltm1, b.tm[1][0]);
+        const cplx cte0 = cadd(lte0, b.te[0][0]), cte1 = cadd(lte1, b.te[1][0]);
+        pend_tm0 = b.tm[0][1]; pend_tm1 = b.tm[1][1]; pend_te0 = b.te[0][1]; pend_te1 = b.te[1][1];
+        cplx utm, ute;
+        utm.re = __shfl_up(ctm1.re, 1, 64); utm.im = __shfl_up(ctm1.im, 1, 64);
+        ute.re = __shfl_up(cte1.re, 1, 64); ute.im = __shfl_up(cte1.im, 1, 64);
+        if (active) {
+            cplx vtm = ctm0, vte = cte0;
+            if (rowok) { vtm = cadd(vtm, utm); vte = cadd(vte, ute); }
+            double* t = ztm + 2 * ((int64_t)e + (int64_t)f * ldtm);
+            t[0] = vtm.re; t[1] = vtm.im;
+            double* u = zte + 2 * ((int64_t)e + (int64_t)f * ldte);
+            u[0] = vte.re; u[1] = vte.im;
+        }
+    }
+}
+
+// Completes the rows far_kernel could not: Z[p, q] += blk(prev p, prev q)[1][1] + blk(prev p, q)[1][0].
+template <int Q>
+__global__ void __launch_bounds__(256) far_fixup(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                 const int* __restrict__ next, const int* __restrict__ prev,
+                                                 double k, const int* __restrict__ rows, double* __restrict__ ztm,
+                                                 int64_t ldtm, double* __restrict__ zte, int64_t ldte) {
+    const int p = rows[blockIdx.y];
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const int e = prev[p];
+    const Elem o = load_elem(p0, p1, e);
+    const int ne = next[e];
+    const int pq = prev[q];
+    Blk b1, b2;
+    far_or_zero<Q>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
+    far_or_zero<Q>(o, e, ne, load_elem(p0, p1, q), q, next[q], k, true, b2);
+    const cplx vtm = cadd(b1.tm[1][1], b2.tm[1][0]);
+    const cplx vte = cadd(b1.te[1][1], b2.te[1][0]);
+    double* t = ztm + 2 * ((int64_t)p + (int64_t)q * ldtm);
+    t[0] += vtm.re; t[1] += vtm.im;
+    double* u = zte + 2 * ((int64_t)p + (int64_t)q * ldte);
+    u[0] += vte.re; u[1] += vte.im;
+}
+
+// ---------------------------------------------------------------------------
+// Near pairs
+// ---------------------------------------------------------------------------
+// One wavefront per observation element: count (pass 0) or list (pass 1) the
+// source elements that are not far, in ascending order.
+__global__ void __launch_bounds__(64) near_list(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                const int* __restrict__ next, int pass, int* __restrict__ counts,
+                                                const int* __restrict__ offsets, int capacity,
+                                                int* __restrict__ pair_e, int* __restrict__ pair_f,
+                                                unsigned* __restrict__ status) {
+    const int e = blockIdx.x;
+    const int lane = threadIdx.x;
+    const Elem o = load_elem(p0, p1, e);
+    const int ne = next[e];
+    int base = pass ? offsets[e] : 0;
+    int total = 0;
+    if (pass && offsets[n] > capacity) {
+        if (lane == 0 && e == 0) atomicOr(status, (unsigned)GRIM_DEV_NEAR_CAPACITY);
+        return;
+    }
+    for (int f0 = 0; f0 < n; f0 += 64) {
+        const int f = f0 + lane;
+        bool near = false;
+        if (f < n) {
+            const Elem s = load_elem(p0, p1, f);
+            near = !is_far(o, e, ne, s, f, next[f]);
+        }
+        const unsigned long long mask = __ballot(near);
+        if (pass && near) {
+            const int idx = base + __popcll(mask & ((1ull << lane) - 1ull));
+            pair_e[idx] = e;
+            pair_f[idx] = f;
+        }
+        base += __popcll(mask);
+        total += __popcll(mask);
+    }
+    if (!pass && lane == 0) counts[e] = total;
+}
+
+// Exclusive scan of n counts into offsets[0..n] (single workgroup).
+__global__ void __launch_bounds__(1024) scan_counts(int n, const int* __restrict__ counts, int* __restrict__ offsets) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = min(n, t * per), hi = min(n, lo + per);
+    int s = 0;
+    for (int i = lo; i < hi; ++i) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t ? part[t - 1] : 0;
+    for (int i = lo; i < hi; ++i) {
+        offsets[i] = run;
+        run += counts[i];
+    }
+    if (t == 1023) offsets[n] = part[1023];
+}
+
+// Touching test of one ordered pair (operators.py:1912-1940): the first endpoint
+// pair within the 1e-9 m snap, or one mesh node within two snap widths.
+__device__ __forceinline__ bool touching(const Elem& o, int e, int ne, const Elem& s, int f, int nf,
+                                         bool& obs_start, bool& src_start) {
+    const double ox[2] = {o.x0, o.x0 + o.sx}, oy[2] = {o.y0, o.y0 + o.sy};
+    const double sx[2] = {s.x0, s.x0 + s.sx}, sy[2] = {s.y0, s.y0 + s.sy};
+    const int on[2] = {e, ne}, sn[2] = {f, nf};
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            const double dx = ox[a] - sx[b], dy = oy[a] - sy[b];
+            const double d = sqrt(dx * dx + dy * dy);
+            if (d <= 1e-9 || (on[a] == sn[b] && d <= 2e-9)) {
+                obs_start = a == 0;
+                src_start = b == 0;
+                return true;
+            }
+        }
+    return false;
+}
+
+__device__ __forceinline__ double point_seg_sq(double x, double y, double ax, double ay, double bx, double by) {
+    const double ex = bx - ax, ey = by - ay;
+    const double den = ex * ex + ey * ey;
+    double t = den > 0 ? ((x - ax) * ex + (y - ay) * ey) / den : 0.0;
+    t = t < 0 ? 0 : (t > 1 ? 1 : t);
+    const double dx = x - ax - t * ex, dy = y - ay - t * ey;
+    return dx * dx + dy * dy;
+}
+
+// separation.py:18-78
+__device__ bool requires_adaptive(const Elem& o, const Elem& s) {
+    const double lo = o.len, ls = s.len;
+    const double scale = lo > ls ? lo : ls;
+    const double cdx = o.cx - s.cx, cdy = o.cy - s.cy;
+    const double distance = sqrt(cdx * cdx + cdy * cdy);
+    if (distance < 0.75 * scale) return true;
+    if (distance >= 0.5 * (lo + ls) + 0.5 * scale) return false;
+    const double p0x = o.x0, p0y = o.y0, p1x = o.x0 + o.sx, p1y = o.y0 + o.sy;
+    const double q0x = s.x0, q0y = s.y0, q1x = s.x0 + s.sx, q1y = s.y0 + s.sy;
+    if (distance < 0.5 * (lo + ls) + 0.25 * scale) {
+        double sq = point_seg_sq(p0x, p0y, q0x, q0y, q1x, q1y);
+        sq = fmin(sq, point_seg_sq(p1x, p1y, q0x, q0y, q1x, q1y));
+        sq = fmin(sq, point_seg_sq(q0x, q0y, p0x, p0y, p1x, p1y));
+        sq = fmin(sq, point_seg_sq(q1x, q1y, p0x, p0y, p1x, p1y));
+        const double ux = p1x - p0x, uy = p1y - p0y, vx = q1x - q0x, vy = q1y - q0y;
+        const double wx = q0x - p0x, wy = q0y - p0y;
+        const double den = ux * vy - uy * vx;
+        double segd = sqrt(fmax(sq, 0.0));
+        if (den != 0) {
+            const double t = (wx * vy - wy * vx) / den, sp = (wx * uy - wy * ux) / den;
+            if (t >= 0 && t <= 1 && sp >= 0 && sp <= 1) segd = 0.0;
+        }
+        if (segd < 0.25 * scale) return true;
+    }
+    double best = 1e300;
+    const double X[4] = {p0x, p1x, q0x, q1x}, Y[4] = {p0y, p1y, q0y, q1y};
+    const double AX[4] = {q0x, q0x, p0x, p0x}, AY[4] = {q0y, q0y, p0y, p0y};
+    const double BX[4] = {q1x, q1x, p1x, p1x}, BY[4] = {q1y, q1y, p1y, p1y};
+    for (int i = 0; i < 4; ++i) {
+        const double ex = BX[i] - AX[i], ey = BY[i] - AY[i];
+        const double den = ex * ex + ey * ey;
+        const double t = den > 0 ? ((X[i] - AX[i]) * ex + (Y[i] - AY[i]) * ey) / den : -1.0;
+        if (t > 0.0 && t < 1.0) {
+            const double dx = X[i] - AX[i] - t * ex, dy = Y[i] - AY[i] - t * ey;
+            best = fmin(best, sqrt(dx * dx + dy * dy));
+        }
+    }
+    return best < 0.5 * scale;
+}
+
+// Exact linear-Galerkin self S block for |k L| <= 8 (operators.py:418-501).
+__device__ bool self_block(double len, double k, cplx (&S)[2][2]) {
+    const double z = k * len;
+    if (!(z <= 8.0) || z <= 1e-30) return false;
+    const double cd[4] = {2.0 / 3.0, -1.0, 0.0, 1.0 / 3.0};
+    const double co[4] = {1.0 / 3.0, 0.0, 0.0, -1.0 / 3.0};
+    const double tp = 2.0 / GRIM_PI;
+    const double lt = log(0.5 * z) + 0.5772156649015329;
+    const double zq = 0.25 * z * z;
+    cplx bd = {0, 0}, bo = {0, 0};
+    double alpha = 1.0, harmonic = 0.0;
+    for (int m = 0;; ) {
+        const int p = 2 * m;
+        const cplx am = {alpha, -alpha * tp * (lt - harmonic)};
+        double md = 0, mo = 0, ld = 0, lo = 0;
+        for (int q = 0; q < 4; ++q) {
+            const double dd = (double)(p + q + 1);
+            md += cd[q] / dd; mo += co[q] / dd;
+            ld -= cd[q] / (dd * dd); lo -= co[q] / (dd * dd);
+        }
+        // a_m * moment - j (2/pi) alpha log_moment
+        bd.re += am.re * md; bd.im += am.im * md - tp * alpha * ld;
+        bo.re += am.re * mo; bo.im += am.im * mo - tp * alpha * lo;
+        ++m;
+        alpha *= -zq / ((double)m * (double)m);
+        harmonic += 1.0 / m;
+        if (m > 60) return false;
+        const double mag = sqrt(bd.re * bd.re + bd.im * bd.im);
+        if (fabs(alpha) < 1e-18 * (mag > 1.0 ? mag : 1.0)) break;
+    }
+    // 0.25 j L^2 [[bd, bo], [bo, bd]]
+    const double c = 0.25 * len * len;
+    const cplx d = {-c * bd.im, c * bd.re}, f = {-c * bo.im, c * bo.re};
+    S[0][0] = d; S[0][1] = f; S[1][0] = f; S[1][1] = d;
+    return true;
+}
+
+struct SK {
+    double v[16];   // S[a][b] re,im (8) then K'[a][b] re,im (8)
+};
+
+// Tensor Gauss 16 x 16 box on sub-intervals; wave-parallel, result in every lane.
+__device__ void box16(const Elem& o, const Elem& s, double k, double oa, double ob, double sa, double sb, SK& out) {
+    const int lane = threadIdx.x & 63;
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+    const double ho = ob - oa, hs = sb - sa;
+    for (int pt = lane; pt < 256; pt += 64) {
+        const int i = pt >> 4, j = pt & 15;
+        const double xo = oa + ho * c_gl16_t[i], xs = sa + hs * c_gl16_t[j];
+        const double w = c_gl16_w[i] * c_gl16_w[j];
+        const double dx = (o.x0 + xo * o.sx) - (s.x0 + xs * s.sx);
+        const double dy = (o.y0 + xo * o.sy) - (s.y0 + xs * s.sy);
+        const double dist = sqrt(dx * dx + dy * dy);
+        cplx g, h;
+        green_kernels(k, dist, g, h);
+        const double safe = dist > GRIM_EPS ? dist : GRIM_EPS;
+        const double proj = (dx * o.nx + dy * o.ny) / safe;
+        cplx dk = {-h.re * proj, -h.im * proj};
+        if (dist <= GRIM_EPS) dk = {0.0, 0.0};
+        const double po[2] = {1.0 - xo, xo}, ps[2] = {1.0 - xs, xs};
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const double c = w * po[a] * ps[b];
+                acc[2 * (2 * a + b)] += c * g.re;
+                acc[2 * (2 * a + b) + 1] += c * g.im;
+                acc[8 + 2 * (2 * a + b)] += c * dk.re;
+                acc[8 + 2 * (2 * a + b) + 1] += c * dk.im;
+            }
+    }
+    const double scale = o.len * ho * s.len * hs;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) out.v[i] = wave_sum(acc[i]) * scale;
+}
+
+// Graded t^4 two-triangle corner rule (operators.py:549-757, polynomial_quadrature.py:319-335).
+__device__ void touching16(const Elem& o, const Elem& s, double k, bool obs_start, bool src_start, SK& out) {
+    const int lane = threadIdx.x & 63;
+    const double cox = obs_start ? o.x0 : o.x0 + o.sx, coy = obs_start ? o.y0 : o.y0 + o.sy;
+    const double csx = src_start ? s.x0 : s.x0 + s.sx, csy = src_start ? s.y0 : s.y0 + s.sy;
+    const double dox = obs_start ? o.sx : -o.sx, doy = obs_start ? o.sy : -o.sy;
+    const double dsx = src_start ? s.sx : -s.sx, dsy = src_start ? s.sy : -s.sy;
+    double orx = cox - csx, ory = coy - csy;
+    const double eps = 2.220446049250313e-16;
+    double big = fmax(fmax(sqrt(cox * cox + coy * coy), sqrt(csx * csx + csy * csy)), fmax(o.len, s.len));
+    if (sqrt(orx * orx + ory * ory) <= 64.0 * eps * big) { orx = 0.0; ory = 0.0; }
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0;
+    for (int pt = lane; pt < 800; pt += 64) {
+        const double x = c_touch_x[pt], y = c_touch_y[pt], w = c_touch_w[pt];
+        const double dx = orx + x * dox - y * dsx;
+        const double dy = ory + x * doy - y * dsy;
+        const double xo = obs_start ? x : 1.0 - x, xs = src_start ? y : 1.0 - y;
+        const double dist = sqrt(dx * dx + dy * dy);
+        const double safe = dist > GRIM_EPS ? dist : GRIM_EPS;
+        cplx g, h;
+        green_kernels(k, safe, g, h);
+        const double proj = (dx * o.nx + dy * o.ny) / safe;
+        cplx dk = {-h.re * proj, -h.im * proj};
+        if (dist <= GRIM_EPS) dk = {0.0, 0.0};
+        const double po[2] = {1.0 - xo, xo}, ps[2] = {1.0 - xs, xs};
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const double c = w * po[a] * ps[b];
+                acc[2 * (2 * a + b)] += c * g.re;
+                acc[2 * (2 * a + b) + 1] += c * g.im;
+                acc[8 + 2 * (2 * a + b)] += c * dk.re;
+                acc[8 + 2 * (2 * a + b) + 1] += c * dk.im;
+            }
+    }
+    const double scale = o.len * s.len;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) out.v[i] = wave_sum(acc[i]) * scale;
+}
+
+// Pair classes: 0 self, 1 touching, 2 fixed, 3 adaptive.
+__global__ void __launch_bounds__(64) near_blocks(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                  const int* __restrict__ next, const int* __restrict__ offsets,
+                                                  const int* __restrict__ pair_e, const int* __restrict__ pair_f,
+                                                  double k, double* __restrict__ blocks, int* __restrict__ kind,
+                                                  unsigned* __restrict__ status) {
+    const int idx = blockIdx.x;
+    if (idx >= offsets[n]) return;
+    const int e = pair_e[idx], f = pair_f[idx];
+    const Elem o = load_elem(p0, p1, e), s = load_elem(p0, p1, f);
+    SK r;
+    int cls;
+    bool os = false, ss = false;
+    if (e == f) {
+        cls = 0;
+        cplx S[2][2];
+        if (!self_block(o.len, k, S)) {
+            if (threadIdx.x == 0) atomicOr(status, (unsigned)GRIM_DEV_SELF_SERIES);
+            for (int i = 0; i < 16; ++i) r.v[i] = 0.0;
+        } else {
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    r.v[2 * (2 * a + b)] = S[a][b].re;
+                    r.v[2 * (2 * a + b) + 1] = S[a][b].im;
+                }
+            for (int i = 8; i < 16; ++i) r.v[i] = 0.0;
+        }
+    } else if (touching(o, e, next[e], s, f, next[f], os, ss)) {
+        cls = 1;
+        touching16(o, s, k, os, ss, r);
+    } else if (requires_adaptive(o, s)) {
+        cls = 3;   // integrated by near_adaptive
+        if (threadIdx.x == 0) kind[idx] = cls;
+        return;
+    } else {
+        cls = 2;
+        box16(o, s, k, 0.0, 1.0, 0.0, 1.0, r);
+    }
+    if (threadIdx.x < 16) blocks[16 * (int64_t)idx + threadIdx.x] = r.v[threadIdx.x];
+    if (threadIdx.x == 0) kind[idx] = cls;
+}
+
+// Recursive four-way bisection with rtol 1e-9 and depth 12 (operators.py:1172-1292),
+// run depth first with an explicit stack; one wavefront per adaptive pair.
+#define GRIM_ADAPT_STACK 48
+__global__ void __launch_bounds__(64) near_adaptive(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                    const int* __restrict__ offsets, const int* __restrict__ pair_e,
+                                                    const int* __restrict__ pair_f, double k,
+                                                    double* __restrict__ blocks, const int* __restrict__ kind,
+                                                    unsigned* __restrict__ status) {
+    const int idx = blockIdx.x;
+    if (idx >= offsets[n] || kind[idx] != 3) return;
+    __shared__ double st_iv[GRIM_ADAPT_STACK][4];
+    __shared__ double st_blk[GRIM_ADAPT_STACK][16];
+    __shared__ int st_depth[GRIM_ADAPT_STACK];
+    const int lane = threadIdx.x;
+    const int e = pair_e[idx], f = pair_f[idx];
+    const Elem o = load_elem(p0, p1, e), s = load_elem(p0, p1, f);
+    const double floor_ = 2.220446049250313e-16 * fmax(1.0, o.len * s.len);
+    SK root;
+    box16(o, s, k, 0.0, 1.0, 0.0, 1.0, root);
+    int top = 0;
+    if (lane == 0) {
+        st_iv[0][0] = 0.0; st_iv[0][1] = 1.0; st_iv[0][2] = 0.0; st_iv[0][3] = 1.0;
+        st_depth[0] = 0;
+    }
+    if (lane < 16) st_blk[0][lane] = root.v[lane];
+    top = 1;
+    __syncthreads();
+    double total[16];
+    for (int i = 0; i < 16; ++i) total[i] = 0.0;
+    bool failed = false;
+    while (top > 0) {
+        --top;
+        const double oa = st_iv[top][0], ob = st_iv[top][1], sa = st_iv[top][2], sb = st_iv[top][3];
+        const int depth = st_depth[top];
+        double coarse[16];
+        for (int i = 0; i < 16; ++i) coarse[i] = st_blk[top][i];
+        __syncthreads();
+        const double om = 0.5 * (oa + ob), sm = 0.5 * (sa + sb);
+        const double iv[4][4] = {{oa, om, sa, sm}, {oa, om, sm, sb}, {om, ob, sa, sm}, {om, ob, sm, sb}};
+        SK kid[4];
+        for (int c = 0; c < 4; ++c) box16(o, s, k, iv[c][0], iv[c][1], iv[c][2], iv[c][3], kid[c]);
+        double err = 0.0;
+        for (int part = 0; part < 2; ++part) {
+            double diff2 = 0.0, denom = 0.0;
+            for (int c = 0; c < 4; ++c) {
+                double n2 = 0.0;
+                for (int i = 0; i < 8; ++i) n2 += kid[c].v[8 * part + i] * kid[c].v[8 * part + i];
+                denom += sqrt(n2);
+            }
+            for (int i = 0; i < 8; ++i) {
+                const double fine = kid[0].v[8 * part + i] + kid[1].v[8 * part + i] + kid[2].v[8 * part + i] +
+                                    kid[3].v[8 * part + i];
+                const double d = fine - coarse[8 * part + i];
+                diff2 += d * d;
+            }
+            err = fmax(err, sqrt(diff2) / fmax(denom, floor_));
+        }
+        if (err <= 1e-9 || depth >= 12 || top + 4 > GRIM_ADAPT_STACK) {
+            if (err > 1e-9) failed = true;
+            for (int c = 0; c < 4; ++c)
+                for (int i = 0; i < 16; ++i) total[i] += kid[c].v[i];
+        } else {
+            for (int c = 3; c >= 0; --c) {
+                if (lane == 0) {
+                    for (int j = 0; j < 4; ++j) st_iv[top][j] = iv[c][j];
+                    st_depth[top] = depth + 1;
+                }
+                if (lane < 16) st_blk[top][lane] = kid[c].v[lane];
+                ++top;
+            }
+        }
+        __syncthreads();
+    }
+    if (failed && lane == 0) atomicOr(status, (unsigned)GRIM_DEV_ADAPTIVE_DEPTH);
+    if (lane < 16) blocks[16 * (int64_t)idx + lane] = total[lane];
+}
+
+__device__ __forceinline__ int find_pair(const int* __restrict__ offsets, const int* __restrict__ pair_f, int row, int col) {
+    int lo = offsets[row], hi = offsets[row + 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = pair_f[mid];
+        if (v == col) return mid;
+        if (v < col) lo = mid + 1; else hi = mid;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void add_entry(double* z, int64_t ld, int p, int q, cplx v) {
+    double* t = z + 2 * ((int64_t)p + (int64_t)q * ld);
+    t[0] += v.re;
+    t[1] += v.im;
+}
+
+// Row-owner gather of near blocks and mass terms: thread p adds everything of node row p.
+__global__ void __launch_bounds__(256) near_gather(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                   const int* __restrict__ next, const int* __restrict__ prev,
+                                                   const int* __restrict__ offsets, const int* __restrict__ pair_f,
+                                                   const double* __restrict__ blocks, double k,
+                                                   double* __restrict__ ztm, int64_t ldtm, double* __restrict__ zte,
+                                                   int64_t ldte) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double g = -1.0 / k;
+    const int elems[2] = {p, prev[p]};
+    for (int loc = 0; loc < 2; ++loc) {
+        const int e = elems[loc];
+        const Elem o = load_elem(p0, p1, e);
+        for (int idx = offsets[e]; idx < offsets[e + 1]; ++idx) {
+            const int f = pair_f[idx];
+            const Elem s = load_elem(p0, p1, f);
+            const double* B = blocks + 16 * (int64_t)idx;
+            const int rev = find_pair(offsets, pair_f, f, e);
+            const double* R = blocks + 16 * (int64_t)(rev < 0 ? idx : rev);
+            cplx S[2][2], K[2][2], D[2][2];
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    S[a][b] = {B[2 * (2 * a + b)], B[2 * (2 * a + b) + 1]};
+                    K[a][b] = {B[8 + 2 * (2 * a + b)], B[8 + 2 * (2 * a + b) + 1]};
+                    // D(e, f) = K'(f, e)^T
+                    D[a][b] = rev < 0 ? cplx{0.0, 0.0} : cplx{R[8 + 2 * (2 * b + a)], R[8 + 2 * (2 * b + a) + 1]};
+                }
+            Blk blk;
+            combine(S, K, D, o, s, k, blk);
+            const int cols[2] = {f, next[f]};
+            for (int b = 0; b < 2; ++b) {
+                add_entry(ztm, ldtm, p, cols[b], blk.tm[loc][b]);
+                add_entry(zte, ldte, p, cols[b], blk.te[loc][b]);
+            }
+        }
+        // mass of element e: L [[1/3, 1/6], [1/6, 1/3]]; TM + gamma/2 M, TE - M/2
+        const int cols[2] = {e, next[e]};
+        for (int b = 0; b < 2; ++b) {
+            const double m = o.len * (loc == b ? 1.0 / 3.0 : 1.0 / 6.0);
+            add_entry(ztm, ldtm, p, cols[b], cplx{0.0, 0.5 * m * g});
+            add_entry(zte, ldte, p, cols[b], cplx{-0.5 * m, 0.0});
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plane-wave loads and far field
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void sinc_d(double u, double& f, double& fp) {
+    if (fabs(u) < 0.01) {
+        const double v2 = u * u;
+        f = 1 + v2 * (-1.0 / 6 + v2 * (1.0 / 120 + v2 * (-1.0 / 5040 + v2 / 362880)));
+        fp = u * (-1.0 / 6 + v2 * (1.0 / 60 + v2 * (-1.0 / 1680 + v2 * (1.0 / 90720 - v2 / 7983360))));
+    } else {
+        double s, c;
+        sincos(u, &s, &c);
+        f = s / u;
+        fp = (u * c - s) / (2 * u * u);
+    }
+}
+
+// Moments of element e for direction (dx, dy): i0, i1 and j k d.n (kernels.py:20-39, 260-285).
+__device__ __forceinline__ void elem_moments(const Elem& E, double k, double dx, double dy, cplx& i0, cplx& i1,
+                                             cplx& dn) {
+    const double z = k * (E.sx * dx + E.sy * dy);
+    double f, fp;
+    sinc_d(0.5 * z, f, fp);
+    double s, c;
+    sincos(k * (E.cx * dx + E.cy * dy), &s, &c);
+    const cplx ph = {E.len * c, E.len * s};
+    i0 = cmul(ph, cplx{0.5 * f, fp});
+    i1 = cmul(ph, cplx{0.5 * f, -fp});
+    dn = {0.0, k * (E.nx * dx + E.ny * dy)};
+}
+
+// bu[p, a] = i0(p) + i1(prev p); bdn likewise with j k d.n.  angles in degrees.
+__global__ void __launch_bounds__(256) loads_kernel(int n, int na, const double* __restrict__ p0,
+                                                    const double* __restrict__ p1, const int* __restrict__ prev,
+                                                    double k, const double* __restrict__ angles_deg,
+                                                    double* __restrict__ bu, double* __restrict__ bdn, int64_t ld) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (p >= n) return;
+    const double phi = angles_deg[a] * (GRIM_PI / 180.0);
+    double dy, dx;
+    sincos(phi, &dy, &dx);
+    const Elem A = load_elem(p0, p1, p), B = load_elem(p0, p1, prev[p]);
+    cplx i0, i1, dnA, j0_, j1_, dnB;
+    elem_moments(A, k, dx, dy, i0, i1, dnA);
+    elem_moments(B, k, dx, dy, j0_, j1_, dnB);
+    const cplx u = cadd(i0, j1_);
+    const cplx v = cadd(cmul(i0, dnA), cmul(j1_, dnB));
+    const int64_t at = 2 * ((int64_t)p + (int64_t)a * ld);
+    if (bu) { bu[at] = u.re; bu[at + 1] = u.im; }
+    if (bdn) { bdn[at] = v.re; bdn[at + 1] = v.im; }
+}
+
+// B(a) = sum_p x[p,a] bu[p,a] + gamma sum_p x[p,a] bdn[p,a]; sigma = |B|^2 / (4k).
+__global__ void __launch_bounds__(256) farfield_kernel(int n, const double* __restrict__ x, int64_t ldx,
+                                                       const double* __restrict__ bu, const double* __restrict__ bdn,
+                                                       int64_t ldb, double k, double* __restrict__ amp,
+                                                       double* __restrict__ sigma) {
+    const int a = blockIdx.x;
+    double s0r = 0, s0i = 0, s1r = 0, s1i = 0;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const int64_t ix = 2 * ((int64_t)p + (int64_t)a * ldx), ib = 2 * ((int64_t)p + (int64_t)a * ldb);
+        const double xr = x[ix], xi = x[ix + 1];
+        s0r += xr * bu[ib] - xi * bu[ib + 1];
+        s0i += xr * bu[ib + 1] + xi * bu[ib];
+        s1r += xr * bdn[ib] - xi * bdn[ib + 1];
+        s1i += xr * bdn[ib + 1] + xi * bdn[ib];
+    }
+    __shared__ double red[4][4];
+    s0r = wave_sum(s0r); s0i = wave_sum(s0i); s1r = wave_sum(s1r); s1i = wave_sum(s1i);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[w][0] = s0r; red[w][1] = s0i; red[w][2] = s1r; red[w][3] = s1i; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[4] = {0, 0, 0, 0};
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i)
+            for (int j = 0; j < 4; ++j) t[j] += red[i][j];
+        const double g = -1.0 / k;   // gamma = (0, g)
+        const double br = t[0] - g * t[3], bi = t[1] + g * t[2];
+        amp[2 * a] = br;
+        amp[2 * a + 1] = bi;
+        if (sigma) sigma[a] = 0.25 / k * (br * br + bi * bi);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers (called from api.cpp)
+// ---------------------------------------------------------------------------
+extern "C" hipError_t grim_upload_rules(const double* gl8t, const double* gl8w, const double* gl16t,
+                                         const double* gl16w, const double* tx, const double* ty, const double* tw) {
+    hipError_t err;
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_gl8_t), gl8t, 8 * sizeof(double))) != hipSuccess) return err;
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_gl8_w), gl8w, 8 * sizeof(double))) != hipSuccess) return err;
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_gl16_t), gl16t, 16 * sizeof(double))) != hipSuccess) return err;
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_gl16_w), gl16w, 16 * sizeof(double))) != hipSuccess) return err;
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_touch_x), tx, 800 * sizeof(double))) != hipSuccess) return err;
+    if ((err = hipMemcpyToSymbol(HIP_SYMBOL(c_touch_y), ty, 800 * sizeof(double))) != hipSuccess) return err;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_touch_w), tw, 800 * sizeof(double));
+}
+
+extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
+                                      const int* prev, double k, int chunk, double* ztm, int64_t ldtm, double* zte,
+                                      int64_t ldte, const int* fix_rows, int nfix, hipStream_t st) {
+    dim3 grid((n + 63) / 64, (n + chunk - 1) / chunk);
+    if (q == 8) far_kernel<8><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+    else far_kernel<16><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess || nfix == 0) return err;
+    dim3 g2((n + 255) / 256, nfix);
+    if (q == 8) far_fixup<8><<<g2, 256, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    else far_fixup<16><<<g2, 256, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1, const int* next, const int* prev,
+                                       double k, int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
+                                       double* blocks, int* kind, unsigned* status, double* ztm, int64_t ldtm,
+                                       double* zte, int64_t ldte, hipStream_t st) {
+    near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 0, counts, offsets, capacity, pair_e, pair_f, status);
+    scan_counts<<<1, 1024, 0, st>>>(n, counts, offsets);
+    near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 1, counts, offsets, capacity, pair_e, pair_f, status);
+    near_blocks<<<capacity, 64, 0, st>>>(n, p0, p1, next, offsets, pair_e, pair_f, k, blocks, kind, status);
+    near_adaptive<<<capacity, 64, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, k, blocks, kind, status);
+    near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, blocks, k, ztm, ldtm, zte,
+                                                 ldte);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t grim_launch_loads(int n, int na, const double* p0, const double* p1, const int* prev, double k,
+                                        const double* angles_deg, double* bu, double* bdn, int64_t ld,
+                                        hipStream_t st) {
+    dim3 grid((n + 255) / 256, na);
+    loads_kernel<<<grid, 256, 0, st>>>(n, na, p0, p1, prev, k, angles_deg, bu, bdn, ld);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t grim_launch_farfield(int n, int na, const double* x, int64_t ldx, const double* bu,
+                                           const double* bdn, int64_t ldb, double k, double* amp, double* sigma,
+                                           hipStream_t st) {
+    farfield_kernel<<<na, 256, 0, st>>>(n, x, ldx, bu, bdn, ldb, k, amp, sigma);
+    return hipGetLastError();
+}

@@ -1,0 +1,135 @@
+// Shared device helpers of the GRIM-2D HIP library (gfx950, FP64 VALU).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hankel_coeffs.h"
+
+#define GRIM_EPS 1e-12
+#define GRIM_PI 3.14159265358979323846
+#define GRIM_SQRT_HALF 0.70710678118654752440
+
+// Status codes of the C ABI (same style as the reference's native seam:
+// 0 ok, 1 index/size out of range, 3 invalid arguments; plus HIP failures).
+enum {
+    GRIM_OK = 0,
+    GRIM_ERR_RANGE = 1,
+    GRIM_ERR_ARGS = 3,
+    GRIM_ERR_HIP = 5,
+    GRIM_ERR_NOT_CONVERGED = 6,
+    GRIM_ERR_CAPACITY = 7,
+    GRIM_ERR_SINGULAR = 8,
+};
+
+// Device status word bits (written by kernels, read by the host once per call).
+enum {
+    GRIM_DEV_ADAPTIVE_DEPTH = 1,
+    GRIM_DEV_NEAR_CAPACITY = 2,
+    GRIM_DEV_SELF_SERIES = 4,
+    GRIM_DEV_ZERO_PIVOT = 8,
+};
+
+struct cplx {
+    double re, im;
+};
+
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.re * s, a.im * s}; }
+
+template <int N>
+__device__ __forceinline__ double horner(const double (&c)[N], double t) {
+    double r = c[N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) r = fma(r, t, c[i]);
+    return r;
+}
+
+// (j/4) H0^(2)(x) and (j/4) H1^(2)(x) for real x > 0 (x = k r, floored at 1e-12).
+// Returns g = (Y0/4, J0/4) and h1 = (Y1/4, J1/4); the caller multiplies h1 by k.
+// Small x: Taylor series of the entire parts (DLMF 10.8.1); large x: modulus /
+// phase form with fitted P_nu, Q_nu (DLMF 10.17.3).  Coefficients generated by
+// gen_hankel_coeffs.py (max relative error 5e-15 for x up to 1e3).
+__device__ __noinline__ void hankel_quarter(double x, cplx& g, cplx& h1) {
+    if (x < GRIM_HANKEL_XS) {
+        const double y = 0.25 * x * x;
+        const double L = log(0.5 * x);
+        const double j0v = horner(GRIM_HK_A0, y);
+        const double j1v = 0.5 * x * horner(GRIM_HK_A1, y);
+        const double y0v = (2.0 / GRIM_PI) * L * j0v + horner(GRIM_HK_B0, y);
+        const double y1v = (2.0 / GRIM_PI) * L * j1v - 2.0 / (GRIM_PI * x) + 0.5 * x * horner(GRIM_HK_B1, y);
+        g = {0.25 * y0v, 0.25 * j0v};
+        h1 = {0.25 * y1v, 0.25 * j1v};
+    } else {
+        const double inv = 1.0 / x;
+        const double r = GRIM_HANKEL_XS * inv;
+        const double w = r * r;
+        const double amp = sqrt(2.0 / GRIM_PI * inv) * GRIM_SQRT_HALF;
+        const double p0 = horner(GRIM_HK_P0, w), q0 = r * horner(GRIM_HK_Q0, w);
+        const double p1 = horner(GRIM_HK_P1, w), q1 = r * horner(GRIM_HK_Q1, w);
+        double s, c;
+        sincos(x, &s, &c);
+        // H0 = amp (P0 - j Q0) (u0 + j v0), u0 = c + s, v0 = c - s
+        const double u0 = c + s, v0 = c - s;
+        const double h0re = amp * (p0 * u0 + q0 * v0), h0im = amp * (p0 * v0 - q0 * u0);
+        // H1 = amp (P1 - j Q1) (u1 + j v1), u1 = s - c, v1 = c + s
+        const double u1 = s - c, v1 = c + s;
+        const double h1re = amp * (p1 * u1 + q1 * v1), h1im = amp * (p1 * v1 - q1 * u1);
+        g = {-0.25 * h0im, 0.25 * h0re};
+        h1 = {-0.25 * h1im, 0.25 * h1re};
+    }
+}
+
+// Kernels at distance r for real k: g = (j/4) H0(k r), h = (j/4) k H1(k r).
+__device__ __forceinline__ void green_kernels(double k, double r, cplx& g, cplx& h) {
+    double rr = r > GRIM_EPS ? r : GRIM_EPS;
+    double x = k * rr;
+    if (x < 1e-12) x = 1e-12;
+    cplx h1;
+    hankel_quarter(x, g, h1);
+    h = cscale(h1, k);
+}
+
+// Geometry of one straight element (element e runs from node e to node next[e]).
+struct Elem {
+    double x0, y0, sx, sy;   // start point and p1 - p0
+    double len, nx, ny;      // length and stored normal (-t_y, t_x)
+    double cx, cy;           // centre 0.5 (p0 + p1)
+};
+
+__device__ __forceinline__ Elem load_elem(const double* p0, const double* p1, int e) {
+    Elem E;
+    E.x0 = p0[2 * e];
+    E.y0 = p0[2 * e + 1];
+    const double x1 = p1[2 * e], y1 = p1[2 * e + 1];
+    E.sx = x1 - E.x0;
+    E.sy = y1 - E.y0;
+    E.len = sqrt(E.sx * E.sx + E.sy * E.sy);
+    const double tx = E.sx / E.len, ty = E.sy / E.len;
+    E.nx = -ty;
+    E.ny = tx;
+    E.cx = 0.5 * (E.x0 + x1);
+    E.cy = 0.5 * (E.y0 + y1);
+    return E;
+}
+
+// Far pair test of the fused traversal: centre distance / max length >= 3,
+// no shared mesh node, not the same element.
+__device__ __forceinline__ bool is_far(const Elem& a, int ea, int na, const Elem& b, int eb, int nb) {
+    if (ea == eb) return false;
+    if (ea == eb || ea == nb || na == eb || na == nb) return false;
+    const double dx = a.cx - b.cx, dy = a.cy - b.cy;
+    const double dist = sqrt(dx * dx + dy * dy);
+    double scale = a.len > b.len ? a.len : b.len;
+    if (scale < GRIM_EPS) scale = GRIM_EPS;
+    return dist / scale >= 3.0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}

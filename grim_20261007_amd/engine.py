@@ -1,0 +1,214 @@
+"""Device-side solve pipeline: assembly, factorization, many-angle solves and RCS.
+
+Every numerical step runs in the HIP library through the C ABI; this module
+only owns device buffers (PyTorch-ROCm tensors), streams and bookkeeping.
+Matrices are stored column-major: a tensor ``t`` of shape (ncols, nrows)
+holds the column-major matrix ``t.T``.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import native
+from .mesh import ContourMesh
+
+NEAR_PER_ELEMENT = 48
+REFINE_TOL = 2e-12           # linalg/refined_lu.py:54
+REFINE_MAX_STEPS = 10        # linalg/refined_lu.py:49
+BACKWARD_ERROR_MAX = 1e-12   # twod/constants.py:24
+
+
+def _stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class DeviceContour:
+    """Contour geometry resident on one GPU."""
+
+    def __init__(self, mesh: ContourMesh, device: torch.device | str = "cuda"):
+        self.mesh = mesh
+        self.device = torch.device(device)
+        native.ensure_device(self.device.index if self.device.index is not None else torch.cuda.current_device())
+        self.n = mesh.n
+        self.p0 = torch.as_tensor(np.ascontiguousarray(mesh.p0, np.float64), device=self.device)
+        self.p1 = torch.as_tensor(np.ascontiguousarray(mesh.p1, np.float64), device=self.device)
+        self.next = torch.as_tensor(np.ascontiguousarray(mesh.next, np.int32), device=self.device)
+        self.prev = torch.as_tensor(np.ascontiguousarray(mesh.prev, np.int32), device=self.device)
+        self.fix_rows = torch.as_tensor(mesh.fix_rows(), device=self.device)
+
+    def matrix_bytes(self, precision: str = "double") -> int:
+        return self.n * self.n * (16 if precision == "double" else 8)
+
+
+def assembly_workspace(n: int, device) -> torch.Tensor:
+    nbytes = native.lib().grim_assemble_workspace_bytes(n, NEAR_PER_ELEMENT)
+    return torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+
+
+def assemble(contour: DeviceContour, k: float, workspace: torch.Tensor | None = None, check: bool = True):
+    """TM and TE system matrices (complex128, column-major) from one fused traversal."""
+    n = contour.n
+    dev = contour.device
+    z_tm = torch.empty((n, n), dtype=torch.complex128, device=dev)
+    z_te = torch.empty((n, n), dtype=torch.complex128, device=dev)
+    ws = workspace if workspace is not None else assembly_workspace(n, dev)
+    st = _stream(dev)
+    code = native.lib().grim_assemble(
+        n, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.next.data_ptr(), contour.prev.data_ptr(),
+        contour.fix_rows.data_ptr(), int(contour.fix_rows.numel()), float(k), contour.mesh.far_order(k),
+        z_tm.data_ptr(), n, z_te.data_ptr(), n, ws.data_ptr(), ws.numel(), NEAR_PER_ELEMENT, st)
+    native.check("grim_assemble", code)
+    if check:
+        native.check("grim_assemble (device)", native.lib().grim_workspace_status(ws.data_ptr(), st))
+    return z_tm, z_te
+
+
+def loads(contour: DeviceContour, k: float, angles_deg: torch.Tensor, want_u=True, want_dn=True):
+    """Nodal plane-wave loads (A, N) complex128 tensors (column-major N x A)."""
+    na = int(angles_deg.numel())
+    dev = contour.device
+    bu = torch.empty((na, contour.n), dtype=torch.complex128, device=dev) if want_u else None
+    bdn = torch.empty((na, contour.n), dtype=torch.complex128, device=dev) if want_dn else None
+    code = native.lib().grim_loads(contour.n, na, contour.p0.data_ptr(), contour.p1.data_ptr(),
+                                   contour.prev.data_ptr(), float(k), angles_deg.data_ptr(), native.ptr(bu),
+                                   native.ptr(bdn), contour.n, _stream(dev))
+    native.check("grim_loads", code)
+    return bu, bdn
+
+
+def farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
+    """Matched monostatic amplitude B (A,) and sigma_2D (A,)."""
+    na, n = x.shape
+    amp = torch.empty(na, dtype=torch.complex128, device=x.device)
+    sigma = torch.empty(na, dtype=torch.float64, device=x.device)
+    code = native.lib().grim_farfield(n, na, x.data_ptr(), n, bu.data_ptr(), bdn.data_ptr(), n, float(k),
+                                      amp.data_ptr(), sigma.data_ptr(), _stream(x.device))
+    native.check("grim_farfield", code)
+    return amp, sigma
+
+
+@dataclass
+class Factor:
+    precision: str              # "double" or "mixed"
+    lu: torch.Tensor            # complex128 or complex64 factors (column-major)
+    piv: torch.Tensor           # int32
+    z: torch.Tensor | None      # complex128 system (kept for residuals)
+    n: int
+
+
+def factor(z: torch.Tensor, precision: str = "double", keep_matrix: bool = True,
+           status: torch.Tensor | None = None, check: bool = True) -> Factor:
+    n = z.shape[0]
+    dev = z.device
+    st = _stream(dev)
+    if precision == "double":
+        lu = z.clone()
+        prec = 64
+    elif precision == "mixed":
+        lu = torch.empty((n, n), dtype=torch.complex64, device=dev)
+        native.check("grim_to_c64", native.lib().grim_to_c64(n, n, z.data_ptr(), n, lu.data_ptr(), n, st))
+        prec = 32
+    else:
+        raise ValueError("precision must be 'double' or 'mixed'")
+    piv = torch.empty(n, dtype=torch.int32, device=dev)
+    word = status if status is not None else torch.zeros(64, dtype=torch.uint8, device=dev)
+    native.check("grim_lu_factor", native.lib().grim_lu_factor(n, lu.data_ptr(), n, prec, piv.data_ptr(),
+                                                               word.data_ptr(), st))
+    if check:
+        code = native.lib().grim_workspace_status(word.data_ptr(), st)
+        if code == 8:
+            raise np.linalg.LinAlgError("zero pivot in the LU factorization")
+        native.check("grim_lu_factor (device)", code)
+    return Factor(precision, lu, piv, z if keep_matrix else None, n)
+
+
+def residual(z: torch.Tensor, x: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """r = b - Z x in FP64 (complex128)."""
+    na, n = b.shape
+    r = b.clone()
+    native.check("grim_gemm_minus", native.lib().grim_gemm_minus(n, na, n, z.data_ptr(), n, x.data_ptr(), n,
+                                                                 r.data_ptr(), n, 64, _stream(z.device)))
+    return r
+
+
+def solve(f: Factor, b: torch.Tensor, gate: bool = True):
+    """x with Z x = b for every column of b (A, N); returns (x, info)."""
+    na, n = b.shape
+    dev = b.device
+    st = _stream(dev)
+    info = {"refinement_steps": 0}
+    if f.precision == "double":
+        x = b.clone()
+        native.check("grim_lu_solve", native.lib().grim_lu_solve(n, na, f.lu.data_ptr(), n, 64, f.piv.data_ptr(),
+                                                                 x.data_ptr(), n, st))
+        if gate and f.z is not None:
+            x, info = _backward_error_gate(f, x, b)
+        return x, info
+    x = torch.empty_like(b)
+    nbytes = native.lib().grim_refine_workspace_bytes(n, na)
+    ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+    steps = ctypes_int()
+    err = ctypes_double()
+    code = native.lib().grim_refined_solve(n, na, f.z.data_ptr(), n, f.lu.data_ptr(), n, f.piv.data_ptr(),
+                                           b.data_ptr(), n, x.data_ptr(), n, REFINE_MAX_STEPS, REFINE_TOL,
+                                           ws.data_ptr(), ws.numel(), steps, err, st)
+    info["refinement_steps"] = int(steps.value)
+    info["refinement_error"] = float(err.value)
+    if code == 6:
+        raise np.linalg.LinAlgError("Mixed-precision refinement stalled; double LU required.")
+    native.check("grim_refined_solve", code)
+    return x, info
+
+
+def ctypes_int():
+    import ctypes
+    return ctypes.c_int(0)
+
+
+def ctypes_double():
+    import ctypes
+    return ctypes.c_double(0.0)
+
+
+def _backward_error_gate(f: Factor, x: torch.Tensor, b: torch.Tensor):
+    """Normwise backward error <= 1e-12 with at most two corrections (linalg/dense.py:257-280)."""
+    z = f.z
+    a_inf = float(torch.max(torch.sum(torch.abs(z), dim=0)))   # column-major: rows of Z are dim 0 sums
+    steps = 0
+
+    def metrics(xx):
+        r = residual(z, xx, b)
+        ri = torch.max(torch.abs(r), dim=1).values
+        den = a_inf * torch.max(torch.abs(xx), dim=1).values + torch.max(torch.abs(b), dim=1).values
+        return r, float(torch.max(ri / den))
+
+    r, err = metrics(x)
+    for _ in range(2):
+        if err <= BACKWARD_ERROR_MAX:
+            break
+        d = r.clone()
+        native.check("grim_lu_solve", native.lib().grim_lu_solve(f.n, d.shape[0], f.lu.data_ptr(), f.n, 64,
+                                                                 f.piv.data_ptr(), d.data_ptr(), f.n,
+                                                                 _stream(z.device)))
+        cand = x + d
+        r2, err2 = metrics(cand)
+        if err2 >= err:
+            break
+        x, r, err = cand, r2, err2
+        steps += 1
+    if not math.isfinite(err) or err > BACKWARD_ERROR_MAX:
+        raise RuntimeError(f"normwise backward error {err} exceeds the release limit {BACKWARD_ERROR_MAX}.")
+    return x, {"refinement_steps": steps, "backward_error": err, "residual": r}
+
+
+def relative_residual(z: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor | None = None):
+    """||b - Z x||_2 / ||b||_2 per column (the reference's linear_residual)."""
+    if r is None:
+        r = residual(z, x, b)
+    nr = torch.linalg.vector_norm(r, dim=1)
+    nb = torch.linalg.vector_norm(b, dim=1)
+    return nr / torch.where(nb <= 1e-12, torch.ones_like(nb), nb)

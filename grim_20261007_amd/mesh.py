@@ -1,0 +1,199 @@
+"""Host geometry: reference geometry snapshot -> closed-contour linear mesh.
+
+Follows the reference's panel builder for PEC (TYPE 2) segments
+(tools/GHOST/ghost_backend/twod/geometry.py): units (:709-715), the TYPE 2
+user-convention flip that reverses each chain (:1009-1065), straight-primitive
+splitting with the primitive's own end vertices (:717-732), panel counts from
+the N property (:906-928, explicit counts checked against the 4-panels-per-
+wavelength floor, :1865-1893), panel normal (-t_y, t_x) (:1935) and node
+welding on the 1e-9 m snap (:1983-2050).  The node numbering is the
+reference's (first appearance, panel by panel), and every contour must close:
+element e then runs from node e to node next[e].
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+C0 = 299_792_458.0
+EPS = 1e-12
+NODE_SNAP = 1.0e-9
+DEFAULT_PANELS_PER_WAVELENGTH = 20
+MIN_EXPLICIT_PANELS_PER_WAVELENGTH = 4
+PANEL_COUNT_CEIL_TOLERANCE = 1e-9
+
+
+@dataclass
+class ContourMesh:
+    nodes: np.ndarray        # (N, 2) welded node coordinates
+    p0: np.ndarray           # (N, 2) element start points
+    p1: np.ndarray           # (N, 2) element end points
+    next: np.ndarray         # (N,) node at the end of element e (element e starts at node e)
+    prev: np.ndarray         # (N,) element ending at node e
+    contour_starts: list = field(default_factory=list)
+
+    @property
+    def n(self) -> int:
+        return len(self.p0)
+
+    @property
+    def elements(self) -> np.ndarray:
+        return np.stack((np.arange(self.n), self.next), axis=1)
+
+    @property
+    def lengths(self) -> np.ndarray:
+        return np.linalg.norm(self.p1 - self.p0, axis=1)
+
+    @property
+    def normals(self) -> np.ndarray:
+        seg = self.p1 - self.p0
+        t = seg / self.lengths[:, None]
+        return np.stack((-t[:, 1], t[:, 0]), axis=1)
+
+    def fix_rows(self, wave: int = 64) -> np.ndarray:
+        """Rows whose previous element is not the row before (wavefront heads, contour starts)."""
+        rows = np.arange(self.n)
+        mask = (rows % wave == 0) | (self.prev != rows - 1)
+        return rows[mask].astype(np.int32)
+
+    def far_order(self, k: float) -> int:
+        """W-calibrated far rule of the paired traversal (twod/operators.py:1509-1527)."""
+        lengths = self.lengths
+        lmax, lmin = float(lengths.max()), float(lengths.min())
+        if 1e-4 <= abs(k) * lmax <= 3.0 and lmin >= 1e-3 * lmax:
+            return 8
+        return 16
+
+
+def unit_scale(units: str) -> float:
+    value = (units or "").strip().lower()
+    if value in {"inch", "inches", "in"}:
+        return 0.0254
+    if value in {"meter", "meters", "m"}:
+        return 1.0
+    raise ValueError(f"Unsupported geometry units '{units}'. Use inches or meters.")
+
+
+def _parse_props(seg: dict):
+    props = list(seg.get("properties", []) or [])
+    vals = []
+    for i in range(5):
+        try:
+            vals.append(int(float(props[i])) if i < len(props) else 0)
+        except (TypeError, ValueError):
+            vals.append(0)
+    seg_type = vals[0] if props else int(seg.get("seg_type", 0))
+    return seg_type, vals[1], vals[2]
+
+
+def _panel_count(n_prop: int, length: float, wavelength: float) -> int:
+    if length <= EPS:
+        return 1
+    if n_prop > 0:
+        return n_prop
+    per = abs(n_prop) if n_prop < 0 else DEFAULT_PANELS_PER_WAVELENGTH
+    target = wavelength / per
+    if not (math.isfinite(target) and target > 0):
+        raise ValueError("The controlling mesh wavelength must be positive and finite.")
+    return max(1, int(math.ceil(length / target - PANEL_COUNT_CEIL_TOLERANCE)))
+
+
+def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int = 20000) -> ContourMesh:
+    """Panels and welded nodes of the PEC contours of ``snapshot`` at ``frequency_ghz``."""
+    scale = unit_scale(units)
+    wavelength = C0 / (float(frequency_ghz) * 1e9)
+    segments = list(snapshot.get("segments", []) or [])
+    if not segments:
+        raise ValueError("Geometry does not contain any segments.")
+    panels0, panels1 = [], []
+    for seg in segments:
+        seg_type, n_prop, ibc = _parse_props(seg)
+        name = str(seg.get("name", "segment"))
+        if seg_type != 2 or ibc != 0:
+            raise ValueError(f"Segment '{name}': only PEC (TYPE 2, no impedance) contours are supported "
+                             "by the GPU path.")
+        pairs = list(seg.get("point_pairs", []) or [])
+        # TYPE 2: the user draws the normal into air; reverse the chain.
+        pairs = [{"x1": p.get("x2", 0.0), "y1": p.get("y2", 0.0), "x2": p.get("x1", 0.0), "y2": p.get("y1", 0.0)}
+                 for p in reversed(pairs)]
+        for pair in pairs:
+            a = np.array([float(pair["x1"]) * scale, float(pair["y1"]) * scale])
+            b = np.array([float(pair["x2"]) * scale, float(pair["y2"]) * scale])
+            length = float(np.linalg.norm(b - a))
+            if EPS < length <= NODE_SNAP:
+                raise ValueError(f"Segment '{name}' has a primitive {length:.3g} m long, not longer than the "
+                                 "mesh node tolerance.")
+            count = _panel_count(n_prop, length, wavelength)
+            if n_prop > 0:
+                minimum = max(1, int(math.ceil(length * MIN_EXPLICIT_PANELS_PER_WAVELENGTH / wavelength)))
+                if count < minimum:
+                    raise ValueError(f"Segment '{name}' explicit N={n_prop} under-resolves a {length:.6g} m "
+                                     f"primitive at wavelength {wavelength:.6g} m: at least N={minimum} is "
+                                     "required.")
+            if len(panels0) + count > max_panels:
+                raise ValueError("Discretization exceeds the configured panel limit.")
+            pts = [a + (b - a) * (i / count) for i in range(count + 1)]
+            pts[0], pts[-1] = a.copy(), b.copy()
+            for i in range(count):
+                if np.linalg.norm(pts[i + 1] - pts[i]) <= EPS:
+                    continue
+                panels0.append(pts[i])
+                panels1.append(pts[i + 1])
+    if not panels0:
+        raise ValueError("Geometry does not contain any valid discretized panels.")
+    p0 = np.array(panels0)
+    p1 = np.array(panels1)
+    # node welding: first appearance, snap key or within the tolerance
+    keys: dict = {}
+    nodes: list = []
+    ids = np.zeros((len(p0), 2), np.int64)
+    for e in range(len(p0)):
+        for c, xy in enumerate((p0[e], p1[e])):
+            key = (int(round(xy[0] / NODE_SNAP)), int(round(xy[1] / NODE_SNAP)))
+            found = keys.get(key)
+            if found is None:
+                for dx in (-2, -1, 0, 1, 2):
+                    for dy in (-2, -1, 0, 1, 2):
+                        idx = keys.get((key[0] + dx, key[1] + dy))
+                        if idx is not None and np.linalg.norm(nodes[idx] - xy) <= NODE_SNAP:
+                            found = idx
+                            break
+                    if found is not None:
+                        break
+            if found is None:
+                found = len(nodes)
+                nodes.append(xy.copy())
+            keys.setdefault(key, found)
+            ids[e, c] = found
+    n = len(p0)
+    if len(nodes) != n or not np.array_equal(ids[:, 0], np.arange(n)):
+        raise ValueError("The GPU path needs closed PEC contours drawn as chained primitives "
+                         "(every node shared by exactly two panels).")
+    nxt = ids[:, 1].astype(np.int32)
+    if sorted(nxt.tolist()) != list(range(n)):
+        raise ValueError("Every PEC contour must be closed.")
+    prev = np.empty(n, np.int32)
+    prev[nxt] = np.arange(n, dtype=np.int32)
+    starts = [int(e) for e in range(n) if prev[e] != e - 1]
+    mesh = ContourMesh(np.array(nodes), p0, p1, nxt, prev, starts)
+    _check_orientation(mesh)
+    return mesh
+
+
+def _check_orientation(mesh: ContourMesh) -> None:
+    """Each closed contour must run counterclockwise internally (normals into the conductor)."""
+    seen = np.zeros(mesh.n, bool)
+    for s in range(mesh.n):
+        if seen[s]:
+            continue
+        area, e = 0.0, s
+        while not seen[e]:
+            seen[e] = True
+            a, b = mesh.p0[e], mesh.p1[e]
+            area += a[0] * b[1] - b[0] * a[1]
+            e = int(mesh.next[e])
+        if area <= 0:
+            raise ValueError("Geometry orientation check failed: a TYPE 2 contour must be drawn with its "
+                             "normal into the air (clockwise around the body).")

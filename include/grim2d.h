@@ -1,0 +1,119 @@
+/* GRIM-2D: MI355X-native 2-D boundary-element RCS path (C ABI).
+ *
+ * Drop-in native seam for the reference's 2-D PEC solve path
+ * (14cole/GRIM_20261007, tools/GHOST/ghost_backend).  Conventions follow the
+ * reference's own native seam (twod/assembly/native/far.c, far.py):
+ *   - int status return: 0 ok, 1 size/index out of range, 3 invalid arguments,
+ *     5 HIP runtime failure, 6 adaptive near rule not converged / refinement
+ *     stalled, 7 near-pair capacity exceeded, 8 zero pivot;
+ *   - sizes are int64_t; complex arrays are interleaved double[2] (complex128)
+ *     or float[2] (complex64), column-major, leading dimension in elements;
+ *   - caller-owned outputs and workspace (all pointers are DEVICE pointers
+ *     unless stated), no allocation inside the library, no internal threads;
+ *   - `stream` is a hipStream_t (NULL = default stream); calls are
+ *     asynchronous on that stream unless documented otherwise.
+ *
+ * Geometry of a set of closed contours with n elements and n nodes: element e
+ * runs from node e (start point p0[e]) to node next[e] (end point p1[e]);
+ * prev[next[e]] == e.  Stored normals are (-t_y, t_x) of the element tangent,
+ * i.e. pointing into the conductor for a contour drawn with the reference's
+ * TYPE 2 convention (twod/geometry.py:1009-1065, 1935).
+ */
+#ifndef GRIM2D_H
+#define GRIM2D_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int grim_version(void);
+
+/* Load the quadrature rules into the current device's constant memory.
+ * Must be called once per device before any other call on that device. */
+int grim_init(int device);
+
+/* Bytes of workspace grim_assemble needs for n elements and at most
+ * near_per_element near pairs per element on average. */
+int64_t grim_assemble_workspace_bytes(int64_t n, int64_t near_per_element);
+
+/* Fused TM + TE system matrices of closed PEC contours in air, one traversal.
+ * Replaces the reference's fused assembly (twod/operators.py:2087-2607 with
+ * twod/assembly/native/far.c:100-207 ghost_far_block and far.c:333-363
+ * ghost_scatter_tile, routed by twod/assembly/scatter.py:303-368 and
+ * twod/formulations/combined_regions.py:149-213):
+ *   z_tm = S + gamma (D + M/2),  z_te = -M/2 + K' - gamma W,  gamma = 1/(j k).
+ * k: real wavenumber (rad/m).  far_order: Gauss points per axis of far pairs,
+ * 8 or 16 (the reference's W-calibrated far rule: 8 when 1e-4 <= k Lmax <= 3
+ * and Lmin >= 1e-3 Lmax, else 16; twod/operators.py:1509-1527).  fix_rows: the nfix
+ * rows r with r % 64 == 0 or prev[r] != r - 1.  Outputs complex128 n x n.
+ * Asynchronous; kernel-side failures are reported by grim_workspace_status. */
+int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* next, const int32_t* prev,
+                  const int32_t* fix_rows, int64_t nfix, double k, int far_order, double* z_tm, int64_t ld_tm,
+                  double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
+                  int64_t near_per_element, void* stream);
+
+/* Synchronises `stream` and returns the kernel status of the last
+ * grim_assemble / grim_lu_factor that used `workspace` (0 ok, else a status
+ * code as above). */
+int grim_workspace_status(void* workspace, void* stream);
+
+/* Plane-wave loads of all nodes for na incidence angles (degrees, device
+ * array): bu = int phi_i exp(j k d.r) ds, bdn = int phi_i (j k d.n) exp(j k d.r) ds,
+ * d = (cos phi, sin phi).  Replaces twod/assembly/kernels.py:222-294
+ * (incident_loads).  RHS of the TM system is -bu, of the TE system -bdn.
+ * Either output may be NULL.  complex128 n x na, leading dimension ld. */
+int grim_loads(int64_t n, int64_t na, const double* p0, const double* p1, const int32_t* prev, double k,
+               const double* angles_deg, double* bu, double* bdn, int64_t ld, void* stream);
+
+/* Matched monostatic far field B[a] = sum_p x[p,a] (bu[p,a] + gamma bdn[p,a])
+ * and sigma_2D[a] = |B[a]|^2 / (4 k).  Replaces twod/assembly/kernels.py:162-219
+ * (farfield, matched projection reusing the RHS loads) and
+ * twod/solver.py:838-859 (_rcs_sigma_from_amp).  amp: complex128 [na];
+ * sigma may be NULL. */
+int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx, const double* bu, const double* bdn,
+                  int64_t ldb, double k, double* amp, double* sigma, void* stream);
+
+/* In-place LU with partial pivoting of an n x n complex matrix: precision 64
+ * (complex128) or 32 (complex64).  piv: int32 [n].  Replaces
+ * scipy.linalg.lu_factor as used by linalg/dense.py:114-155 and
+ * linalg/refined_lu.py:26-39.  Pivots use the library's panel-delayed
+ * convention; factors are only meaningful to grim_lu_solve.  workspace: a
+ * device uint32 status word (read with grim_workspace_status) that receives
+ * the zero-pivot flag.  n <= 32768. */
+int grim_lu_factor(int64_t n, void* a, int64_t lda, int precision, int32_t* piv, void* workspace, void* stream);
+
+/* Solve A X = B in place for nrhs columns with the factors of grim_lu_factor.
+ * Replaces scipy.linalg.lu_solve (linalg/dense.py:239-301). */
+int grim_lu_solve(int64_t n, int64_t nrhs, const void* lu, int64_t ldlu, int precision, const int32_t* piv,
+                  void* b, int64_t ldb, void* stream);
+
+/* C -= A B (complex; precision 64 or 32); the FP64 refinement residual. */
+int grim_gemm_minus(int64_t m, int64_t n, int64_t k, const void* a, int64_t lda, const void* b, int64_t ldb,
+                    void* c, int64_t ldc, int precision, void* stream);
+
+/* Bytes of workspace grim_refined_solve needs. */
+int64_t grim_refine_workspace_bytes(int64_t n, int64_t nrhs);
+
+/* Mixed-precision solve (linalg/refined_lu.py:41-61): x = LU32^{-1} b, then
+ * r = b - Z x in FP64, x += LU32^{-1} r until max_col(|r|_inf / |b|_inf) <= tol
+ * (at most max_steps corrections; a step that does not reduce the error below
+ * 0.98 of the previous one returns 6).  z: complex128 system; lu/piv: complex64
+ * factors of the same matrix.  Synchronises the stream once per step.
+ * steps_out / error_out: host pointers (may be NULL). */
+int grim_refined_solve(int64_t n, int64_t nrhs, const double* z, int64_t ldz, const float* lu, int64_t ldlu,
+                       const int32_t* piv, const double* b, int64_t ldb, double* x, int64_t ldx, int max_steps,
+                       double tol, void* workspace, int64_t workspace_bytes, int* steps_out, double* error_out,
+                       void* stream);
+
+/* complex128 -> complex64 copy (m x n) and complex128 copy. */
+int grim_to_c64(int64_t m, int64_t n, const double* src, int64_t lds, float* dst, int64_t ldd, void* stream);
+int grim_copy_c128(int64_t m, int64_t n, const double* src, int64_t lds, double* dst, int64_t ldd, void* stream);
+
+/* Per-column 2-norm of an m x n complex128 matrix into out[n] (device). */
+int grim_column_norms(int64_t m, int64_t n, const double* v, int64_t ld, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

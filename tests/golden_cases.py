@@ -1,0 +1,17 @@
+"""Loaders of the committed golden fixtures (tests/golden/*.npz)."""
+import os
+
+import numpy as np
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SMALL = ["circle64_ka1", "naca80", "star96", "star128_aspect4"]
+ALL = SMALL + ["c1_circle400_ka10"]
+
+
+def load(name):
+    return dict(np.load(os.path.join(HERE, f"{name}.npz"), allow_pickle=False))
+
+
+def snapshot(case):
+    from grim_20261007_amd.workloads import _snapshot
+    return _snapshot(case["points"], "contour")

@@ -1,0 +1,131 @@
+"""GPU parity of the HIP path against the oracle and the reference's golden vectors."""
+import numpy as np
+import pytest
+
+from tests import golden_cases
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def _mat(t):
+    """Column-major device tensor -> numpy matrix."""
+    return t.cpu().numpy().T
+
+
+def _lib():
+    from grim_20261007_amd import native
+    native.ensure_device(0)
+    return native
+
+
+@pytest.mark.parametrize("prec,tol", [(64, 1e-13), (32, 2e-5)])
+@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (130, 77, 35), (200, 300, 129), (1, 5, 3)])
+def test_gemm_minus(prec, tol, m, n, k):
+    dev = _dev()
+    native = _lib()
+    rng = np.random.default_rng(m * 1000 + n * 10 + k)
+    dt = np.complex128 if prec == 64 else np.complex64
+    a = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))).astype(dt)
+    b = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(dt)
+    c = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(dt)
+    ta, tb, tc = (torch.as_tensor(np.ascontiguousarray(x.T), device=dev) for x in (a, b, c))
+    code = native.lib().grim_gemm_minus(m, n, k, ta.data_ptr(), m, tb.data_ptr(), k, tc.data_ptr(), m, prec,
+                                        torch.cuda.current_stream().cuda_stream)
+    assert code == 0
+    want = c.astype(np.complex128) - a.astype(np.complex128) @ b.astype(np.complex128)
+    got = _mat(tc)
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < tol
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 100, 257, 1100, 2100])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_lu_solve(n, precision):
+    from grim_20261007_amd import engine
+    dev = _dev()
+    _lib()
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * np.sqrt(n) * np.eye(n) * (1 + 1j)
+    b = rng.standard_normal((n, 5)) + 1j * rng.standard_normal((n, 5))
+    z = torch.as_tensor(np.ascontiguousarray(a.T), device=dev)
+    tb = torch.as_tensor(np.ascontiguousarray(b.T), device=dev)
+    f = engine.factor(z, precision)
+    x, info = engine.solve(f, tb)
+    want = np.linalg.solve(a, b)
+    err = np.max(np.abs(x.cpu().numpy().T - want)) / np.max(np.abs(want))
+    print(f"lu n={n} {precision} err={err:.2e} steps={info.get('refinement_steps')}", flush=True)
+    assert err < 1e-11
+
+
+def test_lu_pivoting_needed():
+    """A matrix with a zero leading entry needs row interchanges."""
+    from grim_20261007_amd import engine
+    dev = _dev()
+    _lib()
+    rng = np.random.default_rng(5)
+    n = 300
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a[0, 0] = 0.0
+    b = rng.standard_normal((n, 3)) + 0j
+    f = engine.factor(torch.as_tensor(np.ascontiguousarray(a.T), device=dev), "double")
+    x, _ = engine.solve(f, torch.as_tensor(np.ascontiguousarray(b.T), device=dev), gate=False)
+    want = np.linalg.solve(a, b)
+    assert np.max(np.abs(x.cpu().numpy().T - want)) / np.max(np.abs(want)) < 1e-10
+
+
+@pytest.mark.parametrize("name", golden_cases.ALL)
+def test_assembly_matches_reference(name):
+    from grim_20261007_amd import engine
+    from grim_20261007_amd.mesh import build_mesh
+    from grim_20261007_amd.solver import wavenumber
+    dev = _dev()
+    case = golden_cases.load(name)
+    mesh = build_mesh(golden_cases.snapshot(case), "meters", float(case["frequency_ghz"]))
+    assert np.array_equal(mesh.elements, case["elements"])
+    k = wavenumber(float(case["frequency_ghz"]))
+    contour = engine.DeviceContour(mesh, dev)
+    z_tm, z_te = engine.assemble(contour, k)
+    for pol, z in (("TM", _mat(z_tm)), ("TE", _mat(z_te))):
+        if f"Z_{pol}" in case:
+            ref = case[f"Z_{pol}"]
+        else:
+            rows = case["matrix_rows"]
+            ref = case[f"Zrows_{pol}"]
+            z = z[rows]
+        err = np.max(np.abs(z - ref)) / np.max(np.abs(ref))
+        print(f"{name} {pol} assembly max rel err {err:.2e}", flush=True)
+        assert err < 1e-12
+
+
+@pytest.mark.parametrize("name", golden_cases.ALL)
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_solve_matches_reference(name, precision):
+    from grim_20261007_amd.mesh import build_mesh
+    from grim_20261007_amd.solver import solve_frequency
+    dev = _dev()
+    case = golden_cases.load(name)
+    f = float(case["frequency_ghz"])
+    mesh = build_mesh(golden_cases.snapshot(case), "meters", f)
+    angles = case["angles_deg"].tolist()
+    res = solve_frequency(mesh, f, angles, ("TE", "TM"), precision, dev, return_density=True)
+    for pol in ("TE", "TM"):
+        r = res[pol]
+        cols = case["density_columns"]
+        xref = case[f"density_{pol}"]
+        xerr = np.max(np.abs(r["density"][:, cols] - xref)) / np.max(np.abs(xref))
+        aref = case[f"sample_amp_{pol}"]
+        aerr = np.max(np.abs(r["amplitude"] - aref) / np.abs(aref))
+        dberr = np.max(np.abs(10 * np.log10(r["sigma"]) - case[f"sample_db_{pol}"]))
+        print(f"{name} {pol} {precision}: density {xerr:.2e} amplitude {aerr:.2e} dB {dberr:.2e} "
+              f"residual {np.max(r['residual']):.2e}", flush=True)
+        assert xerr < 1e-10
+        assert aerr < 1e-10
+        assert dberr < 1e-8
+        assert np.max(r["residual"]) < 1e-10
