@@ -1,0 +1,266 @@
+"""Benchmark: RCS solves/s and p99 latency, mixed precision vs plain FP64 on one MI355X.
+
+Workload (the NACA 0012 configuration): NACA 0012 section, chord 1 m, closed
+trailing edge, cosine-spaced stations, 1000 panels per surface (N = 2000
+unknowns), frequency set so the longest panel is lambda/10, 720 monostatic
+angles (0.5 degree steps), both polarizations.  One step is one co-polarized
+request on synthetic geometry: one fused TM+TE assembly, then per polarization
+LU factorization, the 720-column solve and the far-field projection (2 RCS
+solves).  Inputs (geometry) are resident on the GPU before the timed region.
+
+Prints ONE JSON line (rank 0, N = 1).  --gpus > 1 exits: this build runs on
+one GPU (requests shard across GPUs without communication).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6       # TFLOP/s, MI355X dense FP64 (vector and matrix)
+PEAK_FP32 = 157.3      # TFLOP/s, MI355X dense FP32
+PEAK_HBM = 8000.0      # GB/s
+FLOPS_PER_EVAL = 125   # FP64 flops per kernel evaluation (distance, Hankel pair, projections, accumulation)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-side", type=int, default=1000)
+    ap.add_argument("--angles", type=int, default=720)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    return ap.parse_args()
+
+
+def workload(n_side: int, nangles: int):
+    from grim_20261007_amd import workloads
+    from grim_20261007_amd.mesh import build_mesh
+    pts = workloads.naca0012_points(n_side)
+    f = workloads.frequency_for_segments_per_wavelength(pts, 10.0)
+    mesh = build_mesh(workloads._snapshot(pts, "naca0012"), "meters", f)
+    angles = np.arange(nangles) * (360.0 / nangles)
+    return pts, mesh, f, angles
+
+
+class Timer:
+    """HIP events on the stream the library launches on (torch's current stream)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.acc = {}
+
+    def run(self, name, fn):
+        t = self.torch
+        a = t.cuda.Event(enable_timing=True)
+        b = t.cuda.Event(enable_timing=True)
+        a.record()
+        out = fn()
+        b.record()
+        self.acc.setdefault(name, []).append((a, b))
+        return out
+
+    def mean_ms(self, name):
+        self.torch.cuda.synchronize()
+        v = [a.elapsed_time(b) for a, b in self.acc.get(name, [])]
+        return float(np.mean(v)) if v else float("nan")
+
+
+def request(engine, contour, k, angles_t, precision, timer=None, keep=False):
+    """One co-polarized request; returns per-polarization (amplitude, sigma, residual, steps)."""
+    import torch
+    from grim_20261007_amd import native
+    tm = timer.run if timer else (lambda name, fn: fn())
+    n = contour.n
+    z_tm = torch.empty((n, n), dtype=torch.complex128, device=contour.device)
+    z_te = torch.empty((n, n), dtype=torch.complex128, device=contour.device)
+    ws = request.ws
+    st = torch.cuda.current_stream().cuda_stream
+    args = lambda: (n, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.next.data_ptr(),
+                    contour.prev.data_ptr(), contour.fix_rows.data_ptr(), int(contour.fix_rows.numel()), float(k),
+                    contour.mesh.far_order(k), z_tm.data_ptr(), n, z_te.data_ptr(), n, ws.data_ptr(), ws.numel(),
+                    engine.NEAR_PER_ELEMENT, st)
+    native.check("far", tm("assemble_far", lambda: native.lib().grim_assemble_far(*args())))
+    native.check("near", tm("assemble_near", lambda: native.lib().grim_assemble_near(*args())))
+    bu, bdn = tm("loads", lambda: engine.loads(contour, k, angles_t))
+    out = {}
+    for pol, z, b in (("TE", z_te, -bdn), ("TM", z_tm, -bu)):
+        fac = tm(f"factor_{precision}", lambda: engine.factor(z, precision, check=False))
+        x, info = tm(f"solve_{precision}", lambda: engine.solve(fac, b, gate=(precision == "double")))
+        rel = tm("residual", lambda: engine.relative_residual(z, x, b, info.get("residual")))
+        amp, sig = tm("farfield", lambda: engine.farfield(x, bu, bdn, k))
+        out[pol] = (amp, sig, rel, info.get("refinement_steps", 0))
+    native.check("assembly status", native.lib().grim_workspace_status(ws.data_ptr(), st))
+    return out
+
+
+def far_pair_count(mesh):
+    """Unordered far pairs (centre ratio >= 3, no shared node) -- the algorithmic count."""
+    c = 0.5 * (mesh.p0 + mesh.p1)
+    L = mesh.lengths
+    n = mesh.n
+    total = 0
+    for s in range(0, n, 256):
+        rows = np.arange(s, min(n, s + 256))
+        d = np.linalg.norm(c[rows][:, None, :] - c[None, :, :], axis=2)
+        far = d / np.maximum(np.maximum(L[rows][:, None], L[None, :]), 1e-12) >= 3.0
+        for a in (rows, mesh.next[rows]):
+            for b in (np.arange(n), mesh.next):
+                far &= a[:, None] != b[None, :]
+        total += int(np.sum(far))
+    return total // 2
+
+
+def cpu_baseline(mesh, f, angles, seconds):
+    """The oracle (NumPy/SciPy restatement) on a bounded sample of one request, scaled to solves/s."""
+    import scipy.linalg as sl
+    from oracle import bem2d
+    om = bem2d.Mesh(mesh.nodes, mesh.elements, mesh.p0, mesh.p1, 0.5 * (mesh.p0 + mesh.p1), mesh.lengths,
+                    mesh.normals)
+    k = bem2d.frequency_k(f)
+    n = mesh.n
+    q = bem2d.far_order(om, k)
+    # far rows sample
+    rows_done, t_far = 0, 0.0
+    t0 = time.perf_counter()
+    while rows_done < n and time.perf_counter() - t0 < 0.3 * seconds:
+        rows = np.arange(rows_done, min(n, rows_done + 16))
+        far = bem2d.far_mask(om, rows)
+        bem2d._far_rows(om, k, rows, far, q)
+        rows_done += len(rows)
+    t_far = (time.perf_counter() - t0) * n / rows_done
+    # near pairs sample
+    near = np.argwhere(~bem2d.far_mask(om, np.arange(min(n, 64))))
+    t0 = time.perf_counter()
+    done = 0
+    for e, g in near:
+        bem2d.near_blocks(om, k, int(e), int(g))
+        done += 1
+        if time.perf_counter() - t0 > 0.2 * seconds:
+            break
+    per_pair = (time.perf_counter() - t0) / max(done, 1)
+    total_near = len(near) * n / min(n, 64)
+    t_near = per_pair * total_near
+    # LU + 720-column solve + far field at the full size, one polarization
+    rng = np.random.default_rng(0)
+    z = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + n * np.eye(n)
+    t0 = time.perf_counter()
+    lu = sl.lu_factor(z)
+    t_lu = time.perf_counter() - t0
+    b = rng.standard_normal((n, len(angles))) + 0j
+    t0 = time.perf_counter()
+    x = sl.lu_solve(lu, b)
+    t_solve = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bem2d.farfield(om, k, angles, x)
+    bem2d.rhs(om, k, angles, "TM")
+    t_ff = time.perf_counter() - t0
+    per_request = t_far + t_near + 2 * (t_lu + t_solve + t_ff)
+    cores = len(os.sched_getaffinity(0))
+    return {"value": 2.0 / per_request, "unit": "RCS solves/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle far rows {rows_done}/{n} and {done}/{int(total_near)} near pairs timed and "
+                       f"scaled; full-size LAPACK LU ({t_lu:.2f} s), {len(angles)}-column solve "
+                       f"({t_solve:.2f} s) and far field; per co-polarized request {per_request:.1f} s")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    from grim_20261007_amd import engine, native
+    dev = torch.device("cuda", 0)
+    pts, mesh, f, angles = workload(args.n_side, args.angles)
+    from grim_20261007_amd.solver import wavenumber
+    k = wavenumber(f)
+    contour = engine.DeviceContour(mesh, dev)
+    angles_t = torch.tensor(angles, dtype=torch.float64, device=dev)
+    request.ws = engine.assembly_workspace(mesh.n, dev)
+    results = {}
+    timers = {}
+    lat = {}
+    for precision in ("double", "mixed"):
+        for _ in range(args.warmup):
+            request(engine, contour, k, angles_t, precision)
+        torch.cuda.synchronize()
+        timer = Timer(torch)
+        lat[precision] = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            res = request(engine, contour, k, angles_t, precision, timer)
+            torch.cuda.synchronize()
+            lat[precision].append(time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t_start
+        results[precision] = (res, elapsed)
+        timers[precision] = timer
+    # accuracy of the mixed form against the FP64 path on the same request
+    db_err, amp_err, resid = 0.0, 0.0, 0.0
+    steps_mixed = []
+    for pol in ("TE", "TM"):
+        a64, s64, r64, _ = results["double"][0][pol]
+        a32, s32, r32, st = results["mixed"][0][pol]
+        s64, s32 = s64.cpu().numpy(), s32.cpu().numpy()
+        db_err = max(db_err, float(np.max(np.abs(10 * np.log10(s32) - 10 * np.log10(s64)))))
+        amp_err = max(amp_err, float(torch.max(torch.abs(a32 - a64) / torch.abs(a64))))
+        resid = max(resid, float(torch.max(r32)))
+        steps_mixed.append(st)
+    solves = 2 * args.steps
+    t64 = results["double"][1]
+    t32 = results["mixed"][1]
+    value = solves / t32
+    lat32 = np.array(lat["mixed"]) * 1e3
+    lat64 = np.array(lat["double"]) * 1e3
+    # roofline of the dominant kernel (far-pair assembly pass)
+    tm = timers["mixed"]
+    far_ms = tm.mean_ms("assemble_far")
+    pairs = far_pair_count(mesh)
+    q = mesh.far_order(k)
+    flops = pairs * q * q * FLOPS_PER_EVAL
+    achieved = flops / (far_ms * 1e-3) / 1e12
+    stage_ms = {name: round(tm.mean_ms(name), 3) for name in tm.acc}
+    stage64 = {name: round(timers["double"].mean_ms(name), 3) for name in timers["double"].acc}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(mesh, f, angles, args.cpu_seconds)
+    line = {
+        "metric": "RCS solves/sec + p99 latency, mixed-prec vs FP64 GPU at <=0.05 dB RCS error",
+        "value": round(value, 3), "unit": "RCS solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t32 / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 LU + f64 refinement",
+        "data": "synthetic NACA 0012 geometry",
+        "config": {"workload": f"NACA 0012, chord 1 m, N={mesh.n} pulse-free linear Galerkin unknowns, "
+                               f"f={f:.4f} GHz (longest panel lambda/10), {len(angles)} monostatic angles, "
+                               "TE+TM co-polarized request", "n": mesh.n, "angles": len(angles),
+                   "frequency_ghz": f},
+        "p50_ms": round(float(np.percentile(lat32, 50)), 3), "p99_ms": round(float(np.percentile(lat32, 99)), 3),
+        "fp64_path": {"value": round(solves / t64, 3), "p50_ms": round(float(np.percentile(lat64, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat64, 99)), 3), "stage_ms": stage64},
+        "mixed_vs_fp64": {"speedup": round(t64 / t32, 3), "max_rcs_db_error": db_err,
+                          "max_amplitude_rel_error": amp_err, "max_refined_residual": resid,
+                          "refinement_steps": steps_mixed},
+        "stage_ms": stage_ms,
+        "roofline": {"kernel": "far_kernel (grim_assemble_far)", "bound": "mfma",
+                     "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64, 4), "traffic": None,
+                     "model": f"{pairs} unordered far pairs x {q}x{q} points x {FLOPS_PER_EVAL} FP64 flops "
+                              f"per launch; FP64 VALU-bound (MI355X FP64 vector peak = FP64 matrix peak)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

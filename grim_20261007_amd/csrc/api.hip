@@ -10,7 +10,7 @@
 extern "C" hipError_t grim_upload_rules(const double*, const double*, const double*, const double*, const double*,
                                         const double*, const double*);
 extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
-                                      const int* prev, double k, int chunk, double* ztm, int64_t ldtm, double* zte,
+                                      const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
                                       int64_t ldte, const int* fix_rows, int nfix, hipStream_t st);
 extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1, const int* next, const int* prev,
                                        double k, int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
@@ -115,10 +115,10 @@ extern "C" int64_t grim_assemble_workspace_bytes(int64_t n, int64_t near_per_ele
 }
 
 
-extern "C" int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* next,
-                             const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
-                             double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, void* workspace,
-                             int64_t workspace_bytes, int64_t near_per_element, void* stream) {
+static int assemble_pass(int passes, int64_t n, const double* p0, const double* p1, const int32_t* next,
+                         const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
+                         double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, void* workspace,
+                         int64_t workspace_bytes, int64_t near_per_element, void* stream) {
     if (n < 2 || n > (1 << 30) || !p0 || !p1 || !next || !prev || !z_tm || !z_te || !workspace ||
         ld_tm < n || ld_te < n || !(k > 0) || !isfinite(k) || near_per_element <= 0 || nfix < 0 ||
         (nfix > 0 && !fix_rows))
@@ -129,15 +129,40 @@ extern "C" int grim_assemble(int64_t n, const double* p0, const double* p1, cons
     AssemblyWork w = carve(workspace, n, cap);
     if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
     hipStream_t st = (hipStream_t)stream;
-    zero_words<<<1, 64, 0, st>>>(w.status, 1);
-    CHECK(hipGetLastError());
-    const int q = far_order;
-    const int chunk = 64;
-    CHECK(grim_launch_far(q, (int)n, p0, p1, next, prev, k, chunk, z_tm, ld_tm, z_te, ld_te, fix_rows, (int)nfix,
-                          st));
-    CHECK(grim_launch_near((int)n, p0, p1, next, prev, k, w.counts, w.offsets, (int)cap, w.pair_e, w.pair_f,
-                           w.blocks, w.kind, w.status, z_tm, ld_tm, z_te, ld_te, st));
+    if (passes & 1) {
+        zero_words<<<1, 64, 0, st>>>(w.status, 1);
+        CHECK(hipGetLastError());
+        CHECK(grim_launch_far(far_order, (int)n, p0, p1, next, prev, k, z_tm, ld_tm, z_te, ld_te, fix_rows,
+                              (int)nfix, st));
+    }
+    if (passes & 2)
+        CHECK(grim_launch_near((int)n, p0, p1, next, prev, k, w.counts, w.offsets, (int)cap, w.pair_e, w.pair_f,
+                               w.blocks, w.kind, w.status, z_tm, ld_tm, z_te, ld_te, st));
     return GRIM_OK;
+}
+
+extern "C" int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                             const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
+                             double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, void* workspace,
+                             int64_t workspace_bytes, int64_t near_per_element, void* stream) {
+    return assemble_pass(3, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te,
+                         workspace, workspace_bytes, near_per_element, stream);
+}
+
+extern "C" int grim_assemble_far(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                 const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
+                                 double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, void* workspace,
+                                 int64_t workspace_bytes, int64_t near_per_element, void* stream) {
+    return assemble_pass(1, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te,
+                         workspace, workspace_bytes, near_per_element, stream);
+}
+
+extern "C" int grim_assemble_near(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                  const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k,
+                                  int far_order, double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te,
+                                  void* workspace, int64_t workspace_bytes, int64_t near_per_element, void* stream) {
+    return assemble_pass(2, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te,
+                         workspace, workspace_bytes, near_per_element, stream);
 }
 
 extern "C" int grim_workspace_status(void* workspace, void* stream) {

@@ -63,10 +63,25 @@ __device__ __forceinline__ void combine(const cplx (&S)[2][2], const cplx (&K)[2
         }
 }
 
-// Far element-pair block with the Q x Q Gauss rule (far.c:100-191 semantics,
+// Far blocks of one unordered element pair, both orientations:
+//   tm  = S + gamma D,        te  = K' - gamma W            (obs e, src f)
+//   tmt = (S + gamma K')^T,   tet = (D - gamma W)^T         (obs f, src e)
+// since S(f,e) = S(e,f)^T, K'(f,e) = D(e,f)^T, D(f,e) = K'(e,f)^T, W(f,e) = W(e,f)^T.
+struct Blk2 {
+    cplx tm[2][2], te[2][2], tmt[2][2], tet[2][2];
+};
+
+__device__ __forceinline__ void zero_blk2(Blk2& b) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) b.tm[a][c] = b.te[a][c] = b.tmt[a][c] = b.tet[a][c] = {0.0, 0.0};
+}
+
+// Far element-pair quadrature with the Q x Q Gauss rule (far.c:100-191 semantics,
 // kernels evaluated directly instead of through an interpolation table).
 template <int Q>
-__device__ __noinline__ void far_pair(const Elem& o, const Elem& s, double k, Blk& out) {
+__device__ __noinline__ void far_pair(const Elem& o, const Elem& s, double k, Blk2& out) {
     cplx S[2][2], K[2][2], D[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -84,8 +99,9 @@ __device__ __noinline__ void far_pair(const Elem& o, const Elem& s, double k, Bl
             cplx g, h;
             green_kernels(k, r, g, h);
             const double rr = r > GRIM_EPS ? r : GRIM_EPS;
-            const double po = (dx * o.nx + dy * o.ny) / rr;
-            const double pso = (dx * s.nx + dy * s.ny) / rr;
+            const double inv = 1.0 / rr;
+            const double po = (dx * o.nx + dy * o.ny) * inv;
+            const double pso = (dx * s.nx + dy * s.ny) * inv;
             const double w0 = wj * (1.0 - tj), w1 = wj * tj;
             ps0.re = fma(w0, g.re, ps0.re); ps0.im = fma(w0, g.im, ps0.im);
             ps1.re = fma(w1, g.re, ps1.re); ps1.im = fma(w1, g.im, ps1.im);
@@ -113,89 +129,140 @@ __device__ __noinline__ void far_pair(const Elem& o, const Elem& s, double k, Bl
             K[a][b] = cscale(K[a][b], scale);
             D[a][b] = cscale(D[a][b], scale);
         }
-    combine(S, K, D, o, s, k, out);
+    const double g = -1.0 / k;   // gamma = (0, g)
+    const double nn = o.nx * s.nx + o.ny * s.ny;
+    double denom = scale < GRIM_EPS * GRIM_EPS ? GRIM_EPS * GRIM_EPS : scale;
+    const cplx sum = cadd(cadd(S[0][0], S[0][1]), cadd(S[1][0], S[1][1]));
+    const cplx q = {sum.re / denom, sum.im / denom};
+    const double kk = -k * k * nn;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const double tsign = (a == b) ? 1.0 : -1.0;
+            const cplx w = {kk * S[a][b].re + tsign * q.re, kk * S[a][b].im + tsign * q.im};
+            out.tm[a][b] = {S[a][b].re - g * D[a][b].im, S[a][b].im + g * D[a][b].re};
+            out.te[a][b] = {K[a][b].re + g * w.im, K[a][b].im - g * w.re};
+            out.tmt[b][a] = {S[a][b].re - g * K[a][b].im, S[a][b].im + g * K[a][b].re};
+            out.tet[b][a] = {D[a][b].re + g * w.im, D[a][b].im - g * w.re};
+        }
 }
 
 template <int Q>
 __device__ __forceinline__ void far_or_zero(const Elem& o, int e, int ne, const Elem& s, int f, int nf,
-                                            double k, bool active, Blk& out) {
+                                            double k, bool active, Blk2& out) {
     if (active && is_far(o, e, ne, s, f, nf)) far_pair<Q>(o, s, k, out);
-    else zero_blk(out);
+    else zero_blk2(out);
 }
 
-// Far pass.  One wavefront = 64 consecutive observation elements (lanes) x a
-// chunk of source columns walked in order.  Column q of node row p collects
-// blk(p, prev q)[0][1] + blk(p, q)[0][0] + blk(prev p, prev q)[1][1] + blk(prev p, q)[1][0];
-// the row neighbour comes from lane - 1 by a shuffle.  Rows whose previous
-// element is not lane - 1 (lane 0, contour starts) are completed by far_fixup.
+__device__ __forceinline__ void store_add(double* z, int64_t ld, int p, int q, cplx v, bool add) {
+    double* t = z + 2 * ((int64_t)p + (int64_t)q * ld);
+    if (add) { t[0] += v.re; t[1] += v.im; }
+    else { t[0] = v.re; t[1] = v.im; }
+}
+
+// Far pass over unordered element pairs.  A wavefront owns 64 consecutive
+// observation elements (lanes, block I) and walks a chunk of `chunk` source
+// columns (inside column block J >= I) in order.  Node entry Z[p, q] gathers
+//   blk(p, prev q)[0][1] + blk(p, q)[0][0] + blk(prev p, prev q)[1][1] + blk(prev p, q)[1][0]:
+// the source-side neighbour is carried in registers along the walk, the
+// observation-side neighbour comes from lane - 1 by a shuffle.  For J > I the
+// same evaluations also give the transposed tile (rows in chunk, columns in
+// block I), written with the roles exchanged.  Every entry is written exactly
+// once, in a fixed summation order.  Entries whose observation-side neighbour
+// is not lane - 1 (lane 0, contour starts) are completed by far_fixup.
 template <int Q>
 __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                  const int* __restrict__ next, const int* __restrict__ prev,
                                                  double k, int chunk, double* __restrict__ ztm, int64_t ldtm,
                                                  double* __restrict__ zte, int64_t ldte) {
     const int lane = threadIdx.x;
-    const int e = blockIdx.x * 64 + lane;
+    // tile = (row block I, column chunk j) with j * chunk / 64 >= I, enumerated row block by row block
+    const int nc = (n + chunk - 1) / chunk, cpb = 64 / chunk, nb = (n + 63) / 64;
+    int2 tile = {0, 0};
+    for (int I = 0, idx = blockIdx.x; I < nb; ++I) {
+        const int cnt = nc - I * cpb;
+        if (idx < cnt) { tile.x = I; tile.y = I * cpb + idx; break; }
+        idx -= cnt;
+    }
+    const int e = tile.x * 64 + lane;
     const bool active = e < n;
     const int ee = active ? e : n - 1;
     const Elem o = load_elem(p0, p1, ee);
     const int ne = next[ee];
     const bool rowok = active && lane > 0 && prev[ee] == ee - 1;
-    const int c0 = blockIdx.y * chunk;
+    const int c0 = tile.y * chunk;
     const int c1 = min(n, c0 + chunk);
-    cplx pend_tm0 = {0, 0}, pend_tm1 = {0, 0}, pend_te0 = {0, 0}, pend_te1 = {0, 0};
+    const bool mirror = (c0 / 64) > tile.x;
+    cplx pd_tm0 = {0, 0}, pd_tm1 = {0, 0}, pd_te0 = {0, 0}, pd_te1 = {0, 0};   // direct: column next(f)
+    cplx pt_tm0 = {0, 0}, pt_tm1 = {0, 0}, pt_te0 = {0, 0}, pt_te1 = {0, 0};   // mirror: row next(f)
     for (int f = c0; f < c1; ++f) {
         const int pf = prev[f];
-        cplx ltm0, ltm1, lte0, lte1;
+        cplx ltm0, ltm1, lte0, lte1, mtm0, mtm1, mte0, mte1;
         if (f == c0 || pf != f - 1) {
-            const Elem sp = load_elem(p0, p1, pf);
-            Blk b;
-            far_or_zero<Q>(o, ee, ne, sp, pf, next[pf], k, active, b);
+            Blk2 b;
+            far_or_zero<Q>(o, ee, ne, load_elem(p0, p1, pf), pf, next[pf], k, active, b);
             ltm0 = b.tm[0][1]; ltm1 = b.tm[1][1]; lte0 = b.te[0][1]; lte1 = b.te[1][1];
+            mtm0 = b.tmt[1][0]; mtm1 = b.tmt[1][1]; mte0 = b.tet[1][0]; mte1 = b.tet[1][1];
         } else {
-            ltm0 = pend_tm0; ltm1 = pend_tm1; lte0 = pend_te0; lte1 = pend_te1;
+            ltm0 = pd_tm0; ltm1 = pd_tm1; lte0 = pd_te0; lte1 = pd_te1;
+            mtm0 = pt_tm0; mtm1 = pt_tm1; mte0 = pt_te0; mte1 = pt_te1;
         }
-        const Elem s = load_elem(p0, p1, f);
-        Blk b;
-        far_or_zero<Q>(o, ee, ne, s, f, next[f], k, active, b);
+        Blk2 b;
+        far_or_zero<Q>(o, ee, ne, load_elem(p0, p1, f), f, next[f], k, active, b);
+        // direct tile: column f, rows e (own, local 0) and next(e) (local 1 -> lane + 1)
         const cplx ctm0 = cadd(ltm0, b.tm[0][0]), ctm1 = cadd(ltm1, b.tm[1][0]);
         const cplx cte0 = cadd(lte0, b.te[0][0]), cte1 = cadd(lte1, b.te[1][0]);
-        pend_tm0 = b.tm[0][1]; pend_tm1 = b.tm[1][1]; pend_te0 = b.te[0][1]; pend_te1 = b.te[1][1];
-        cplx utm, ute;
+        pd_tm0 = b.tm[0][1]; pd_tm1 = b.tm[1][1]; pd_te0 = b.te[0][1]; pd_te1 = b.te[1][1];
+        // mirror tile: row f, columns e (local 0) and next(e) (local 1 -> lane + 1)
+        const cplx rtm0 = cadd(mtm0, b.tmt[0][0]), rtm1 = cadd(mtm1, b.tmt[0][1]);
+        const cplx rte0 = cadd(mte0, b.tet[0][0]), rte1 = cadd(mte1, b.tet[0][1]);
+        pt_tm0 = b.tmt[1][0]; pt_tm1 = b.tmt[1][1]; pt_te0 = b.tet[1][0]; pt_te1 = b.tet[1][1];
+        cplx utm, ute, vtm, vte;
         utm.re = __shfl_up(ctm1.re, 1, 64); utm.im = __shfl_up(ctm1.im, 1, 64);
         ute.re = __shfl_up(cte1.re, 1, 64); ute.im = __shfl_up(cte1.im, 1, 64);
+        vtm.re = __shfl_up(rtm1.re, 1, 64); vtm.im = __shfl_up(rtm1.im, 1, 64);
+        vte.re = __shfl_up(rte1.re, 1, 64); vte.im = __shfl_up(rte1.im, 1, 64);
         if (active) {
-            cplx vtm = ctm0, vte = cte0;
-            if (rowok) { vtm = cadd(vtm, utm); vte = cadd(vte, ute); }
-            double* t = ztm + 2 * ((int64_t)e + (int64_t)f * ldtm);
-            t[0] = vtm.re; t[1] = vtm.im;
-            double* u = zte + 2 * ((int64_t)e + (int64_t)f * ldte);
-            u[0] = vte.re; u[1] = vte.im;
+            store_add(ztm, ldtm, e, f, rowok ? cadd(ctm0, utm) : ctm0, false);
+            store_add(zte, ldte, e, f, rowok ? cadd(cte0, ute) : cte0, false);
+            if (mirror) {
+                store_add(ztm, ldtm, f, e, rowok ? cadd(rtm0, vtm) : rtm0, false);
+                store_add(zte, ldte, f, e, rowok ? cadd(rte0, vte) : rte0, false);
+            }
         }
     }
 }
 
-// Completes the rows far_kernel could not: Z[p, q] += blk(prev p, prev q)[1][1] + blk(prev p, q)[1][0].
+// Completes the entries far_kernel leaves short, for each listed index c
+// (c % 64 == 0 or prev[c] != c - 1), with e' = prev(c):
+//   row c    (entries written by a direct tile, block(q) >= block(c)):
+//       Z[c, q] += blk(e', prev q).tm[1][1] + blk(e', q).tm[1][0]
+//   column c (entries written by a mirror tile,  block(q) >  block(c)):
+//       Z[q, c] += blk(e', prev q).tmt[1][1] + blk(e', q).tmt[0][1]
 template <int Q>
 __global__ void __launch_bounds__(256) far_fixup(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                  const int* __restrict__ next, const int* __restrict__ prev,
                                                  double k, const int* __restrict__ rows, double* __restrict__ ztm,
                                                  int64_t ldtm, double* __restrict__ zte, int64_t ldte) {
-    const int p = rows[blockIdx.y];
+    const int c = rows[blockIdx.y];
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= n) return;
-    const int e = prev[p];
+    const int e = prev[c];
     const Elem o = load_elem(p0, p1, e);
     const int ne = next[e];
     const int pq = prev[q];
-    Blk b1, b2;
+    Blk2 b1, b2;
     far_or_zero<Q>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
     far_or_zero<Q>(o, e, ne, load_elem(p0, p1, q), q, next[q], k, true, b2);
-    const cplx vtm = cadd(b1.tm[1][1], b2.tm[1][0]);
-    const cplx vte = cadd(b1.te[1][1], b2.te[1][0]);
-    double* t = ztm + 2 * ((int64_t)p + (int64_t)q * ldtm);
-    t[0] += vtm.re; t[1] += vtm.im;
-    double* u = zte + 2 * ((int64_t)p + (int64_t)q * ldte);
-    u[0] += vte.re; u[1] += vte.im;
+    if (q / 64 >= c / 64) {
+        store_add(ztm, ldtm, c, q, cadd(b1.tm[1][1], b2.tm[1][0]), true);
+        store_add(zte, ldte, c, q, cadd(b1.te[1][1], b2.te[1][0]), true);
+    }
+    if (q / 64 > c / 64) {
+        store_add(ztm, ldtm, q, c, cadd(b1.tmt[1][1], b2.tmt[0][1]), true);
+        store_add(zte, ldte, q, c, cadd(b1.tet[1][1], b2.tet[0][1]), true);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -733,12 +800,16 @@ extern "C" hipError_t grim_upload_rules(const double* gl8t, const double* gl8w, 
     return hipMemcpyToSymbol(HIP_SYMBOL(c_touch_w), tw, 800 * sizeof(double));
 }
 
+// Far pass launcher: one wavefront per (row block I, column chunk j) with j in block J >= I.
 extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
-                                      const int* prev, double k, int chunk, double* ztm, int64_t ldtm, double* zte,
+                                      const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
                                       int64_t ldte, const int* fix_rows, int nfix, hipStream_t st) {
-    dim3 grid((n + 63) / 64, (n + chunk - 1) / chunk);
-    if (q == 8) far_kernel<8><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
-    else far_kernel<16><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+    const int chunk = n <= 4096 ? 16 : 32;
+    const int nc = (n + chunk - 1) / chunk, cpb = 64 / chunk, nb = (n + 63) / 64;
+    long ntiles = 0;
+    for (int I = 0; I < nb; ++I) ntiles += nc - I * cpb;
+    if (q == 8) far_kernel<8><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+    else far_kernel<16><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || nfix == 0) return err;
     dim3 g2((n + 255) / 256, nfix);

@@ -24,14 +24,19 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct Mfma;
+// Output element v of lane l of a 16 x 16 tile sits at column l % 16 and row
+// row(l / 16, v): f64 16x16x4 interleaves rows (l/16 + 4 v), f32 16x16x4 keeps
+// them contiguous (4 (l/16) + v) -- measured on gfx950.
 template <> struct Mfma<double> {
     typedef d4 acc;
+    static __device__ __forceinline__ int row(int fk, int v) { return fk + 4 * v; }
     static __device__ __forceinline__ acc run(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
 };
 template <> struct Mfma<float> {
     typedef f4 acc;
+    static __device__ __forceinline__ int row(int fk, int v) { return 4 * fk + v; }
     static __device__ __forceinline__ acc run(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -120,14 +125,14 @@ __global__ void __launch_bounds__(256) cgemm_minus(int m, int n, int k, const T*
         }
         __syncthreads();
     }
-    // lane holds D[4 * (lane / 16) + v][lane % 16] of each 16 x 16 tile
+    // lane holds D[row(lane / 16, v)][lane % 16] of each 16 x 16 tile
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int gr = m0 + wm + 16 * i + 4 * fk + v;
+                const int gr = m0 + wm + 16 * i + Mfma<T>::row(fk, v);
                 const int gc = n0 + wn + 16 * j + fr;
                 if (gr < m && gc < n) {
                     T* p = C + 2 * ((int64_t)gr + (int64_t)gc * ldc);

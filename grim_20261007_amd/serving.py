@@ -1,0 +1,248 @@
+"""Continuous-batching RCS request server on one GPU.
+
+A request is (geometry, frequency, polarization set, angle set).  The server
+admits requests by their HBM footprint while earlier ones are still running:
+each worker thread owns a HIP stream and runs one request at a time on it, so
+the kernels of several requests overlap on the device, and a new request is
+admitted as soon as a worker slot and enough of the byte budget are free
+(continuous batching, not a FIFO of whole batches).  Assembled systems and
+their LU factors stay resident in a device cache keyed by (geometry digest,
+wavenumber, precision) and are reused by later requests with new angle sets.
+
+Per-request latency (arrival to completion) and throughput are recorded for
+p50 / p99 reporting.  The memory estimate follows the reference's admission
+idea (twod/solver.py:1474-1554): every dense buffer of the request is counted
+before it is allocated.
+"""
+from __future__ import annotations
+
+import hashlib
+import queue
+import threading
+import time
+from collections import OrderedDict
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import engine
+from .mesh import ContourMesh
+from .solver import wavenumber
+
+
+@dataclass
+class Request:
+    mesh: ContourMesh
+    frequency_ghz: float
+    angles_deg: np.ndarray
+    pols: Tuple[str, ...] = ("TE", "TM")
+    precision: str = "mixed"
+    arrival: float = 0.0
+    rid: int = 0
+    key: Optional[str] = None          # cache key; None -> geometry digest
+
+
+@dataclass
+class Result:
+    rid: int
+    amplitude: Dict[str, np.ndarray]
+    sigma: Dict[str, np.ndarray]
+    residual: Dict[str, float]
+    cached: bool
+    latency_s: float
+    started: float
+    finished: float
+    info: Dict[str, dict] = field(default_factory=dict)
+
+
+def geometry_digest(mesh: ContourMesh) -> str:
+    h = hashlib.sha256()
+    for a in (mesh.p0, mesh.p1, mesh.next):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()[:24]
+
+
+def request_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool) -> int:
+    """Device bytes a request allocates (systems, factors, loads, RHS, solutions, workspace)."""
+    z = 16 * n * n
+    lu = (8 if precision == "mixed" else 16) * n * n
+    vec = 16 * n * nangles
+    work = 0 if cached else engine.NEAR_PER_ELEMENT * n * 160 + 64 * n
+    refine = (24 * n * nangles) if precision == "mixed" else 0
+    per_pol = vec * 3 + refine                       # rhs, solution, residual
+    systems = 0 if cached else 2 * (z + lu)          # both polarizations assembled together
+    return systems + 2 * vec + npols * per_pol + work
+
+
+class SystemCache:
+    """Device-resident assembled + factored systems, LRU within a byte budget."""
+
+    def __init__(self, budget_bytes: int):
+        self.budget = int(budget_bytes)
+        self.used = 0
+        self.items: "OrderedDict[tuple, dict]" = OrderedDict()
+        self.lock = threading.Lock()
+        self.hits = self.misses = 0
+
+    def get(self, key):
+        with self.lock:
+            item = self.items.get(key)
+            if item is not None:
+                self.items.move_to_end(key)
+                self.hits += 1
+            else:
+                self.misses += 1
+            return item
+
+    def put(self, key, item, nbytes):
+        with self.lock:
+            if nbytes > self.budget:
+                return False
+            while self.items and self.used + nbytes > self.budget:
+                _, old = self.items.popitem(last=False)
+                self.used -= old["bytes"]
+            item["bytes"] = nbytes
+            self.items[key] = item
+            self.used += nbytes
+            return True
+
+
+class Server:
+    def __init__(self, device=None, workers: int = 4, hbm_budget_bytes: Optional[int] = None,
+                 cache_bytes: Optional[int] = None):
+        self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        total = torch.cuda.get_device_properties(self.device).total_memory
+        self.budget = int(hbm_budget_bytes if hbm_budget_bytes is not None else 0.6 * total)
+        self.cache = SystemCache(cache_bytes if cache_bytes is not None else int(0.25 * total))
+        self.in_use = 0
+        self.cv = threading.Condition()
+        self.queue: "queue.Queue" = queue.Queue()
+        self.results: List[Result] = []
+        self.workers = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(workers)]
+        self.stop = False
+        self.errors: List[BaseException] = []
+        for t in self.workers:
+            t.start()
+
+    # -- admission -----------------------------------------------------------
+    def _admit(self, nbytes: int):
+        with self.cv:
+            if nbytes > self.budget:
+                raise MemoryError(f"request needs {nbytes / 2**30:.2f} GiB, budget {self.budget / 2**30:.2f} GiB")
+            while self.in_use + nbytes > self.budget:
+                self.cv.wait()
+            self.in_use += nbytes
+
+    def _release(self, nbytes: int):
+        with self.cv:
+            self.in_use -= nbytes
+            self.cv.notify_all()
+
+    def submit(self, req: Request) -> Future:
+        fut: Future = Future()
+        if req.arrival == 0.0:
+            req.arrival = time.perf_counter()
+        self.queue.put((req, fut))
+        return fut
+
+    def close(self):
+        for _ in self.workers:
+            self.queue.put(None)
+        for t in self.workers:
+            t.join()
+
+    # -- execution -----------------------------------------------------------
+    def _worker(self, index: int):
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(device=self.device)
+        ws = {}
+        while True:
+            item = self.queue.get()
+            if item is None:
+                return
+            req, fut = item
+            try:
+                with torch.cuda.stream(stream):
+                    res = self._run(req, ws)
+                fut.set_result(res)
+            except BaseException as exc:   # surfaced through the future
+                fut.set_exception(exc)
+
+    def _run(self, req: Request, ws: dict) -> Result:
+        mesh = req.mesh
+        k = wavenumber(req.frequency_ghz)
+        key = (req.key or geometry_digest(mesh), float(k), req.precision)
+        cached = self.cache.get(key)
+        nbytes = request_bytes(mesh.n, len(req.angles_deg), len(req.pols), req.precision, cached is not None)
+        self._admit(nbytes)
+        started = time.perf_counter()
+        try:
+            dev = self.device
+            contour = cached["contour"] if cached else engine.DeviceContour(mesh, dev)
+            if cached is None:
+                if mesh.n not in ws:
+                    ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
+                z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
+                facs = {"TM": engine.factor(z_tm, req.precision), "TE": engine.factor(z_te, req.precision)}
+                sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * mesh.n * mesh.n
+                self.cache.put(key, {"contour": contour, "factors": facs}, sysbytes)
+            else:
+                facs = cached["factors"]
+                # the cached tensors are read on this worker's stream too
+                cur = torch.cuda.current_stream()
+                for t in (contour.p0, contour.p1, contour.next, contour.prev, contour.fix_rows):
+                    t.record_stream(cur)
+                for fac in facs.values():
+                    for t in (fac.lu, fac.piv, fac.z):
+                        if t is not None:
+                            t.record_stream(cur)
+            angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
+            bu, bdn = engine.loads(contour, k, angles)
+            amp, sig, resid, info = {}, {}, {}, {}
+            pending = []
+            for pol in req.pols:
+                fac = facs[pol]
+                b = -(bu if pol == "TM" else bdn)
+                x, inf = engine.solve(fac, b, gate=(req.precision == "double"))
+                rel = engine.relative_residual(fac.z, x, b, inf.get("residual"))
+                a, s = engine.farfield(x, bu, bdn, k)
+                pending.append((pol, a, s, rel))
+                info[pol] = {kk: v for kk, v in inf.items() if kk != "residual"}
+            torch.cuda.current_stream().synchronize()
+            for pol, a, s, rel in pending:
+                amp[pol] = a.cpu().numpy()
+                sig[pol] = s.cpu().numpy()
+                resid[pol] = float(rel.max())
+            finished = time.perf_counter()
+            return Result(req.rid, amp, sig, resid, cached is not None, finished - req.arrival, started, finished,
+                          info)
+        finally:
+            self._release(nbytes)
+
+
+def run_stream(server: Server, requests: List[Request], rate_hz: Optional[float] = None,
+               rng: Optional[np.random.Generator] = None) -> Tuple[List[Result], float]:
+    """Submit requests with Poisson arrivals at rate_hz (None: all at once); wait for all."""
+    rng = rng or np.random.default_rng(0)
+    t0 = time.perf_counter()
+    futs = []
+    t_next = t0
+    for req in requests:
+        if rate_hz:
+            t_next += rng.exponential(1.0 / rate_hz)
+            delay = t_next - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+        req.arrival = time.perf_counter()
+        futs.append(server.submit(req))
+    results = [f.result() for f in futs]
+    return results, time.perf_counter() - t0
+
+
+def latency_summary(results: List[Result]) -> dict:
+    lat = np.array([r.latency_s for r in results]) * 1e3
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max())}

@@ -53,6 +53,19 @@ int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* 
                   double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
                   int64_t near_per_element, void* stream);
 
+/* The two passes of grim_assemble, for callers that time or overlap them:
+ * grim_assemble_far writes every entry of z_tm / z_te (far pairs; zero for
+ * near pairs) and must run first; grim_assemble_near adds the near-pair
+ * blocks and the mass terms.  Same arguments as grim_assemble. */
+int grim_assemble_far(int64_t n, const double* p0, const double* p1, const int32_t* next, const int32_t* prev,
+                      const int32_t* fix_rows, int64_t nfix, double k, int far_order, double* z_tm, int64_t ld_tm,
+                      double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
+                      int64_t near_per_element, void* stream);
+int grim_assemble_near(int64_t n, const double* p0, const double* p1, const int32_t* next, const int32_t* prev,
+                       const int32_t* fix_rows, int64_t nfix, double k, int far_order, double* z_tm, int64_t ld_tm,
+                       double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
+                       int64_t near_per_element, void* stream);
+
 /* Synchronises `stream` and returns the kernel status of the last
  * grim_assemble / grim_lu_factor that used `workspace` (0 ok, else a status
  * code as above). */
