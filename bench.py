@@ -82,8 +82,8 @@ def request(engine, contour, k, angles_t, precision, timer=None, keep=False):
     from grim_20261007_amd import native
     tm = timer.run if timer else (lambda name, fn: fn())
     n = contour.n
-    z_tm = torch.empty((n, n), dtype=torch.complex128, device=contour.device)
-    z_te = torch.empty((n, n), dtype=torch.complex128, device=contour.device)
+    pair_buf = torch.empty((2, n, n), dtype=torch.complex128, device=contour.device)
+    z_te, z_tm = pair_buf[0], pair_buf[1]
     ws = request.ws
     st = torch.cuda.current_stream().cuda_stream
     args = lambda: (n, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.next.data_ptr(),
@@ -93,13 +93,15 @@ def request(engine, contour, k, angles_t, precision, timer=None, keep=False):
     native.check("far", tm("assemble_far", lambda: native.lib().grim_assemble_far(*args())))
     native.check("near", tm("assemble_near", lambda: native.lib().grim_assemble_near(*args())))
     bu, bdn = tm("loads", lambda: engine.loads(contour, k, angles_t))
+    pair = z_tm._base                                   # [TE, TM]
+    rhs = torch.stack((-bdn, -bu))
+    fac = tm(f"factor_{precision}", lambda: engine.factor(pair, precision, check=False))
+    x, info = tm(f"solve_{precision}", lambda: engine.solve(fac, rhs, gate=(precision == "double")))
+    rel = tm("residual", lambda: engine.relative_residual(pair, x, rhs, info.get("residual")))
     out = {}
-    for pol, z, b in (("TE", z_te, -bdn), ("TM", z_tm, -bu)):
-        fac = tm(f"factor_{precision}", lambda: engine.factor(z, precision, check=False))
-        x, info = tm(f"solve_{precision}", lambda: engine.solve(fac, b, gate=(precision == "double")))
-        rel = tm("residual", lambda: engine.relative_residual(z, x, b, info.get("residual")))
-        amp, sig = tm("farfield", lambda: engine.farfield(x, bu, bdn, k))
-        out[pol] = (amp, sig, rel, info.get("refinement_steps", 0))
+    for i, pol in enumerate(("TE", "TM")):
+        amp, sig = tm("farfield", lambda: engine.farfield(x[i], bu, bdn, k))
+        out[pol] = (amp, sig, rel[i], info.get("refinement_steps", 0))
     native.check("assembly status", native.lib().grim_workspace_status(ws.data_ptr(), st))
     return out
 

@@ -22,19 +22,23 @@ extern "C" hipError_t grim_launch_loads(int n, int na, const double* p0, const d
 extern "C" hipError_t grim_launch_farfield(int n, int na, const double* x, int64_t ldx, const double* bu,
                                            const double* bdn, int64_t ldb, double k, double* amp, double* sigma,
                                            hipStream_t st);
-extern "C" hipError_t grim_dense_lu_factor(int n, void* a, int64_t lda, int single, int* piv, unsigned* status,
-                                           hipStream_t st);
-extern "C" hipError_t grim_dense_lu_solve(int n, int nrhs, const void* a, int64_t lda, int single, const int* piv,
-                                          void* b, int64_t ldb, hipStream_t st);
-extern "C" hipError_t grim_dense_gemm_minus(int m, int n, int k, const void* a, int64_t lda, const void* b,
-                                            int64_t ldb, void* c, int64_t ldc, int single, hipStream_t st);
-extern "C" hipError_t grim_dense_to_c64(int m, int n, const double* src, int64_t lds, float* dst, int64_t ldd,
-                                        hipStream_t st);
-extern "C" hipError_t grim_dense_copy(int m, int n, const double* src, int64_t lds, double* dst, int64_t ldd,
-                                      hipStream_t st);
-extern "C" hipError_t grim_dense_add_c64(int m, int n, const float* d, int64_t ldd, double* x, int64_t ldx,
-                                        hipStream_t st);
-extern "C" hipError_t grim_dense_colmax(int m, int n, const double* v, int64_t ld, double* out, hipStream_t st);
+extern "C" int64_t grim_dense_aux_bytes(int n, int single);
+extern "C" hipError_t grim_dense_lu_factor(int batch, int n, void* a, int64_t lda, int64_t sa, int single, int* piv,
+                                           int64_t spiv, void* aux, int64_t saux, unsigned* status, hipStream_t st);
+extern "C" hipError_t grim_dense_lu_solve(int batch, int n, int nrhs, const void* a, int64_t lda, int64_t sa,
+                                          int single, const void* aux, int64_t saux, void* b, int64_t ldb,
+                                          int64_t sb, void* w, int64_t sw, hipStream_t st);
+extern "C" hipError_t grim_dense_gemm_minus(int batch, int m, int n, int k, const void* a, int64_t lda, int64_t sa,
+                                            const void* b, int64_t ldb, int64_t sb, void* c, int64_t ldc,
+                                            int64_t sc, int single, hipStream_t st);
+extern "C" hipError_t grim_dense_to_c64(int batch, int m, int n, const double* src, int64_t lds, int64_t ss,
+                                        float* dst, int64_t ldd, int64_t sd, hipStream_t st);
+extern "C" hipError_t grim_dense_copy(int batch, int m, int n, const double* src, int64_t lds, int64_t ss,
+                                      double* dst, int64_t ldd, int64_t sd, hipStream_t st);
+extern "C" hipError_t grim_dense_add_c64(int batch, int m, int n, const float* d, int64_t ldd, int64_t sd, double* x,
+                                        int64_t ldx, int64_t sx, hipStream_t st);
+extern "C" hipError_t grim_dense_colmax(int batch, int m, int n, const double* v, int64_t ld, int64_t sv,
+                                        double* out, hipStream_t st);
 extern "C" hipError_t grim_dense_colnorm(int m, int n, const double* v, int64_t ld, double* out, hipStream_t st);
 
 #define CHECK(x)                                   \
@@ -50,10 +54,10 @@ __global__ void zero_words(unsigned* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-__global__ void zero_c128(int m, int n, double* x, int64_t ld) {
+__global__ void zero_c128(int m, int n, double* x, int64_t ld, int64_t stride) {
     const int r = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
     if (r >= m) return;
-    double* t = x + 2 * ((int64_t)r + (int64_t)c * ld);
+    double* t = x + 2 * (stride * blockIdx.z + (int64_t)r + (int64_t)c * ld);
     t[0] = 0.0;
     t[1] = 0.0;
 }
@@ -195,104 +199,128 @@ extern "C" int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx
     return GRIM_OK;
 }
 
-extern "C" int grim_lu_factor(int64_t n, void* a, int64_t lda, int precision, int32_t* piv, void* workspace,
+static bool batch_ok(int64_t batch) { return batch >= 1 && batch <= 65535; }
+
+extern "C" int64_t grim_lu_aux_bytes(int64_t n, int precision) {
+    if (n < 1 || n > 32768 || (precision != 64 && precision != 32)) return -1;
+    return grim_dense_aux_bytes((int)n, precision == 32);
+}
+
+extern "C" int grim_lu_factor(int64_t batch, int64_t n, void* a, int64_t lda, int64_t stride_a, int precision,
+                              int32_t* piv, int64_t stride_piv, void* aux, int64_t stride_aux, void* status,
                               void* stream) {
-    if (n < 1 || n > 32768 || !a || !piv || lda < n || (precision != 64 && precision != 32)) return GRIM_ERR_ARGS;
-    hipStream_t st = (hipStream_t)stream;
-    unsigned* status = (unsigned*)workspace;
-    if (status) {
-        zero_words<<<1, 64, 0, st>>>(status, 1);
-        CHECK(hipGetLastError());
-    } else {
+    if (!batch_ok(batch) || n < 1 || n > 32768 || !a || !piv || !aux || !status || lda < n ||
+        (precision != 64 && precision != 32))
         return GRIM_ERR_ARGS;
-    }
-    CHECK(grim_dense_lu_factor((int)n, a, lda, precision == 32, piv, status, st));
+    if (batch > 1 && (stride_a < lda * n || stride_piv < n || stride_aux < grim_lu_aux_bytes(n, precision) ||
+                      stride_aux % 256 != 0))
+        return GRIM_ERR_ARGS;
+    hipStream_t st = (hipStream_t)stream;
+    zero_words<<<1, 64, 0, st>>>((unsigned*)status, 1);
+    CHECK(hipGetLastError());
+    CHECK(grim_dense_lu_factor((int)batch, (int)n, a, lda, stride_a, precision == 32, piv, stride_piv, aux,
+                               stride_aux, (unsigned*)status, st));
     return GRIM_OK;
 }
 
-extern "C" int grim_lu_solve(int64_t n, int64_t nrhs, const void* lu, int64_t ldlu, int precision,
-                             const int32_t* piv, void* b, int64_t ldb, void* stream) {
-    if (n < 1 || nrhs < 0 || !lu || !piv || !b || ldlu < n || ldb < n || (precision != 64 && precision != 32))
+extern "C" int64_t grim_lu_solve_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs, int precision) {
+    if (!batch_ok(batch) || n < 1 || nrhs < 1) return -1;
+    return batch * n * nrhs * (precision == 32 ? 8 : 16);
+}
+
+extern "C" int grim_lu_solve(int64_t batch, int64_t n, int64_t nrhs, const void* lu, int64_t ldlu, int64_t stride_lu,
+                             int precision, const void* aux, int64_t stride_aux, void* b, int64_t ldb,
+                             int64_t stride_b, void* work, int64_t work_bytes, void* stream) {
+    if (!batch_ok(batch) || n < 1 || nrhs < 0 || !lu || !aux || !b || ldlu < n || ldb < n || nrhs > 65535 ||
+        (precision != 64 && precision != 32))
         return GRIM_ERR_ARGS;
     if (nrhs == 0) return GRIM_OK;
-    CHECK(grim_dense_lu_solve((int)n, (int)nrhs, lu, ldlu, precision == 32, piv, b, ldb, (hipStream_t)stream));
+    if (!work || work_bytes < grim_lu_solve_workspace_bytes(batch, n, nrhs, precision)) return GRIM_ERR_RANGE;
+    CHECK(grim_dense_lu_solve((int)batch, (int)n, (int)nrhs, lu, ldlu, stride_lu, precision == 32, aux, stride_aux,
+                              b, ldb, stride_b, work, n * nrhs, (hipStream_t)stream));
     return GRIM_OK;
 }
 
-extern "C" int grim_gemm_minus(int64_t m, int64_t n, int64_t k, const void* a, int64_t lda, const void* b,
-                               int64_t ldb, void* c, int64_t ldc, int precision, void* stream) {
-    if (m < 0 || n < 0 || k < 0 || !a || !b || !c || lda < m || ldb < k || ldc < m ||
-        (precision != 64 && precision != 32) || n > 65535 * 64)
+extern "C" int grim_gemm_minus(int64_t batch, int64_t m, int64_t n, int64_t k, const void* a, int64_t lda,
+                               int64_t stride_a, const void* b, int64_t ldb, int64_t stride_b, void* c, int64_t ldc,
+                               int64_t stride_c, int precision, void* stream) {
+    if (!batch_ok(batch) || m < 0 || n < 0 || k < 0 || !a || !b || !c || lda < m || ldb < k || ldc < m ||
+        (precision != 64 && precision != 32) || n > 65535 * 64 || m > 65535 * 64)
         return GRIM_ERR_ARGS;
-    CHECK(grim_dense_gemm_minus((int)m, (int)n, (int)k, a, lda, b, ldb, c, ldc, precision == 32,
-                                (hipStream_t)stream));
+    CHECK(grim_dense_gemm_minus((int)batch, (int)m, (int)n, (int)k, a, lda, stride_a, b, ldb, stride_b, c, ldc,
+                                stride_c, precision == 32, (hipStream_t)stream));
     return GRIM_OK;
 }
 
-extern "C" int64_t grim_refine_workspace_bytes(int64_t n, int64_t nrhs) {
-    if (n < 1 || nrhs < 1) return -1;
-    return align_up(16 * n * nrhs) + align_up(8 * n * nrhs) + 2 * align_up(8 * nrhs);
+extern "C" int64_t grim_refine_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs) {
+    if (!batch_ok(batch) || n < 1 || nrhs < 1) return -1;
+    return align_up(batch * 16 * n * nrhs) + 2 * align_up(batch * 8 * n * nrhs) + 2 * align_up(batch * 8 * nrhs);
 }
 
-extern "C" int grim_refined_solve(int64_t n, int64_t nrhs, const double* z, int64_t ldz, const float* lu,
-                                  int64_t ldlu, const int32_t* piv, const double* b, int64_t ldb, double* x,
-                                  int64_t ldx, int max_steps, double tol, void* workspace, int64_t workspace_bytes,
-                                  int* steps_out, double* error_out, void* stream) {
-    if (n < 1 || nrhs < 1 || !z || !lu || !piv || !b || !x || ldz < n || ldlu < n || ldb < n || ldx < n ||
-        max_steps < 0 || !(tol > 0) || !workspace || nrhs > 65535)
+extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz,
+                                  int64_t stride_z, const float* lu, int64_t ldlu, int64_t stride_lu, const void* aux,
+                                  int64_t stride_aux, const double* b, int64_t ldb, int64_t stride_b, double* x,
+                                  int64_t ldx, int64_t stride_x, int max_steps, double tol, void* workspace,
+                                  int64_t workspace_bytes, int* steps_out, double* error_out, void* stream) {
+    if (!batch_ok(batch) || n < 1 || nrhs < 1 || !z || !lu || !aux || !b || !x || ldz < n || ldlu < n || ldb < n ||
+        ldx < n || max_steps < 0 || !(tol > 0) || !workspace || nrhs > 65535)
         return GRIM_ERR_ARGS;
-    if (grim_refine_workspace_bytes(n, nrhs) > workspace_bytes) return GRIM_ERR_RANGE;
+    if (grim_refine_workspace_bytes(batch, n, nrhs) > workspace_bytes) return GRIM_ERR_RANGE;
     hipStream_t st = (hipStream_t)stream;
+    const int B = (int)batch, N = (int)n, R = (int)nrhs;
+    const int64_t snr = n * nrhs;   // per-matrix stride of the n x nrhs work arrays
     char* p = (char*)workspace;
-    double* r = (double*)p; p += align_up(16 * n * nrhs);
-    float* d = (float*)p; p += align_up(8 * n * nrhs);
-    double* bmax = (double*)p; p += align_up(8 * nrhs);
+    double* r = (double*)p; p += align_up(batch * 16 * n * nrhs);
+    float* d = (float*)p; p += align_up(batch * 8 * n * nrhs);
+    float* w = (float*)p; p += align_up(batch * 8 * n * nrhs);
+    double* bmax = (double*)p; p += align_up(batch * 8 * nrhs);
     double* rmax = (double*)p;
-    const int N = (int)n, R = (int)nrhs;
-    std::vector<double> hb(R), hr(R);
-    CHECK(grim_dense_colmax(N, R, b, ldb, bmax, st));
-    CHECK(hipMemcpyAsync(hb.data(), bmax, 8 * R, hipMemcpyDeviceToHost, st));
+    std::vector<double> hb((size_t)B * R), hr((size_t)B * R);
+    CHECK(grim_dense_colmax(B, N, R, b, ldb, stride_b, bmax, st));
+    CHECK(hipMemcpyAsync(hb.data(), bmax, 8 * (size_t)B * R, hipMemcpyDeviceToHost, st));
     // x = LU32^-1 b
-    CHECK(grim_dense_to_c64(N, R, b, ldb, d, n, st));
-    CHECK(grim_dense_lu_solve(N, R, lu, ldlu, 1, piv, d, n, st));
-    zero_c128<<<dim3((N + 255) / 256, R), 256, 0, st>>>(N, R, x, ldx);
+    CHECK(grim_dense_to_c64(B, N, R, b, ldb, stride_b, d, n, snr, st));
+    CHECK(grim_dense_lu_solve(B, N, R, lu, ldlu, stride_lu, 1, aux, stride_aux, d, n, snr, w, snr, st));
+    zero_c128<<<dim3((N + 255) / 256, R, B), 256, 0, st>>>(N, R, x, ldx, stride_x);
     CHECK(hipGetLastError());
-    CHECK(grim_dense_add_c64(N, R, d, n, x, ldx, st));
+    CHECK(grim_dense_add_c64(B, N, R, d, n, snr, x, ldx, stride_x, st));
     CHECK(hipStreamSynchronize(st));
-    for (int c = 0; c < R; ++c) hb[c] = hb[c] > 0 ? hb[c] : 1.0;
+    for (auto& v : hb) v = v > 0 ? v : 1.0;
     double previous = INFINITY;
     for (int step = 0; step < max_steps; ++step) {
         // r = b - Z x (FP64)
-        CHECK(grim_dense_copy(N, R, b, ldb, r, n, st));
-        CHECK(grim_dense_gemm_minus(N, R, N, z, ldz, x, ldx, r, n, 0, st));
-        CHECK(grim_dense_colmax(N, R, r, n, rmax, st));
-        CHECK(hipMemcpyAsync(hr.data(), rmax, 8 * R, hipMemcpyDeviceToHost, st));
+        CHECK(grim_dense_copy(B, N, R, b, ldb, stride_b, r, n, snr, st));
+        CHECK(grim_dense_gemm_minus(B, N, R, N, z, ldz, stride_z, x, ldx, stride_x, r, n, snr, 0, st));
+        CHECK(grim_dense_colmax(B, N, R, r, n, snr, rmax, st));
+        CHECK(hipMemcpyAsync(hr.data(), rmax, 8 * (size_t)B * R, hipMemcpyDeviceToHost, st));
         CHECK(hipStreamSynchronize(st));
         double err = 0.0;
-        for (int c = 0; c < R; ++c) err = fmax(err, hr[c] / hb[c]);
+        for (size_t c = 0; c < hr.size(); ++c) err = fmax(err, hr[c] / hb[c]);
         if (error_out) *error_out = err;
         if (steps_out) *steps_out = step;
         if (isfinite(err) && err <= tol) return GRIM_OK;
         if (!isfinite(err) || err >= 0.98 * previous) return GRIM_ERR_NOT_CONVERGED;
         previous = err;
-        CHECK(grim_dense_to_c64(N, R, r, n, d, n, st));
-        CHECK(grim_dense_lu_solve(N, R, lu, ldlu, 1, piv, d, n, st));
-        CHECK(grim_dense_add_c64(N, R, d, n, x, ldx, st));
+        CHECK(grim_dense_to_c64(B, N, R, r, n, snr, d, n, snr, st));
+        CHECK(grim_dense_lu_solve(B, N, R, lu, ldlu, stride_lu, 1, aux, stride_aux, d, n, snr, w, snr, st));
+        CHECK(grim_dense_add_c64(B, N, R, d, n, snr, x, ldx, stride_x, st));
     }
     return GRIM_ERR_NOT_CONVERGED;
 }
 
-extern "C" int grim_to_c64(int64_t m, int64_t n, const double* src, int64_t lds, float* dst, int64_t ldd,
-                           void* stream) {
-    if (m < 0 || n < 0 || !src || !dst || lds < m || ldd < m || n > 65535) return GRIM_ERR_ARGS;
-    CHECK(grim_dense_to_c64((int)m, (int)n, src, lds, dst, ldd, (hipStream_t)stream));
+extern "C" int grim_to_c64(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds, int64_t stride_src,
+                           float* dst, int64_t ldd, int64_t stride_dst, void* stream) {
+    if (!batch_ok(batch) || m < 0 || n < 0 || !src || !dst || lds < m || ldd < m || n > 65535) return GRIM_ERR_ARGS;
+    CHECK(grim_dense_to_c64((int)batch, (int)m, (int)n, src, lds, stride_src, dst, ldd, stride_dst,
+                            (hipStream_t)stream));
     return GRIM_OK;
 }
 
-extern "C" int grim_copy_c128(int64_t m, int64_t n, const double* src, int64_t lds, double* dst, int64_t ldd,
-                              void* stream) {
-    if (m < 0 || n < 0 || !src || !dst || lds < m || ldd < m || n > 65535) return GRIM_ERR_ARGS;
-    CHECK(grim_dense_copy((int)m, (int)n, src, lds, dst, ldd, (hipStream_t)stream));
+extern "C" int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds,
+                              int64_t stride_src, double* dst, int64_t ldd, int64_t stride_dst, void* stream) {
+    if (!batch_ok(batch) || m < 0 || n < 0 || !src || !dst || lds < m || ldd < m || n > 65535) return GRIM_ERR_ARGS;
+    CHECK(grim_dense_copy((int)batch, (int)m, (int)n, src, lds, stride_src, dst, ldd, stride_dst,
+                          (hipStream_t)stream));
     return GRIM_OK;
 }
 

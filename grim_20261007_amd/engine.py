@@ -50,11 +50,15 @@ def assembly_workspace(n: int, device) -> torch.Tensor:
 
 
 def assemble(contour: DeviceContour, k: float, workspace: torch.Tensor | None = None, check: bool = True):
-    """TM and TE system matrices (complex128, column-major) from one fused traversal."""
+    """TM and TE system matrices (complex128, column-major) from one fused traversal.
+
+    Returns (z_tm, z_te), two views of one (2, N, N) tensor ``pair`` = [TE, TM]
+    (also reachable as ``z_tm._base``) so that both can be factored and solved in
+    batched launches."""
     n = contour.n
     dev = contour.device
-    z_tm = torch.empty((n, n), dtype=torch.complex128, device=dev)
-    z_te = torch.empty((n, n), dtype=torch.complex128, device=dev)
+    pair = torch.empty((2, n, n), dtype=torch.complex128, device=dev)
+    z_te, z_tm = pair[0], pair[1]
     ws = workspace if workspace is not None else assembly_workspace(n, dev)
     st = _stream(dev)
     code = native.lib().grim_assemble(
@@ -93,75 +97,107 @@ def farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
 
 @dataclass
 class Factor:
+    """LU factors of a batch of equally sized systems (column-major, batch first)."""
     precision: str              # "double" or "mixed"
-    lu: torch.Tensor            # complex128 or complex64 factors (column-major)
-    piv: torch.Tensor           # int32
-    z: torch.Tensor | None      # complex128 system (kept for residuals)
+    lu: torch.Tensor            # (B, N, N) complex128 or complex64
+    piv: torch.Tensor           # (B, N) int32
+    aux: torch.Tensor           # (B, aux_bytes) uint8: permutation + inverse diagonal blocks
+    z: torch.Tensor | None      # (B, N, N) complex128 systems (for residuals)
     n: int
+    batch: int
+
+    @property
+    def prec(self) -> int:
+        return 64 if self.precision == "double" else 32
+
+
+def _as_batch(t: torch.Tensor) -> torch.Tensor:
+    return t.unsqueeze(0) if t.dim() == 2 else t
 
 
 def factor(z: torch.Tensor, precision: str = "double", keep_matrix: bool = True,
            status: torch.Tensor | None = None, check: bool = True) -> Factor:
-    n = z.shape[0]
+    """Factor one system (N, N) or a batch (B, N, N) of complex128 systems."""
+    zb = _as_batch(z)
+    if not zb.is_contiguous():
+        raise ValueError("systems must be contiguous")
+    b, n = zb.shape[0], zb.shape[-1]
     dev = z.device
     st = _stream(dev)
     if precision == "double":
-        lu = z.clone()
+        lu = zb.clone()
         prec = 64
     elif precision == "mixed":
-        lu = torch.empty((n, n), dtype=torch.complex64, device=dev)
-        native.check("grim_to_c64", native.lib().grim_to_c64(n, n, z.data_ptr(), n, lu.data_ptr(), n, st))
+        lu = torch.empty((b, n, n), dtype=torch.complex64, device=dev)
+        native.check("grim_to_c64", native.lib().grim_to_c64(b, n, n, zb.data_ptr(), n, n * n, lu.data_ptr(), n,
+                                                             n * n, st))
         prec = 32
     else:
         raise ValueError("precision must be 'double' or 'mixed'")
-    piv = torch.empty(n, dtype=torch.int32, device=dev)
+    piv = torch.empty((b, n), dtype=torch.int32, device=dev)
+    aux_bytes = int(native.lib().grim_lu_aux_bytes(n, prec))
+    aux = torch.empty((b, aux_bytes), dtype=torch.uint8, device=dev)
     word = status if status is not None else torch.zeros(64, dtype=torch.uint8, device=dev)
-    native.check("grim_lu_factor", native.lib().grim_lu_factor(n, lu.data_ptr(), n, prec, piv.data_ptr(),
-                                                               word.data_ptr(), st))
+    native.check("grim_lu_factor", native.lib().grim_lu_factor(b, n, lu.data_ptr(), n, n * n, prec, piv.data_ptr(),
+                                                               n, aux.data_ptr(), aux_bytes, word.data_ptr(), st))
     if check:
         code = native.lib().grim_workspace_status(word.data_ptr(), st)
         if code == 8:
             raise np.linalg.LinAlgError("zero pivot in the LU factorization")
         native.check("grim_lu_factor (device)", code)
-    return Factor(precision, lu, piv, z if keep_matrix else None, n)
+    return Factor(precision, lu, piv, aux, zb if keep_matrix else None, n, b)
+
+
+def lu_solve_inplace(f: Factor, x: torch.Tensor) -> torch.Tensor:
+    """x <- A^{-1} x for x of shape (B, A, N) in the factor's precision dtype."""
+    b, na, n = x.shape
+    st = _stream(x.device)
+    nbytes = int(native.lib().grim_lu_solve_workspace_bytes(b, n, na, f.prec))
+    work = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    native.check("grim_lu_solve", native.lib().grim_lu_solve(b, n, na, f.lu.data_ptr(), n, n * n, f.prec,
+                                                             f.aux.data_ptr(), f.aux.shape[1], x.data_ptr(), n,
+                                                             na * n, work.data_ptr(), nbytes, st))
+    return x
 
 
 def residual(z: torch.Tensor, x: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """r = b - Z x in FP64 (complex128)."""
-    na, n = b.shape
-    r = b.clone()
-    native.check("grim_gemm_minus", native.lib().grim_gemm_minus(n, na, n, z.data_ptr(), n, x.data_ptr(), n,
-                                                                 r.data_ptr(), n, 64, _stream(z.device)))
-    return r
+    """r = b - Z x in FP64 (complex128); batched shapes (B, N, N), (B, A, N)."""
+    zb, xb, bb = _as_batch(z), _as_batch(x), _as_batch(b)
+    nb, na, n = bb.shape
+    r = bb.clone()
+    native.check("grim_gemm_minus", native.lib().grim_gemm_minus(nb, n, na, n, zb.data_ptr(), n, n * n,
+                                                                 xb.data_ptr(), n, na * n, r.data_ptr(), n, na * n,
+                                                                 64, _stream(z.device)))
+    return r if b.dim() == 3 else r[0]
 
 
 def solve(f: Factor, b: torch.Tensor, gate: bool = True):
-    """x with Z x = b for every column of b (A, N); returns (x, info)."""
-    na, n = b.shape
+    """x with Z x = b for every column: b is (A, N) (batch 1) or (B, A, N); returns (x, info)."""
+    bb = _as_batch(b).contiguous()
+    nb, na, n = bb.shape
     dev = b.device
     st = _stream(dev)
     info = {"refinement_steps": 0}
     if f.precision == "double":
-        x = b.clone()
-        native.check("grim_lu_solve", native.lib().grim_lu_solve(n, na, f.lu.data_ptr(), n, 64, f.piv.data_ptr(),
-                                                                 x.data_ptr(), n, st))
+        x = lu_solve_inplace(f, bb.clone())
         if gate and f.z is not None:
-            x, info = _backward_error_gate(f, x, b)
-        return x, info
-    x = torch.empty_like(b)
-    nbytes = native.lib().grim_refine_workspace_bytes(n, na)
+            x, info = _backward_error_gate(f, x, bb)
+        return (x if b.dim() == 3 else x[0]), info
+    x = torch.empty_like(bb)
+    nbytes = native.lib().grim_refine_workspace_bytes(nb, n, na)
     ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
     steps = ctypes_int()
     err = ctypes_double()
-    code = native.lib().grim_refined_solve(n, na, f.z.data_ptr(), n, f.lu.data_ptr(), n, f.piv.data_ptr(),
-                                           b.data_ptr(), n, x.data_ptr(), n, REFINE_MAX_STEPS, REFINE_TOL,
-                                           ws.data_ptr(), ws.numel(), steps, err, st)
+    code = native.lib().grim_refined_solve(nb, n, na, f.z.data_ptr(), n, n * n, f.lu.data_ptr(), n, n * n,
+                                           f.aux.data_ptr(), f.aux.shape[1], bb.data_ptr(), n, na * n,
+                                           x.data_ptr(), n, na * n, REFINE_MAX_STEPS, REFINE_TOL, ws.data_ptr(),
+                                           ws.numel(), steps, err, st)
     info["refinement_steps"] = int(steps.value)
     info["refinement_error"] = float(err.value)
     if code == 6:
         raise np.linalg.LinAlgError("Mixed-precision refinement stalled; double LU required.")
     native.check("grim_refined_solve", code)
-    return x, info
+    return (x if b.dim() == 3 else x[0]), info
 
 
 def ctypes_int():
@@ -177,24 +213,21 @@ def ctypes_double():
 def _backward_error_gate(f: Factor, x: torch.Tensor, b: torch.Tensor):
     """Normwise backward error <= 1e-12 with at most two corrections (linalg/dense.py:257-280)."""
     z = f.z
-    a_inf = float(torch.max(torch.sum(torch.abs(z), dim=0)))   # column-major: rows of Z are dim 0 sums
+    # column-major storage: row sums of |Z| are sums over dim 1 of each (N, N) tensor
+    a_inf = torch.amax(torch.sum(torch.abs(z), dim=1), dim=1)          # (B,)
     steps = 0
 
     def metrics(xx):
         r = residual(z, xx, b)
-        ri = torch.max(torch.abs(r), dim=1).values
-        den = a_inf * torch.max(torch.abs(xx), dim=1).values + torch.max(torch.abs(b), dim=1).values
+        ri = torch.amax(torch.abs(r), dim=2)                            # (B, A)
+        den = a_inf[:, None] * torch.amax(torch.abs(xx), dim=2) + torch.amax(torch.abs(b), dim=2)
         return r, float(torch.max(ri / den))
 
     r, err = metrics(x)
     for _ in range(2):
         if err <= BACKWARD_ERROR_MAX:
             break
-        d = r.clone()
-        native.check("grim_lu_solve", native.lib().grim_lu_solve(f.n, d.shape[0], f.lu.data_ptr(), f.n, 64,
-                                                                 f.piv.data_ptr(), d.data_ptr(), f.n,
-                                                                 _stream(z.device)))
-        cand = x + d
+        cand = x + lu_solve_inplace(f, r.clone())
         r2, err2 = metrics(cand)
         if err2 >= err:
             break
@@ -206,9 +239,9 @@ def _backward_error_gate(f: Factor, x: torch.Tensor, b: torch.Tensor):
 
 
 def relative_residual(z: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor | None = None):
-    """||b - Z x||_2 / ||b||_2 per column (the reference's linear_residual)."""
+    """||b - Z x||_2 / ||b||_2 per column (the reference's linear_residual); batched or not."""
     if r is None:
         r = residual(z, x, b)
-    nr = torch.linalg.vector_norm(r, dim=1)
-    nb = torch.linalg.vector_norm(b, dim=1)
+    nr = torch.linalg.vector_norm(r, dim=-1)
+    nb = torch.linalg.vector_norm(b, dim=-1)
     return nr / torch.where(nb <= 1e-12, torch.ones_like(nb), nb)

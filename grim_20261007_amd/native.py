@@ -31,14 +31,18 @@ SIGNATURES = {
     "grim_workspace_status": (_i32, [_vp, _vp]),
     "grim_loads": (_i32, [_i64, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _i64, _vp]),
     "grim_farfield": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp, _i64, _dbl, _vp, _vp, _vp]),
-    "grim_lu_factor": (_i32, [_i64, _vp, _i64, _i32, _vp, _vp, _vp]),
-    "grim_lu_solve": (_i32, [_i64, _i64, _vp, _i64, _i32, _vp, _vp, _i64, _vp]),
-    "grim_gemm_minus": (_i32, [_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _vp]),
-    "grim_refine_workspace_bytes": (_i64, [_i64, _i64]),
-    "grim_refined_solve": (_i32, [_i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _i32, _dbl, _vp,
-                                  _i64, ctypes.POINTER(_i32), ctypes.POINTER(_dbl), _vp]),
-    "grim_to_c64": (_i32, [_i64, _i64, _vp, _i64, _vp, _i64, _vp]),
-    "grim_copy_c128": (_i32, [_i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    "grim_lu_aux_bytes": (_i64, [_i64, _i32]),
+    "grim_lu_factor": (_i32, [_i64, _i64, _vp, _i64, _i64, _i32, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "grim_lu_solve_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32]),
+    "grim_lu_solve": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _i32, _vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp]),
+    "grim_gemm_minus": (_i32, [_i64, _i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i32,
+                               _vp]),
+    "grim_refine_workspace_bytes": (_i64, [_i64, _i64, _i64]),
+    "grim_refined_solve": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64,
+                                  _vp, _i64, _i64, _i32, _dbl, _vp, _i64, ctypes.POINTER(_i32),
+                                  ctypes.POINTER(_dbl), _vp]),
+    "grim_to_c64": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
+    "grim_copy_c128": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "grim_column_norms": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp]),
 }
 

@@ -186,36 +186,34 @@ class Server:
                 if mesh.n not in ws:
                     ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
                 z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
-                facs = {"TM": engine.factor(z_tm, req.precision), "TE": engine.factor(z_te, req.precision)}
+                fac = engine.factor(z_tm._base, req.precision)          # batch [TE, TM]
                 sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * mesh.n * mesh.n
-                self.cache.put(key, {"contour": contour, "factors": facs}, sysbytes)
+                self.cache.put(key, {"contour": contour, "factor": fac}, sysbytes)
             else:
-                facs = cached["factors"]
+                fac = cached["factor"]
                 # the cached tensors are read on this worker's stream too
                 cur = torch.cuda.current_stream()
-                for t in (contour.p0, contour.p1, contour.next, contour.prev, contour.fix_rows):
+                for t in (contour.p0, contour.p1, contour.next, contour.prev, contour.fix_rows,
+                          fac.lu, fac.piv, fac.aux, fac.z):
                     t.record_stream(cur)
-                for fac in facs.values():
-                    for t in (fac.lu, fac.piv, fac.z):
-                        if t is not None:
-                            t.record_stream(cur)
             angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
             bu, bdn = engine.loads(contour, k, angles)
+            rhs = torch.stack((-bdn, -bu))                             # [TE, TM]
+            x, inf = engine.solve(fac, rhs, gate=(req.precision == "double"))
+            rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
             amp, sig, resid, info = {}, {}, {}, {}
             pending = []
-            for pol in req.pols:
-                fac = facs[pol]
-                b = -(bu if pol == "TM" else bdn)
-                x, inf = engine.solve(fac, b, gate=(req.precision == "double"))
-                rel = engine.relative_residual(fac.z, x, b, inf.get("residual"))
-                a, s = engine.farfield(x, bu, bdn, k)
-                pending.append((pol, a, s, rel))
+            for i, pol in enumerate(("TE", "TM")):
+                if pol not in req.pols:
+                    continue
+                a, s = engine.farfield(x[i], bu, bdn, k)
+                pending.append((pol, a, s, rel[i]))
                 info[pol] = {kk: v for kk, v in inf.items() if kk != "residual"}
             torch.cuda.current_stream().synchronize()
-            for pol, a, s, rel in pending:
+            for pol, a, s, r in pending:
                 amp[pol] = a.cpu().numpy()
                 sig[pol] = s.cpu().numpy()
-                resid[pol] = float(rel.max())
+                resid[pol] = float(r.max())
             finished = time.perf_counter()
             return Result(req.rid, amp, sig, resid, cached is not None, finished - req.arrival, started, finished,
                           info)

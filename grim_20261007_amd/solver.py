@@ -72,22 +72,24 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     z_tm, z_te = engine.assemble(contour, k)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, angles)
+    if abort_event is not None and abort_event.is_set():
+        raise InterruptedError("Solve canceled by user.")
+    pair = z_tm._base if z_tm._base is not None else torch.stack((z_te, z_tm))   # [TE, TM]
+    order = [p for p in ("TE", "TM") if p in pols]
+    systems = pair if len(order) == 2 else (z_te if order[0] == "TE" else z_tm).unsqueeze(0)
+    rhs = torch.stack([-(bdn if p == "TE" else bu) for p in order])
+    fac = engine.factor(systems.contiguous(), precision)
+    x, info = engine.solve(fac, rhs)
+    rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
     out = {}
-    for pol in pols:
-        if abort_event is not None and abort_event.is_set():
-            raise InterruptedError("Solve canceled by user.")
-        z = z_tm if pol == "TM" else z_te
-        b = -(bu if pol == "TM" else bdn)
-        fac = engine.factor(z, precision)
-        x, info = engine.solve(fac, b)
-        rel = engine.relative_residual(z, x, b, info.get("residual"))
-        amp, sigma = engine.farfield(x, bu, bdn, k)
+    for i, pol in enumerate(order):
+        amp, sigma = engine.farfield(x[i], bu, bdn, k)
         res = {"amplitude": amp.cpu().numpy(), "sigma": sigma.cpu().numpy(),
-               "residual": rel.cpu().numpy(), "info": info, "k": k, "n": mesh.n}
+               "residual": rel[i].cpu().numpy(), "info": {kk: v for kk, v in info.items() if kk != "residual"},
+               "k": k, "n": mesh.n}
         if return_density:
-            res["density"] = x.cpu().numpy().T      # (N, A) nodal densities
+            res["density"] = x[i].cpu().numpy().T      # (N, A) nodal densities
         out[pol] = res
-        del fac
     return out
 
 

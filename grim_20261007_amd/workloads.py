@@ -115,3 +115,50 @@ def frequency_for_segments_per_wavelength(points: np.ndarray, per_wavelength: fl
     longest = float(np.max(np.linalg.norm(np.roll(p, -1, axis=0) - p, axis=1)))
     wavelength = longest * per_wavelength
     return C0 / wavelength / 1e9
+
+
+def star_sweep(seed: int = 20261203, nfreq: int = 64, fmin_ghz: float = 0.3, fmax_ghz: float = 3.0,
+               per_wavelength: float = 100.0, radius: float = 0.5, nangles: int = 360):
+    """Batched frequency sweep of one random star-shaped PEC curve.
+
+    N follows the mesh density: ``per_wavelength`` panels per wavelength of the
+    LONGEST panel at each frequency.  At 10 panels per wavelength the
+    R = 0.5 m curves give N of about 35-370; the sweep's stated range of about
+    100-3200 unknowns needs about 100 panels per wavelength, the default here.
+    Returns a list of (points, frequency_ghz, angles).
+    """
+    rng = np.random.default_rng(seed)
+    a, phi = star_shape(rng)
+    freqs = np.logspace(np.log10(fmin_ghz), np.log10(fmax_ghz), nfreq)
+    probe = star_points(4096, radius, a, phi)
+    perim = contour_perimeter(probe)
+    out = []
+    for f in freqs:
+        lam = C0 / (f * 1e9)
+        n = max(32, int(math.ceil(perim / lam * per_wavelength)))
+        out.append((star_points(n, radius, a, phi), float(f), np.arange(nangles) * (360.0 / nangles)))
+    return out
+
+
+def serving_stream(count: int, seed: int = 7, nmin: int = 200, nmax: int = 12000, reuse: float = 0.3,
+                   angle_sets=(90, 360, 1440), per_wavelength: float = 10.0, radius: float = 0.5):
+    """Requests of the serving configuration: star-shaped curves, N log-uniform in [nmin, nmax],
+    angle-set size drawn from angle_sets, a fraction ``reuse`` reusing an earlier (geometry,
+    frequency) with a new angle set.  Frequency is set so the longest panel is
+    lambda / per_wavelength.  Returns a list of dicts (points, frequency_ghz, angles, key)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nangles = int(rng.choice(angle_sets))
+        angles = np.sort(rng.uniform(0.0, 360.0, nangles))
+        if out and rng.uniform() < reuse:
+            base = out[int(rng.integers(len(out)))]
+            out.append({"points": base["points"], "frequency_ghz": base["frequency_ghz"], "angles": angles,
+                        "key": base["key"], "reused": True})
+            continue
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        a, phi = star_shape(rng)
+        pts = star_points(n, radius, a, phi)
+        f = frequency_for_segments_per_wavelength(pts, per_wavelength)
+        out.append({"points": pts, "frequency_ghz": f, "angles": angles, "key": f"req{i}", "reused": False})
+    return out

@@ -87,41 +87,59 @@ int grim_loads(int64_t n, int64_t na, const double* p0, const double* p1, const 
 int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx, const double* bu, const double* bdn,
                   int64_t ldb, double k, double* amp, double* sigma, void* stream);
 
-/* In-place LU with partial pivoting of an n x n complex matrix: precision 64
- * (complex128) or 32 (complex64).  piv: int32 [n].  Replaces
- * scipy.linalg.lu_factor as used by linalg/dense.py:114-155 and
- * linalg/refined_lu.py:26-39.  Pivots use the library's panel-delayed
- * convention; factors are only meaningful to grim_lu_solve.  workspace: a
- * device uint32 status word (read with grim_workspace_status) that receives
- * the zero-pivot flag.  n <= 32768. */
-int grim_lu_factor(int64_t n, void* a, int64_t lda, int precision, int32_t* piv, void* workspace, void* stream);
+/* Batched dense LU.  All dense calls take `batch` equally strided matrices
+ * (strides in complex elements for matrices/vectors, in elements for int32
+ * arrays, in bytes for aux), e.g. the TE and TM systems of one request.
+ *
+ * Bytes of factor-side auxiliary storage per matrix (the row permutation the
+ * pivots compose to, 4 n bytes rounded up to 256); precision 64 or 32. */
+int64_t grim_lu_aux_bytes(int64_t n, int precision);
 
-/* Solve A X = B in place for nrhs columns with the factors of grim_lu_factor.
- * Replaces scipy.linalg.lu_solve (linalg/dense.py:239-301). */
-int grim_lu_solve(int64_t n, int64_t nrhs, const void* lu, int64_t ldlu, int precision, const int32_t* piv,
-                  void* b, int64_t ldb, void* stream);
+/* In-place LU with partial pivoting, P A = L U (LAPACK pivot rule: first row of
+ * largest |re| + |im|), complex128 (precision 64) or complex64 (32), n <= 32768.
+ * piv: int32 [n] per matrix (absolute row exchanged with row c, LAPACK getrf
+ * order, 0-based); aux: grim_lu_aux_bytes per matrix, stride a multiple of 256.
+ * status: device uint32 word (read with grim_workspace_status).  Replaces
+ * scipy.linalg.lu_factor as used by linalg/dense.py:114-155 (zgetrf) and
+ * linalg/refined_lu.py:26-39 (cgetrf on the complex64 copy). */
+int grim_lu_factor(int64_t batch, int64_t n, void* a, int64_t lda, int64_t stride_a, int precision, int32_t* piv,
+                   int64_t stride_piv, void* aux, int64_t stride_aux, void* status, void* stream);
+
+/* Bytes of workspace grim_lu_solve needs. */
+int64_t grim_lu_solve_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs, int precision);
+
+/* Solve A X = B in place for nrhs columns with the factors and aux of
+ * grim_lu_factor.  Replaces scipy.linalg.lu_solve (linalg/dense.py:239-301). */
+int grim_lu_solve(int64_t batch, int64_t n, int64_t nrhs, const void* lu, int64_t ldlu, int64_t stride_lu,
+                  int precision, const void* aux, int64_t stride_aux, void* b, int64_t ldb, int64_t stride_b,
+                  void* work, int64_t work_bytes, void* stream);
 
 /* C -= A B (complex; precision 64 or 32); the FP64 refinement residual. */
-int grim_gemm_minus(int64_t m, int64_t n, int64_t k, const void* a, int64_t lda, const void* b, int64_t ldb,
-                    void* c, int64_t ldc, int precision, void* stream);
+int grim_gemm_minus(int64_t batch, int64_t m, int64_t n, int64_t k, const void* a, int64_t lda, int64_t stride_a,
+                    const void* b, int64_t ldb, int64_t stride_b, void* c, int64_t ldc, int64_t stride_c,
+                    int precision, void* stream);
 
 /* Bytes of workspace grim_refined_solve needs. */
-int64_t grim_refine_workspace_bytes(int64_t n, int64_t nrhs);
+int64_t grim_refine_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs);
 
 /* Mixed-precision solve (linalg/refined_lu.py:41-61): x = LU32^{-1} b, then
  * r = b - Z x in FP64, x += LU32^{-1} r until max_col(|r|_inf / |b|_inf) <= tol
- * (at most max_steps corrections; a step that does not reduce the error below
- * 0.98 of the previous one returns 6).  z: complex128 system; lu/piv: complex64
- * factors of the same matrix.  Synchronises the stream once per step.
- * steps_out / error_out: host pointers (may be NULL). */
-int grim_refined_solve(int64_t n, int64_t nrhs, const double* z, int64_t ldz, const float* lu, int64_t ldlu,
-                       const int32_t* piv, const double* b, int64_t ldb, double* x, int64_t ldx, int max_steps,
-                       double tol, void* workspace, int64_t workspace_bytes, int* steps_out, double* error_out,
-                       void* stream);
+ * over all matrices of the batch (at most max_steps corrections; a step that
+ * does not reduce the error below 0.98 of the previous one returns 6).
+ * z: complex128 systems; lu/aux: complex64 factors of the same matrices.
+ * Synchronises the stream once per step.  steps_out / error_out: host
+ * pointers (may be NULL). */
+int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz, int64_t stride_z,
+                       const float* lu, int64_t ldlu, int64_t stride_lu, const void* aux, int64_t stride_aux,
+                       const double* b, int64_t ldb, int64_t stride_b, double* x, int64_t ldx, int64_t stride_x,
+                       int max_steps, double tol, void* workspace, int64_t workspace_bytes, int* steps_out,
+                       double* error_out, void* stream);
 
-/* complex128 -> complex64 copy (m x n) and complex128 copy. */
-int grim_to_c64(int64_t m, int64_t n, const double* src, int64_t lds, float* dst, int64_t ldd, void* stream);
-int grim_copy_c128(int64_t m, int64_t n, const double* src, int64_t lds, double* dst, int64_t ldd, void* stream);
+/* complex128 -> complex64 copy (m x n) and complex128 copy, batched. */
+int grim_to_c64(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds, int64_t stride_src, float* dst,
+                int64_t ldd, int64_t stride_dst, void* stream);
+int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds, int64_t stride_src,
+                   double* dst, int64_t ldd, int64_t stride_dst, void* stream);
 
 /* Per-column 2-norm of an m x n complex128 matrix into out[n] (device). */
 int grim_column_norms(int64_t m, int64_t n, const double* v, int64_t ld, double* out, void* stream);
