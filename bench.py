@@ -40,7 +40,21 @@ def parse():
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mib: int):
+    """A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE counters calibrate HBM traffic."""
+    import torch
+    n = mib * 2**20 // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "launches": 3}))
 
 
 def workload(n_side: int, nangles: int):
@@ -180,6 +194,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import torch
     from grim_20261007_amd import engine, native
     dev = torch.device("cuda", 0)

@@ -81,7 +81,7 @@ __device__ __forceinline__ void zero_blk2(Blk2& b) {
 // Far element-pair quadrature with the Q x Q Gauss rule (far.c:100-191 semantics,
 // kernels evaluated directly instead of through an interpolation table).
 template <int Q>
-__device__ __noinline__ void far_pair(const Elem& o, const Elem& s, double k, Blk2& out) {
+__device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k, Blk2& out) {
     cplx S[2][2], K[2][2], D[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)

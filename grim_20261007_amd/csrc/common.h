@@ -53,7 +53,9 @@ __device__ __forceinline__ double horner(const double (&c)[N], double t) {
 // Small x: Taylor series of the entire parts (DLMF 10.8.1); large x: modulus /
 // phase form with fitted P_nu, Q_nu (DLMF 10.17.3).  Coefficients generated by
 // gen_hankel_coeffs.py (max relative error 5e-15 for x up to 1e3).
-__device__ __noinline__ void hankel_quarter(double x, cplx& g, cplx& h1) {
+struct HK { double gr, gi, hr, hi; };
+__device__ __noinline__ HK hankel_quarter_v(double x) {
+    cplx g, h1;
     if (x < GRIM_HANKEL_XS) {
         const double y = 0.25 * x * x;
         const double L = log(0.5 * x);
@@ -81,6 +83,12 @@ __device__ __noinline__ void hankel_quarter(double x, cplx& g, cplx& h1) {
         g = {-0.25 * h0im, 0.25 * h0re};
         h1 = {-0.25 * h1im, 0.25 * h1re};
     }
+    return HK{g.re, g.im, h1.re, h1.im};
+}
+__device__ __forceinline__ void hankel_quarter(double x, cplx& g, cplx& h1) {
+    const HK v = hankel_quarter_v(x);
+    g = {v.gr, v.gi};
+    h1 = {v.hr, v.hi};
 }
 
 // Kernels at distance r for real k: g = (j/4) H0(k r), h = (j/4) k H1(k r).

@@ -11,11 +11,14 @@
 // (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32).
 //
 // Factorization: right-looking, panels of GRIM_LU_NB = 64 columns, each panel
-// factored recursively down to register-resident leaves; row interchanges are
-// applied to the whole row (LAPACK convention, P A = L U) and also folded into
-// a permutation vector.  After each panel the inverses of its 64 x 64 unit
-// lower and upper diagonal blocks are formed, so every triangular step of the
-// factorization and of the many-column solve is an MFMA matrix product.
+// factored recursively down to register-resident leaves.  Row interchanges
+// follow LAPACK (whole rows, P A = L U): each leaf applies its own to the rest
+// of the panel, and each panel's interchanges are composed into one list that
+// is applied as a gather to the columns left of the panel and, fused into the
+// U12 triangular solve, to the columns right of it; the same list updates the
+// permutation vector used by the solves.  Triangular solves of up to 64 rows
+// run lane-parallel (one row per lane, pivots broadcast with v_readlane);
+// everything else is MFMA matrix products.
 // All kernels take a batch of equally strided matrices (blockIdx.z or one
 // workgroup per matrix), e.g. the TE and TM systems of one request.
 #include "common.h"
@@ -153,23 +156,142 @@ __global__ void __launch_bounds__(256) cgemm(int m, int n, int k, const T* __res
 }
 
 // ---------------------------------------------------------------------------
-// Panel leaf: factor columns [0, w) of an m x w block (rows cached in registers).
+// (value, index) argmax across lanes without LDS: larger value wins, ties go to
+// the smaller index (LAPACK izamax picks the first maximum).  DPP quad
+// permutations and row mirrors reduce within rows of 16 lanes, then gfx950's
+// v_permlane16_swap / v_permlane32_swap combine the rows; every lane ends with
+// the result.  The comparisons are branch-free selects.
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ void am_take(T& v, int& i, T ov, int oi) {
+    const bool t = (ov > v) | ((ov == v) & (oi < i));
+    v = t ? ov : v;
+    i = t ? oi : i;
+}
+template <int CTRL> __device__ __forceinline__ int dpp_mov(int x) {
+    return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
+}
+template <int CTRL> __device__ __forceinline__ void am_dpp(double& v, int& i) {
+    const int lo = dpp_mov<CTRL>(__double2loint(v)), hi = dpp_mov<CTRL>(__double2hiint(v));
+    am_take(v, i, __hiloint2double(hi, lo), dpp_mov<CTRL>(i));
+}
+template <int CTRL> __device__ __forceinline__ void am_dpp(float& v, int& i) {
+    am_take(v, i, __int_as_float(dpp_mov<CTRL>(__float_as_int(v))), dpp_mov<CTRL>(i));
+}
+template <typename T> __device__ __forceinline__ void argmax16(T& v, int& i) {
+    am_dpp<0xB1>(v, i);    // quad_perm [1,0,3,2]
+    am_dpp<0x4E>(v, i);    // quad_perm [2,3,0,1]
+    am_dpp<0x141>(v, i);   // row_half_mirror
+    am_dpp<0x140>(v, i);   // row_mirror
+}
+// v_permlane{16,32}_swap(x, x) returns (x of the even rows / low half, x of
+// the odd rows / high half) in every lane, so one select combines partners.
+#define GRIM_AM_SWAP(NAME, BUILTIN)                                                     \
+    __device__ __forceinline__ void NAME(double& v, int& i) {                           \
+        const auto lo = BUILTIN(__double2loint(v), __double2loint(v), false, false);    \
+        const auto hi = BUILTIN(__double2hiint(v), __double2hiint(v), false, false);    \
+        const auto ix = BUILTIN(i, i, false, false);                                    \
+        v = __hiloint2double(hi[0], lo[0]);                                             \
+        i = ix[0];                                                                      \
+        am_take(v, i, __hiloint2double(hi[1], lo[1]), (int)ix[1]);                      \
+    }                                                                                   \
+    __device__ __forceinline__ void NAME(float& v, int& i) {                            \
+        const auto x = BUILTIN(__float_as_int(v), __float_as_int(v), false, false);     \
+        const auto ix = BUILTIN(i, i, false, false);                                    \
+        v = __int_as_float(x[0]);                                                       \
+        i = ix[0];                                                                      \
+        am_take(v, i, __int_as_float(x[1]), (int)ix[1]);                                \
+    }
+GRIM_AM_SWAP(am_swap16, __builtin_amdgcn_permlane16_swap)
+GRIM_AM_SWAP(am_swap32, __builtin_amdgcn_permlane32_swap)
+#undef GRIM_AM_SWAP
+template <typename T> __device__ __forceinline__ void argmax64(T& v, int& i) {
+    argmax16(v, i);
+    am_swap16(v, i);
+    am_swap32(v, i);
+}
+
+// ---------------------------------------------------------------------------
+// Composite of a sequence of row interchanges, built by one wave.
+// The interchanges are (r0 + i) <-> piv[i] for i = 0 .. cnt-1 (cnt <= 64,
+// piv[i] >= r0 + i, absolute rows, applied in order).  On return lane l holds
+//   top  = the row whose content ends at position r0 + l   (l < cnt),
+//   ldst / lsrc = a lower position (>= r0 + cnt) that changes and the row
+//          whose content ends there (l < nlow).
+// Every position whose content changes is a top slot or a lower entry, and
+// every source is one of those positions, so applying the interchanges to a
+// column is one gather: read all sources, then write all destinations.
+// ---------------------------------------------------------------------------
+struct Composite {
+    int top, ldst, lsrc, nlow;
+};
+
+__device__ Composite wave_composite(int r0, int cnt, const int* piv) {
+    const int lane = threadIdx.x & 63;
+    const int pl = lane < cnt ? piv[lane] : 0;   // lane i holds piv[i]
+    Composite x;
+    x.top = r0 + lane;
+    x.ldst = -1;
+    x.lsrc = -1;
+    int nlow = 0;
+    for (int i = 0; i < cnt; ++i) {
+        const int p = __builtin_amdgcn_readlane(pl, i);
+        const int vi = __builtin_amdgcn_readlane(x.top, i);
+        if (p - r0 < cnt) {
+            const int q = p - r0;
+            const int vq = __builtin_amdgcn_readlane(x.top, q);
+            if (lane == i) x.top = vq;
+            if (lane == q) x.top = vi;
+        } else {
+            const unsigned long long hit = __ballot(x.ldst == p);
+            int q;
+            if (hit) {
+                q = __ffsll((long long)hit) - 1;
+            } else {
+                q = nlow++;
+                if (lane == q) { x.ldst = p; x.lsrc = p; }
+            }
+            const int vq = __builtin_amdgcn_readlane(x.lsrc, q);
+            if (lane == i) x.top = vq;
+            if (lane == q) x.lsrc = vi;
+        }
+    }
+    x.nlow = nlow;
+    return x;
+}
+
+// Interchange list of one 64-column panel in the auxiliary storage:
+// [0] = nlow, [1 + l] = top source of slot l, [65 + l] / [129 + l] = lower
+// destination / source of entry l (absolute rows).
+#define GRIM_LIST_INTS 256
+
+// ---------------------------------------------------------------------------
+// Panel leaf: factor leaf columns [lc0, lc0 + w) of the panel whose first
+// column starts at `pan` (row r0 = first row of the leaf, panel width pw),
+// rows r0 .. r0 + m - 1, with the leaf's block cached in registers.
 // One workgroup of 1024 threads per matrix; thread t owns rows t, t + 1024, ...
-// Pivot = first row maximising |re| + |im| (LAPACK izamax).  Two barriers per
-// column: wave candidates and the pivot / current rows go through
-// double-buffered LDS slots, and every thread reduces the 16 wave candidates
-// itself, so no thread has to publish the pivot index separately.
+// Pivot = first row maximising |re| + |im| (LAPACK izamax).  Per column: a
+// register-only wave argmax, one LDS slot per wave and a barrier, a 16-lane
+// argmax of the slots, then the pivot row and row c through double-buffered
+// LDS and a second barrier.  Pivots stay in LDS until the end.  Finally the
+// leaf's interchanges are applied, as one gather, to the other columns of the
+// panel, so the panel recursion needs no separate row-swap passes.
 // ---------------------------------------------------------------------------
 template <typename T, int R, int W>
-__global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__ a, int64_t lda, int64_t sa,
-                                                   int row0, int* __restrict__ piv, int64_t spiv,
+__global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__ pan, int64_t lda, int64_t sa,
+                                                   int pw, int lc0, int row0, int* __restrict__ piv, int64_t spiv,
                                                    unsigned* __restrict__ status) {
     __shared__ T cand_v[2][16];
     __shared__ int cand_i[2][16];
     __shared__ T prow_re[2][W], prow_im[2][W], crow_re[2][W], crow_im[2][W];
-    a += 2 * sa * blockIdx.x;
+    __shared__ int lpiv[W];
+    __shared__ int zero_pivot;
+    __shared__ int list_dst[2 * W], list_src[2 * W];
+    __shared__ int list_n;
+    pan += 2 * sa * blockIdx.x;
     piv += spiv * blockIdx.x;
+    T* a = pan + 2 * (int64_t)lc0 * lda;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) zero_pivot = 0;
     T vr[R][W], vi[R][W];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -188,77 +310,71 @@ __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__
     }
 #pragma unroll
     for (int c = 0; c < W; ++c) {
-        if (c >= w) break;
-        const int buf = c & 1;
-        T best = -1;
-        int bi = 0x7fffffff;
+        if (c < w) {
+            const int buf = c & 1;
+            T best = -1;
+            int bi = 0x7fffffff;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int row = tid + 1024 * r;
-            if (row >= c && row < m) {
-                const T v = fabs(vr[r][c]) + fabs(vi[r][c]);
-                if (v > best || (v == best && row < bi)) { best = v; bi = row; }
-            }
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bi, off, 64);
-            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
-        }
-        if (lane == 0) { cand_v[buf][wave] = best; cand_i[buf][wave] = bi; }
-        __syncthreads();
-        T bv = cand_v[buf][0];
-        int p = cand_i[buf][0];
-#pragma unroll
-        for (int i = 1; i < 16; ++i) {
-            const T v = cand_v[buf][i];
-            const int q = cand_i[buf][i];
-            if (v > bv || (v == bv && q < p)) { bv = v; p = q; }
-        }
-        if (tid == 0) {
-            piv[c] = row0 + p;
-            if (!(bv > 0)) atomicOr(status, (unsigned)GRIM_DEV_ZERO_PIVOT);
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int row = tid + 1024 * r;
-            if (row == p) {
-#pragma unroll
-                for (int cc = 0; cc < W; ++cc) { prow_re[buf][cc] = vr[r][cc]; prow_im[buf][cc] = vi[r][cc]; }
-            }
-            if (row == c) {
-#pragma unroll
-                for (int cc = 0; cc < W; ++cc) { crow_re[buf][cc] = vr[r][cc]; crow_im[buf][cc] = vi[r][cc]; }
-            }
-        }
-        __syncthreads();
-        const T dr = prow_re[buf][c], di = prow_im[buf][c];
-        const T den = dr * dr + di * di;
-        const T ir = den > 0 ? dr / den : 0, ii = den > 0 ? -di / den : 0;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int row = tid + 1024 * r;
-            if (p != c) {
-                if (row == c) {
-#pragma unroll
-                    for (int cc = 0; cc < W; ++cc) { vr[r][cc] = prow_re[buf][cc]; vi[r][cc] = prow_im[buf][cc]; }
-                } else if (row == p) {
-#pragma unroll
-                    for (int cc = 0; cc < W; ++cc) { vr[r][cc] = crow_re[buf][cc]; vi[r][cc] = crow_im[buf][cc]; }
+            for (int r = 0; r < R; ++r) {
+                const int row = tid + 1024 * r;
+                if (row >= c && row < m) {
+                    T v = fabs(vr[r][c]) + fabs(vi[r][c]);
+                    v = v == v ? v : T(-0.5);   // NaN ranks below every number: a valid row always wins
+                    am_take(best, bi, v, row);
                 }
             }
-            if (row > c && row < m) {
-                const T lr = vr[r][c] * ir - vi[r][c] * ii;
-                const T li = vr[r][c] * ii + vi[r][c] * ir;
-                vr[r][c] = lr;
-                vi[r][c] = li;
+            argmax64(best, bi);
+            if (lane == 0) { cand_v[buf][wave] = best; cand_i[buf][wave] = bi; }
+            __syncthreads();
+            T bv = T(-2);
+            int p = 0x7fffffff;
+            if (lane < 16) { bv = cand_v[buf][lane]; p = cand_i[buf][lane]; }
+            argmax16(bv, p);
+            p = __builtin_amdgcn_readfirstlane(p);
+            if (tid == 0) {
+                lpiv[c] = p;
+                if (!(bv > 0)) zero_pivot = 1;
+            }
 #pragma unroll
-                for (int cc = 0; cc < W; ++cc) {
-                    if (cc > c) {
-                        const T ur = prow_re[buf][cc], ui = prow_im[buf][cc];
-                        vr[r][cc] -= lr * ur - li * ui;
-                        vi[r][cc] -= lr * ui + li * ur;
+            for (int r = 0; r < R; ++r) {
+                const int row = tid + 1024 * r;
+                if (row == p) {
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) { prow_re[buf][cc] = vr[r][cc]; prow_im[buf][cc] = vi[r][cc]; }
+                }
+                if (row == c) {
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) { crow_re[buf][cc] = vr[r][cc]; crow_im[buf][cc] = vi[r][cc]; }
+                }
+            }
+            __syncthreads();
+            const T dr = prow_re[buf][c], di = prow_im[buf][c];
+            const T den = dr * dr + di * di;
+            const T ir = den > 0 ? dr / den : 0, ii = den > 0 ? -di / den : 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int row = tid + 1024 * r;
+                if (p != c) {
+                    if (row == c) {
+#pragma unroll
+                        for (int cc = 0; cc < W; ++cc) { vr[r][cc] = prow_re[buf][cc]; vi[r][cc] = prow_im[buf][cc]; }
+                    } else if (row == p) {
+#pragma unroll
+                        for (int cc = 0; cc < W; ++cc) { vr[r][cc] = crow_re[buf][cc]; vi[r][cc] = crow_im[buf][cc]; }
+                    }
+                }
+                if (row > c && row < m) {
+                    const T lr = vr[r][c] * ir - vi[r][c] * ii;
+                    const T li = vr[r][c] * ii + vi[r][c] * ir;
+                    vr[r][c] = lr;
+                    vi[r][c] = li;
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) {
+                        if (cc > c) {
+                            const T ur = prow_re[buf][cc], ui = prow_im[buf][cc];
+                            vr[r][cc] -= lr * ur - li * ui;
+                            vi[r][cc] -= lr * ui + li * ur;
+                        }
                     }
                 }
             }
@@ -278,31 +394,92 @@ __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__
             }
         }
     }
-}
-
-// Apply interchanges piv[c0..c1) (absolute rows) to columns [0, ncols) of a,
-// skipping columns [s0, s1).  One thread per column; batch in blockIdx.z.
-template <typename T>
-__global__ void __launch_bounds__(256) laswp(int ncols, T* __restrict__ a, int64_t lda, int64_t sa,
-                                             const int* __restrict__ piv, int64_t spiv, int c0, int c1, int s0,
-                                             int s1) {
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= ncols || (col >= s0 && col < s1)) return;
-    a += 2 * sa * blockIdx.z;
-    piv += spiv * blockIdx.z;
-    T* base = a + 2 * (int64_t)col * lda;
-    for (int c = c0; c < c1; ++c) {
-        const int p = piv[c];
-        if (p != c) {
-            T* x = base + 2 * (int64_t)c;
-            T* y = base + 2 * (int64_t)p;
-            const T xr = x[0], xi = x[1];
-            x[0] = y[0];
-            x[1] = y[1];
-            y[0] = xr;
-            y[1] = xi;
+    __syncthreads();
+    if (tid < w) piv[tid] = row0 + lpiv[tid];
+    if (tid == 0 && zero_pivot) atomicOr(status, (unsigned)GRIM_DEV_ZERO_PIVOT);
+    if (pw == w) return;
+    // the leaf's interchanges on the panel's other columns (rows local to r0)
+    if (wave == 0) {
+        const Composite x = wave_composite(0, w, lpiv);
+        if (lane < w) { list_dst[lane] = lane; list_src[lane] = x.top; }
+        if (lane < x.nlow) { list_dst[w + lane] = x.ldst; list_src[w + lane] = x.lsrc; }
+        if (lane == 0) list_n = w + x.nlow;
+    }
+    __syncthreads();
+    const int cnt = list_n;
+    const int others = pw - w;
+    const int items = cnt * others;
+    T sr[2], si[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int idx = tid + 1024 * k;
+        sr[k] = 0;
+        si[k] = 0;
+        if (idx < items) {
+            const int e = idx % cnt, oc = idx / cnt;
+            const int col = oc < lc0 ? oc : oc + w;
+            const T* p = pan + 2 * ((int64_t)list_src[e] + (int64_t)col * lda);
+            sr[k] = p[0];
+            si[k] = p[1];
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int idx = tid + 1024 * k;
+        if (idx < items) {
+            const int e = idx % cnt, oc = idx / cnt;
+            const int col = oc < lc0 ? oc : oc + w;
+            T* p = pan + 2 * ((int64_t)list_dst[e] + (int64_t)col * lda);
+            p[0] = sr[k];
+            p[1] = si[k];
+        }
+    }
+}
+
+// Interchange list of the panel [pj, pj + pw) from its pivots; also applied to
+// the running row permutation (perm[i] = original row now at position i).
+// One wave per matrix.
+__global__ void __launch_bounds__(64) panel_perm(int pj, int pw, const int* __restrict__ piv, int64_t spiv,
+                                                 int* __restrict__ perm, int64_t sperm, int* __restrict__ list,
+                                                 int64_t slist) {
+    piv += spiv * blockIdx.x;
+    perm += sperm * blockIdx.x;
+    list += slist * blockIdx.x;
+    const int lane = threadIdx.x;
+    const Composite x = wave_composite(pj, pw, piv + pj);
+    if (lane == 0) list[0] = x.nlow;
+    list[1 + lane] = x.top;
+    list[65 + lane] = x.ldst;
+    list[129 + lane] = x.lsrc;
+    const int vt = lane < pw ? perm[x.top] : 0;
+    const int vl = lane < x.nlow ? perm[x.lsrc] : 0;
+    if (lane < pw) perm[pj + lane] = vt;
+    if (lane < x.nlow) perm[x.ldst] = vl;
+}
+
+__global__ void __launch_bounds__(256) perm_init(int n, int* __restrict__ perm, int64_t sperm) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) perm[sperm * blockIdx.z + i] = i;
+}
+
+// Apply a panel's interchange list to columns [0, ncols) of a (rows absolute).
+// One wave per column, lanes over list entries; batch in blockIdx.z.
+template <typename T>
+__global__ void __launch_bounds__(256) apply_list(int ncols, int pj, int pw, T* __restrict__ a, int64_t lda,
+                                                  int64_t sa, const int* __restrict__ list, int64_t slist) {
+    const int lane = threadIdx.x & 63;
+    const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (col >= ncols) return;
+    a += 2 * sa * blockIdx.z + 2 * (int64_t)col * lda;
+    list += slist * blockIdx.z;
+    const int nlow = list[0];
+    const int ts = list[1 + lane], ld = list[65 + lane], ls = list[129 + lane];
+    T tr = 0, ti = 0, lr = 0, li = 0;
+    if (lane < pw) { tr = a[2 * (int64_t)ts]; ti = a[2 * (int64_t)ts + 1]; }
+    if (lane < nlow) { lr = a[2 * (int64_t)ls]; li = a[2 * (int64_t)ls + 1]; }
+    if (lane < pw) { a[2 * (int64_t)(pj + lane)] = tr; a[2 * (int64_t)(pj + lane) + 1] = ti; }
+    if (lane < nlow) { a[2 * (int64_t)ld] = lr; a[2 * (int64_t)ld + 1] = li; }
 }
 
 #define GRIM_LU_NB 64
@@ -318,15 +495,21 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 
 // X = T^{-1} B in place for a w x w triangle (w <= 64) and ncols columns:
 // LOWER: unit lower triangle; else upper with its diagonal.  Lane i of a wave
-// holds row i of TRSM_CPW columns; step j broadcasts x_j from lane j with
+// holds row i of CPW columns; step j broadcasts x_j from lane j with
 // v_readlane (wave-uniform, so it lands in SGPRs) and every lane below (above)
 // subtracts its T[i][j] x_j.  The triangle is staged transposed in LDS so lanes
-// read consecutive words.  256 threads = 4 waves; batch in blockIdx.z.
-#define TRSM_CPW 4
-template <typename T, bool LOWER>
+// read consecutive words, with the reciprocals of its diagonal.  256 threads =
+// 4 waves; batch in blockIdx.z.
+// With `list` (a panel's interchange list; rows minus list_base are relative
+// to the top of B) each wave first applies the panel's row interchanges to its
+// columns of B: it reads every source row, writes the lower destinations and
+// solves on the permuted top rows, so no separate row-swap pass is needed.
+template <typename T, bool LOWER, int CPW>
 __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __restrict__ tri, int64_t ldt, int64_t st,
-                                                  T* __restrict__ b, int64_t ldb, int64_t sb) {
+                                                  T* __restrict__ b, int64_t ldb, int64_t sb,
+                                                  const int* __restrict__ list, int64_t slist, int list_base) {
     __shared__ T tr[GRIM_LU_NB][GRIM_LU_NB], ti[GRIM_LU_NB][GRIM_LU_NB];   // [col j][row i]
+    __shared__ T dinv_r[GRIM_LU_NB], dinv_i[GRIM_LU_NB];
     tri += 2 * st * blockIdx.z;
     b += 2 * sb * blockIdx.z;
     for (int idx = threadIdx.x; idx < GRIM_LU_NB * GRIM_LU_NB; idx += 256) {
@@ -340,26 +523,62 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
         tr[j][i] = x;
         ti[j][i] = y;
     }
-    __syncthreads();
+    if (!LOWER && threadIdx.x < w) {
+        const int j = threadIdx.x;
+        const T* p = tri + 2 * ((int64_t)j + (int64_t)j * ldt);
+        const T dr = p[0], di = p[1];
+        const T den = dr * dr + di * di;
+        dinv_r[j] = dr / den;
+        dinv_i[j] = -di / den;
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col0 = (blockIdx.x * 4 + wave) * TRSM_CPW;
-    T xr[TRSM_CPW], xi[TRSM_CPW];
+    const int col0 = (blockIdx.x * 4 + wave) * CPW;
+    T xr[CPW], xi[CPW];
+    if (list) {
+        list += slist * blockIdx.z;
+        const int nlow = list[0];
+        const int ts = list[1 + lane] - list_base, ld = list[65 + lane] - list_base,
+                  ls = list[129 + lane] - list_base;
+        T lr[CPW], li[CPW];
 #pragma unroll
-    for (int c = 0; c < TRSM_CPW; ++c) {
-        xr[c] = 0;
-        xi[c] = 0;
-        if (lane < w && col0 + c < ncols) {
-            const T* p = b + 2 * ((int64_t)lane + (int64_t)(col0 + c) * ldb);
-            xr[c] = p[0];
-            xi[c] = p[1];
+        for (int c = 0; c < CPW; ++c) {
+            xr[c] = 0;
+            xi[c] = 0;
+            lr[c] = 0;
+            li[c] = 0;
+            if (col0 + c < ncols) {
+                const T* bc = b + 2 * (int64_t)(col0 + c) * ldb;
+                if (lane < w) { xr[c] = bc[2 * (int64_t)ts]; xi[c] = bc[2 * (int64_t)ts + 1]; }
+                if (lane < nlow) { lr[c] = bc[2 * (int64_t)ls]; li[c] = bc[2 * (int64_t)ls + 1]; }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            if (col0 + c < ncols && lane < nlow) {
+                T* bc = b + 2 * (int64_t)(col0 + c) * ldb;
+                bc[2 * (int64_t)ld] = lr[c];
+                bc[2 * (int64_t)ld + 1] = li[c];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            xr[c] = 0;
+            xi[c] = 0;
+            if (lane < w && col0 + c < ncols) {
+                const T* p = b + 2 * ((int64_t)lane + (int64_t)(col0 + c) * ldb);
+                xr[c] = p[0];
+                xi[c] = p[1];
+            }
         }
     }
+    __syncthreads();
     if (LOWER) {
         for (int j = 0; j < w - 1; ++j) {
             const T lr = tr[j][lane], li = ti[j][lane];
             const bool below = lane > j;
 #pragma unroll
-            for (int c = 0; c < TRSM_CPW; ++c) {
+            for (int c = 0; c < CPW; ++c) {
                 const T ur = lane_value(xr[c], j), ui = lane_value(xi[c], j);
                 if (below) {
                     xr[c] -= lr * ur - li * ui;
@@ -369,13 +588,11 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
         }
     } else {
         for (int j = w - 1; j >= 0; --j) {
-            const T dr = tr[j][j], di = ti[j][j];
-            const T den = dr * dr + di * di;
-            const T ir = dr / den, ii = -di / den;
+            const T ir = dinv_r[j], ii = dinv_i[j];
             const T lr = tr[j][lane], li = ti[j][lane];
             const bool above = lane < j;
 #pragma unroll
-            for (int c = 0; c < TRSM_CPW; ++c) {
+            for (int c = 0; c < CPW; ++c) {
                 const T br = lane_value(xr[c], j), bi = lane_value(xi[c], j);
                 const T ur = br * ir - bi * ii, ui = br * ii + bi * ir;
                 if (lane == j) {
@@ -389,27 +606,12 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
         }
     }
 #pragma unroll
-    for (int c = 0; c < TRSM_CPW; ++c) {
+    for (int c = 0; c < CPW; ++c) {
         if (lane < w && col0 + c < ncols) {
             T* p = b + 2 * ((int64_t)lane + (int64_t)(col0 + c) * ldb);
             p[0] = xr[c];
             p[1] = xi[c];
         }
-    }
-}
-
-// perm[i] = original row that ends at row i after the interchanges piv[0..n).
-__global__ void build_perm(int n, const int* __restrict__ piv, int64_t spiv, int* __restrict__ perm,
-                           int64_t sperm) {
-    if (threadIdx.x != 0) return;
-    piv += spiv * blockIdx.x;
-    perm += sperm * blockIdx.x;
-    for (int i = 0; i < n; ++i) perm[i] = i;
-    for (int c = 0; c < n; ++c) {
-        const int p = piv[c];
-        const int t = perm[c];
-        perm[c] = perm[p];
-        perm[p] = t;
     }
 }
 
@@ -508,16 +710,19 @@ static hipError_t gemm(const Batch& bt, int m, int n, int k, const T* A, int64_t
 }
 
 template <typename T>
-static hipError_t leaf(const Batch& bt, int m, int w, T* a, int64_t lda, int64_t sa, int row0, int* piv,
-                       int64_t spiv, unsigned* status) {
+static hipError_t leaf(const Batch& bt, int m, int w, T* pan, int64_t lda, int64_t sa, int pw, int lc0, int row0,
+                       int* piv, int64_t spiv, unsigned* status) {
     hipStream_t st = bt.st;
     const int b = bt.count;
-    if (m <= 1024) panel_leaf<T, 1, 16><<<b, 1024, 0, st>>>(m, w, a, lda, sa, row0, piv, spiv, status);
-    else if (m <= 2048) panel_leaf<T, 2, 8><<<b, 1024, 0, st>>>(m, w, a, lda, sa, row0, piv, spiv, status);
-    else if (m <= 4096) panel_leaf<T, 4, 4><<<b, 1024, 0, st>>>(m, w, a, lda, sa, row0, piv, spiv, status);
-    else if (m <= 8192) panel_leaf<T, 8, 2><<<b, 1024, 0, st>>>(m, w, a, lda, sa, row0, piv, spiv, status);
-    else if (m <= 16384) panel_leaf<T, 16, 1><<<b, 1024, 0, st>>>(m, w, a, lda, sa, row0, piv, spiv, status);
-    else panel_leaf<T, 32, 1><<<b, 1024, 0, st>>>(m, w, a, lda, sa, row0, piv, spiv, status);
+#define GRIM_LEAF(R, W) \
+    panel_leaf<T, R, W><<<b, 1024, 0, st>>>(m, w, pan, lda, sa, pw, lc0, row0, piv, spiv, status)
+    if (m <= 1024) GRIM_LEAF(1, 16);
+    else if (m <= 2048) GRIM_LEAF(2, 8);
+    else if (m <= 4096) GRIM_LEAF(4, 4);
+    else if (m <= 8192) GRIM_LEAF(8, 2);
+    else if (m <= 16384) GRIM_LEAF(16, 1);
+    else GRIM_LEAF(32, 1);
+#undef GRIM_LEAF
     return hipGetLastError();
 }
 
@@ -529,77 +734,86 @@ static int leaf_width(int m) {
     return 1;
 }
 
-template <typename T>
-static hipError_t swap_cols(const Batch& bt, int ncols, T* a, int64_t lda, int64_t sa, const int* piv, int64_t spiv,
-                            int c0, int c1, int s0, int s1) {
-    if (ncols <= 0 || c1 <= c0) return hipSuccess;
-    laswp<T><<<dim3((ncols + 255) / 256, 1, bt.count), 256, 0, bt.st>>>(ncols, a, lda, sa, piv, spiv, c0, c1, s0,
-                                                                         s1);
-    return hipGetLastError();
-}
-
 template <typename T, bool LOWER>
 static hipError_t trsm(const Batch& bt, int w, int ncols, const T* tri, int64_t ldt, int64_t st, T* b, int64_t ldb,
-                       int64_t sb) {
+                       int64_t sb, const int* list = nullptr, int64_t slist = 0, int list_base = 0) {
     if (w <= 0 || ncols <= 0) return hipSuccess;
-    const int per_block = 4 * TRSM_CPW;
-    trsm_lanes<T, LOWER><<<dim3((ncols + per_block - 1) / per_block, 1, bt.count), 256, 0, bt.st>>>(
-        w, ncols, tri, ldt, st, b, ldb, sb);
+    // columns per wave: enough waves to cover the chip first, then ILP
+    const int64_t cols = (int64_t)ncols * bt.count;
+#define GRIM_TRSM(CPW)                                                                                            \
+    trsm_lanes<T, LOWER, CPW><<<dim3((ncols + 4 * CPW - 1) / (4 * CPW), 1, bt.count), 256, 0, bt.st>>>(        \
+        w, ncols, tri, ldt, st, b, ldb, sb, list, slist, list_base)
+    if (cols >= 8192) GRIM_TRSM(4);
+    else if (cols >= 2048) GRIM_TRSM(2);
+    else GRIM_TRSM(1);
+#undef GRIM_TRSM
     return hipGetLastError();
 }
 
-// Recursive factorization of columns [j0, j0 + w) (rows j0..n-1); interchanges
-// are applied to all columns of this column range.
+// Recursive factorization of columns [j0, j0 + w) of the panel starting at
+// column pj (width pw), rows j0 .. n-1.  Each leaf applies its interchanges to
+// the whole panel, so the right-hand half sees permuted rows without a swap
+// pass.
 template <typename T>
-static hipError_t panel_rec(const Batch& bt, int n, T* a, int64_t lda, int64_t sa, int j0, int w, int* piv,
-                            int64_t spiv, unsigned* status) {
+static hipError_t panel_rec(const Batch& bt, int n, T* a, int64_t lda, int64_t sa, int pj, int pw, int j0, int w,
+                            int* piv, int64_t spiv, unsigned* status) {
     const int m = n - j0;
     const int lw = leaf_width(m);
     T* top = a + 2 * ((int64_t)j0 + (int64_t)j0 * lda);
-    if (w <= lw) return leaf<T>(bt, m, w, top, lda, sa, j0, piv + j0, spiv, status);
+    if (w <= lw)
+        return leaf<T>(bt, m, w, a + 2 * ((int64_t)j0 + (int64_t)pj * lda), lda, sa, pw, j0 - pj, j0, piv + j0,
+                       spiv, status);
     int w1 = w / 2;
     if (w1 > lw) w1 = (w1 / lw) * lw;
-    hipError_t err = panel_rec<T>(bt, n, a, lda, sa, j0, w1, piv, spiv, status);
+    hipError_t err = panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0, w1, piv, spiv, status);
     if (err != hipSuccess) return err;
     const int w2 = w - w1;
     T* right = a + 2 * ((int64_t)(j0 + w1) * lda);
-    if ((err = swap_cols<T>(bt, w2, right, lda, sa, piv, spiv, j0, j0 + w1, -1, -1)) != hipSuccess) return err;
     if ((err = trsm<T, true>(bt, w1, w2, top, lda, sa, right + 2 * (int64_t)j0, lda, sa)) != hipSuccess) return err;
     if ((err = gemm<T>(bt, m - w1, w2, w1, top + 2 * (int64_t)w1, lda, sa, right + 2 * (int64_t)j0, lda, sa,
                        right + 2 * (int64_t)(j0 + w1), lda, sa)) != hipSuccess)
         return err;
-    if ((err = panel_rec<T>(bt, n, a, lda, sa, j0 + w1, w2, piv, spiv, status)) != hipSuccess) return err;
-    T* left = a + 2 * (int64_t)j0 * lda;
-    return swap_cols<T>(bt, w1, left, lda, sa, piv, spiv, j0 + w1, j0 + w, -1, -1);
+    return panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0 + w1, w2, piv, spiv, status);
 }
 
-// Factor-side auxiliary storage per matrix: the row permutation (n int32)
-// that the pivots compose to, applied to right-hand sides in one gather.
+// Factor-side auxiliary storage per matrix: the row permutation the pivots
+// compose to (n int32, applied to right-hand sides in one gather) and the
+// interchange list of the panel being factored.
 static int64_t align256(int64_t v) { return (v + 255) / 256 * 256; }
-static int64_t aux_bytes(int n) { return align256(4 * (int64_t)n); }
+static int64_t aux_bytes(int n) { return align256(4 * (int64_t)n) + align256(4 * GRIM_LIST_INTS); }
 
 template <typename T>
 static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t sa, int* piv, int64_t spiv,
                             void* auxp, int64_t saux, unsigned* status) {
     hipError_t err;
+    int* perm = (int*)auxp;
+    int* list = (int*)((char*)auxp + align256(4 * (int64_t)n));
+    const int64_t sperm = saux / 4;
+    perm_init<<<dim3((n + 255) / 256, 1, bt.count), 256, 0, bt.st>>>(n, perm, sperm);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
     for (int j = 0; j < n; j += GRIM_LU_NB) {
         const int jb = min(GRIM_LU_NB, n - j);
-        if ((err = panel_rec<T>(bt, n, a, lda, sa, j, jb, piv, spiv, status)) != hipSuccess) return err;
-        // the panel's interchanges on every other column (LAPACK convention)
-        if ((err = swap_cols<T>(bt, n, a, lda, sa, piv, spiv, j, j + jb, j, j + jb)) != hipSuccess) return err;
+        if ((err = panel_rec<T>(bt, n, a, lda, sa, j, jb, j, jb, piv, spiv, status)) != hipSuccess) return err;
+        panel_perm<<<bt.count, 64, 0, bt.st>>>(j, jb, piv, spiv, perm, sperm, list, sperm);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+        // the panel's interchanges on the columns left of it (LAPACK keeps L permuted)
+        if (j > 0) {
+            apply_list<T><<<dim3((j + 3) / 4, 1, bt.count), 256, 0, bt.st>>>(j, j, jb, a, lda, sa, list, sperm);
+            if ((err = hipGetLastError()) != hipSuccess) return err;
+        }
         const int rest = n - j - jb;
         if (rest > 0) {
             T* diag = a + 2 * ((int64_t)j + (int64_t)j * lda);
             T* a12 = a + 2 * ((int64_t)j + (int64_t)(j + jb) * lda);
-            // U12 = L11^{-1} A12 in place, then A22 -= L21 U12
-            if ((err = trsm<T, true>(bt, jb, rest, diag, lda, sa, a12, lda, sa)) != hipSuccess) return err;
+            // interchanges on the right-hand columns fused with U12 = L11^{-1} A12, then A22 -= L21 U12
+            if ((err = trsm<T, true>(bt, jb, rest, diag, lda, sa, a12, lda, sa, list, sperm, j)) != hipSuccess)
+                return err;
             if ((err = gemm<T>(bt, rest, rest, jb, diag + 2 * (int64_t)jb, lda, sa, a12, lda, sa,
                                a12 + 2 * (int64_t)jb, lda, sa)) != hipSuccess)
                 return err;
         }
     }
-    build_perm<<<bt.count, 64, 0, bt.st>>>(n, piv, spiv, (int*)auxp, saux / 4);
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 // Solve in place: b <- A^{-1} b with work (n x nrhs per matrix, ld n).

@@ -103,6 +103,47 @@ def test_lu_pivoting_needed():
     assert np.max(np.abs(x.cpu().numpy().T - want)) / np.max(np.abs(want)) < 1e-10
 
 
+@pytest.mark.parametrize("n", [300, 1500, 2500])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_lu_random_pivoting(n, precision):
+    """Gaussian matrices pivot on almost every column: the factors reproduce P A = L U,
+    the pivot sequence is LAPACK getrf's (|re| + |im| rule) and the solve is accurate."""
+    import scipy.linalg
+    from grim_20261007_amd import engine
+    dev = _dev()
+    _lib()
+    rng = np.random.default_rng(1000 + n)
+    a = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    b = rng.standard_normal((2, n, 4)) + 1j * rng.standard_normal((2, n, 4))
+    z = torch.as_tensor(np.ascontiguousarray(np.transpose(a, (0, 2, 1))), device=dev)
+    f = engine.factor(z, precision)
+    x, _ = engine.solve(f, torch.as_tensor(np.ascontiguousarray(np.transpose(b, (0, 2, 1))), device=dev))
+    piv = f.piv.cpu().numpy()
+    lu = np.transpose(f.lu.cpu().numpy(), (0, 2, 1)).astype(np.complex128)
+    for i in range(2):
+        lu_ref, piv_ref = scipy.linalg.lu_factor(a[i].astype(np.complex64) if precision == "mixed" else a[i])
+        agree = np.mean(piv[i] == piv_ref)
+        # the first 64-column panel sees identical data; later FP64 pivots may differ by rounding
+        # near ties, FP32 ones (different summation order from cgetrf, 24-bit mantissa) often do
+        assert np.array_equal(piv[i, :64], piv_ref[:64])
+        if precision == "double":
+            assert agree > 0.95, agree
+        # P A = L U with the returned pivots
+        perm = np.arange(n)
+        for c in range(n):
+            perm[[c, piv[i, c]]] = perm[[piv[i, c], c]]
+        low = np.tril(lu[i], -1) + np.eye(n)
+        up = np.triu(lu[i])
+        rel = np.max(np.abs(a[i][perm] - low @ up)) / np.max(np.abs(a[i]))
+        tol = 1e-12 if precision == "double" else 2e-4
+        want = np.linalg.solve(a[i], b[i])
+        err = np.max(np.abs(x[i].cpu().numpy().T - want)) / np.max(np.abs(want))
+        print(f"pivoting n={n} {precision} matrix {i}: pivots agree {agree:.4f}, |PA-LU| {rel:.2e}, "
+              f"solve {err:.2e}", flush=True)
+        assert rel < tol
+        assert err < 1e-10
+
+
 @pytest.mark.parametrize("name", golden_cases.ALL)
 def test_assembly_matches_reference(name):
     from grim_20261007_amd import engine
