@@ -197,6 +197,8 @@ def solve(f: Factor, b: torch.Tensor, gate: bool = True):
     if code == 6:
         raise np.linalg.LinAlgError("Mixed-precision refinement stalled; double LU required.")
     native.check("grim_refined_solve", code)
+    # the workspace starts with the final residual b - Z x of the returned x
+    info["residual"] = ws[: 16 * nb * na * n].view(torch.complex128).view(nb, na, n)
     return (x if b.dim() == 3 else x[0]), info
 
 

@@ -91,8 +91,9 @@ int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx, const dou
  * (strides in complex elements for matrices/vectors, in elements for int32
  * arrays, in bytes for aux), e.g. the TE and TM systems of one request.
  *
- * Bytes of factor-side auxiliary storage per matrix (the row permutation the
- * pivots compose to, 4 n bytes rounded up to 256); precision 64 or 32. */
+ * Bytes of factor-side auxiliary storage per matrix: the row permutation the
+ * pivots compose to (4 n bytes) and the interchange list of the panel being
+ * factored (1 KiB), each rounded up to 256 bytes; precision 64 or 32. */
 int64_t grim_lu_aux_bytes(int64_t n, int precision);
 
 /* In-place LU with partial pivoting, P A = L U (LAPACK pivot rule: first row of
@@ -128,7 +129,10 @@ int64_t grim_refine_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs);
  * does not reduce the error below 0.98 of the previous one returns 6).
  * z: complex128 systems; lu/aux: complex64 factors of the same matrices.
  * Synchronises the stream once per step.  steps_out / error_out: host
- * pointers (may be NULL). */
+ * pointers (may be NULL).  On GRIM_OK the workspace starts with the final
+ * residual b - Z x (complex128, leading dimension n, stride n * nrhs per
+ * matrix), the residual linalg/refined_lu.py:41-61 returns with
+ * return_residual=True. */
 int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz, int64_t stride_z,
                        const float* lu, int64_t ldlu, int64_t stride_lu, const void* aux, int64_t stride_aux,
                        const double* b, int64_t ldb, int64_t stride_b, double* x, int64_t ldx, int64_t stride_x,
