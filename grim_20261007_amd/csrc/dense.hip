@@ -50,20 +50,31 @@ template <> struct Mfma<float> {
 // ---------------------------------------------------------------------------
 // C[m x n] (-)= A[m x k] * B[k x n]   (complex, column-major, interleaved)
 // SET: C = A B, else C -= A B.  256 threads, 64 x 64 output tile, 4 waves of
-// 32 x 32 (2 x 2 MFMA tiles), k staged through LDS 16 at a time; blockIdx.z
-// selects the matrix of a strided batch (strides in complex elements).
+// 32 x 32 (2 x 2 MFMA tiles); k staged through LDS GB_K<T> at a time (16 for
+// complex128, 32 for complex64) with the next k-slab prefetched into
+// registers while the current one is multiplied.  The MFMA computes the
+// transposed tile (B^T A^T), so a lane's outputs run down a column of C and
+// the read-modify-write of C is coalesced.  blockIdx.z selects the matrix of a
+// strided batch (strides in complex elements).
 // ---------------------------------------------------------------------------
 #define GB_M 64
 #define GB_N 64
-#define GB_K 16
+template <typename T> struct Cplx2;
+template <> struct Cplx2<double> { typedef double2 type; };
+template <> struct Cplx2<float> { typedef float2 type; };
+template <typename T> constexpr int gb_k() { return sizeof(T) == 8 ? 16 : 32; }
 
 template <typename T, bool SET>
 __global__ void __launch_bounds__(256) cgemm(int m, int n, int k, const T* __restrict__ A, int64_t lda, int64_t sa,
                                              const T* __restrict__ B, int64_t ldb, int64_t sb, T* __restrict__ C,
                                              int64_t ldc, int64_t sc) {
     typedef typename Mfma<T>::acc acc_t;
-    __shared__ T a_re[GB_K][GB_M + 1], a_im[GB_K][GB_M + 1];
-    __shared__ T b_re[GB_K][GB_N + 1], b_im[GB_K][GB_N + 1];
+    typedef typename Cplx2<T>::type v2;
+    constexpr int BK = gb_k<T>();
+    constexpr int PA = BK / 4;   // complex A elements each thread stages per k-slab (64 x BK / 256)
+    constexpr int PB = BK / 4;   // complex B elements (BK x 64 / 256)
+    __shared__ T a_re[BK][GB_M + 1], a_im[BK][GB_M + 1];
+    __shared__ T b_re[BK][GB_N + 1], b_im[BK][GB_N + 1];
     A += 2 * sa * blockIdx.z;
     B += 2 * sb * blockIdx.z;
     C += 2 * sc * blockIdx.z;
@@ -80,37 +91,39 @@ __global__ void __launch_bounds__(256) cgemm(int m, int n, int k, const T* __res
             ci[i][j] = acc_t{0, 0, 0, 0};
         }
     const int fr = lane & 15, fk = lane >> 4;
-    for (int k0 = 0; k0 < k; k0 += GB_K) {
+    // staging map: A rows tid & 63, columns (tid >> 6) + 4 i;  B rows tid % BK, columns tid / BK + (256 / BK) i
+    const int ar0 = tid & 63, ac0 = tid >> 6;
+    const int br0 = tid % BK, bc0 = tid / BK;
+    v2 pa[PA], pb[PB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = tid & 63, c = (tid >> 6) + 4 * i;
-            const int gr = m0 + r, gc = k0 + c;
-            T vr = 0, vi = 0;
-            if (gr < m && gc < k) {
-                const T* p = A + 2 * ((int64_t)gr + (int64_t)gc * lda);
-                vr = p[0];
-                vi = p[1];
-            }
-            a_re[c][r] = vr;
-            a_im[c][r] = vi;
+        for (int i = 0; i < PA; ++i) {
+            const int gr = m0 + ar0, gc = k0 + ac0 + 4 * i;
+            pa[i] = (gr < m && gc < k) ? *(const v2*)(A + 2 * ((int64_t)gr + (int64_t)gc * lda)) : v2{0, 0};
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = tid & 15, c = (tid >> 4) + 16 * i;
-            const int gr = k0 + r, gc = n0 + c;
-            T vr = 0, vi = 0;
-            if (gr < k && gc < n) {
-                const T* p = B + 2 * ((int64_t)gr + (int64_t)gc * ldb);
-                vr = p[0];
-                vi = p[1];
-            }
-            b_re[r][c] = vr;
-            b_im[r][c] = vi;
+        for (int i = 0; i < PB; ++i) {
+            const int gr = k0 + br0, gc = n0 + bc0 + (256 / BK) * i;
+            pb[i] = (gr < k && gc < n) ? *(const v2*)(B + 2 * ((int64_t)gr + (int64_t)gc * ldb)) : v2{0, 0};
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < k; k0 += BK) {
+#pragma unroll
+        for (int i = 0; i < PA; ++i) {
+            a_re[ac0 + 4 * i][ar0] = pa[i].x;
+            a_im[ac0 + 4 * i][ar0] = pa[i].y;
+        }
+#pragma unroll
+        for (int i = 0; i < PB; ++i) {
+            b_re[br0][bc0 + (256 / BK) * i] = pb[i].x;
+            b_im[br0][bc0 + (256 / BK) * i] = pb[i].y;
         }
         __syncthreads();
+        if (k0 + BK < k) fetch(k0 + BK);
 #pragma unroll
-        for (int kk = 0; kk < GB_K; kk += 4) {
-            T ar[2], ai[2], br[2], bi[2];
+        for (int kk = 0; kk < BK; kk += 4) {
+            T ar[2], ai[2], br[2], bi[2], nbi[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 ar[i] = a_re[kk + fk][wm + 16 * i + fr];
@@ -120,36 +133,40 @@ __global__ void __launch_bounds__(256) cgemm(int m, int n, int k, const T* __res
             for (int j = 0; j < 2; ++j) {
                 br[j] = b_re[kk + fk][wn + 16 * j + fr];
                 bi[j] = b_im[kk + fk][wn + 16 * j + fr];
+                nbi[j] = -bi[j];
             }
+            // transposed tile: D[j-col][i-row] = sum_k B[k][j] A[i][k]
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    cr[i][j] = Mfma<T>::run(ar[i], br[j], cr[i][j]);
-                    cr[i][j] = Mfma<T>::run(-ai[i], bi[j], cr[i][j]);
-                    ci[i][j] = Mfma<T>::run(ar[i], bi[j], ci[i][j]);
-                    ci[i][j] = Mfma<T>::run(ai[i], br[j], ci[i][j]);
+                    cr[i][j] = Mfma<T>::run(br[j], ar[i], cr[i][j]);
+                    cr[i][j] = Mfma<T>::run(nbi[j], ai[i], cr[i][j]);
+                    ci[i][j] = Mfma<T>::run(bi[j], ar[i], ci[i][j]);
+                    ci[i][j] = Mfma<T>::run(br[j], ai[i], ci[i][j]);
                 }
         }
         __syncthreads();
     }
-    // lane holds D[row(lane / 16, v)][lane % 16] of each 16 x 16 tile
+    // lane holds D[row(lane / 16, v)][lane % 16] of each transposed 16 x 16 tile:
+    // C row = lane % 16 (consecutive lanes, consecutive rows), C column = row(lane / 16, v)
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int gr = m0 + wm + 16 * i + Mfma<T>::row(fk, v);
-                const int gc = n0 + wn + 16 * j + fr;
+                const int gr = m0 + wm + 16 * i + fr;
+                const int gc = n0 + wn + 16 * j + Mfma<T>::row(fk, v);
                 if (gr < m && gc < n) {
-                    T* p = C + 2 * ((int64_t)gr + (int64_t)gc * ldc);
+                    v2* p = (v2*)(C + 2 * ((int64_t)gr + (int64_t)gc * ldc));
                     if (SET) {
-                        p[0] = cr[i][j][v];
-                        p[1] = ci[i][j][v];
+                        *p = v2{cr[i][j][v], ci[i][j][v]};
                     } else {
-                        p[0] -= cr[i][j][v];
-                        p[1] -= ci[i][j][v];
+                        v2 c = *p;
+                        c.x -= cr[i][j][v];
+                        c.y -= ci[i][j][v];
+                        *p = c;
                     }
                 }
             }
