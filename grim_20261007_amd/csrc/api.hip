@@ -177,6 +177,7 @@ extern "C" int grim_workspace_status(void* workspace, void* stream) {
     CHECK(hipStreamSynchronize(st));
     if (h & GRIM_DEV_NEAR_CAPACITY) return GRIM_ERR_CAPACITY;
     if (h & (GRIM_DEV_ADAPTIVE_DEPTH | GRIM_DEV_SELF_SERIES)) return GRIM_ERR_NOT_CONVERGED;
+    if (h & GRIM_DEV_BARRIER_TIMEOUT) return GRIM_ERR_TIMEOUT;
     if (h & GRIM_DEV_ZERO_PIVOT) return GRIM_ERR_SINGULAR;
     return GRIM_OK;
 }
@@ -202,14 +203,14 @@ extern "C" int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx
 static bool batch_ok(int64_t batch) { return batch >= 1 && batch <= 65535; }
 
 extern "C" int64_t grim_lu_aux_bytes(int64_t n, int precision) {
-    if (n < 1 || n > 32768 || (precision != 64 && precision != 32)) return -1;
+    if (n < 1 || n > 65536 || (precision != 64 && precision != 32)) return -1;
     return grim_dense_aux_bytes((int)n, precision == 32);
 }
 
 extern "C" int grim_lu_factor(int64_t batch, int64_t n, void* a, int64_t lda, int64_t stride_a, int precision,
                               int32_t* piv, int64_t stride_piv, void* aux, int64_t stride_aux, void* status,
                               void* stream) {
-    if (!batch_ok(batch) || n < 1 || n > 32768 || !a || !piv || !aux || !status || lda < n ||
+    if (!batch_ok(batch) || n < 1 || n > 65536 || !a || !piv || !aux || !status || lda < n ||
         (precision != 64 && precision != 32))
         return GRIM_ERR_ARGS;
     if (batch > 1 && (stride_a < lda * n || stride_piv < n || stride_aux < grim_lu_aux_bytes(n, precision) ||

@@ -19,6 +19,7 @@ enum {
     GRIM_ERR_NOT_CONVERGED = 6,
     GRIM_ERR_CAPACITY = 7,
     GRIM_ERR_SINGULAR = 8,
+    GRIM_ERR_TIMEOUT = 9,
 };
 
 // Device status word bits (written by kernels, read by the host once per call).
@@ -27,6 +28,7 @@ enum {
     GRIM_DEV_NEAR_CAPACITY = 2,
     GRIM_DEV_SELF_SERIES = 4,
     GRIM_DEV_ZERO_PIVOT = 8,
+    GRIM_DEV_BARRIER_TIMEOUT = 16,
 };
 
 struct cplx {

@@ -221,6 +221,13 @@ template <typename T> __device__ __forceinline__ void argmax16(T& v, int& i) {
 GRIM_AM_SWAP(am_swap16, __builtin_amdgcn_permlane16_swap)
 GRIM_AM_SWAP(am_swap32, __builtin_amdgcn_permlane32_swap)
 #undef GRIM_AM_SWAP
+__device__ __forceinline__ double first_lane(double v) {
+    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                            __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+__device__ __forceinline__ float first_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 template <typename T> __device__ __forceinline__ void argmax64(T& v, int& i) {
     argmax16(v, i);
     am_swap16(v, i);
@@ -454,6 +461,250 @@ __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tall-panel leaf: the same column steps as panel_leaf for panels taller than
+// one workgroup's registers hold.  G workgroups (grid.x, G <= GRIM_TALL_G)
+// each own a contiguous block of 1024 * R rows; per column every workgroup
+// publishes its candidate (value, row and the row's TW values) and the owner
+// of row c publishes row c in double-buffered device-memory slots, then all
+// meet at a device-wide barrier: one atomic counter per matrix that only
+// grows during a factorization (`base` = arrivals of earlier tall leaves), so
+// it needs no reset between columns.  The wait is bounded: on expiry the
+// kernel flags GRIM_DEV_BARRIER_TIMEOUT and carries on with clamped indices,
+// so it always terminates and never leaves the matrix.  Matrices of a batch
+// are grid.y.  TW = 2 leaf columns.
+// ---------------------------------------------------------------------------
+#define GRIM_TALL_G 8
+#define GRIM_TALL_TW 2
+template <typename T> struct TallScratch {
+    unsigned counter;
+    unsigned pad[63];
+    T cand_v[2][GRIM_TALL_G];
+    int cand_i[2][GRIM_TALL_G];
+    T cand_re[2][GRIM_TALL_G][GRIM_TALL_TW], cand_im[2][GRIM_TALL_G][GRIM_TALL_TW];
+    T crow_re[2][GRIM_TALL_TW], crow_im[2][GRIM_TALL_TW];
+};
+
+template <typename T> __device__ __forceinline__ void pub(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ T sub(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(1024) panel_leaf_tall(int m, int w, T* __restrict__ pan, int64_t lda, int64_t sa,
+                                                        int pw, int lc0, int row0, int* __restrict__ piv,
+                                                        int64_t spiv, unsigned* __restrict__ status, char* scratch,
+                                                        int64_t sscr, unsigned base) {
+    constexpr int W = GRIM_TALL_TW;
+    __shared__ T wv[16];
+    __shared__ int wi[16];
+    __shared__ int lpiv[W];
+    __shared__ int zero_pivot;
+    const int G = gridDim.x, g = blockIdx.x;
+    pan += 2 * sa * blockIdx.y;
+    piv += spiv * blockIdx.y;
+    TallScratch<T>* ts = (TallScratch<T>*)(scratch + sscr * blockIdx.y);
+    T* a = pan + 2 * (int64_t)lc0 * lda;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rbase = g * 1024 * R;
+    if (tid == 0) zero_pivot = 0;
+    T vr[R][W], vi[R][W];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = rbase + tid + 1024 * r;
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            T x = 0, y = 0;
+            if (row < m && c < w) {
+                const T* p = a + 2 * ((int64_t)row + (int64_t)c * lda);
+                x = p[0];
+                y = p[1];
+            }
+            vr[r][c] = x;
+            vi[r][c] = y;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        if (c < w) {
+            const int buf = c & 1;
+            T best = -1;
+            int bi = 0x7fffffff;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int row = rbase + tid + 1024 * r;
+                if (row >= c && row < m) {
+                    T v = fabs(vr[r][c]) + fabs(vi[r][c]);
+                    v = v == v ? v : T(-0.5);
+                    am_take(best, bi, v, row);
+                }
+            }
+            argmax64(best, bi);
+            if (lane == 0) { wv[wave] = best; wi[wave] = bi; }
+            __syncthreads();
+            T gv = T(-2);
+            int gp = 0x7fffffff;
+            if (lane < 16) { gv = wv[lane]; gp = wi[lane]; }
+            argmax16(gv, gp);   // lanes 0-15 hold the workgroup's result
+            gp = __builtin_amdgcn_readfirstlane(gp);
+            gv = first_lane(gv);
+            // publish this workgroup's candidate and (if owned) row c
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int row = rbase + tid + 1024 * r;
+                if (row == gp) {
+                    pub(&ts->cand_v[buf][g], gv);
+                    pub(&ts->cand_i[buf][g], gp);
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) {
+                        pub(&ts->cand_re[buf][g][cc], vr[r][cc]);
+                        pub(&ts->cand_im[buf][g][cc], vi[r][cc]);
+                    }
+                }
+                if (row == c) {
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) {
+                        pub(&ts->crow_re[buf][cc], vr[r][cc]);
+                        pub(&ts->crow_im[buf][cc], vi[r][cc]);
+                    }
+                }
+            }
+            if (gp >= m) {   // no row >= c in this block: an empty candidate
+                if (tid == 0) {
+                    pub(&ts->cand_v[buf][g], T(-2));
+                    pub(&ts->cand_i[buf][g], 0x7fffffff);
+                }
+            }
+            __syncthreads();
+            if (tid == 0) {
+                __hip_atomic_fetch_add(&ts->counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned target = base + (unsigned)(c + 1) * (unsigned)G;
+                unsigned spins = 0;
+                while (__hip_atomic_load(&ts->counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    if (++spins > (1u << 24)) {
+                        atomicOr(status, (unsigned)GRIM_DEV_BARRIER_TIMEOUT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();
+            T bv = T(-2);
+            int p = 0x7fffffff;
+            if (lane < G) {
+                int ci = sub(&ts->cand_i[buf][lane]);
+                bv = sub(&ts->cand_v[buf][lane]);
+                if (ci < 0 || ci >= m) {   // empty slot (or stale after an expired wait)
+                    ci = m;
+                    bv = T(-2);
+                }
+                p = ci * 16 + lane;
+            }
+            argmax16(bv, p);
+            p = __builtin_amdgcn_readfirstlane(p);
+            const int win = p & 15;
+            p >>= 4;
+            if (p < c || p >= m) p = c;   // only after an expired wait
+            T ure[W], uim[W], cre[W], cim[W];
+#pragma unroll
+            for (int cc = 0; cc < W; ++cc) {
+                ure[cc] = sub(&ts->cand_re[buf][win][cc]);
+                uim[cc] = sub(&ts->cand_im[buf][win][cc]);
+                cre[cc] = sub(&ts->crow_re[buf][cc]);
+                cim[cc] = sub(&ts->crow_im[buf][cc]);
+            }
+            if (p == c) {
+#pragma unroll
+                for (int cc = 0; cc < W; ++cc) { ure[cc] = cre[cc]; uim[cc] = cim[cc]; }
+            }
+            if (tid == 0) {
+                lpiv[c] = p;
+                if (!(bv > 0)) zero_pivot = 1;
+            }
+            const T dr = ure[c], di = uim[c];
+            const T den = dr * dr + di * di;
+            const T ir = den > 0 ? dr / den : 0, ii = den > 0 ? -di / den : 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int row = rbase + tid + 1024 * r;
+                if (p != c) {
+                    if (row == c) {
+#pragma unroll
+                        for (int cc = 0; cc < W; ++cc) { vr[r][cc] = ure[cc]; vi[r][cc] = uim[cc]; }
+                    } else if (row == p) {
+#pragma unroll
+                        for (int cc = 0; cc < W; ++cc) { vr[r][cc] = cre[cc]; vi[r][cc] = cim[cc]; }
+                    }
+                }
+                if (row > c && row < m) {
+                    const T lr = vr[r][c] * ir - vi[r][c] * ii;
+                    const T li = vr[r][c] * ii + vi[r][c] * ir;
+                    vr[r][c] = lr;
+                    vi[r][c] = li;
+#pragma unroll
+                    for (int cc = 0; cc < W; ++cc) {
+                        if (cc > c) {
+                            vr[r][cc] -= lr * ure[cc] - li * uim[cc];
+                            vi[r][cc] -= lr * uim[cc] + li * ure[cc];
+                        }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = rbase + tid + 1024 * r;
+        if (row < m) {
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                if (c < w) {
+                    T* p = a + 2 * ((int64_t)row + (int64_t)c * lda);
+                    p[0] = vr[r][c];
+                    p[1] = vi[r][c];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (g != 0) return;
+    if (tid < w) piv[tid] = row0 + lpiv[tid];
+    if (tid == 0 && zero_pivot) atomicOr(status, (unsigned)GRIM_DEV_ZERO_PIVOT);
+    if (pw == w) return;
+    // the leaf's interchanges on the panel's other columns: at most 2 W rows,
+    // applied by the first workgroup (rows of other columns are in memory only)
+    __shared__ int list_dst[2 * W], list_src[2 * W];
+    __shared__ int list_n;
+    if (wave == 0) {
+        const Composite x = wave_composite(0, w, lpiv);
+        if (lane < w) { list_dst[lane] = lane; list_src[lane] = x.top; }
+        if (lane < x.nlow) { list_dst[w + lane] = x.ldst; list_src[w + lane] = x.lsrc; }
+        if (lane == 0) list_n = w + x.nlow;
+    }
+    __syncthreads();
+    const int cnt = list_n;
+    const int others = pw - w;
+    const int items = cnt * others;
+    T sr = 0, si = 0;
+    int e = 0, col = 0;
+    if (tid < items) {
+        e = tid % cnt;
+        const int oc = tid / cnt;
+        col = oc < lc0 ? oc : oc + w;
+        const T* p = pan + 2 * ((int64_t)list_src[e] + (int64_t)col * lda);
+        sr = p[0];
+        si = p[1];
+    }
+    __syncthreads();
+    if (tid < items) {
+        T* p = pan + 2 * ((int64_t)list_dst[e] + (int64_t)col * lda);
+        p[0] = sr;
+        p[1] = si;
+    }
+}
+
 // Interchange list of the panel [pj, pj + pw) from its pivots; also applied to
 // the running row permutation (perm[i] = original row now at position i).
 // One wave per matrix.
@@ -475,9 +726,11 @@ __global__ void __launch_bounds__(64) panel_perm(int pj, int pw, const int* __re
     if (lane < x.nlow) perm[x.ldst] = vl;
 }
 
-__global__ void __launch_bounds__(256) perm_init(int n, int* __restrict__ perm, int64_t sperm) {
+__global__ void __launch_bounds__(256) perm_init(int n, int* __restrict__ perm, int64_t sperm,
+                                                 unsigned* __restrict__ counter) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) perm[sperm * blockIdx.z + i] = i;
+    if (i == 0) counter[sperm * blockIdx.z] = 0;   // the tall leaves' barrier counter
 }
 
 // Apply a panel's interchange list to columns [0, ncols) of a (rows absolute).
@@ -726,19 +979,34 @@ static hipError_t gemm(const Batch& bt, int m, int n, int k, const T* A, int64_t
     return hipGetLastError();
 }
 
+#define GRIM_LEAF_MAX_M 16384
+
 template <typename T>
 static hipError_t leaf(const Batch& bt, int m, int w, T* pan, int64_t lda, int64_t sa, int pw, int lc0, int row0,
-                       int* piv, int64_t spiv, unsigned* status) {
+                       int* piv, int64_t spiv, unsigned* status, char* tall, int64_t stall, unsigned& arrivals) {
     hipStream_t st = bt.st;
     const int b = bt.count;
+    if (m > GRIM_LEAF_MAX_M) {
+        // rows over G <= GRIM_TALL_G workgroups of 1024 * R rows each
+        const int R = m <= 2 * GRIM_LEAF_MAX_M ? 4 : 8;
+        const int G = (m + 1024 * R - 1) / (1024 * R);
+        if (G > GRIM_TALL_G) return hipErrorInvalidValue;
+        if (R == 4)
+            panel_leaf_tall<T, 4><<<dim3(G, b), 1024, 0, st>>>(m, w, pan, lda, sa, pw, lc0, row0, piv, spiv, status,
+                                                                tall, stall, arrivals);
+        else
+            panel_leaf_tall<T, 8><<<dim3(G, b), 1024, 0, st>>>(m, w, pan, lda, sa, pw, lc0, row0, piv, spiv, status,
+                                                                tall, stall, arrivals);
+        arrivals += (unsigned)(w * G);
+        return hipGetLastError();
+    }
 #define GRIM_LEAF(R, W) \
     panel_leaf<T, R, W><<<b, 1024, 0, st>>>(m, w, pan, lda, sa, pw, lc0, row0, piv, spiv, status)
     if (m <= 1024) GRIM_LEAF(1, 16);
     else if (m <= 2048) GRIM_LEAF(2, 8);
     else if (m <= 4096) GRIM_LEAF(4, 4);
     else if (m <= 8192) GRIM_LEAF(8, 2);
-    else if (m <= 16384) GRIM_LEAF(16, 1);
-    else GRIM_LEAF(32, 1);
+    else GRIM_LEAF(16, 1);
 #undef GRIM_LEAF
     return hipGetLastError();
 }
@@ -748,7 +1016,8 @@ static int leaf_width(int m) {
     if (m <= 2048) return 8;
     if (m <= 4096) return 4;
     if (m <= 8192) return 2;
-    return 1;
+    if (m <= GRIM_LEAF_MAX_M) return 1;
+    return GRIM_TALL_TW;
 }
 
 template <typename T, bool LOWER>
@@ -773,16 +1042,17 @@ static hipError_t trsm(const Batch& bt, int w, int ncols, const T* tri, int64_t 
 // pass.
 template <typename T>
 static hipError_t panel_rec(const Batch& bt, int n, T* a, int64_t lda, int64_t sa, int pj, int pw, int j0, int w,
-                            int* piv, int64_t spiv, unsigned* status) {
+                            int* piv, int64_t spiv, unsigned* status, char* tall, int64_t stall,
+                            unsigned& arrivals) {
     const int m = n - j0;
     const int lw = leaf_width(m);
     T* top = a + 2 * ((int64_t)j0 + (int64_t)j0 * lda);
     if (w <= lw)
         return leaf<T>(bt, m, w, a + 2 * ((int64_t)j0 + (int64_t)pj * lda), lda, sa, pw, j0 - pj, j0, piv + j0,
-                       spiv, status);
+                       spiv, status, tall, stall, arrivals);
     int w1 = w / 2;
     if (w1 > lw) w1 = (w1 / lw) * lw;
-    hipError_t err = panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0, w1, piv, spiv, status);
+    hipError_t err = panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0, w1, piv, spiv, status, tall, stall, arrivals);
     if (err != hipSuccess) return err;
     const int w2 = w - w1;
     T* right = a + 2 * ((int64_t)(j0 + w1) * lda);
@@ -790,14 +1060,16 @@ static hipError_t panel_rec(const Batch& bt, int n, T* a, int64_t lda, int64_t s
     if ((err = gemm<T>(bt, m - w1, w2, w1, top + 2 * (int64_t)w1, lda, sa, right + 2 * (int64_t)j0, lda, sa,
                        right + 2 * (int64_t)(j0 + w1), lda, sa)) != hipSuccess)
         return err;
-    return panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0 + w1, w2, piv, spiv, status);
+    return panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0 + w1, w2, piv, spiv, status, tall, stall, arrivals);
 }
 
 // Factor-side auxiliary storage per matrix: the row permutation the pivots
 // compose to (n int32, applied to right-hand sides in one gather) and the
 // interchange list of the panel being factored.
 static int64_t align256(int64_t v) { return (v + 255) / 256 * 256; }
-static int64_t aux_bytes(int n) { return align256(4 * (int64_t)n) + align256(4 * GRIM_LIST_INTS); }
+static int64_t aux_bytes(int n) {
+    return align256(4 * (int64_t)n) + align256(4 * GRIM_LIST_INTS) + align256(sizeof(TallScratch<double>));
+}
 
 template <typename T>
 static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t sa, int* piv, int64_t spiv,
@@ -805,12 +1077,16 @@ static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t s
     hipError_t err;
     int* perm = (int*)auxp;
     int* list = (int*)((char*)auxp + align256(4 * (int64_t)n));
+    char* tall = (char*)auxp + align256(4 * (int64_t)n) + align256(4 * GRIM_LIST_INTS);
     const int64_t sperm = saux / 4;
-    perm_init<<<dim3((n + 255) / 256, 1, bt.count), 256, 0, bt.st>>>(n, perm, sperm);
+    perm_init<<<dim3((n + 255) / 256, 1, bt.count), 256, 0, bt.st>>>(n, perm, sperm, (unsigned*)tall);
     if ((err = hipGetLastError()) != hipSuccess) return err;
+    unsigned arrivals = 0;
     for (int j = 0; j < n; j += GRIM_LU_NB) {
         const int jb = min(GRIM_LU_NB, n - j);
-        if ((err = panel_rec<T>(bt, n, a, lda, sa, j, jb, j, jb, piv, spiv, status)) != hipSuccess) return err;
+        if ((err = panel_rec<T>(bt, n, a, lda, sa, j, jb, j, jb, piv, spiv, status, tall, saux, arrivals)) !=
+            hipSuccess)
+            return err;
         panel_perm<<<bt.count, 64, 0, bt.st>>>(j, jb, piv, spiv, perm, sperm, list, sperm);
         if ((err = hipGetLastError()) != hipSuccess) return err;
         // the panel's interchanges on the columns left of it (LAPACK keeps L permuted)

@@ -5,7 +5,8 @@
  * reference's own native seam (twod/assembly/native/far.c, far.py):
  *   - int status return: 0 ok, 1 size/index out of range, 3 invalid arguments,
  *     5 HIP runtime failure, 6 adaptive near rule not converged / refinement
- *     stalled, 7 near-pair capacity exceeded, 8 zero pivot;
+ *     stalled, 7 near-pair capacity exceeded, 8 zero pivot, 9 a device-wide
+ *     barrier of the tall-panel LU did not complete (bounded wait expired);
  *   - sizes are int64_t; complex arrays are interleaved double[2] (complex128)
  *     or float[2] (complex64), column-major, leading dimension in elements;
  *   - caller-owned outputs and workspace (all pointers are DEVICE pointers
@@ -97,7 +98,7 @@ int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx, const dou
 int64_t grim_lu_aux_bytes(int64_t n, int precision);
 
 /* In-place LU with partial pivoting, P A = L U (LAPACK pivot rule: first row of
- * largest |re| + |im|), complex128 (precision 64) or complex64 (32), n <= 32768.
+ * largest |re| + |im|), complex128 (precision 64) or complex64 (32), n <= 65536.
  * piv: int32 [n] per matrix (absolute row exchanged with row c, LAPACK getrf
  * order, 0-based); aux: grim_lu_aux_bytes per matrix, stride a multiple of 256.
  * status: device uint32 word (read with grim_workspace_status).  Replaces
