@@ -51,6 +51,12 @@ def parse():
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=40.0)
+    ap.add_argument("--workload", choices=("naca", "sweep", "stream", "large"), default="naca",
+                    help="naca: the headline NACA 0012 N=2000 request (default); sweep: 64-frequency star-curve "
+                         "sweep; stream: Poisson serving stream with cache reuse; large: one N=40000 request")
+    ap.add_argument("--large-n", type=int, default=40000)
+    ap.add_argument("--stream-count", type=int, default=64)
+    ap.add_argument("--stream-rate", type=float, default=25.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -227,6 +233,162 @@ def serve(engine_mod, mesh, f, angles, precision, workers, nreq, warm):
     return res, elapsed, svc
 
 
+def _requests_plain(engine, meshes, freqs, angle_sets, precision):
+    """One request at a time on the current stream, no reuse: (elapsed seconds, per-request ms)."""
+    import torch
+    from grim_20261007_amd import serving
+    dev = torch.device("cuda", 0)
+    lat, ws = [], {}
+    t0 = time.perf_counter()
+    for mesh, f, ang in zip(meshes, freqs, angle_sets):
+        t1 = time.perf_counter()
+        serving.execute(serving.Request(mesh, f, ang, precision=precision), dev, ws)
+        torch.cuda.synchronize()
+        lat.append(1e3 * (time.perf_counter() - t1))
+    return time.perf_counter() - t0, np.array(lat)
+
+
+def sweep_workload(args):
+    """64-frequency sweep of one random star-shaped PEC curve (R = 0.5 m, seed 20261203), N from
+    100 panels per wavelength of the longest panel (about 360-3600), 360 angles, TE + TM."""
+    import torch
+    from grim_20261007_amd import engine, serving, workloads
+    from grim_20261007_amd.mesh import build_mesh
+    cases = workloads.star_sweep()
+    meshes, freqs, angle_sets = [], [], []
+    for pts, f, ang in cases:
+        meshes.append(build_mesh(workloads._snapshot(pts, "star"), "meters", f))
+        freqs.append(f)
+        angle_sets.append(ang)
+    ns = [m.n for m in meshes]
+    # plain FP64 path, one request at a time (warm once on the smallest)
+    _requests_plain(engine, meshes[:1], freqs[:1], angle_sets[:1], "double")
+    t64, lat64 = _requests_plain(engine, meshes, freqs, angle_sets, "double")
+    # mixed-precision serving form
+    srv = serving.Server(workers=args.workers)
+    try:
+        serving.run_stream(srv, [serving.Request(meshes[0], freqs[0], angle_sets[0], key="warm")])
+        reqs = [serving.Request(m, f, a, precision="mixed", key=f"s{i}")
+                for i, (m, f, a) in enumerate(zip(meshes, freqs, angle_sets))]
+        res, tm = serving.run_stream(srv, reqs)
+    finally:
+        srv.close()
+    lat = np.array([r.latency_s for r in res]) * 1e3
+    solves = 2 * len(meshes)
+    print(json.dumps({
+        "workload": "star-curve frequency sweep", "requests": len(meshes), "n_min": min(ns), "n_max": max(ns),
+        "angles": 360, "frequencies_ghz": [round(freqs[0], 4), round(freqs[-1], 4)],
+        "mixed_serving": {"value": round(solves / tm, 3), "unit": "RCS solves/s", "workers": args.workers,
+                          "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                          "max_residual": max(max(r.residual.values()) for r in res)},
+        "fp64_plain": {"value": round(solves / t64, 3), "unit": "RCS solves/s",
+                       "p50_ms": round(float(np.percentile(lat64, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat64, 99)), 3)},
+        "speedup": round(t64 / tm, 3)}))
+
+
+def stream_workload(args):
+    """Poisson serving stream of star-shaped curves: N log-uniform in [200, 12000], angle sets of
+    90 / 360 / 1440, 30 % of requests reusing an earlier (geometry, frequency) with new angles."""
+    import torch
+    from grim_20261007_amd import engine, serving, workloads
+    from grim_20261007_amd.mesh import build_mesh
+    stream = workloads.serving_stream(args.stream_count)
+    meshes, cache = [], {}
+    for item in stream:
+        if item["key"] not in cache:
+            cache[item["key"]] = build_mesh(workloads._snapshot(item["points"], "star"), "meters",
+                                            item["frequency_ghz"])
+        meshes.append(cache[item["key"]])
+    freqs = [it["frequency_ghz"] for it in stream]
+    angle_sets = [it["angles"] for it in stream]
+    ns = [m.n for m in meshes]
+    _requests_plain(engine, meshes[:1], freqs[:1], angle_sets[:1], "double")
+    t64, lat64 = _requests_plain(engine, meshes, freqs, angle_sets, "double")
+    srv = serving.Server(workers=args.workers)
+    try:
+        serving.run_stream(srv, [serving.Request(meshes[0], freqs[0], angle_sets[0], key="warm")])
+        reqs = [serving.Request(m, f, a, precision="mixed", key=it["key"])
+                for m, f, a, it in zip(meshes, freqs, angle_sets, stream)]
+        res, tm = serving.run_stream(srv, reqs, rate_hz=args.stream_rate, rng=np.random.default_rng(11))
+        hits = srv.cache.hits
+    finally:
+        srv.close()
+    lat = np.array([r.latency_s for r in res]) * 1e3
+    solves = 2 * len(meshes)
+    print(json.dumps({
+        "workload": "Poisson serving stream", "requests": len(meshes), "n_min": min(ns), "n_max": max(ns),
+        "reused": int(sum(it["reused"] for it in stream)), "cache_hits": hits,
+        "arrival_rate_hz": args.stream_rate,
+        "mixed_serving": {"value": round(solves / tm, 3), "unit": "RCS solves/s", "workers": args.workers,
+                          "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                          "note": "value includes waiting for Poisson arrivals; latency is arrival to result",
+                          "max_residual": max(max(r.residual.values()) for r in res)},
+        "fp64_plain": {"value": round(solves / t64, 3), "unit": "RCS solves/s",
+                       "p50_ms": round(float(np.percentile(lat64, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat64, 99)), 3),
+                       "note": "back-to-back, no arrivals, no reuse"}}))
+
+
+def large_workload(args):
+    """One elongated star-shaped body (R = 2 m, aspect 4:1), N = args.large_n, longest panel
+    lambda/10, 1440 angles, TE + TM: complex64 LU + FP64 refinement against the FP64 LU."""
+    import torch
+    from grim_20261007_amd import engine, native, workloads
+    from grim_20261007_amd.mesh import build_mesh
+    from grim_20261007_amd.solver import wavenumber
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(20261203)
+    a, phi = workloads.star_shape(rng)
+    pts = workloads.star_points(args.large_n, 2.0, a, phi, aspect=4.0)
+    f = workloads.frequency_for_segments_per_wavelength(pts, 10.0)
+    mesh = build_mesh(workloads._snapshot(pts, "star"), "meters", f, max_panels=max(20000, args.large_n))
+    n = mesh.n
+    k = wavenumber(f)
+    angles = torch.tensor(np.arange(1440) * 0.25, dtype=torch.float64, device=dev)
+    contour = engine.DeviceContour(mesh, dev)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    out = {"workload": "large single request", "n": n, "angles": 1440, "frequency_ghz": f}
+    e0, e1 = ev(), ev()
+    e0.record()
+    z_tm, z_te = engine.assemble(contour, k)
+    e1.record()
+    torch.cuda.synchronize()
+    out["assemble_ms"] = e0.elapsed_time(e1)
+    pair = z_tm._base
+    bu, bdn = engine.loads(contour, k, angles)
+    rhs = torch.stack((-bdn, -bu))
+    lu_flops = 2 * 8.0 / 3.0 * n ** 3
+    trsm_flops = 2 * 8.0 * n * n * 1440
+    sig = {}
+    for precision in ("mixed", "double"):
+        e0, e1, e2 = ev(), ev(), ev()
+        e0.record()
+        fac = engine.factor(pair, precision)
+        e1.record()
+        x, info = engine.solve(fac, rhs, gate=(precision == "double"))
+        e2.record()
+        torch.cuda.synchronize()
+        rel = engine.relative_residual(pair, x, rhs, info.get("residual"))
+        amps = [engine.farfield(x[i], bu, bdn, k) for i in range(2)]
+        sig[precision] = torch.stack([s for _, s in amps]).cpu().numpy()
+        fm, sm = e0.elapsed_time(e1), e1.elapsed_time(e2)
+        peak = PEAK_FP32 if precision == "mixed" else PEAK_FP64
+        out[precision] = {"factor_ms": round(fm, 1), "solve_ms": round(sm, 1),
+                          "lu_tflops": round(lu_flops / fm / 1e9, 2),
+                          "lu_frac_of_peak": round(lu_flops / fm / 1e9 / peak, 4),
+                          "refinement_steps": info.get("refinement_steps"),
+                          "max_relative_residual": float(torch.max(rel)),
+                          "solves_per_s": round(2000.0 / (out["assemble_ms"] + fm + sm), 4)}
+        del fac, x, info
+        torch.cuda.empty_cache()
+    out["mixed_vs_fp64_max_db"] = float(np.max(np.abs(10 * np.log10(sig["mixed"]) - 10 * np.log10(sig["double"]))))
+    out["trsm_flops_per_solve"] = trsm_flops
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -234,6 +396,11 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload != "naca":
+        from grim_20261007_amd import native
+        native.ensure_device(0)
+        {"sweep": sweep_workload, "stream": stream_workload, "large": large_workload}[args.workload](args)
         return
     import torch
     from grim_20261007_amd import engine, native

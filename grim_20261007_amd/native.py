@@ -64,6 +64,9 @@ _initialised = set()
 def load(path: Path | str = LIB_PATH) -> ctypes.CDLL:
     """Load the shared library (raises OSError if it was not built)."""
     global _lib
+    # PyTorch's HIP runtime is loaded first, so that the library's libamdhip64
+    # dependency binds to that same runtime (one runtime, one set of streams).
+    import torch  # noqa: F401
     with _lock:
         if _lib is None:
             p = Path(path)
@@ -88,10 +91,15 @@ def check(name: str, code: int) -> None:
 
 
 def ensure_device(device: int) -> None:
-    """grim_init once per device (uploads quadrature rules to constant memory)."""
+    """grim_init once per device (uploads quadrature rules to constant memory).
+
+    PyTorch's HIP runtime is initialised first: it must own the device context
+    that the library's launches then share."""
     with _lock:
         if device in _initialised:
             return
+    import torch
+    torch.cuda.init()
     check("grim_init", lib().grim_init(int(device)))
     with _lock:
         _initialised.add(device)

@@ -178,47 +178,56 @@ class Server:
         cached = self.cache.get(key)
         nbytes = request_bytes(mesh.n, len(req.angles_deg), len(req.pols), req.precision, cached is not None)
         self._admit(nbytes)
-        started = time.perf_counter()
         try:
-            dev = self.device
-            contour = cached["contour"] if cached else engine.DeviceContour(mesh, dev)
-            if cached is None:
-                if mesh.n not in ws:
-                    ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
-                z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
-                fac = engine.factor(z_tm._base, req.precision)          # batch [TE, TM]
-                sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * mesh.n * mesh.n
-                self.cache.put(key, {"contour": contour, "factor": fac}, sysbytes)
-            else:
-                fac = cached["factor"]
-                # the cached tensors are read on this worker's stream too
-                cur = torch.cuda.current_stream()
-                for t in (contour.p0, contour.p1, contour.next, contour.prev, contour.fix_rows,
-                          fac.lu, fac.piv, fac.aux, fac.z):
-                    t.record_stream(cur)
-            angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
-            bu, bdn = engine.loads(contour, k, angles)
-            rhs = torch.stack((-bdn, -bu))                             # [TE, TM]
-            x, inf = engine.solve(fac, rhs, gate=(req.precision == "double"))
-            rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
-            amp, sig, resid, info = {}, {}, {}, {}
-            pending = []
-            for i, pol in enumerate(("TE", "TM")):
-                if pol not in req.pols:
-                    continue
-                a, s = engine.farfield(x[i], bu, bdn, k)
-                pending.append((pol, a, s, rel[i]))
-                info[pol] = {kk: v for kk, v in inf.items() if kk != "residual"}
-            torch.cuda.current_stream().synchronize()
-            for pol, a, s, r in pending:
-                amp[pol] = a.cpu().numpy()
-                sig[pol] = s.cpu().numpy()
-                resid[pol] = float(r.max())
-            finished = time.perf_counter()
-            return Result(req.rid, amp, sig, resid, cached is not None, finished - req.arrival, started, finished,
-                          info)
+            return execute(req, self.device, ws, cached, lambda item, nb: self.cache.put(key, item, nb))
         finally:
             self._release(nbytes)
+
+
+def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store=None) -> Result:
+    """Run one request on the current stream: assemble and factor (or reuse `cached`), then loads,
+    solve and far field.  `store(item, nbytes)` receives the assembled + factored systems."""
+    mesh = req.mesh
+    k = wavenumber(req.frequency_ghz)
+    started = time.perf_counter()
+    dev = torch.device(device)
+    contour = cached["contour"] if cached else engine.DeviceContour(mesh, dev)
+    if cached is None:
+        if mesh.n not in ws:
+            ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
+        z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
+        fac = engine.factor(z_tm._base, req.precision)          # batch [TE, TM]
+        if store is not None:
+            sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * mesh.n * mesh.n
+            store({"contour": contour, "factor": fac}, sysbytes)
+    else:
+        fac = cached["factor"]
+        # the cached tensors are read on this worker's stream too
+        cur = torch.cuda.current_stream()
+        for t in (contour.p0, contour.p1, contour.next, contour.prev, contour.fix_rows,
+                  fac.lu, fac.piv, fac.aux, fac.z):
+            t.record_stream(cur)
+    angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
+    bu, bdn = engine.loads(contour, k, angles)
+    rhs = torch.stack((-bdn, -bu))                             # [TE, TM]
+    x, inf = engine.solve(fac, rhs, gate=(req.precision == "double"))
+    rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
+    amp, sig, resid, info = {}, {}, {}, {}
+    pending = []
+    for i, pol in enumerate(("TE", "TM")):
+        if pol not in req.pols:
+            continue
+        a, s = engine.farfield(x[i], bu, bdn, k)
+        pending.append((pol, a, s, rel[i]))
+        info[pol] = {kk: v for kk, v in inf.items() if kk != "residual"}
+    torch.cuda.current_stream().synchronize()
+    for pol, a, s, r in pending:
+        amp[pol] = a.cpu().numpy()
+        sig[pol] = s.cpu().numpy()
+        resid[pol] = float(r.max())
+    finished = time.perf_counter()
+    arrival = req.arrival or started
+    return Result(req.rid, amp, sig, resid, cached is not None, finished - arrival, started, finished, info)
 
 
 def run_stream(server: Server, requests: List[Request], rate_hz: Optional[float] = None,
