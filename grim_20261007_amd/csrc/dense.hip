@@ -753,6 +753,7 @@ __global__ void __launch_bounds__(256) apply_list(int ncols, int pj, int pw, T* 
 }
 
 #define GRIM_LU_NB 64
+#define GRIM_SOLVE_NB 256
 
 __device__ __forceinline__ double lane_value(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -764,12 +765,12 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 }
 
 // X = T^{-1} B in place for a w x w triangle (w <= 64) and ncols columns:
-// LOWER: unit lower triangle; else upper with its diagonal.  Lane i of a wave
-// holds row i of CPW columns; step j broadcasts x_j from lane j with
-// v_readlane (wave-uniform, so it lands in SGPRs) and every lane below (above)
-// subtracts its T[i][j] x_j.  The triangle is staged transposed in LDS so lanes
-// read consecutive words, with the reciprocals of its diagonal.  256 threads =
-// 4 waves; batch in blockIdx.z.
+// LOWER: unit lower triangle; else upper with its diagonal, solved as the unit
+// upper triangle D^{-1} U after scaling B by D^{-1}.  Lane i of a wave holds
+// row i of CPW columns; step j broadcasts x_j from lane j with v_readlane
+// (wave-uniform, so it lands in SGPRs) and every lane subtracts T[i][j] x_j.
+// The strict triangle is staged transposed in LDS (zeros elsewhere, so the
+// update needs no lane mask).  256 threads = 4 waves; batch in blockIdx.z.
 // With `list` (a panel's interchange list; rows minus list_base are relative
 // to the top of B) each wave first applies the panel's row interchanges to its
 // columns of B: it reads every source row, writes the lower destinations and
@@ -782,24 +783,37 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
     __shared__ T dinv_r[GRIM_LU_NB], dinv_i[GRIM_LU_NB];
     tri += 2 * st * blockIdx.z;
     b += 2 * sb * blockIdx.z;
+    if (!LOWER) {
+        if (threadIdx.x < GRIM_LU_NB) {
+            const int j = threadIdx.x;
+            T ir = 0, ii = 0;
+            if (j < w) {
+                const T* p = tri + 2 * ((int64_t)j + (int64_t)j * ldt);
+                const T dr = p[0], di = p[1];
+                const T den = dr * dr + di * di;
+                ir = dr / den;
+                ii = -di / den;
+            }
+            dinv_r[j] = ir;
+            dinv_i[j] = ii;
+        }
+        __syncthreads();
+    }
     for (int idx = threadIdx.x; idx < GRIM_LU_NB * GRIM_LU_NB; idx += 256) {
         const int i = idx & 63, j = idx >> 6;
         T x = 0, y = 0;
-        if (i < w && j < w) {
+        if (i < w && j < w && (LOWER ? i > j : i < j)) {
             const T* p = tri + 2 * ((int64_t)i + (int64_t)j * ldt);
             x = p[0];
             y = p[1];
+            if (!LOWER) {   // row i of D^{-1} U
+                const T sr = x * dinv_r[i] - y * dinv_i[i], si = x * dinv_i[i] + y * dinv_r[i];
+                x = sr;
+                y = si;
+            }
         }
         tr[j][i] = x;
         ti[j][i] = y;
-    }
-    if (!LOWER && threadIdx.x < w) {
-        const int j = threadIdx.x;
-        const T* p = tri + 2 * ((int64_t)j + (int64_t)j * ldt);
-        const T dr = p[0], di = p[1];
-        const T den = dr * dr + di * di;
-        dinv_r[j] = dr / den;
-        dinv_i[j] = -di / den;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col0 = (blockIdx.x * 4 + wave) * CPW;
@@ -846,32 +860,28 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
     if (LOWER) {
         for (int j = 0; j < w - 1; ++j) {
             const T lr = tr[j][lane], li = ti[j][lane];
-            const bool below = lane > j;
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const T ur = lane_value(xr[c], j), ui = lane_value(xi[c], j);
-                if (below) {
-                    xr[c] -= lr * ur - li * ui;
-                    xi[c] -= lr * ui + li * ur;
-                }
+                xr[c] -= lr * ur - li * ui;
+                xi[c] -= lr * ui + li * ur;
             }
         }
     } else {
-        for (int j = w - 1; j >= 0; --j) {
-            const T ir = dinv_r[j], ii = dinv_i[j];
+        const T ir = dinv_r[lane], ii = dinv_i[lane];
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const T br = xr[c], bi = xi[c];
+            xr[c] = br * ir - bi * ii;
+            xi[c] = br * ii + bi * ir;
+        }
+        for (int j = w - 1; j > 0; --j) {
             const T lr = tr[j][lane], li = ti[j][lane];
-            const bool above = lane < j;
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
-                const T br = lane_value(xr[c], j), bi = lane_value(xi[c], j);
-                const T ur = br * ir - bi * ii, ui = br * ii + bi * ir;
-                if (lane == j) {
-                    xr[c] = ur;
-                    xi[c] = ui;
-                } else if (above) {
-                    xr[c] -= lr * ur - li * ui;
-                    xi[c] -= lr * ui + li * ur;
-                }
+                const T ur = lane_value(xr[c], j), ui = lane_value(xi[c], j);
+                xr[c] -= lr * ur - li * ui;
+                xi[c] -= lr * ui + li * ur;
             }
         }
     }
@@ -1118,25 +1128,45 @@ static hipError_t lu_solve(const Batch& bt, int n, int nrhs, const T* a, int64_t
     gather_rows<T><<<dim3((n + 255) / 256, nrhs, bt.count), 256, 0, bt.st>>>(n, (const int*)auxp, saux / 4, b, ldb,
                                                                               sb, w, ldw, sw);
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    // forward: w_j <- L_jj^{-1} w_j ; w_{j+} -= L_{j+, j} w_j
-    for (int j = 0; j < n; j += GRIM_LU_NB) {
-        const int jb = min(GRIM_LU_NB, n - j);
-        const T* djj = a + 2 * ((int64_t)j + (int64_t)j * lda);
-        if ((err = trsm<T, true>(bt, jb, nrhs, djj, lda, sa, w + 2 * (int64_t)j, ldw, sw)) != hipSuccess) return err;
-        if (j + jb < n)
-            if ((err = gemm<T>(bt, n - j - jb, nrhs, jb, djj + 2 * (int64_t)jb, lda, sa, w + 2 * (int64_t)j, ldw, sw,
-                               w + 2 * (int64_t)(j + jb), ldw, sw)) != hipSuccess)
+    // Blocks of GRIM_SOLVE_NB rows: inside a block, 64-row triangles (lanes) and
+    // rank-64 updates; between blocks, one rank-GRIM_SOLVE_NB update, so most
+    // of the solve's flops run as deep-k GEMMs.
+    // forward: w_J <- L_JJ^{-1} w_J ; w_{J+} -= L_{J+, J} w_J
+    for (int J = 0; J < n; J += GRIM_SOLVE_NB) {
+        const int jend = min(n, J + GRIM_SOLVE_NB);
+        for (int j = J; j < jend; j += GRIM_LU_NB) {
+            const int jb = min(GRIM_LU_NB, jend - j);
+            const T* djj = a + 2 * ((int64_t)j + (int64_t)j * lda);
+            if ((err = trsm<T, true>(bt, jb, nrhs, djj, lda, sa, w + 2 * (int64_t)j, ldw, sw)) != hipSuccess)
+                return err;
+            if (j + jb < jend)
+                if ((err = gemm<T>(bt, jend - j - jb, nrhs, jb, djj + 2 * (int64_t)jb, lda, sa, w + 2 * (int64_t)j,
+                                   ldw, sw, w + 2 * (int64_t)(j + jb), ldw, sw)) != hipSuccess)
+                    return err;
+        }
+        if (jend < n)
+            if ((err = gemm<T>(bt, n - jend, nrhs, jend - J, a + 2 * ((int64_t)jend + (int64_t)J * lda), lda, sa,
+                               w + 2 * (int64_t)J, ldw, sw, w + 2 * (int64_t)jend, ldw, sw)) != hipSuccess)
                 return err;
     }
-    // backward: w_j <- U_jj^{-1} w_j ; w_{<j} -= U_{<j, j} w_j
-    const int last = ((n - 1) / GRIM_LU_NB) * GRIM_LU_NB;
-    for (int j = last; j >= 0; j -= GRIM_LU_NB) {
-        const int jb = min(GRIM_LU_NB, n - j);
-        const T* djj = a + 2 * ((int64_t)j + (int64_t)j * lda);
-        if ((err = trsm<T, false>(bt, jb, nrhs, djj, lda, sa, w + 2 * (int64_t)j, ldw, sw)) != hipSuccess) return err;
-        if (j > 0)
-            if ((err = gemm<T>(bt, j, nrhs, jb, a + 2 * (int64_t)j * lda, lda, sa, w + 2 * (int64_t)j, ldw, sw, w,
-                               ldw, sw)) != hipSuccess)
+    // backward: w_J <- U_JJ^{-1} w_J ; w_{<J} -= U_{<J, J} w_J
+    const int lastJ = ((n - 1) / GRIM_SOLVE_NB) * GRIM_SOLVE_NB;
+    for (int J = lastJ; J >= 0; J -= GRIM_SOLVE_NB) {
+        const int jend = min(n, J + GRIM_SOLVE_NB);
+        const int lastj = J + ((jend - 1 - J) / GRIM_LU_NB) * GRIM_LU_NB;
+        for (int j = lastj; j >= J; j -= GRIM_LU_NB) {
+            const int jb = min(GRIM_LU_NB, jend - j);
+            const T* djj = a + 2 * ((int64_t)j + (int64_t)j * lda);
+            if ((err = trsm<T, false>(bt, jb, nrhs, djj, lda, sa, w + 2 * (int64_t)j, ldw, sw)) != hipSuccess)
+                return err;
+            if (j > J)
+                if ((err = gemm<T>(bt, j - J, nrhs, jb, a + 2 * ((int64_t)J + (int64_t)j * lda), lda, sa,
+                                   w + 2 * (int64_t)j, ldw, sw, w + 2 * (int64_t)J, ldw, sw)) != hipSuccess)
+                    return err;
+        }
+        if (J > 0)
+            if ((err = gemm<T>(bt, J, nrhs, jend - J, a + 2 * (int64_t)J * lda, lda, sa, w + 2 * (int64_t)J, ldw, sw,
+                               w, ldw, sw)) != hipSuccess)
                 return err;
     }
     copy_cast<T, T><<<dim3((n + 255) / 256, nrhs, bt.count), 256, 0, bt.st>>>(n, w, ldw, sw, b, ldb, sb);
