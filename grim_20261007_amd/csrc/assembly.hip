@@ -804,7 +804,10 @@ extern "C" hipError_t grim_upload_rules(const double* gl8t, const double* gl8w, 
 extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
                                       const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
                                       int64_t ldte, const int* fix_rows, int nfix, hipStream_t st) {
-    const int chunk = n <= 4096 ? 16 : 32;
+    // columns per wavefront: enough wavefronts (about 8-16 K) to fill 1024 SIMDs
+    // with their tails overlapping; each chunk re-evaluates one column pair.
+    int chunk = 4;
+    while (chunk < 32 && (double)n * n / (128.0 * chunk) > 16384.0) chunk *= 2;
     const int nc = (n + chunk - 1) / chunk, cpb = 64 / chunk, nb = (n + 63) / 64;
     long ntiles = 0;
     for (int I = 0; I < nb; ++I) ntiles += nc - I * cpb;
