@@ -56,7 +56,8 @@ def parse():
                          "sweep; stream: Poisson serving stream with cache reuse; large: one N=40000 request")
     ap.add_argument("--large-n", type=int, default=40000)
     ap.add_argument("--stream-count", type=int, default=64)
-    ap.add_argument("--stream-rate", type=float, default=25.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--stream-rate", type=float, default=200.0,
+                    help="Poisson arrival rate (requests/s); the default saturates the server")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -306,16 +307,21 @@ def stream_workload(args):
     ns = [m.n for m in meshes]
     _requests_plain(engine, meshes[:1], freqs[:1], angle_sets[:1], "double")
     t64, lat64 = _requests_plain(engine, meshes, freqs, angle_sets, "double")
-    srv = serving.Server(workers=args.workers)
-    try:
-        serving.run_stream(srv, [serving.Request(meshes[0], freqs[0], angle_sets[0], key="warm")])
-        reqs = [serving.Request(m, f, a, precision="mixed", key=it["key"])
-                for m, f, a, it in zip(meshes, freqs, angle_sets, stream)]
-        res, tm = serving.run_stream(srv, reqs, rate_hz=args.stream_rate, rng=np.random.default_rng(11))
-        hits = srv.cache.hits
-    finally:
-        srv.close()
+    forms = {}
+    for precision in ("mixed", "auto"):
+        srv = serving.Server(workers=args.workers)
+        try:
+            serving.run_stream(srv, [serving.Request(meshes[0], freqs[0], angle_sets[0], key="warm")])
+            reqs = [serving.Request(m, f, a, precision=precision, key=it["key"])
+                    for m, f, a, it in zip(meshes, freqs, angle_sets, stream)]
+            res, tm = serving.run_stream(srv, reqs, rate_hz=args.stream_rate, rng=np.random.default_rng(11))
+            forms[precision] = (res, tm, srv.cache.hits, sum(r.precision == "mixed" for r in reqs))
+        finally:
+            srv.close()
+    res, tm, hits, _ = forms["mixed"]
     lat = np.array([r.latency_s for r in res]) * 1e3
+    res_a, tm_a, hits_a, n_mixed = forms["auto"]
+    lat_a = np.array([r.latency_s for r in res_a]) * 1e3
     solves = 2 * len(meshes)
     print(json.dumps({
         "workload": "Poisson serving stream", "requests": len(meshes), "n_min": min(ns), "n_max": max(ns),
@@ -326,6 +332,11 @@ def stream_workload(args):
                           "p99_ms": round(float(np.percentile(lat, 99)), 3),
                           "note": "value includes waiting for Poisson arrivals; latency is arrival to result",
                           "max_residual": max(max(r.residual.values()) for r in res)},
+        "auto_serving": {"value": round(solves / tm_a, 3), "unit": "RCS solves/s", "workers": args.workers,
+                         "p50_ms": round(float(np.percentile(lat_a, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lat_a, 99)), 3), "requests_mixed": int(n_mixed),
+                         "cache_hits": hits_a, "max_residual": max(max(r.residual.values()) for r in res_a),
+                         "note": "per request the cheaper form by the flop model (mixed iff angles < N/18)"},
         "fp64_plain": {"value": round(solves / t64, 3), "unit": "RCS solves/s",
                        "p50_ms": round(float(np.percentile(lat64, 50)), 3),
                        "p99_ms": round(float(np.percentile(lat64, 99)), 3),
@@ -461,6 +472,17 @@ def main():
     q = mesh.far_order(k)
     flops = pairs * q * q * FLOPS_PER_EVAL
     achieved = flops / (far_ms * 1e-3) / 1e12
+    # the refinement residual b - Z x (one batched FP64 complex GEMM, N x A x N per system)
+    pair = torch.empty((2, mesh.n, mesh.n), dtype=torch.complex128, device=dev)
+    pair.normal_()
+    xs = torch.randn((2, len(angles), mesh.n), dtype=torch.complex128, device=dev)
+    bs = torch.randn((2, len(angles), mesh.n), dtype=torch.complex128, device=dev)
+    gt = Timer(torch)
+    for i in range(6):
+        (gt.run if i else (lambda name, fn: fn()))("residual_gemm", lambda: engine.residual(pair, xs, bs))
+    gemm_ms = gt.mean_ms("residual_gemm")
+    gemm_tf = 2 * 8.0 * mesh.n * mesh.n * len(angles) / (gemm_ms * 1e-3) / 1e12
+    del pair, xs, bs
     stage_ms = {name: round(tm.mean_ms(name), 3) for name in tm.acc}
     stage64 = {name: round(timers["double"].mean_ms(name), 3) for name in timers["double"].acc}
     cpu = None
@@ -495,6 +517,10 @@ def main():
                      "frac": round(achieved / PEAK_FP64, 4), "traffic": far_traffic(),
                      "model": f"{pairs} unordered far pairs x {q}x{q} points x {FLOPS_PER_EVAL} FP64 flops "
                               f"per launch; FP64 VALU-bound (MI355X FP64 vector peak = FP64 matrix peak)"},
+        "roofline_fp64_gemm": {"kernel": "cgemm<double> (refinement residual b - Z x, 2 x N x A x N)",
+                               "bound": "mfma", "achieved": round(gemm_tf, 2), "peak": PEAK_FP64,
+                               "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4),
+                               "ms": round(gemm_ms, 3)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -39,7 +39,7 @@ class Request:
     frequency_ghz: float
     angles_deg: np.ndarray
     pols: Tuple[str, ...] = ("TE", "TM")
-    precision: str = "mixed"
+    precision: str = "mixed"          # "mixed", "double" or "auto" (choose_precision)
     arrival: float = 0.0
     rid: int = 0
     key: Optional[str] = None          # cache key; None -> geometry digest
@@ -63,6 +63,21 @@ def geometry_digest(mesh: ContourMesh) -> str:
     for a in (mesh.p0, mesh.p1, mesh.next):
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()[:24]
+
+
+def choose_precision(n: int, nangles: int, corrections: Optional[int] = None) -> str:
+    """Cheaper form for one request by the flop model (FP32 MFMA at twice the FP64 rate):
+    plain  = (8/3) N^3 + 2 * 8 N^2 A                       (LU, solve, backward-error gate)
+    mixed  = (8/3) N^3 / 2 + (s + 1) 8 N^2 A / 2 + s 8 N^2 A   (s corrections, FP64 residuals)
+    so the mixed form pays only when A < N / 18 (s = 3).  s grows with N (the complex64
+    factor's error does): 3 at N <= 2000 (measured on the NACA 0012 systems), one more per
+    doubling (7 measured at N = 40 000)."""
+    if corrections is None:
+        corrections = 3 + max(0, int(np.ceil(np.log2(max(n, 1) / 2000.0))))
+    n3, n2a = n ** 3, n * n * nangles
+    plain = 8.0 / 3.0 * n3 + 16.0 * n2a
+    mixed = 4.0 / 3.0 * n3 + 4.0 * (corrections + 1) * n2a + 8.0 * corrections * n2a
+    return "mixed" if mixed < plain else "double"
 
 
 def request_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool) -> int:
@@ -173,6 +188,8 @@ class Server:
 
     def _run(self, req: Request, ws: dict) -> Result:
         mesh = req.mesh
+        if req.precision == "auto":
+            req.precision = choose_precision(mesh.n, len(req.angles_deg))
         k = wavenumber(req.frequency_ghz)
         key = (req.key or geometry_digest(mesh), float(k), req.precision)
         cached = self.cache.get(key)
@@ -188,6 +205,8 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     """Run one request on the current stream: assemble and factor (or reuse `cached`), then loads,
     solve and far field.  `store(item, nbytes)` receives the assembled + factored systems."""
     mesh = req.mesh
+    if req.precision == "auto":
+        req.precision = choose_precision(mesh.n, len(req.angles_deg))
     k = wavenumber(req.frequency_ghz)
     started = time.perf_counter()
     dev = torch.device(device)
