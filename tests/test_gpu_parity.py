@@ -227,6 +227,59 @@ def test_assembly_matches_oracle_large_kr():
         assert err < 1e-12
 
 
+def _two_bodies():
+    from grim_20261007_amd import workloads
+    a = workloads.circle_points(0.5, 120) + np.array([-1.5, 0.0])
+    b = workloads.star_points(150, 0.6, [0.05, 0.02, 0, 0, 0, 0, 0, 0], [0.3, 1.0, 0, 0, 0, 0, 0, 0]) + \
+        np.array([1.5, 0.2])
+    segs = workloads._snapshot(a, "left")["segments"] + workloads._snapshot(b, "right")["segments"]
+    return {"segments": segs, "ibcs": [], "dielectrics": []}, workloads.frequency_for_ka(8.0, 0.5)
+
+
+def test_two_contours_match_oracle():
+    """Two closed bodies (a circle and a star, 270 nodes): contour starts inside a row block, far
+    and near pairs between the bodies, and the full solve, against the oracle."""
+    from grim_20261007_amd import engine
+    from grim_20261007_amd.mesh import build_mesh
+    from grim_20261007_amd.solver import solve_frequency, wavenumber
+    from oracle import bem2d
+    dev = _dev()
+    _lib()
+    snap, f = _two_bodies()
+    mesh = build_mesh(snap, "meters", f)
+    om = bem2d.build_mesh(snap, "meters", f)
+    assert np.array_equal(mesh.elements, om.elements)
+    k = wavenumber(f)
+    z_tm, z_te = engine.assemble(engine.DeviceContour(mesh, dev), k)
+    ref_tm, ref_te = bem2d.assemble(om, bem2d.frequency_k(f))
+    for pol, z, ref in (("TM", _mat(z_tm), ref_tm), ("TE", _mat(z_te), ref_te)):
+        err = np.max(np.abs(z - ref)) / np.max(np.abs(ref))
+        print(f"two bodies {pol} assembly vs oracle {err:.2e}", flush=True)
+        assert err < 1e-12
+    angles = np.arange(0.0, 360.0, 5.0)
+    res = solve_frequency(mesh, f, list(angles), ("TE", "TM"), "double", dev)
+    want = bem2d.solve_monostatic(om, f, angles)
+    for pol in ("TE", "TM"):
+        amp_ref = want[pol]["amplitude"]
+        err = np.max(np.abs(res[pol]["amplitude"] - amp_ref)) / np.max(np.abs(amp_ref))
+        print(f"two bodies {pol} amplitude vs oracle {err:.2e}", flush=True)
+        assert err < 1e-10
+
+
+def test_singular_system_is_reported():
+    """A zero column gives a zero pivot: the status word turns it into LinAlgError."""
+    from grim_20261007_amd import engine
+    dev = _dev()
+    _lib()
+    rng = np.random.default_rng(3)
+    n = 200
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a[:, 77] = 0.0
+    for precision in ("double", "mixed"):
+        with pytest.raises(np.linalg.LinAlgError):
+            engine.factor(torch.as_tensor(np.ascontiguousarray(a.T), device=dev), precision)
+
+
 @pytest.mark.parametrize("name", golden_cases.ALL)
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_solve_matches_reference(name, precision):
