@@ -22,7 +22,9 @@ EPS = 1e-12
 NODE_SNAP = 1.0e-9
 DEFAULT_PANELS_PER_WAVELENGTH = 20
 MIN_EXPLICIT_PANELS_PER_WAVELENGTH = 4
-PANEL_COUNT_CEIL_TOLERANCE = 1e-9
+# A wavelength-controlled primitive gets another panel only once it exceeds the
+# target panel length by this fraction of a panel (twod/geometry.py:894-903).
+PANEL_COUNT_CEIL_TOLERANCE = 0.05
 
 
 @dataclass
@@ -76,14 +78,17 @@ def unit_scale(units: str) -> float:
     raise ValueError(f"Unsupported geometry units '{units}'. Use inches or meters.")
 
 
+def _parse_int(token, default: int = 0) -> int:
+    """Nearest integer of a numeric token (twod/geometry.py:447-451)."""
+    try:
+        return int(round(float(token)))
+    except (TypeError, ValueError):
+        return default
+
+
 def _parse_props(seg: dict):
     props = list(seg.get("properties", []) or [])
-    vals = []
-    for i in range(5):
-        try:
-            vals.append(int(float(props[i])) if i < len(props) else 0)
-        except (TypeError, ValueError):
-            vals.append(0)
+    vals = [_parse_int(props[i]) if i < len(props) else 0 for i in range(5)]
     seg_type = vals[0] if props else int(seg.get("seg_type", 0))
     return seg_type, vals[1], vals[2]
 
@@ -100,15 +105,97 @@ def _panel_count(n_prop: int, length: float, wavelength: float) -> int:
     return max(1, int(math.ceil(length / target - PANEL_COUNT_CEIL_TOLERANCE)))
 
 
+def _bit_reversed_fraction(index: int) -> float:
+    """Van der Corput (base 2) value of ``index``: 0, 1/2, 1/4, 3/4, ... ; ordering a run of
+    equal primitives by it picks them spread along the chain (twod/geometry.py:1569-1578)."""
+    value, step = 0.0, 0.5
+    while index:
+        value += step * (index & 1)
+        index >>= 1
+        step *= 0.5
+    return value
+
+
+def refined_chain_counts(base_counts, lengths, factor: float) -> list:
+    """Panel counts of one segment chain of the certification fine mesh
+    (twod/geometry.py:1581-1640).
+
+    The chain gets T = max(B + 1, ceil(factor B)) panels for its B base panels.  Every
+    primitive keeps at least its base count; the extra panels go where panels are longest:
+    a bisection finds the smallest panel-length threshold h whose counts
+    max(base, ceil(length / h)) stay within T, and what is left of T is handed out one panel
+    per primitive from the longest remaining panels down.  Primitives whose panels are equally
+    long (the mirror halves of a symmetric body) are refined as one group, also when that
+    exceeds T by up to max(2, ceil(T / 100)) panels; a larger group takes the remaining panels
+    in bit-reversed index order.
+    """
+    base = [max(1, int(c)) for c in base_counts]
+    ls = [float(v) for v in lengths]
+    b_total = sum(base)
+    target = max(b_total + 1, int(math.ceil(b_total * float(factor))))
+    if len(base) == 1:
+        return [target]
+
+    def counts_at(h: float) -> list:
+        return [max(c, int(math.ceil(v / h))) for c, v in zip(base, ls)]
+
+    hi = max(v / c for c, v in zip(base, ls))
+    if not hi > 0.0:
+        out = list(base)
+        out[0] += target - b_total
+        return out
+    lo = hi / max(2.0, float(factor)) / 2.0
+    while sum(counts_at(lo)) <= target and lo > EPS:
+        lo *= 0.5
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if sum(counts_at(mid)) <= target:
+            hi = mid
+        else:
+            lo = mid
+        if hi - lo <= 1.0e-12 * hi:
+            break
+    out = counts_at(hi)
+    left = target - sum(out)
+    if left <= 0:
+        return out
+    longest = max(v / c for c, v in zip(out, ls))
+    # panel length relative to the longest, quantised so mirror images compare equal
+    level = [round(ls[i] / out[i] / longest * 1.0e9) for i in range(len(out))]
+    slack = max(2, int(math.ceil(0.01 * target)))
+    for value in sorted(set(level), reverse=True):
+        if left <= 0:
+            break
+        members = [i for i in range(len(out)) if level[i] == value]
+        if len(members) > left + slack:
+            members = sorted(members, key=_bit_reversed_fraction)[:left]
+        for i in members:
+            out[i] += 1
+        left -= len(members)
+    return out
+
+
 def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int = 20000) -> ContourMesh:
-    """Panels and welded nodes of the PEC contours of ``snapshot`` at ``frequency_ghz``."""
+    """Panels and welded nodes of the PEC contours of ``snapshot`` at ``frequency_ghz``.
+
+    A snapshot that carries ``_2d_certification_refinement_factor`` (> 1) and
+    ``_2d_certification_base_segment_n`` (each segment's base N) is the fine mesh of a
+    certification pair: its chains get ``refined_chain_counts`` of the base counts
+    (twod/geometry.py:1788-1850)."""
     scale = unit_scale(units)
     wavelength = C0 / (float(frequency_ghz) * 1e9)
     segments = list(snapshot.get("segments", []) or [])
     if not segments:
         raise ValueError("Geometry does not contain any segments.")
+    refine = float(snapshot.get("_2d_certification_refinement_factor", 1.0) or 1.0)
+    base_n = list(snapshot.get("_2d_certification_base_segment_n", []) or [])
+    if not math.isfinite(refine) or refine < 1.0:
+        raise ValueError("Internal 2-D certification refinement factor must be finite and >= 1.")
+    if refine > 1.0 and len(base_n) != len(segments):
+        raise ValueError("Internal 2-D certification refinement metadata does not match the geometry "
+                         "segment count.")
     panels0, panels1 = [], []
-    for seg in segments:
+    for s, seg in enumerate(segments):
         seg_type, n_prop, ibc = _parse_props(seg)
         name = str(seg.get("name", "segment"))
         if seg_type != 2 or ibc != 0:
@@ -118,6 +205,7 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
         # TYPE 2: the user draws the normal into air; reverse the chain.
         pairs = [{"x1": p.get("x2", 0.0), "y1": p.get("y2", 0.0), "x2": p.get("x1", 0.0), "y2": p.get("y1", 0.0)}
                  for p in reversed(pairs)]
+        ends = []
         for pair in pairs:
             a = np.array([float(pair["x1"]) * scale, float(pair["y1"]) * scale])
             b = np.array([float(pair["x2"]) * scale, float(pair["y2"]) * scale])
@@ -125,7 +213,17 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
             if EPS < length <= NODE_SNAP:
                 raise ValueError(f"Segment '{name}' has a primitive {length:.3g} m long, not longer than the "
                                  "mesh node tolerance.")
-            count = _panel_count(n_prop, length, wavelength)
+            ends.append((a, b, length))
+        if refine > 1.0:
+            seg_base_n = _parse_int(base_n[s])
+            base_counts = [_panel_count(seg_base_n, length, wavelength) for _, _, length in ends]
+            fine_total = max(sum(base_counts) + 1, int(math.ceil(sum(base_counts) * refine)))
+            if len(panels0) + fine_total > max_panels:
+                raise ValueError("Discretization exceeds the configured panel limit.")
+            counts = refined_chain_counts(base_counts, [e[2] for e in ends], refine) if ends else []
+        else:
+            counts = [_panel_count(n_prop, length, wavelength) for _, _, length in ends]
+        for (a, b, length), count in zip(ends, counts):
             if n_prop > 0:
                 minimum = max(1, int(math.ceil(length * MIN_EXPLICIT_PANELS_PER_WAVELENGTH / wavelength)))
                 if count < minimum:
@@ -134,6 +232,9 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
                                      "required.")
             if len(panels0) + count > max_panels:
                 raise ValueError("Discretization exceeds the configured panel limit.")
+            if count > 1 and length / count <= NODE_SNAP:
+                raise ValueError(f"Segment '{name}' N={n_prop} would split a {length:.3g} m primitive into "
+                                 "panels no longer than the mesh node tolerance. Reduce N.")
             pts = [a + (b - a) * (i / count) for i in range(count + 1)]
             pts[0], pts[-1] = a.copy(), b.copy()
             for i in range(count):

@@ -15,7 +15,7 @@ reference factors it.  Outputs are small ``.npz`` fixtures next to this file:
 inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [--certified | --meshes]
 """
 from __future__ import annotations
 
@@ -115,6 +115,55 @@ def run_case(name, points, frequency_ghz, angles, keep_matrix=True, matrix_rows=
           f"reference {seconds:.2f} s", flush=True)
 
 
+def run_certified_case(name, points, frequency_ghz, angles, policy=None):
+    """The reference's certified co-polarized entry (twod/solver.py:3979-4005) on one snapshot:
+    records, per channel, the base and fine sample amplitudes and dB that its mesh gate
+    compared (runs/quality.py:265) and the gate's metrics."""
+    import ghost_backend.runs.quality as quality
+    import ghost_backend.twod.solver as rcs
+
+    captured = []
+    original = quality.evaluate_mesh_convergence
+
+    def recording(base_result, fine_result, **kwargs):
+        gate = original(base_result=base_result, fine_result=fine_result, **kwargs)
+        captured.append((base_result["samples"], fine_result["samples"], kwargs, gate))
+        return gate
+
+    quality.evaluate_mesh_convergence = recording
+    snapshot = workloads._snapshot(points, name)
+    error = ""
+    try:
+        try:
+            rcs.solve_monostatic_rcs_2d_certified(
+                snapshot, frequencies_ghz=[frequency_ghz], elevations_deg=list(map(float, angles)),
+                geometry_units="meters", mesh_convergence_policy=policy, max_panels=100000,
+                solver_method="direct")
+        except ValueError as exc:
+            error = str(exc)
+    finally:
+        quality.evaluate_mesh_convergence = original
+    out = {"points": np.asarray(points, float), "frequency_ghz": float(frequency_ghz),
+           "angles_deg": np.asarray(angles, float), "error": np.array(error),
+           "policy_json": np.array(json.dumps(policy or {}))}
+    keys = ("passed", "sample_count", "rms_db", "max_abs_db", "mean_db", "median_abs_db",
+            "db_sample_count_above_floor", "rms_db_unfloored", "max_abs_db_unfloored",
+            "complex_rms_normalized", "complex_max_normalized", "complex_reference_peak_amplitude",
+            "phase_sample_count", "phase_rms_deg", "phase_max_deg")
+    for i, (base, fine, kwargs, gate) in enumerate(captured):
+        for tag, rows in (("base", base), ("fine", fine)):
+            out[f"c{i}_{tag}_amp"] = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+            out[f"c{i}_{tag}_db"] = np.array([r["rcs_db"] for r in rows])
+            out[f"c{i}_{tag}_theta"] = np.array([r["theta_inc_deg"] for r in rows])
+        out[f"c{i}_kwargs_json"] = np.array(json.dumps(kwargs))
+        out[f"c{i}_gate"] = np.array([float(gate[k]) for k in keys])
+        out[f"c{i}_violations_json"] = np.array(json.dumps(gate["violations"]))
+    out["gate_keys"] = np.array(keys)
+    out["channels"] = np.array(len(captured))
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print(f"{name}: certified, {len(captured)} channel gates, error={error[:60]!r}", flush=True)
+
+
 def main():
     # PEC circle, R = 0.1 m, 64 chords, ka = 1 (the reference's physics fixture).
     r = 0.1
@@ -141,5 +190,94 @@ def main():
              keep_rhs=False)
 
 
+def main_certified():
+    # converged pair: 64-chord circle at ka = 1, explicit 1 panel per chord (fine: 96 panels)
+    r = 0.1
+    run_certified_case("cert_circle64_ka1", workloads.circle_points(r, 64), workloads.frequency_for_ka(1.0, r),
+                       np.arange(0.0, 360.0, 15.0))
+    # under-resolved pair: a 4-lambda star meshed at 6 panels per wavelength fails the gate
+    rng = np.random.default_rng(20261203)
+    a, phi = workloads.star_shape(rng)
+    pts = workloads.star_points(60, 0.5, a, phi)
+    run_certified_case("cert_star60_coarse", pts, workloads.frequency_for_segments_per_wavelength(pts, 6.0),
+                       np.arange(0.0, 360.0, 10.0))
+    # the same pair under a tight policy: the VV gate fails and the call raises
+    run_certified_case("cert_star60_tight", pts, workloads.frequency_for_segments_per_wavelength(pts, 6.0),
+                       np.arange(0.0, 360.0, 10.0),
+                       policy={"complex_max_limit": 0.001, "phase_max_limit_deg": 0.5})
+
+
+def _with_n(snapshot, n_prop):
+    snap = json.loads(json.dumps(snapshot))
+    for seg in snap["segments"]:
+        seg["properties"] = ["2", str(n_prop), "0", "0", "0"]
+    return snap
+
+
+def main_meshes():
+    """Panel endpoints of the reference's panel builder (twod/geometry.py:1753-1960) for base
+    and certification-fine snapshots, and its fine-count rule (twod/geometry.py:1581-1640) on
+    synthetic chains: meshes.npz."""
+    import ghost_backend.runs.quality as quality
+    import ghost_backend.twod.geometry as geo
+
+    c0 = 299_792_458.0
+    out = {}
+    rng = np.random.default_rng(20261203)
+    a, phi = workloads.star_shape(rng)
+    star60 = workloads.star_points(60, 0.5, a, phi)
+    two = workloads._snapshot(workloads.circle_points(0.1, 24), "left")
+    right = workloads._snapshot(workloads.circle_points(0.1, 40) + np.array([0.5, 0.0]), "right")
+    two["segments"] += right["segments"]
+    cases = [
+        ("circle64_n1", workloads._snapshot(workloads.circle_points(0.1, 64), "c"), 0.477, 1, 1.5),
+        ("star60_n1", workloads._snapshot(star60, "s"),
+         workloads.frequency_for_segments_per_wavelength(star60, 6.0), 1, 1.5),
+        ("naca80_auto", workloads._snapshot(workloads.naca0012_points(40), "n"), 3.0, 0, 1.5),
+        ("star96_ppw10", workloads._snapshot(workloads.star_points(96, 0.5, a, phi), "s"), 2.0, -10, 2.0),
+        ("circle400_auto", workloads._snapshot(workloads.circle_points(1.0, 400), "c"), 4.77, 0, 1.5),
+        ("two_circles_n2", two, 1.3, 2, 1.5),
+    ]
+    names = []
+    for name, snap, freq, n_prop, factor in cases:
+        snap = _with_n(snap, n_prop)
+        wavelength = c0 / (freq * 1e9)
+        fine = quality.scale_snapshot_panel_density(snap, factor)
+        fine["_2d_certification_refinement_factor"] = float(factor)
+        fine["_2d_certification_base_segment_n"] = [str(n_prop)] * len(snap["segments"])
+        for tag, s in (("base", snap), ("fine", fine)):
+            panels = geo._build_panels(s, 1.0, wavelength, max_panels=100000)
+            out[f"{name}_{tag}_p0"] = np.array([p.p0 for p in panels], float)
+            out[f"{name}_{tag}_p1"] = np.array([p.p1 for p in panels], float)
+        out[f"{name}_snapshot_json"] = np.array(json.dumps(snap))
+        out[f"{name}_freq_factor"] = np.array([freq, factor])
+        names.append(name)
+        print(f"{name}: base {len(out[f'{name}_base_p0'])} fine {len(out[f'{name}_fine_p0'])} panels", flush=True)
+    out["mesh_cases"] = np.array(names)
+    chains = []
+    for trial in range(40):
+        m = int(rng.integers(2, 40))
+        if trial % 4 == 0:      # mirror-symmetric chain: equal panel lengths in pairs
+            half = rng.uniform(0.1, 1.0, (m + 1) // 2)
+            lengths = np.concatenate([half, half[::-1]])[:m]
+        elif trial % 4 == 1:    # a densely drawn polyline: all primitives equal
+            lengths = np.full(m, 0.01)
+        else:
+            lengths = rng.uniform(0.01, 1.0, m)
+        counts = rng.integers(1, 6, m) if trial % 3 else np.ones(m, np.int64)
+        factor = float(rng.choice([1.25, 1.5, 2.0, 3.0]))
+        fine_counts = geo._certification_fine_counts(counts.tolist(), lengths.tolist(), factor)
+        chains.append({"base": counts.tolist(), "lengths": lengths.tolist(), "factor": factor,
+                       "fine": [int(c) for c in fine_counts]})
+    out["chains_json"] = np.array(json.dumps(chains))
+    np.savez_compressed(HERE / "meshes.npz", **out)
+    print(f"meshes: {len(names)} meshes, {len(chains)} chains", flush=True)
+
+
 if __name__ == "__main__":
-    main()
+    if "--certified" in sys.argv:
+        main_certified()
+    elif "--meshes" in sys.argv:
+        main_meshes()
+    else:
+        main()
