@@ -95,11 +95,17 @@ __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k,
             const double tj = Rule<Q>::t(qj), wj = Rule<Q>::w(qj);
             const double dx = xo - fma(tj, s.sx, s.x0);
             const double dy = yo - fma(tj, s.sy, s.y0);
-            const double r = sqrt(dx * dx + dy * dy);
+            // r and 1/r from the hardware 1/sqrt estimate: two Newton steps for 1/r, one
+            // residual correction for r (r is the Hankel phase argument, so it is kept
+            // to the rounding of a correctly rounded sqrt)
+            const double r2 = fmax(fma(dx, dx, dy * dy), GRIM_EPS * GRIM_EPS);
+            double inv = __builtin_amdgcn_rsq(r2);
+            inv = inv * fma(-0.5 * r2, inv * inv, 1.5);
+            inv = inv * fma(-0.5 * r2, inv * inv, 1.5);
+            double r = r2 * inv;
+            r = fma(0.5 * inv, fma(-r, r, r2), r);
             cplx g, h;
             green_kernels(k, r, g, h);
-            const double rr = r > GRIM_EPS ? r : GRIM_EPS;
-            const double inv = 1.0 / rr;
             const double po = (dx * o.nx + dy * o.ny) * inv;
             const double pso = (dx * s.nx + dy * s.ny) * inv;
             const double w0 = wj * (1.0 - tj), w1 = wj * tj;
@@ -241,27 +247,44 @@ __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict
 //   column c (entries written by a mirror tile,  block(q) >  block(c)):
 //       Z[q, c] += blk(e', prev q).tmt[1][1] + blk(e', q).tmt[0][1]
 template <int Q>
-__global__ void __launch_bounds__(256) far_fixup(int n, const double* __restrict__ p0, const double* __restrict__ p1,
-                                                 const int* __restrict__ next, const int* __restrict__ prev,
-                                                 double k, const int* __restrict__ rows, double* __restrict__ ztm,
-                                                 int64_t ldtm, double* __restrict__ zte, int64_t ldte) {
+__global__ void __launch_bounds__(64) far_fixup(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                const int* __restrict__ next, const int* __restrict__ prev,
+                                                double k, const int* __restrict__ rows, double* __restrict__ ztm,
+                                                int64_t ldtm, double* __restrict__ zte, int64_t ldte) {
+    // one wavefront per 63 consecutive q: lane l >= 1 owns q = qb + l - 1 and evaluates
+    // blk(e', q); lane 0 evaluates blk(e', prev qb), the neighbour of lane 1.  Each lane
+    // takes blk(e', prev q) from lane - 1 (prev q == q - 1) or, at a contour start inside
+    // the wavefront, evaluates it itself.
     const int c = rows[blockIdx.y];
-    const int q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= n) return;
+    const int lane = threadIdx.x;
+    const int qb = blockIdx.x * 63;
+    const int q = qb + lane - 1;
     const int e = prev[c];
     const Elem o = load_elem(p0, p1, e);
     const int ne = next[e];
-    const int pq = prev[q];
-    Blk2 b1, b2;
-    far_or_zero<Q>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
-    far_or_zero<Q>(o, e, ne, load_elem(p0, p1, q), q, next[q], k, true, b2);
+    const int qc = q < n ? q : n - 1;
+    const int col = lane == 0 ? prev[qb] : qc;
+    Blk2 b;
+    far_or_zero<Q>(o, e, ne, load_elem(p0, p1, col), col, next[col], k, lane == 0 || q < n, b);
+    cplx ltm, lte, lmt, lme;
+    ltm.re = __shfl_up(b.tm[1][1].re, 1, 64); ltm.im = __shfl_up(b.tm[1][1].im, 1, 64);
+    lte.re = __shfl_up(b.te[1][1].re, 1, 64); lte.im = __shfl_up(b.te[1][1].im, 1, 64);
+    lmt.re = __shfl_up(b.tmt[1][1].re, 1, 64); lmt.im = __shfl_up(b.tmt[1][1].im, 1, 64);
+    lme.re = __shfl_up(b.tet[1][1].re, 1, 64); lme.im = __shfl_up(b.tet[1][1].im, 1, 64);
+    const int pq = prev[qc];
+    if (lane >= 2 && q < n && pq != q - 1) {
+        Blk2 b1;
+        far_or_zero<Q>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
+        ltm = b1.tm[1][1]; lte = b1.te[1][1]; lmt = b1.tmt[1][1]; lme = b1.tet[1][1];
+    }
+    if (lane == 0 || q >= n) return;
     if (q / 64 >= c / 64) {
-        store_add(ztm, ldtm, c, q, cadd(b1.tm[1][1], b2.tm[1][0]), true);
-        store_add(zte, ldte, c, q, cadd(b1.te[1][1], b2.te[1][0]), true);
+        store_add(ztm, ldtm, c, q, cadd(ltm, b.tm[1][0]), true);
+        store_add(zte, ldte, c, q, cadd(lte, b.te[1][0]), true);
     }
     if (q / 64 > c / 64) {
-        store_add(ztm, ldtm, q, c, cadd(b1.tmt[1][1], b2.tmt[0][1]), true);
-        store_add(zte, ldte, q, c, cadd(b1.tet[1][1], b2.tet[0][1]), true);
+        store_add(ztm, ldtm, q, c, cadd(lmt, b.tmt[0][1]), true);
+        store_add(zte, ldte, q, c, cadd(lme, b.tet[0][1]), true);
     }
 }
 
@@ -815,9 +838,9 @@ extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const doub
     else far_kernel<16><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || nfix == 0) return err;
-    dim3 g2((n + 255) / 256, nfix);
-    if (q == 8) far_fixup<8><<<g2, 256, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
-    else far_fixup<16><<<g2, 256, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    dim3 g2((n + 62) / 63, nfix);
+    if (q == 8) far_fixup<8><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    else far_fixup<16><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
     return hipGetLastError();
 }
 

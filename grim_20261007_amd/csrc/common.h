@@ -52,9 +52,11 @@ __device__ __forceinline__ double horner(const double (&c)[N], double t) {
 
 // (j/4) H0^(2)(x) and (j/4) H1^(2)(x) for real x > 0 (x = k r, floored at 1e-12).
 // Returns g = (Y0/4, J0/4) and h1 = (Y1/4, J1/4); the caller multiplies h1 by k.
-// Small x: Taylor series of the entire parts (DLMF 10.8.1); large x: modulus /
-// phase form with fitted P_nu, Q_nu (DLMF 10.17.3).  Coefficients generated by
-// gen_hankel_coeffs.py (max relative error 5e-15 for x up to 1e3).
+// Small x: Taylor series of the entire parts (DLMF 10.8.1); x >= 6: modulus /
+// phase form with fitted P_nu, Q_nu (DLMF 10.17.3), 11 terms on [6, 12) and 8
+// terms from 12 on, the common case of far pairs.  Coefficients generated by
+// gen_hankel_coeffs.py (max relative error 3e-15 for x up to 50; beyond, the
+// eps * x conditioning of the phase dominates).
 struct HK { double gr, gi, hr, hi; };
 __device__ __noinline__ HK hankel_quarter_v(double x) {
     cplx g, h1;
@@ -68,19 +70,69 @@ __device__ __noinline__ HK hankel_quarter_v(double x) {
         g = {0.25 * y0v, 0.25 * j0v};
         h1 = {0.25 * y1v, 0.25 * j1v};
     } else {
-        const double inv = 1.0 / x;
-        const double r = GRIM_HANKEL_XS * inv;
-        const double w = r * r;
-        const double amp = sqrt(2.0 / GRIM_PI * inv) * GRIM_SQRT_HALF;
-        const double p0 = horner(GRIM_HK_P0, w), q0 = r * horner(GRIM_HK_Q0, w);
-        const double p1 = horner(GRIM_HK_P1, w), q1 = r * horner(GRIM_HK_Q1, w);
-        double s, c;
-        sincos(x, &s, &c);
+        // 1/sqrt(x) by the hardware estimate and two Newton steps; 1/x = its square
+        double rs = __builtin_amdgcn_rsq(x);
+        rs = rs * fma(-0.5 * x, rs * rs, 1.5);
+        rs = rs * fma(-0.5 * x, rs * rs, 1.5);
+        const double inv = rs * rs;
+        const double amp = (0.7978845608028654 * GRIM_SQRT_HALF) * rs;
+        double p0, q0, p1, q1;
+        if (x < GRIM_HANKEL_XM) {
+            const double r = GRIM_HANKEL_XS * inv;
+            const double u = (r * r) * (8.0 / 3.0) - 5.0 / 3.0;
+            p0 = horner(GRIM_HK_P0M, u);
+            q0 = r * horner(GRIM_HK_Q0M, u);
+            p1 = horner(GRIM_HK_P1M, u);
+            q1 = r * horner(GRIM_HK_Q1M, u);
+        } else {
+            const double r = GRIM_HANKEL_XM * inv;
+            const double w = r * r;
+            p0 = horner(GRIM_HK_P0, w);
+            q0 = r * horner(GRIM_HK_Q0, w);
+            p1 = horner(GRIM_HK_P1, w);
+            q1 = r * horner(GRIM_HK_Q1, w);
+        }
+        // cos x + sin x and cos x - sin x: x = n pi/2 + t, |t| <= pi/4 (two-part
+        // Cody-Waite reduction with FMA, exact to an ulp of t for n < 2^30), then
+        // Taylor polynomials of sin t and cos t (remainders below 7e-17)
+        double cps, cms;
+        if (x < 1.0e9) {
+            const double n = rint(x * 0.6366197723675814);
+            double t = fma(-n, 1.5707963267948966, x);
+            t = fma(-n, 6.123233995736766e-17, t);
+            const double t2 = t * t;
+            double ps = -7.647163731819816e-13;
+            ps = fma(ps, t2, 1.6059043836821613e-10);
+            ps = fma(ps, t2, -2.505210838544172e-08);
+            ps = fma(ps, t2, 2.7557319223985893e-06);
+            ps = fma(ps, t2, -1.984126984126984e-04);
+            ps = fma(ps, t2, 8.333333333333333e-03);
+            ps = fma(ps, t2, -1.6666666666666666e-01);
+            const double sn = fma(ps * t2, t, t);
+            double pc = 4.779477332387385e-14;
+            pc = fma(pc, t2, -1.1470745597729725e-11);
+            pc = fma(pc, t2, 2.08767569878681e-09);
+            pc = fma(pc, t2, -2.755731922398589e-07);
+            pc = fma(pc, t2, 2.48015873015873e-05);
+            pc = fma(pc, t2, -1.388888888888889e-03);
+            pc = fma(pc, t2, 4.1666666666666664e-02);
+            const double cs = fma(fma(pc, t2, -0.5), t2, 1.0);
+            const int q = (int)n & 3;
+            const double a = cs + sn, b = cs - sn;
+            cps = (q & 1) ? b : a;
+            cms = (q & 1) ? -a : b;
+            if (q & 2) { cps = -cps; cms = -cms; }
+        } else {
+            double s0, c0;
+            sincos(x, &s0, &c0);
+            cps = c0 + s0;
+            cms = c0 - s0;
+        }
         // H0 = amp (P0 - j Q0) (u0 + j v0), u0 = c + s, v0 = c - s
-        const double u0 = c + s, v0 = c - s;
+        const double u0 = cps, v0 = cms;
         const double h0re = amp * (p0 * u0 + q0 * v0), h0im = amp * (p0 * v0 - q0 * u0);
         // H1 = amp (P1 - j Q1) (u1 + j v1), u1 = s - c, v1 = c + s
-        const double u1 = s - c, v1 = c + s;
+        const double u1 = -cms, v1 = cps;
         const double h1re = amp * (p1 * u1 + q1 * v1), h1im = amp * (p1 * v1 - q1 * u1);
         g = {-0.25 * h0im, 0.25 * h0re};
         h1 = {-0.25 * h1im, 0.25 * h1re};

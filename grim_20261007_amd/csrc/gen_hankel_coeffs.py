@@ -1,17 +1,22 @@
 """Generate hankel_coeffs.h: polynomial coefficients of the device Hankel kernels.
 
 The device evaluates (j/4) H0^(2)(x) and (j/4) H1^(2)(x) for real x > 0 in
-FP64 from two representations (DLMF 10.8.1, 10.17.3):
+FP64 from three representations (DLMF 10.8.1, 10.17.3):
 
 * x < XS, with y = x^2/4 and L = ln(x/2):
       J0 = A0(y)                 J1 = (x/2) A1(y)
       Y0 = (2/pi) L J0 + B0(y)   Y1 = (2/pi) L J1 - 2/(pi x) + (x/2) B1(y)
   A0, A1, B0, B1 are entire in y; their exact Taylor coefficients (to
   degree DEG_SMALL) are used with Horner on the device.
-* x >= XS, with w = (XS/x)^2:
+* x >= XS, the modulus / phase form
       H_nu^(2)(x) = sqrt(2/(pi x)) (P_nu(x) - j Q_nu(x)) exp(-j (x - (2 nu + 1) pi/4))
-  P_nu(x) = Pn(w) and Q_nu(x) = (XS/x) Qn(w); Pn, Qn are Chebyshev fits in w
-  on [0, 1] converted to monomials in w.
+  with P_nu, Q_nu from two Chebyshev fits converted to monomials:
+  - XS <= x < XM: w = (XS/x)^2 in (1/4, 1], u = (8 w - 5)/3 in (-1, 1];
+    P_nu = PnM(u), Q_nu = (XS/x) QnM(u), degree DEG_MID;
+  - x >= XM: w = (XM/x)^2 in (0, 1]; P_nu = Pn(w), Q_nu = (XM/x) Qn(w),
+    degree DEG_LARGE.  Far from the origin the asymptotic expansion
+    converges fast, so this range needs few terms; it is the common case of
+    the far-pair assembly.
 
 The header also carries the Gauss-Legendre rules (8, 16 and 20 points,
 mapped to [0, 1] as t = (x + 1)/2, w/2) of the element-pair quadratures.
@@ -30,8 +35,10 @@ from numpy.polynomial import polynomial as P
 
 mp.mp.dps = 40
 XS = 6.0
+XM = 12.0
 DEG_SMALL = 18
-DEG_LARGE = 19
+DEG_MID = 10
+DEG_LARGE = 7
 NODES = 600
 
 
@@ -59,19 +66,29 @@ def small_parts(y):
     return a0, a1, b0, b1
 
 
-def large_parts(w):
-    """P0, Q0n, P1, Q1n at w = (XS/x)^2 (mpmath); Qn = Q * x / XS."""
+def modulus_parts(w, x0):
+    """P0, Q0n, P1, Q1n at w = (x0/x)^2 (mpmath); Qn = Q * x / x0."""
     w = mp.mpf(w)
     out = []
     for nu in (0, 1):
         if w == 0:
-            out += [mp.mpf(1), mp.mpf(4 * nu * nu - 1) / 8 / XS]
+            out += [mp.mpf(1), mp.mpf(4 * nu * nu - 1) / 8 / x0]
             continue
-        x = XS / mp.sqrt(w)
+        x = x0 / mp.sqrt(w)
         h = mp.hankel2(nu, x)
         v = h * mp.exp(1j * (x - (2 * nu + 1) * mp.pi / 4)) * mp.sqrt(mp.pi * x / 2)
-        out += [mp.re(v), -mp.im(v) * x / XS]
+        out += [mp.re(v), -mp.im(v) * x / x0]
     return out
+
+
+def mid_parts(u):
+    """P0, Q0n, P1, Q1n of XS <= x < XM at u = (8 w - 5)/3, w = (XS/x)^2."""
+    return modulus_parts((3 * mp.mpf(u) + 5) / 8, XS)
+
+
+def large_parts(w):
+    """P0, Q0n, P1, Q1n of x >= XM at w = (XM/x)^2."""
+    return modulus_parts(w, XM)
 
 
 def taylor_small(deg):
@@ -112,14 +129,17 @@ def fit(fun, a, b, deg):
 
 def main(out=sys.stdout):
     small = taylor_small(DEG_SMALL)
+    mid = fit(mid_parts, -1.0, 1.0, DEG_MID)
     large = fit(large_parts, 0.0, 1.0, DEG_LARGE)
-    names = ("A0", "A1", "B0", "B1"), ("P0", "Q0", "P1", "Q1")
+    names = ("A0", "A1", "B0", "B1"), ("P0M", "Q0M", "P1M", "Q1M"), ("P0", "Q0", "P1", "Q1")
     print("// Generated by gen_hankel_coeffs.py -- do not edit.", file=out)
     print("#pragma once", file=out)
     print(f"#define GRIM_HANKEL_XS {XS!r}", file=out)
+    print(f"#define GRIM_HANKEL_XM {XM!r}", file=out)
     print(f"#define GRIM_HANKEL_DEG_SMALL {DEG_SMALL}", file=out)
+    print(f"#define GRIM_HANKEL_DEG_MID {DEG_MID}", file=out)
     print(f"#define GRIM_HANKEL_DEG_LARGE {DEG_LARGE}", file=out)
-    for group, arrays in zip(names, (small, large)):
+    for group, arrays in zip(names, (small, mid, large)):
         for name, arr in zip(group, arrays):
             body = ", ".join(repr(float(v)) for v in arr)
             print(f"static constexpr double GRIM_HK_{name}[{len(arr)}] = {{{body}}};", file=out)
