@@ -53,8 +53,8 @@ __device__ __forceinline__ double horner(const double (&c)[N], double t) {
 // (j/4) H0^(2)(x) and (j/4) H1^(2)(x) for real x > 0 (x = k r, floored at 1e-12).
 // Returns g = (Y0/4, J0/4) and h1 = (Y1/4, J1/4); the caller multiplies h1 by k.
 // Small x: Taylor series of the entire parts (DLMF 10.8.1); x >= 6: modulus /
-// phase form with fitted P_nu, Q_nu (DLMF 10.17.3), 11 terms on [6, 12) and 8
-// terms from 12 on, the common case of far pairs.  Coefficients generated by
+// phase form with fitted P_nu, Q_nu (DLMF 10.17.3), 12 terms on [6, 20) and 6
+// terms from 20 on, the common case of far pairs.  Coefficients generated by
 // gen_hankel_coeffs.py (max relative error 3e-15 for x up to 50; beyond, the
 // eps * x conditioning of the phase dominates).
 struct HK { double gr, gi, hr, hi; };
@@ -79,7 +79,7 @@ __device__ __noinline__ HK hankel_quarter_v(double x) {
         double p0, q0, p1, q1;
         if (x < GRIM_HANKEL_XM) {
             const double r = GRIM_HANKEL_XS * inv;
-            const double u = (r * r) * (8.0 / 3.0) - 5.0 / 3.0;
+            const double u = fma(r * r, GRIM_HANKEL_MID_SCALE, GRIM_HANKEL_MID_SHIFT);
             p0 = horner(GRIM_HK_P0M, u);
             q0 = r * horner(GRIM_HK_Q0M, u);
             p1 = horner(GRIM_HK_P1M, u);

@@ -11,8 +11,9 @@ FP64 from three representations (DLMF 10.8.1, 10.17.3):
 * x >= XS, the modulus / phase form
       H_nu^(2)(x) = sqrt(2/(pi x)) (P_nu(x) - j Q_nu(x)) exp(-j (x - (2 nu + 1) pi/4))
   with P_nu, Q_nu from two Chebyshev fits converted to monomials:
-  - XS <= x < XM: w = (XS/x)^2 in (1/4, 1], u = (8 w - 5)/3 in (-1, 1];
-    P_nu = PnM(u), Q_nu = (XS/x) QnM(u), degree DEG_MID;
+  - XS <= x < XM: w = (XS/x)^2 in (W_LO, 1], W_LO = (XS/XM)^2, mapped to
+    u = MID_SCALE w + MID_SHIFT in (-1, 1]; P_nu = PnM(u), Q_nu = (XS/x) QnM(u),
+    degree DEG_MID;
   - x >= XM: w = (XM/x)^2 in (0, 1]; P_nu = Pn(w), Q_nu = (XM/x) Qn(w),
     degree DEG_LARGE.  Far from the origin the asymptotic expansion
     converges fast, so this range needs few terms; it is the common case of
@@ -35,10 +36,13 @@ from numpy.polynomial import polynomial as P
 
 mp.mp.dps = 40
 XS = 6.0
-XM = 12.0
+XM = 20.0
 DEG_SMALL = 18
-DEG_MID = 10
-DEG_LARGE = 7
+DEG_MID = 11
+DEG_LARGE = 5
+W_LO = (XS / XM) ** 2                  # 0.09
+MID_SCALE = 2.0 / (1.0 - W_LO)
+MID_SHIFT = -(1.0 + W_LO) / (1.0 - W_LO)
 NODES = 600
 
 
@@ -82,8 +86,9 @@ def modulus_parts(w, x0):
 
 
 def mid_parts(u):
-    """P0, Q0n, P1, Q1n of XS <= x < XM at u = (8 w - 5)/3, w = (XS/x)^2."""
-    return modulus_parts((3 * mp.mpf(u) + 5) / 8, XS)
+    """P0, Q0n, P1, Q1n of XS <= x < XM at u = MID_SCALE w + MID_SHIFT, w = (XS/x)^2."""
+    lo = (mp.mpf(XS) / XM) ** 2
+    return modulus_parts(((1 - lo) * mp.mpf(u) + 1 + lo) / 2, XS)
 
 
 def large_parts(w):
@@ -138,6 +143,8 @@ def main(out=sys.stdout):
     print(f"#define GRIM_HANKEL_XM {XM!r}", file=out)
     print(f"#define GRIM_HANKEL_DEG_SMALL {DEG_SMALL}", file=out)
     print(f"#define GRIM_HANKEL_DEG_MID {DEG_MID}", file=out)
+    print(f"#define GRIM_HANKEL_MID_SCALE {MID_SCALE!r}", file=out)
+    print(f"#define GRIM_HANKEL_MID_SHIFT {MID_SHIFT!r}", file=out)
     print(f"#define GRIM_HANKEL_DEG_LARGE {DEG_LARGE}", file=out)
     for group, arrays in zip(names, (small, mid, large)):
         for name, arr in zip(group, arrays):

@@ -17,7 +17,9 @@ def _tables():
           for m in re.finditer(r"GRIM_HK_(\w+)\[\d+\] = \{([^}]*)\}", src)}
     xs = float(re.search(r"GRIM_HANKEL_XS ([\d.]+)", src).group(1))
     xm = float(re.search(r"GRIM_HANKEL_XM ([\d.]+)", src).group(1))
-    return co, xs, xm
+    scale = float(re.search(r"GRIM_HANKEL_MID_SCALE (\S+)", src).group(1))
+    shift = float(re.search(r"GRIM_HANKEL_MID_SHIFT (\S+)", src).group(1))
+    return co, xs, xm, scale, shift
 
 
 def _horner(c, t):
@@ -29,7 +31,7 @@ def _horner(c, t):
 
 def _device_form(x):
     """(j/4) H0^(2)(x), (j/4) H1^(2)(x) by the device's three ranges."""
-    co, xs, xm = _tables()
+    co, xs, xm, scale, shift = _tables()
     x = np.asarray(x, float)
     g = np.empty(x.shape, complex)
     h = np.empty(x.shape, complex)
@@ -42,7 +44,7 @@ def _device_form(x):
     y1 = (2 / np.pi) * lg * j1 - 2 / (np.pi * x[s]) + 0.5 * x[s] * _horner(co["B1"], y)
     g[s], h[s] = 0.25 * y0 + 0.25j * j0, 0.25 * y1 + 0.25j * j1
     for sel, x0, keys, var in (((x >= xs) & (x < xm), xs, ("P0M", "Q0M", "P1M", "Q1M"),
-                                lambda w: w * (8.0 / 3.0) - 5.0 / 3.0),
+                                lambda w: w * scale + shift),
                                (x >= xm, xm, ("P0", "Q0", "P1", "Q1"), lambda w: w)):
         xl = x[sel]
         r = x0 / xl
@@ -55,8 +57,8 @@ def _device_form(x):
     return g, h
 
 
-@pytest.mark.parametrize("lo,hi,tol", [(1e-6, 1.0, 2e-14), (1.0, 6.0, 2e-14), (6.0, 12.0, 2e-14),
-                                       (12.0, 60.0, 2e-14), (60.0, 2000.0, 1e-12)])
+@pytest.mark.parametrize("lo,hi,tol", [(1e-6, 1.0, 2e-14), (1.0, 6.0, 2e-14), (6.0, 20.0, 2e-14),
+                                       (20.0, 60.0, 2e-14), (60.0, 2000.0, 1e-12)])
 def test_device_hankel_tables_match_scipy(lo, hi, tol):
     x = np.geomspace(lo, hi, 4001)
     g, h = _device_form(x)
@@ -68,7 +70,7 @@ def test_device_hankel_tables_match_scipy(lo, hi, tol):
 
 
 def test_range_boundaries_are_continuous():
-    _, xs, xm = _tables()
+    _, xs, xm, _, _ = _tables()
     for b in (xs, xm):
         x = np.array([np.nextafter(b, 0.0), b])
         g, h = _device_form(x)
