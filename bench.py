@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--n-side", type=int, default=1000)
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default="mi355x",
+                    help="launch-shape table: 'mi355x' (shipped, measured by the autotuner), a JSON path, "
+                         "'tune' (measure now) or 'none' (built-in rules)")
     ap.add_argument("--cpu-seconds", type=float, default=40.0)
     ap.add_argument("--workload", choices=("naca", "sweep", "stream", "large"), default="naca",
                     help="naca: the headline NACA 0012 N=2000 request (default); sweep: 64-frequency star-curve "
@@ -400,6 +403,18 @@ def large_workload(args):
     print(json.dumps(out))
 
 
+def apply_tuning(choice):
+    """Load the launch-shape table the run uses; returns its entries for the JSON line."""
+    from grim_20261007_amd import autotune, native
+    native.ensure_device(0)
+    if choice == "none":
+        autotune.clear()
+        return None
+    table = autotune.tune() if choice == "tune" else autotune.resolve(choice)
+    autotune.apply(table)
+    return {"far_columns": table["far_columns"], "trsm_columns": table["trsm_columns"], "source": choice}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -408,6 +423,7 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    shapes = apply_tuning(args.tuning)
     if args.workload != "naca":
         from grim_20261007_amd import native
         native.ensure_device(0)
@@ -522,6 +538,7 @@ def main():
                                "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4),
                                "ms": round(gemm_ms, 3)},
         "cpu_baseline": cpu,
+        "launch_shapes": shapes,
     }
     print(json.dumps(line))
 

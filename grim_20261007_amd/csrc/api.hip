@@ -2,6 +2,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "../../include/grim2d.h"
@@ -63,6 +64,72 @@ __global__ void zero_c128(int m, int n, double* x, int64_t ld, int64_t stride) {
 }
 
 extern "C" int grim_version(void) { return 100; }
+
+// Launch-shape table: per knob, entries (min_size, value) sorted by min_size.
+namespace {
+constexpr int kShapeEntries = 16;
+struct ShapeTable {
+    int count = 0;
+    int64_t min_size[kShapeEntries];
+    int value[kShapeEntries];
+};
+ShapeTable g_shapes[GRIM_SHAPE_KNOBS];
+std::mutex g_shape_lock;
+
+bool shape_allowed(int knob, int value) {
+    if (knob == GRIM_SHAPE_FAR_COLUMNS) return value == 4 || value == 8 || value == 16 || value == 32;
+    if (knob == GRIM_SHAPE_TRSM_COLUMNS) return value == 1 || value == 2 || value == 4;
+    return false;
+}
+}  // namespace
+
+extern "C" int grim_set_launch_shape(int knob, int64_t min_size, int value) {
+    if (knob < 0 || knob >= GRIM_SHAPE_KNOBS || min_size < 0) return GRIM_ERR_ARGS;
+    if (value != 0 && !shape_allowed(knob, value)) return GRIM_ERR_ARGS;
+    std::lock_guard<std::mutex> guard(g_shape_lock);
+    ShapeTable& t = g_shapes[knob];
+    int pos = 0;
+    while (pos < t.count && t.min_size[pos] < min_size) ++pos;
+    const bool exists = pos < t.count && t.min_size[pos] == min_size;
+    if (value == 0) {   // remove the entry
+        if (exists) {
+            for (int i = pos; i + 1 < t.count; ++i) {
+                t.min_size[i] = t.min_size[i + 1];
+                t.value[i] = t.value[i + 1];
+            }
+            --t.count;
+        }
+        return GRIM_OK;
+    }
+    if (!exists) {
+        if (t.count == kShapeEntries) return GRIM_ERR_RANGE;
+        for (int i = t.count; i > pos; --i) {
+            t.min_size[i] = t.min_size[i - 1];
+            t.value[i] = t.value[i - 1];
+        }
+        ++t.count;
+        t.min_size[pos] = min_size;
+    }
+    t.value[pos] = value;
+    return GRIM_OK;
+}
+
+extern "C" int grim_clear_launch_shapes(void) {
+    std::lock_guard<std::mutex> guard(g_shape_lock);
+    for (auto& t : g_shapes) t.count = 0;
+    return GRIM_OK;
+}
+
+extern "C" int grim_shape_value(int knob, int64_t size) {
+    if (knob < 0 || knob >= GRIM_SHAPE_KNOBS) return 0;
+    std::lock_guard<std::mutex> guard(g_shape_lock);
+    const ShapeTable& t = g_shapes[knob];
+    int v = 0;
+    for (int i = 0; i < t.count && t.min_size[i] <= size; ++i) v = t.value[i];
+    return v;
+}
+
+extern "C" int grim_launch_shape(int knob, int64_t size) { return grim_shape_value(knob, size); }
 
 // Gauss-Legendre rules (numpy.polynomial.legendre.leggauss mapped to [0, 1],
 // the reference's _quadrature_nodes, twod/operators.py:3319-3323) and the

@@ -829,8 +829,11 @@ extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const doub
                                       int64_t ldte, const int* fix_rows, int nfix, hipStream_t st) {
     // columns per wavefront: enough wavefronts (about 8-16 K) to fill 1024 SIMDs
     // with their tails overlapping; each chunk re-evaluates one column pair.
-    int chunk = 4;
-    while (chunk < 32 && (double)n * n / (128.0 * chunk) > 16384.0) chunk *= 2;
+    int chunk = grim_shape_value(GRIM_SHAPE_FAR_COLUMNS, n);
+    if (chunk == 0) {
+        chunk = 4;
+        while (chunk < 32 && (double)n * n / (128.0 * chunk) > 16384.0) chunk *= 2;
+    }
     const int nc = (n + chunk - 1) / chunk, cpb = 64 / chunk, nb = (n + 63) / 64;
     long ntiles = 0;
     for (int I = 0; I < nb; ++I) ntiles += nc - I * cpb;

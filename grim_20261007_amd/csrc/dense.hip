@@ -1039,8 +1039,10 @@ static hipError_t trsm(const Batch& bt, int w, int ncols, const T* tri, int64_t 
 #define GRIM_TRSM(CPW)                                                                                            \
     trsm_lanes<T, LOWER, CPW><<<dim3((ncols + 4 * CPW - 1) / (4 * CPW), 1, bt.count), 256, 0, bt.st>>>(        \
         w, ncols, tri, ldt, st, b, ldb, sb, list, slist, list_base)
-    if (cols >= 8192) GRIM_TRSM(4);
-    else if (cols >= 2048) GRIM_TRSM(2);
+    int cpw = grim_shape_value(GRIM_SHAPE_TRSM_COLUMNS, cols);
+    if (cpw == 0) cpw = cols >= 8192 ? 4 : cols >= 2048 ? 2 : 1;
+    if (cpw == 4) GRIM_TRSM(4);
+    else if (cpw == 2) GRIM_TRSM(2);
     else GRIM_TRSM(1);
 #undef GRIM_TRSM
     return hipGetLastError();

@@ -44,6 +44,9 @@ SIGNATURES = {
     "grim_to_c64": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "grim_copy_c128": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "grim_column_norms": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp]),
+    "grim_set_launch_shape": (_i32, [_i32, _i64, _i32]),
+    "grim_clear_launch_shapes": (_i32, []),
+    "grim_launch_shape": (_i32, [_i32, _i64]),
 }
 
 STATUS = {0: "ok", 1: "size or index out of range", 3: "invalid arguments", 5: "HIP runtime failure",

@@ -28,7 +28,7 @@ from typing import Dict, List, Optional, Tuple
 import numpy as np
 import torch
 
-from . import engine
+from . import autotune, engine
 from .mesh import ContourMesh
 from .solver import wavenumber
 
@@ -127,8 +127,12 @@ class SystemCache:
 
 class Server:
     def __init__(self, device=None, workers: int = 4, hbm_budget_bytes: Optional[int] = None,
-                 cache_bytes: Optional[int] = None):
+                 cache_bytes: Optional[int] = None, tuning=None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        # launch-shape table from the autotuner (a dict or a JSON path); shapes never change results
+        self.tuning = autotune.resolve(tuning)
+        if self.tuning is not None:
+            autotune.apply(self.tuning)
         total = torch.cuda.get_device_properties(self.device).total_memory
         self.budget = int(hbm_budget_bytes if hbm_budget_bytes is not None else 0.6 * total)
         self.cache = SystemCache(cache_bytes if cache_bytes is not None else int(0.25 * total))

@@ -146,6 +146,22 @@ int grim_to_c64(int64_t batch, int64_t m, int64_t n, const double* src, int64_t 
 int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds, int64_t stride_src,
                    double* dst, int64_t ldd, int64_t stride_dst, void* stream);
 
+/* Launch shapes (no counterpart in the reference; for a project autotuner and
+ * callers that pin their own shapes).  Knob 0: far-pass source columns per
+ * wavefront (4, 8, 16 or 32), keyed by n; knob 1: triangular-solve columns per
+ * wavefront (1, 2 or 4), keyed by the right-hand-side columns of one call times
+ * its batch.  An entry (min_size, value) applies to every size >= min_size up
+ * to the next entry; value 0 removes the entry; without an entry the library's
+ * built-in rule applies.  Shapes never change results: every matrix entry and
+ * solution component is computed by the same operations.  Process-wide; set
+ * them before launching work (calls are serialised by a lock, but a launch
+ * that runs concurrently may see either shape).  grim_launch_shape returns the
+ * table's value for a size (0 = built-in rule).  Status: 3 for an unknown knob
+ * or value, 1 when a knob already has 16 entries. */
+int grim_set_launch_shape(int knob, int64_t min_size, int value);
+int grim_clear_launch_shapes(void);
+int grim_launch_shape(int knob, int64_t size);
+
 /* Per-column 2-norm of an m x n complex128 matrix into out[n] (device). */
 int grim_column_norms(int64_t m, int64_t n, const double* v, int64_t ld, double* out, void* stream);
 
