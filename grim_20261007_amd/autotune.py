@@ -31,10 +31,10 @@ from .mesh import build_mesh
 from .solver import wavenumber
 
 FAR_COLUMNS, TRSM_COLUMNS = 0, 1
-# table measured by tune() on an MI355X (far pass n = 500-8000, solves of 90-1440 angles)
+# table measured by tune() on an MI355X (far pass n = 250-8000, solves of 90-1440 angles)
 MI355X_TABLE = Path(__file__).resolve().parent / "tuning_mi355x.json"
 KNOB_NAMES = {FAR_COLUMNS: "far_columns", TRSM_COLUMNS: "trsm_columns"}
-CHOICES = {FAR_COLUMNS: (4, 8, 16, 32), TRSM_COLUMNS: (1, 2, 4)}
+CHOICES = {FAR_COLUMNS: (1, 2, 4, 8, 16, 32), TRSM_COLUMNS: (1, 2, 4)}
 
 
 def set_shape(knob: int, min_size: int, value: int) -> None:

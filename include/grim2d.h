@@ -148,7 +148,7 @@ int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double* src, int64
 
 /* Launch shapes (no counterpart in the reference; for a project autotuner and
  * callers that pin their own shapes).  Knob 0: far-pass source columns per
- * wavefront (4, 8, 16 or 32), keyed by n; knob 1: triangular-solve columns per
+ * wavefront (1, 2, 4, 8, 16 or 32), keyed by n; knob 1: triangular-solve columns per
  * wavefront (1, 2 or 4), keyed by the right-hand-side columns of one call times
  * its batch.  An entry (min_size, value) applies to every size >= min_size up
  * to the next entry; value 0 removes the entry; without an entry the library's

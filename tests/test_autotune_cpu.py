@@ -28,6 +28,7 @@ def test_table_lookup_ranges(lib):
 
 def test_table_rejects_bad_arguments(lib):
     assert lib.grim_set_launch_shape(0, 0, 5) == 3              # not a far-pass shape
+    assert lib.grim_set_launch_shape(0, 0, 64) == 3
     assert lib.grim_set_launch_shape(1, 0, 8) == 3              # not a solve shape
     assert lib.grim_set_launch_shape(2, 0, 1) == 3              # unknown knob
     assert lib.grim_set_launch_shape(0, -1, 4) == 3
