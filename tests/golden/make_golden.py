@@ -15,7 +15,7 @@ reference factors it.  Outputs are small ``.npz`` fixtures next to this file:
 inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
-Usage:  python tests/golden/make_golden.py [--certified | --meshes]
+Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim]
 """
 from __future__ import annotations
 
@@ -274,10 +274,37 @@ def main_meshes():
     print(f"meshes: {len(names)} meshes, {len(chains)} chains", flush=True)
 
 
+def main_grim():
+    """The reference's .grim exporter (io/grim.py:1034-1170) on the result dicts of
+    grim_cases.py: formats 2 and 1 with the raw amplitude, and format 2 without it."""
+    sys.path.insert(0, str(HERE))
+    import grim_cases
+    from ghost_backend.io.grim import export_result_to_grim
+    out = HERE / "grim"
+    out.mkdir(exist_ok=True)
+    for name, (result, kwargs) in grim_cases.cases().items():
+        for version, raw in ((2, True), (1, True), (2, False)):
+            path = out / f"{name}_v{version}{'' if raw else '_noraw'}.grim"
+            export_result_to_grim(result, str(path), preserve_raw_complex_amplitude=raw, format_version=version,
+                                  **kwargs)
+            print(f"{path.name}: {path.stat().st_size} bytes", flush=True)
+    errors = {}
+    for name, result in grim_cases.error_cases().items():
+        try:
+            export_result_to_grim(result, str(out / "refused.grim"))
+            errors[name] = ""
+        except ValueError as exc:
+            errors[name] = str(exc)
+    (out / "errors.json").write_text(json.dumps(errors, indent=1, sort_keys=True))
+    print(f"errors.json: {errors}", flush=True)
+
+
 if __name__ == "__main__":
     if "--certified" in sys.argv:
         main_certified()
     elif "--meshes" in sys.argv:
         main_meshes()
+    elif "--grim" in sys.argv:
+        main_grim()
     else:
         main()

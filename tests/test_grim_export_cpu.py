@@ -1,0 +1,74 @@
+"""`.grim` archives against the reference exporter's: the same result dicts
+(tests/golden/grim_cases.py) give the same keys, dtypes and values in formats 1 and 2,
+and the same refusals (archives and messages written by tests/golden/make_golden.py --grim)."""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from grim_20261007_amd import grim_export
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+sys.path.insert(0, str(GOLDEN))
+import grim_cases  # noqa: E402
+
+VARIANTS = [(2, True, ""), (1, True, ""), (2, False, "_noraw")]
+
+
+def _arrays(path):
+    with np.load(path, allow_pickle=False) as data:
+        return {k: data[k] for k in data.files}
+
+
+@pytest.mark.parametrize("case", sorted(grim_cases.cases()))
+@pytest.mark.parametrize("version,raw,suffix", VARIANTS)
+def test_archive_matches_reference(tmp_path, case, version, raw, suffix):
+    result, kwargs = grim_cases.cases()[case]
+    out = grim_export.export_result_to_grim(result, str(tmp_path / "out"), preserve_raw_complex_amplitude=raw,
+                                            format_version=version, **kwargs)
+    assert out == [str((tmp_path / "out.grim").resolve())]
+    got = _arrays(out[0])
+    want = _arrays(GOLDEN / "grim" / f"{case}_v{version}{suffix}.grim")
+    assert list(got) == list(want)
+    for key in want:
+        assert got[key].dtype == want[key].dtype, key
+        assert got[key].shape == want[key].shape, key
+        if want[key].dtype.kind in "fc":
+            assert np.array_equal(got[key], want[key], equal_nan=True), key
+        else:
+            assert np.array_equal(got[key], want[key]), key
+
+
+@pytest.mark.parametrize("name", sorted(grim_cases.error_cases()))
+def test_refusals_match_reference(tmp_path, name):
+    messages = json.loads((GOLDEN / "grim" / "errors.json").read_text())
+    with pytest.raises(ValueError) as err:
+        grim_export.export_result_to_grim(grim_cases.error_cases()[name], str(tmp_path / "x"))
+    assert str(err.value) == messages[name]
+    assert not (tmp_path / "x.grim").exists()
+
+
+def test_reader_gives_format1_view_of_format2(tmp_path):
+    result, kwargs = grim_cases.cases()["co_polarized"]
+    v2 = grim_export.load_grim(grim_export.export_result_to_grim(result, str(tmp_path / "a"), format_version=2)[0])
+    v1 = grim_export.load_grim(grim_export.export_result_to_grim(result, str(tmp_path / "b"), format_version=1)[0])
+    np.testing.assert_array_equal(v2["rcs_amp_real"], v1["rcs_amp_real"])
+    np.testing.assert_array_equal(v2["rcs_amp_imag"], v1["rcs_amp_imag"])
+    np.testing.assert_array_equal(v2["rcs_phase"], v1["rcs_phase"])
+    # power derived from the field agrees with the samples' float32 power to float32 rounding
+    np.testing.assert_allclose(v2["rcs_power"], v1["rcs_power"], rtol=2e-7, atol=1e-30)
+    # sigma = |B|^2 / (4 k) for every stored cell
+    k = 2 * np.pi * v1["frequencies"] * 1e9 / grim_export.C0
+    b2 = v1["rcs_amp_real"] ** 2 + v1["rcs_amp_imag"] ** 2
+    np.testing.assert_allclose(v1["rcs_power"], b2 / (4 * k[None, None, :, None]), rtol=1e-6, atol=1e-30)
+
+
+def test_format_version_selection(monkeypatch):
+    monkeypatch.setenv("GHOST_GRIM_FORMAT_VERSION", "1")
+    assert grim_export.resolve_format_version() == 1
+    monkeypatch.delenv("GHOST_GRIM_FORMAT_VERSION")
+    assert grim_export.resolve_format_version() == 2
+    with pytest.raises(ValueError):
+        grim_export.resolve_format_version(3)
