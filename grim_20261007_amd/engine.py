@@ -95,6 +95,25 @@ def farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
     return amp, sigma
 
 
+def bistatic_farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
+    """Bistatic amplitudes B[b, i, s] = sum_p x[b, i, p] (bu[s, p] + gamma bdn[s, p]), gamma = 1/(j k),
+    for densities x (B, A_inc, N) and observation-direction loads (A_obs, N); with s = i this is
+    the matched monostatic projection.  One complex GEMM per batch on the library's FP64 MFMA
+    kernel (the observation operator is shared: stride 0).  Returns (amplitude, sigma_2D), both
+    (B, A_inc, A_obs)."""
+    xb = _as_batch(x).contiguous()
+    nb, ninc, n = xb.shape
+    nobs = bu.shape[0]
+    proj_t = (bu + bdn * complex(0.0, -1.0 / float(k))).transpose(0, 1).contiguous()   # (N, A_obs)
+    c = torch.zeros((nb, ninc, nobs), dtype=torch.complex128, device=x.device)
+    code = native.lib().grim_gemm_minus(nb, nobs, ninc, n, proj_t.data_ptr(), nobs, 0, xb.data_ptr(), n, ninc * n,
+                                        c.data_ptr(), nobs, ninc * nobs, 64, _stream(x.device))
+    native.check("grim_gemm_minus", code)
+    amp = -c
+    sigma = (amp.real * amp.real + amp.imag * amp.imag) / (4.0 * float(k))
+    return amp, sigma
+
+
 @dataclass
 class Factor:
     """LU factors of a batch of equally sized systems (column-major, batch first)."""

@@ -4,7 +4,8 @@ The archive is the one GRIM and the reference's exporter exchange
 (``tools/GHOST/ghost_backend/io/grim.py:1034-1170``, ``export_result_to_grim``;
 layout notes in ``tools/GHOST/README.md:212-221``): a compressed NumPy archive
 holding a rectangular grid over (azimuth, elevation, frequency, polarization)
-with its axes and metadata.
+with its axes and metadata.  A bistatic result becomes one archive per
+incidence angle (``<path>_inc_<angle>.grim``), published together.
 
 * Format 2 (the default) stores the solver's raw complex far-field amplitude
   once, as ``rcs_complex`` (complex128), framed by ``grim_format_version``,
@@ -17,8 +18,7 @@ A co-polarized result (``polarizations == ["VV", "HH"]`` with matching
 ``co_solved_samples``) becomes one archive with both channels on the last
 axis.  Every key, dtype and value follows the reference's writer, so the same
 result dict gives the same arrays; ``tests/test_grim_export_cpu.py`` checks
-that against archives the reference wrote.  Bistatic results are not produced
-by this path and are refused.
+that against archives the reference wrote.
 """
 from __future__ import annotations
 
@@ -247,9 +247,70 @@ def _payload(channels, labels, alias_primary, aliases_json, quantity, log_unit, 
             "source_path": source_path, "history": history}
 
 
+def _grim_path(path: str) -> str:
+    return path if path.lower().endswith(".grim") else f"{path}.grim"
+
+
+def incidence_suffix(theta_inc_deg: float) -> str:
+    """File suffix of one incidence of a bistatic collection: inc_12p5, inc_m30 (io/grim.py:191-194)."""
+    text = f"{theta_inc_deg:.6f}".rstrip("0").rstrip(".")
+    return "inc_" + (text.replace("-", "m").replace(".", "p") or "0")
+
+
+def _publish(entries, version: int, preserve_raw: bool) -> List[str]:
+    """Write a bistatic collection as one unit (io/grim.py:894-1005): every archive is staged
+    beside its target first; targets are then replaced, keeping backups until all are in place,
+    so a failure restores the previous collection."""
+    targets, seen = [], set()
+    for _, path in entries:
+        target = os.path.abspath(_grim_path(path))
+        if os.path.normcase(target) in seen:
+            raise ValueError(f"A GRIM output batch contains the duplicate target {target!r}.")
+        if os.path.exists(target) and not os.path.isfile(target):
+            raise ValueError(f"GRIM output target {target!r} exists but is not a regular file.")
+        seen.add(os.path.normcase(target))
+        targets.append(target)
+    staged, backups, complete = [], {}, False
+    try:
+        for (payload, _), target in zip(entries, targets):
+            os.makedirs(os.path.dirname(target), exist_ok=True)
+            fd, stage = tempfile.mkstemp(prefix=f".{os.path.basename(target)}.", suffix=".stage.grim",
+                                         dir=os.path.dirname(target))
+            os.close(fd)
+            staged.append((stage, target))
+            _write(payload, stage, version, preserve_raw)
+        for stage, target in staged:
+            backup = None
+            if os.path.exists(target):
+                fd, backup = tempfile.mkstemp(prefix=f".{os.path.basename(target)}.", suffix=".backup",
+                                              dir=os.path.dirname(target))
+                os.close(fd)
+                os.replace(target, backup)
+            backups[target] = backup
+            os.replace(stage, target)
+        complete = True
+        return targets
+    except BaseException:
+        for target, backup in reversed(list(backups.items())):
+            if backup is None:
+                if os.path.exists(target):
+                    os.unlink(target)
+            elif os.path.exists(backup):
+                os.replace(backup, target)
+        raise
+    finally:
+        for stage, _ in staged:
+            if os.path.exists(stage):
+                os.unlink(stage)
+        if complete:
+            for backup in backups.values():
+                if backup is not None and os.path.exists(backup):
+                    os.unlink(backup)
+
+
 def _write(payload: Dict[str, Any], path: str, version: int, preserve_raw: bool) -> str:
     """Write the archive atomically (temporary file, fsync, rename), keys in the reference's order."""
-    out = path if path.lower().endswith(".grim") else f"{path}.grim"
+    out = _grim_path(path)
     save = {k: payload[k] for k in ("azimuths", "elevations", "frequencies", "polarizations",
                                     "polarization_alias_primary", "polarization_aliases_json")}
     compact = version == 2 and preserve_raw
@@ -305,29 +366,63 @@ def export_result_to_grim(result: Dict[str, Any], output_path: str, source_path:
     mapping = result.get("polarization_mapping", {}) or {}
     if dual and set(mapping) != {"VV", "HH"}:
         raise ValueError("A co-polarized result must declare its VV/HH physical mapping.")
-    if str(result.get("scattering_mode", "monostatic")).strip().lower() == "bistatic":
-        raise ValueError("Bistatic results are not produced by the 2-D PEC monostatic path; nothing to export.")
+    bistatic = str(result.get("scattering_mode", "monostatic")).strip().lower() == "bistatic"
     quantity = str(result.get("rcs_linear_quantity", "sigma_2d")).strip().lower()
     if quantity not in ("sigma_2d", "sigma_3d"):
         raise ValueError("rcs_linear_quantity must be 'sigma_2d' or 'sigma_3d'.")
     log_unit = str(result.get("rcs_log_unit", "dBke"))
-    if dual:
-        internal = [str(mapping[c]).strip().upper() for c in ("VV", "HH")]
-        if any(not label for label in internal):
-            raise ValueError("Co-polarized GRIM mapping labels cannot be empty.")
-        chans = [_channel_grid(list(co[c] or []), c, quantity) for c in ("VV", "HH")]
-        payload = _payload(chans, ("VV", "HH"), ",".join(internal), json.dumps(internal), quantity, log_unit,
-                           source_path, history)
-    else:
-        label = canonical_polarization(result.get("polarization_export") or result.get("polarization") or "TM")
-        chans = [_channel_grid(list(samples), label, quantity)]
-        aliases = TM_ALIASES if label == "TM" else TE_ALIASES
-        payload = _payload(chans, ("HH" if label == "TM" else "VV",), label, json.dumps(aliases), quantity,
-                           log_unit, source_path, history)
-    if not np.all(np.isfinite(payload["field"].real) & np.isfinite(payload["field"].imag)):
-        raise ValueError("Solver complex amplitudes are not finite after grid construction.")
-    payload["solver_metadata_json"] = solver_metadata_json(result)
-    return [os.path.abspath(_write(payload, output_path, version, bool(preserve_raw_complex_amplitude)))]
+    label = None if dual else canonical_polarization(
+        result.get("polarization_export") or result.get("polarization") or "TM")
+
+    def build(channel_rows: Dict[str, List[Dict[str, Any]]], hist: str) -> Dict[str, Any]:
+        if dual:
+            internal = [str(mapping[c]).strip().upper() for c in ("VV", "HH")]
+            if any(not lab for lab in internal):
+                raise ValueError("Co-polarized GRIM mapping labels cannot be empty.")
+            chans = [_channel_grid(channel_rows[c], c, quantity) for c in ("VV", "HH")]
+            payload = _payload(chans, ("VV", "HH"), ",".join(internal), json.dumps(internal), quantity,
+                               log_unit, source_path, hist)
+        else:
+            chans = [_channel_grid(channel_rows[label], label, quantity)]
+            aliases = TM_ALIASES if label == "TM" else TE_ALIASES
+            payload = _payload(chans, ("HH" if label == "TM" else "VV",), label, json.dumps(aliases), quantity,
+                               log_unit, source_path, hist)
+        if not np.all(np.isfinite(payload["field"].real) & np.isfinite(payload["field"].imag)):
+            raise ValueError("Solver complex amplitudes are not finite after grid construction.")
+        return payload
+
+    channels = {c: list(co[c] or []) for c in ("VV", "HH")} if dual else {label: list(samples)}
+    raw = bool(preserve_raw_complex_amplitude)
+    if not bistatic:
+        payload = build(channels, history)
+        payload["solver_metadata_json"] = solver_metadata_json(result)
+        return [os.path.abspath(_write(payload, output_path, version, raw))]
+    # bistatic: one archive per incidence angle, published together (io/grim.py:1092-1170)
+    groups = {c: _by_incidence(rows, c if dual else "") for c, rows in channels.items()}
+    incidences = [set(g) for g in groups.values()]
+    if any(s != incidences[0] for s in incidences[1:]):
+        raise ValueError("VV and HH bistatic incidence axes differ and cannot share one GRIM collection.")
+    root = _grim_path(output_path)[:-5]
+    metadata = solver_metadata_json(result)
+    entries = []
+    for inc in sorted(incidences[0]):
+        payload = build({c: g[inc] for c, g in groups.items()}, (history + f" | theta_inc_deg={inc:g}").strip(" |"))
+        payload["solver_metadata_json"] = metadata
+        entries.append((payload, f"{root}_{incidence_suffix(inc)}.grim"))
+    return _publish(entries, version, raw)
+
+
+def _by_incidence(rows: List[Dict[str, Any]], channel: str = "") -> Dict[float, List[Dict[str, Any]]]:
+    """Rows grouped by theta_inc_deg, each group in its original order."""
+    out: Dict[float, List[Dict[str, Any]]] = {}
+    for i, row in enumerate(rows):
+        try:
+            inc = _finite(row, "theta_inc_deg", i)
+        except ValueError as exc:
+            raise ValueError(f"Invalid {channel + ' ' if channel else ''}bistatic solver sample at index {i}: "
+                             f"{str(exc).split(': ', 1)[1]}") from exc
+        out.setdefault(inc, []).append(row)
+    return out
 
 
 def load_grim(path: str) -> Dict[str, np.ndarray]:

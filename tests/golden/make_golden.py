@@ -15,7 +15,7 @@ reference factors it.  Outputs are small ``.npz`` fixtures next to this file:
 inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
-Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim]
+Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic]
 """
 from __future__ import annotations
 
@@ -299,6 +299,52 @@ def main_grim():
     print(f"errors.json: {errors}", flush=True)
 
 
+def run_bistatic_case(name, points, frequency_ghz, incidence, observation):
+    """The reference's co-polarized bistatic entry (twod/solver.py:3603-3660) on one snapshot:
+    per channel the amplitude grid (incidence x observation), the result and sample keys, the
+    merged result's samples (as JSON) and the archives its exporter writes for that result."""
+    import ghost_backend.twod.solver as rcs
+    from ghost_backend.io.grim import export_result_to_grim
+    snapshot = workloads._snapshot(points, name)
+    result = rcs.solve_bistatic_rcs_2d(snapshot, frequencies_ghz=[frequency_ghz],
+                                       incidence_angles_deg=list(map(float, incidence)),
+                                       observation_angles_deg=list(map(float, observation)),
+                                       geometry_units="meters", max_panels=100000)
+    out = {"points": np.asarray(points, float), "frequency_ghz": float(frequency_ghz),
+           "incidence_deg": np.asarray(incidence, float), "observation_deg": np.asarray(observation, float),
+           "result_keys": np.array(sorted(result.keys())),
+           "sample_keys": np.array(sorted(result["samples"][0].keys())),
+           "scattering_mode": np.array(result["scattering_mode"])}
+    for export in ("VV", "HH"):
+        rows = result["co_solved_samples"][export]
+        amp = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+        out[f"amp_{export}"] = amp.reshape(len(incidence), len(observation))
+        out[f"db_{export}"] = np.array([r["rcs_db"] for r in rows]).reshape(len(incidence), len(observation))
+        out[f"inc_{export}"] = np.array([r["theta_inc_deg"] for r in rows])
+        out[f"obs_{export}"] = np.array([r["theta_scat_deg"] for r in rows])
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    # the export of exactly this result, for the archive layout of bistatic collections
+    keep = {k: result[k] for k in ("solver", "scattering_mode", "amplitude_convention", "amplitude_version",
+                                   "polarizations", "polarization_mapping")}
+    plain = lambda rows: [{k: (float(v) if isinstance(v, (int, float, np.floating)) else v)
+                           for k, v in r.items()} for r in rows]
+    small = dict(keep, samples=plain(result["samples"]),
+                 co_solved_samples={c: plain(result["co_solved_samples"][c]) for c in ("VV", "HH")},
+                 metadata={"source": "reference bistatic solve", "panel_count": 64})
+    (HERE / "grim" / f"{name}_result.json").write_text(json.dumps(small, indent=None, sort_keys=True))
+    paths = export_result_to_grim(small, str(HERE / "grim" / f"{name}.grim"), history="bistatic case")
+    print(f"{name}: keys {sorted(result.keys())}; archives {[os.path.basename(p) for p in paths]}", flush=True)
+
+
+def main_bistatic():
+    r = 0.1
+    run_bistatic_case("bistatic_circle64_ka1", workloads.circle_points(r, 64), workloads.frequency_for_ka(1.0, r),
+                      [0.0, 90.0, 135.0], np.arange(0.0, 360.0, 30.0))
+    pts = workloads.naca0012_points(40)
+    run_bistatic_case("bistatic_naca80", pts, workloads.frequency_for_segments_per_wavelength(pts, 10.0),
+                      [0.0, 45.0], [0.0, 45.0, 100.0, 180.0, 270.0])
+
+
 if __name__ == "__main__":
     if "--certified" in sys.argv:
         main_certified()
@@ -306,5 +352,7 @@ if __name__ == "__main__":
         main_meshes()
     elif "--grim" in sys.argv:
         main_grim()
+    elif "--bistatic" in sys.argv:
+        main_bistatic()
     else:
         main()

@@ -72,3 +72,26 @@ def test_format_version_selection(monkeypatch):
     assert grim_export.resolve_format_version() == 2
     with pytest.raises(ValueError):
         grim_export.resolve_format_version(3)
+
+
+@pytest.mark.parametrize("name", ["bistatic_circle64_ka1", "bistatic_naca80"])
+def test_bistatic_collection_matches_reference(tmp_path, name):
+    """The reference's bistatic solve result (as written by make_golden.py --bistatic) exported by
+    this build: one archive per incidence, every array equal to the reference exporter's."""
+    result = json.loads((GOLDEN / "grim" / f"{name}_result.json").read_text())
+    paths = grim_export.export_result_to_grim(result, str(tmp_path / name), history="bistatic case")
+    names = [Path(p).name for p in paths]
+    want_names = sorted(p.name for p in (GOLDEN / "grim").glob(f"{name}_inc_*.grim"))
+    assert sorted(names) == want_names
+    for p in paths:
+        got, want = _arrays(p), _arrays(GOLDEN / "grim" / Path(p).name)
+        assert list(got) == list(want)
+        for key in want:
+            assert got[key].dtype == want[key].dtype and np.array_equal(got[key], want[key]), (p, key)
+
+
+def test_incidence_suffix():
+    assert grim_export.incidence_suffix(0.0) == "inc_0"
+    assert grim_export.incidence_suffix(12.5) == "inc_12p5"
+    assert grim_export.incidence_suffix(-30.0) == "inc_m30"
+    assert grim_export.incidence_suffix(1e-7) == "inc_0"
