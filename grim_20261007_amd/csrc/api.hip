@@ -23,6 +23,9 @@ extern "C" hipError_t grim_launch_loads(int n, int na, const double* p0, const d
 extern "C" hipError_t grim_launch_farfield(int n, int na, const double* x, int64_t ldx, const double* bu,
                                            const double* bdn, int64_t ldb, double k, double* amp, double* sigma,
                                            hipStream_t st);
+extern "C" hipError_t grim_launch_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* y,
+                                             int64_t ldy, const double* ax, const double* ay, const int* dirichlet,
+                                             hipStream_t st);
 extern "C" int64_t grim_dense_aux_bytes(int n, int single);
 extern "C" hipError_t grim_dense_lu_factor(int batch, int n, void* a, int64_t lda, int64_t sa, int single, int* piv,
                                            int64_t spiv, void* aux, int64_t saux, unsigned* status, hipStream_t st);
@@ -395,5 +398,13 @@ extern "C" int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double*
 extern "C" int grim_column_norms(int64_t m, int64_t n, const double* v, int64_t ld, double* out, void* stream) {
     if (m < 1 || n < 0 || !v || !out || ld < m) return GRIM_ERR_ARGS;
     CHECK(grim_dense_colnorm((int)m, (int)n, v, ld, out, (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+extern "C" int grim_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* y, int64_t ldy,
+                               const double* alpha_x, const double* alpha_y, const int32_t* dirichlet, void* stream) {
+    if (rows < 0 || cols < 0 || cols > 65535 || ldx < rows || ldy < rows) return GRIM_ERR_ARGS;
+    if (rows > 0 && cols > 0 && (!x || !y || !alpha_x || !alpha_y || !dirichlet)) return GRIM_ERR_ARGS;
+    CHECK(grim_launch_robin_rows(rows, cols, x, ldx, y, ldy, alpha_x, alpha_y, dirichlet, (hipStream_t)stream));
     return GRIM_OK;
 }

@@ -861,6 +861,41 @@ extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1
     return hipGetLastError();
 }
 
+// Impedance (Robin) rows of closed contours, in place over a column-major pair
+// of rows x cols arrays: x <- x + ax[r] y on every row, y <- y on Dirichlet rows
+// and x + ay[r] y elsewhere (both from the original x, y).  With x, y the TE and
+// TM systems (or their loads) this forms the flux rows dn(u) + alpha u of the
+// combined potential (twod/formulations/combined_regions.py:43-62,
+// twod/formulations/robin.py:25-83) for a law that is constant on each contour.
+__global__ void __launch_bounds__(256) robin_rows_kernel(int64_t rows, int64_t cols, double* __restrict__ x,
+                                                         int64_t ldx, double* __restrict__ y, int64_t ldy,
+                                                         const double* __restrict__ ax, const double* __restrict__ ay,
+                                                         const int* __restrict__ dirichlet) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (r >= rows || c >= cols) return;
+    double* px = x + 2 * (r + c * ldx);
+    double* py = y + 2 * (r + c * ldy);
+    const double xr = px[0], xi = px[1], yr = py[0], yi = py[1];
+    const double axr = ax[2 * r], axi = ax[2 * r + 1];
+    px[0] = xr + (axr * yr - axi * yi);
+    px[1] = xi + (axr * yi + axi * yr);
+    if (!dirichlet[r]) {
+        const double ayr = ay[2 * r], ayi = ay[2 * r + 1];
+        py[0] = xr + (ayr * yr - ayi * yi);
+        py[1] = xi + (ayr * yi + ayi * yr);
+    }
+}
+
+extern "C" hipError_t grim_launch_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* y,
+                                             int64_t ldy, const double* ax, const double* ay, const int* dirichlet,
+                                             hipStream_t st) {
+    if (rows == 0 || cols == 0) return hipSuccess;
+    dim3 grid((unsigned)((rows + 255) / 256), (unsigned)cols);
+    robin_rows_kernel<<<grid, 256, 0, st>>>(rows, cols, x, ldx, y, ldy, ax, ay, dirichlet);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t grim_launch_loads(int n, int na, const double* p0, const double* p1, const int* prev, double k,
                                         const double* angles_deg, double* bu, double* bdn, int64_t ld,
                                         hipStream_t st) {

@@ -95,6 +95,50 @@ def farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
     return amp, sigma
 
 
+ETA0 = 376.730313668      # free-space wave impedance (twod/constants.py)
+
+
+def robin_coefficients(mesh: ContourMesh, k: float):
+    """Per-node Robin coefficients of a surface impedance Zs in air (twod/geometry.py:2488-2521):
+    alpha_TE = j k Zs / eta0, alpha_TM = j k eta0 / Zs; 0 and a Dirichlet TM row on PEC contours.
+    Node p takes the law of element p (one law per contour)."""
+    z = np.asarray(mesh.impedance, complex)
+    ibc = np.abs(z) > 1e-12
+    a_te = np.where(ibc, 1j * k * z / ETA0, 0.0)
+    a_tm = np.where(ibc, 1j * k * ETA0 / np.where(ibc, z, 1.0), 0.0)
+    return a_te.astype(np.complex128), a_tm.astype(np.complex128), (~ibc).astype(np.int32)
+
+
+def _robin_device(contour: DeviceContour, k: float, dev):
+    a_te, a_tm, dirichlet = robin_coefficients(contour.mesh, k)
+    return (torch.as_tensor(a_te, device=dev), torch.as_tensor(a_tm, device=dev),
+            torch.as_tensor(dirichlet, device=dev))
+
+
+def impedance_systems(contour: DeviceContour, k: float, pair: torch.Tensor) -> torch.Tensor:
+    """Turn the [TE, TM] PEC systems `pair` (2, N, N) in place into the systems of the contours'
+    impedance laws: flux rows Z_TE + alpha Z_TM for both polarizations, TM rows of PEC contours
+    unchanged (grim_robin_rows)."""
+    n = contour.n
+    ax, ay, dm = _robin_device(contour, k, pair.device)
+    native.check("grim_robin_rows", native.lib().grim_robin_rows(n, n, pair[0].data_ptr(), n, pair[1].data_ptr(), n,
+                                                                 ax.data_ptr(), ay.data_ptr(), dm.data_ptr(),
+                                                                 _stream(pair.device)))
+    return pair
+
+
+def impedance_rhs(contour: DeviceContour, k: float, bu: torch.Tensor, bdn: torch.Tensor) -> torch.Tensor:
+    """Right-hand sides (2, A, N) [TE, TM] of impedance_systems: -(b_dn + alpha b_u) on flux rows,
+    -b_u on TM rows of PEC contours."""
+    n, na = contour.n, int(bu.shape[0])
+    ax, ay, dm = _robin_device(contour, k, bu.device)
+    x, y = bdn.clone(), bu.clone()
+    native.check("grim_robin_rows", native.lib().grim_robin_rows(n, na, x.data_ptr(), n, y.data_ptr(), n,
+                                                                 ax.data_ptr(), ay.data_ptr(), dm.data_ptr(),
+                                                                 _stream(bu.device)))
+    return torch.stack((-x, -y))
+
+
 def bistatic_farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
     """Bistatic amplitudes B[b, i, s] = sum_p x[b, i, p] (bu[s, p] + gamma bdn[s, p]), gamma = 1/(j k),
     for densities x (B, A_inc, N) and observation-direction loads (A_obs, N); with s = i this is

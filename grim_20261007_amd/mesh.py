@@ -35,6 +35,7 @@ class ContourMesh:
     next: np.ndarray         # (N,) node at the end of element e (element e starts at node e)
     prev: np.ndarray         # (N,) element ending at node e
     contour_starts: list = field(default_factory=list)
+    impedance: np.ndarray | None = None   # (N,) complex surface impedance per element (0: PEC), or None
 
     @property
     def n(self) -> int:
@@ -175,6 +176,38 @@ def refined_chain_counts(base_counts, lengths, factor: float) -> list:
     return out
 
 
+def impedance_laws(snapshot: dict) -> dict:
+    """Inline constant surface impedances of the snapshot's ``ibcs`` rows,
+    ``[flag, "constant", R, X, R_end, X_end]`` (twod/geometry.py:261-312: inline rows are
+    ``<kind> R_start X_start R_end X_end``).  Tapered, file-backed and thin-layer laws are not
+    handled by this path and are refused when a segment uses them."""
+    laws = {}
+    for row in snapshot.get("ibcs", []) or []:
+        if not row:
+            continue
+        try:
+            flag = int(round(float(row[0])))
+        except (TypeError, ValueError):
+            raise ValueError(f"Invalid IBC material definition flag {row[0]!r}.")
+        if flag in laws:
+            raise ValueError(f"Duplicate IBC material flag {flag}.")
+        tokens = [str(t).strip() for t in row[1:] if str(t).strip() != ""]
+        if len(tokens) != 5 or tokens[0].lower() != "constant":
+            laws[flag] = None            # a law this path does not evaluate
+            continue
+        try:
+            z = complex(float(tokens[1]), float(tokens[2]))
+        except ValueError:
+            raise ValueError(f"IBC flag {flag}: impedance values must be numeric.")
+        if not (math.isfinite(z.real) and math.isfinite(z.imag)):
+            raise ValueError(f"IBC flag {flag}: impedance must be finite.")
+        if z.real < -1e-12 * max(1.0, abs(z)):
+            raise ValueError(f"IBC flag {flag} has negative resistance Re(Zs)={z.real:g} ohm. Active/gain "
+                             "surface impedances are not supported.")
+        laws[flag] = 0j if abs(z) <= EPS else z
+    return laws
+
+
 def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int = 20000) -> ContourMesh:
     """Panels and welded nodes of the PEC contours of ``snapshot`` at ``frequency_ghz``.
 
@@ -194,13 +227,24 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     if refine > 1.0 and len(base_n) != len(segments):
         raise ValueError("Internal 2-D certification refinement metadata does not match the geometry "
                          "segment count.")
-    panels0, panels1 = [], []
+    laws = impedance_laws(snapshot)
+    panels0, panels1, zs = [], [], []
     for s, seg in enumerate(segments):
         seg_type, n_prop, ibc = _parse_props(seg)
         name = str(seg.get("name", "segment"))
-        if seg_type != 2 or ibc != 0:
-            raise ValueError(f"Segment '{name}': only PEC (TYPE 2, no impedance) contours are supported "
-                             "by the GPU path.")
+        if seg_type != 2:
+            raise ValueError(f"Segment '{name}': only conductor (TYPE 2) contours are supported by the GPU path.")
+        z_seg = 0j
+        if ibc > 0:
+            if ibc not in laws:
+                raise ValueError(f"Segment '{name}' uses IBC flag {ibc}, which the snapshot does not define.")
+            if laws[ibc] is None:
+                raise ValueError(f"Segment '{name}': only constant inline surface impedances are supported by "
+                                 "the GPU path (no tapers, tables or thin layers).")
+            z_seg = laws[ibc]
+            if z_seg != 0 and n_prop <= 0:
+                raise ValueError(f"Segment '{name}': impedance segments need an explicit N > 0 here (the automatic "
+                                 "density follows the bound surface wave).")
         pairs = list(seg.get("point_pairs", []) or [])
         # TYPE 2: the user draws the normal into air; reverse the chain.
         pairs = [{"x1": p.get("x2", 0.0), "y1": p.get("y2", 0.0), "x2": p.get("x1", 0.0), "y2": p.get("y1", 0.0)}
@@ -242,6 +286,7 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
                     continue
                 panels0.append(pts[i])
                 panels1.append(pts[i + 1])
+                zs.append(z_seg)
     if not panels0:
         raise ValueError("Geometry does not contain any valid discretized panels.")
     p0 = np.array(panels0)
@@ -278,9 +323,29 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     prev = np.empty(n, np.int32)
     prev[nxt] = np.arange(n, dtype=np.int32)
     starts = [int(e) for e in range(n) if prev[e] != e - 1]
-    mesh = ContourMesh(np.array(nodes), p0, p1, nxt, prev, starts)
+    impedance = np.array(zs, complex)
+    mesh = ContourMesh(np.array(nodes), p0, p1, nxt, prev, starts,
+                       impedance if np.any(impedance != 0) else None)
     _check_orientation(mesh)
+    if mesh.impedance is not None:
+        _check_uniform_law(mesh)
     return mesh
+
+
+def _check_uniform_law(mesh: ContourMesh) -> None:
+    """Each closed contour carries one law (PEC or one constant impedance): the flux rows
+    Z_TE + alpha Z_TM hold exactly then; zones along a contour need element-weighted terms."""
+    seen = np.zeros(mesh.n, bool)
+    for s in range(mesh.n):
+        if seen[s]:
+            continue
+        e, law = s, mesh.impedance[s]
+        while not seen[e]:
+            seen[e] = True
+            if mesh.impedance[e] != law:
+                raise ValueError("Impedance zones along one closed contour are not supported by the GPU path: "
+                                 "each contour must be PEC or carry one constant surface impedance.")
+            e = int(mesh.next[e])
 
 
 def _check_orientation(mesh: ContourMesh) -> None:

@@ -62,6 +62,8 @@ def geometry_digest(mesh: ContourMesh) -> str:
     h = hashlib.sha256()
     for a in (mesh.p0, mesh.p1, mesh.next):
         h.update(np.ascontiguousarray(a).tobytes())
+    if mesh.impedance is not None:
+        h.update(np.ascontiguousarray(mesh.impedance).tobytes())
     return h.hexdigest()[:24]
 
 
@@ -219,7 +221,10 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
         if mesh.n not in ws:
             ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
         z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
-        fac = engine.factor(z_tm._base, req.precision)          # batch [TE, TM]
+        pair = z_tm._base                                         # batch [TE, TM]
+        if mesh.impedance is not None:
+            engine.impedance_systems(contour, k, pair)
+        fac = engine.factor(pair, req.precision)
         if store is not None:
             sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * mesh.n * mesh.n
             store({"contour": contour, "factor": fac}, sysbytes)
@@ -232,7 +237,8 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
             t.record_stream(cur)
     angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, angles)
-    rhs = torch.stack((-bdn, -bu))                             # [TE, TM]
+    rhs = (torch.stack((-bdn, -bu)) if mesh.impedance is None     # [TE, TM]
+           else engine.impedance_rhs(contour, k, bu, bdn))
     x, inf = engine.solve(fac, rhs, gate=(req.precision == "double"))
     rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
     amp, sig, resid, info = {}, {}, {}, {}

@@ -64,6 +64,16 @@ def _device(device):
     return torch.device(device)
 
 
+def _systems(contour, k, z_te, z_tm, bu, bdn):
+    """[TE, TM] systems (2, N, N) and right-hand sides (2, A, N): the PEC combined potential
+    (TM Dirichlet rows -b_u, TE flux rows -b_dn), or, for contours with a surface impedance,
+    the flux rows dn(u) + alpha u of both polarizations (engine.impedance_rows)."""
+    pair = z_tm._base if z_tm._base is not None else torch.stack((z_te, z_tm))   # [TE, TM]
+    if contour.mesh.impedance is None:
+        return pair, torch.stack((-bdn, -bu))
+    return engine.impedance_systems(contour, k, pair), engine.impedance_rhs(contour, k, bu, bdn)
+
+
 def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("TE", "TM"),
                     precision: str = "double", device=None, return_density: bool = False,
                     abort_event=None) -> Dict[str, Dict[str, Any]]:
@@ -76,10 +86,11 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     bu, bdn = engine.loads(contour, k, angles)
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
-    pair = z_tm._base if z_tm._base is not None else torch.stack((z_te, z_tm))   # [TE, TM]
+    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn)
     order = [p for p in ("TE", "TM") if p in pols]
-    systems = pair if len(order) == 2 else (z_te if order[0] == "TE" else z_tm).unsqueeze(0)
-    rhs = torch.stack([-(bdn if p == "TE" else bu) for p in order])
+    sel = [("TE", "TM").index(p) for p in order]
+    systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
+    rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
     fac = engine.factor(systems.contiguous(), precision)
     x, info = engine.solve(fac, rhs)
     rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
@@ -109,10 +120,11 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
     bu, bdn = engine.loads(contour, k, inc)
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
-    pair = z_tm._base if z_tm._base is not None else torch.stack((z_te, z_tm))   # [TE, TM]
+    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn)
     order = [p for p in ("TE", "TM") if p in pols]
-    systems = pair if len(order) == 2 else (z_te if order[0] == "TE" else z_tm).unsqueeze(0)
-    rhs = torch.stack([-(bdn if p == "TE" else bu) for p in order])
+    sel = [("TE", "TM").index(p) for p in order]
+    systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
+    rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
     fac = engine.factor(systems.contiguous(), precision)
     x, info = engine.solve(fac, rhs)
     rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))

@@ -146,6 +146,21 @@ int grim_to_c64(int64_t batch, int64_t m, int64_t n, const double* src, int64_t 
 int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds, int64_t stride_src,
                    double* dst, int64_t ldd, int64_t stride_dst, void* stream);
 
+/* Impedance (Robin) rows of closed contours with a surface impedance that is
+ * constant on each contour, in place: x <- x + alpha_x[r] y on every row r;
+ * y <- y on rows with dirichlet[r] != 0 and y <- x + alpha_y[r] y elsewhere
+ * (both right-hand sides from the original x, y).  x, y: complex128 rows x
+ * cols, column-major (cols <= 65535); alpha_x, alpha_y: complex128 [rows];
+ * dirichlet: int32 [rows].  With x = Z_TE, y = Z_TM (or x = b_dn, y = b_u),
+ * alpha = j k Zs / eta (TE) and j k eta / Zs (TM), and dirichlet marking TM
+ * rows of PEC contours, this forms the flux rows dn(u) + alpha u of the
+ * combined potential: Z_TE + alpha Z_TM.  Replaces the per-node routing of
+ * twod/formulations/combined_regions.py:43-62 and the Robin system of
+ * twod/formulations/robin.py:25-83 for such laws (Robin coefficient:
+ * twod/geometry.py:2488-2521). */
+int grim_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* y, int64_t ldy,
+                    const double* alpha_x, const double* alpha_y, const int32_t* dirichlet, void* stream);
+
 /* Launch shapes (no counterpart in the reference; for a project autotuner and
  * callers that pin their own shapes).  Knob 0: far-pass source columns per
  * wavefront (1, 2, 4, 8, 16 or 32), keyed by n; knob 1: triangular-solve columns per

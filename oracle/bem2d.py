@@ -38,6 +38,13 @@ What each function follows in the reference (paths under ``tools/GHOST/ghost_bac
                           reuses the RHS loads), twod/solver.py:838-873 (sigma, dB).
 * ``refined_solve``       linalg/refined_lu.py:41-61 (complex64 LU + FP64
                           residual refinement to 2e-12).
+* ``impedance_system``    twod/formulations/combined_regions.py:1-62 (flux rows
+                          dn(u) + alpha u route -gamma W and gamma alpha D; alpha
+                          weights S and the trace-jump mass), robin.py:25-83
+                          (-M/2 + K' + alpha S base rows), twod/geometry.py:2488-2521
+                          (alpha_TM = j k eta/Zs, alpha_TE = j k Zs/eta) and
+                          twod/solver.py:1810-1825 (RHS -(b_dn + alpha b_u); -b_u on
+                          TM PEC rows), for one constant law per contour.
 
 Pinning (see tests/test_oracle_*.py): every function above is checked against
 the reference's own outputs (tests/golden/*.npz, written by
@@ -607,3 +614,25 @@ def solve_monostatic(mesh: Mesh, frequency_ghz: float, angles_deg, pols=("TE", "
         sig = sigma_from_amplitude(amp, k)
         out[pol] = dict(density=x, amplitude=amp, sigma=sig, db=db_from_sigma(sig), matrix=z, rhs=b)
     return out
+
+
+ETA0 = 376.730313668
+
+
+def impedance_system(z_tm, z_te, bu, bdn, node_impedance, k: float, pol: str):
+    """System and right-hand sides of contours with a constant surface impedance per contour.
+
+    A flux row carries dn(u) + alpha u: with the law constant along the contour its operator rows
+    are -M/2 + K' - gamma W + alpha (S + gamma (D + M/2)) = Z_TE + alpha Z_TM and its load is
+    -(b_dn + alpha b_u).  TM rows of PEC contours (Zs = 0) keep the Dirichlet row Z_TM, -b_u."""
+    z = np.asarray(node_impedance, complex)
+    ibc = np.abs(z) > EPS
+    safe = np.where(ibc, z, 1.0)
+    alpha = np.where(ibc, 1j * k * (ETA0 / safe if pol == "TM" else safe / ETA0), 0.0)
+    flux = z_te + alpha[:, None] * z_tm
+    load = -(bdn + alpha[:, None] * bu)
+    if pol == "TM":
+        dirichlet = ~ibc
+        flux = np.where(dirichlet[:, None], z_tm, flux)
+        load = np.where(dirichlet[:, None], -bu, load)
+    return flux, load

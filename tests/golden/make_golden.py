@@ -15,7 +15,7 @@ reference factors it.  Outputs are small ``.npz`` fixtures next to this file:
 inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
-Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic]
+Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc]
 """
 from __future__ import annotations
 
@@ -345,6 +345,54 @@ def main_bistatic():
                       [0.0, 45.0], [0.0, 45.0, 100.0, 180.0, 270.0])
 
 
+def run_ibc_case(name, snapshot, frequency_ghz, angles):
+    """The reference's co-polarized monostatic solve of a snapshot with impedance (IBC) segments:
+    the sample amplitudes and dB per channel."""
+    import ghost_backend.twod.solver as rcs
+    result = rcs.solve_monostatic_rcs_2d(snapshot, frequencies_ghz=[frequency_ghz],
+                                         elevations_deg=list(map(float, angles)), geometry_units="meters",
+                                         strict_quality_gate=False, max_panels=100000, solver_method="direct",
+                                         execution_options=EXECUTION)
+    out = {"snapshot_json": np.array(json.dumps(snapshot)), "frequency_ghz": float(frequency_ghz),
+           "angles_deg": np.asarray(angles, float)}
+    for export in ("VV", "HH"):
+        rows = result["co_solved_samples"][export]
+        out[f"amp_{export}"] = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+        out[f"db_{export}"] = np.array([r["rcs_db"] for r in rows])
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print(f"{name}: {result['metadata'].get('formulation')} panels {result['metadata'].get('panel_count')}",
+          flush=True)
+
+
+def _ibc_snapshot(contours):
+    """contours: list of (points, N, impedance or None); one IBC flag per distinct impedance."""
+    segments, ibcs = [], []
+    for i, (pts, n_prop, z) in enumerate(contours):
+        seg = workloads._snapshot(pts, f"c{i}")["segments"][0]
+        flag = 0
+        if z is not None:
+            flag = len(ibcs) + 1
+            ibcs.append([str(flag), "constant", repr(z.real), repr(z.imag), repr(z.real), repr(z.imag)])
+        seg["properties"] = ["2", str(n_prop), str(flag), "0", "0"]
+        segments.append(seg)
+    return {"segments": segments, "ibcs": ibcs, "dielectrics": []}
+
+
+def main_ibc():
+    r = 0.1
+    f = workloads.frequency_for_ka(1.0, r)
+    run_ibc_case("ibc_circle64_ka1", _ibc_snapshot([(workloads.circle_points(r, 64), 1, complex(150.0, -80.0))]),
+                 f, [0.0, 45.0, 90.0, 180.0])
+    run_ibc_case("ibc_circle64_resistive", _ibc_snapshot([(workloads.circle_points(r, 64), 1, complex(376.0, 0.0))]),
+                 f, [0.0, 30.0, 200.0])
+    two = [(workloads.circle_points(0.1, 24), 1, None),
+           (workloads.circle_points(0.1, 40) + np.array([0.35, 0.05]), 1, complex(60.0, 30.0))]
+    run_ibc_case("ibc_pec_and_ibc", _ibc_snapshot(two), 1.3, [0.0, 90.0, 135.0, 270.0])
+    pts = workloads.naca0012_points(40)
+    run_ibc_case("ibc_naca80", _ibc_snapshot([(pts, 1, complex(20.0, 200.0))]),
+                 workloads.frequency_for_segments_per_wavelength(pts, 10.0), [0.0, 45.0, 180.0, 333.0])
+
+
 if __name__ == "__main__":
     if "--certified" in sys.argv:
         main_certified()
@@ -354,5 +402,7 @@ if __name__ == "__main__":
         main_grim()
     elif "--bistatic" in sys.argv:
         main_bistatic()
+    elif "--ibc" in sys.argv:
+        main_ibc()
     else:
         main()
