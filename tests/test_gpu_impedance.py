@@ -58,3 +58,19 @@ def test_serving_impedance_request_matches_public_entry():
         ref = z[f"amp_{export}"]
         assert np.max(np.abs(got.amplitude[internal] - ref) / np.abs(ref)) < 1e-10
         assert np.max(np.abs(again.amplitude[internal] - ref[:2]) / np.abs(ref[:2])) < 1e-10
+
+
+def test_impedance_bistatic_diagonal_is_monostatic():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from grim_20261007_amd.solver import solve_bistatic_rcs_2d
+    snap, f, angles, z = _case("ibc_pec_and_ibc")
+    res = solve_bistatic_rcs_2d(snap, [f], angles, angles, geometry_units="meters", max_panels=100000)
+    n = len(angles)
+    for export in ("VV", "HH"):
+        rows = res["co_solved_samples"][export]
+        grid = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows]).reshape(n, n)
+        ref = z[f"amp_{export}"]
+        err = np.max(np.abs(np.diagonal(grid) - ref) / np.abs(ref))
+        print(f"bistatic diagonal {export}: {err:.2e}", flush=True)
+        assert err < 1e-10
