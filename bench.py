@@ -412,7 +412,7 @@ def apply_tuning(choice):
         return None
     table = autotune.tune() if choice == "tune" else autotune.resolve(choice)
     autotune.apply(table)
-    return {"far_columns": table["far_columns"], "trsm_columns": table["trsm_columns"], "source": choice}
+    return {name: table[name] for name in autotune.KNOB_NAMES.values() if name in table} | {"source": choice}
 
 
 def main():

@@ -1,13 +1,16 @@
 """Launch-shape autotuner for the HIP library.
 
-Two launch shapes of the request path depend on the problem size and the device:
+Three launch shapes of the request path depend on the problem size and the device:
 
 * far-pass source columns per wavefront (knob 0, keyed by n): few columns give
   more wavefronts (better fill of the 1024 SIMDs at small n), more columns
   re-evaluate fewer chunk-boundary pairs;
-* triangular-solve columns per wavefront (knob 1, keyed by the right-hand-side
-  columns of one call times its batch): more columns give more independent
-  chains per wavefront but fewer wavefronts.
+* columns per wavefront of the factorization's 64-row triangular solves (knob 1,
+  keyed by the columns of one call times its batch): more columns give more
+  independent chains per wavefront but fewer wavefronts;
+* columns per wavefront of the LU solve's 256-row block solves (knob 2, keyed by
+  the right-hand-side columns times the batch): the same trade, against how many
+  workgroups fit on the device at once.
 
 ``tune`` times every candidate on the current device with HIP events, on
 synthetic star-shaped contours of the requested sizes, and returns a table;
@@ -30,11 +33,12 @@ from . import engine, native, workloads
 from .mesh import build_mesh
 from .solver import wavenumber
 
-FAR_COLUMNS, TRSM_COLUMNS = 0, 1
-# table measured by tune() on an MI355X (far pass n = 250-8000, solves of 90-1440 angles)
+FAR_COLUMNS, TRSM_COLUMNS, SOLVE_COLUMNS = 0, 1, 2
+# table measured by tune() on an MI355X (far pass n = 250-8000, factors of n = 2000, solves of
+# 90-1440 angles)
 MI355X_TABLE = Path(__file__).resolve().parent / "tuning_mi355x.json"
-KNOB_NAMES = {FAR_COLUMNS: "far_columns", TRSM_COLUMNS: "trsm_columns"}
-CHOICES = {FAR_COLUMNS: (1, 2, 4, 8, 16, 32), TRSM_COLUMNS: (1, 2, 4)}
+KNOB_NAMES = {FAR_COLUMNS: "far_columns", TRSM_COLUMNS: "trsm_columns", SOLVE_COLUMNS: "solve_columns"}
+CHOICES = {FAR_COLUMNS: (1, 2, 4, 8, 16, 32), TRSM_COLUMNS: (1, 2, 4), SOLVE_COLUMNS: (1, 2, 4)}
 
 
 def set_shape(knob: int, min_size: int, value: int) -> None:
@@ -107,27 +111,53 @@ def tune_far(sizes: Iterable[int], device=None, repeats: int = 5) -> Dict[int, D
     return times
 
 
-def tune_trsm(column_counts: Iterable[int], n: int = 2000, device=None, repeats: int = 5
+def _test_matrix(n: int, dev, g) -> torch.Tensor:
+    z = torch.randn((2, n, n), dtype=torch.complex128, generator=g)
+    z += n * torch.eye(n, dtype=torch.complex128)      # well conditioned: timing only
+    return z.to(dev)
+
+
+def tune_trsm(n: int = 2000, precisions: Sequence[str] = ("double", "mixed"), device=None, repeats: int = 3
               ) -> Dict[int, Dict[int, float]]:
-    """FP64 two-system LU solve time (ms) per candidate columns-per-wavefront, for right-hand
-    sides of A columns (the call then has 2 A columns: both polarizations)."""
+    """Two-system LU factor time (ms, summed over `precisions`) per candidate columns-per-wavefront
+    of the factorization's triangular solves.  One factor calls them with every column count from
+    2 (n - 64) down, so the result is one table entry, under key 0."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", 0)
+    native.ensure_device(dev.index or 0)
+    z = _test_matrix(n, dev, torch.Generator(device="cpu").manual_seed(11))
+    times: Dict[int, Dict[int, float]] = {0: {}}
+    for value in CHOICES[TRSM_COLUMNS]:
+        set_shape(TRSM_COLUMNS, 0, value)
+        try:
+            times[0][value] = sum(_time(lambda: engine.factor(z, p), repeats) for p in precisions)
+        finally:
+            set_shape(TRSM_COLUMNS, 0, 0)
+    return times
+
+
+def tune_solve(column_counts: Iterable[int], n: int = 2000, device=None, repeats: int = 5
+               ) -> Dict[int, Dict[int, float]]:
+    """FP64 plus complex64 two-system LU solve time (ms) per candidate columns-per-wavefront of
+    the block solves, for right-hand sides of A columns (the call then has 2 A columns: both
+    polarizations)."""
     dev = torch.device(device) if device is not None else torch.device("cuda", 0)
     native.ensure_device(dev.index or 0)
     g = torch.Generator(device="cpu").manual_seed(11)
-    z = torch.randn((2, n, n), dtype=torch.complex128, generator=g)
-    z += n * torch.eye(n, dtype=torch.complex128)      # well conditioned: timing only
-    fac = engine.factor(z.to(dev), "double")
+    z = _test_matrix(n, dev, g)
+    facs = [engine.factor(z, p) for p in ("double", "mixed")]
     times: Dict[int, Dict[int, float]] = {}
     for na in sorted(set(int(v) for v in column_counts)):
         b = torch.randn((2, na, n), dtype=torch.complex128, generator=g).to(dev)
+        rhss = [b.to(f.lu.dtype) for f in facs]          # each factor solves in its own precision
         cols = 2 * na
         times[cols] = {}
-        for value in CHOICES[TRSM_COLUMNS]:
-            set_shape(TRSM_COLUMNS, 0, value)
+        for value in CHOICES[SOLVE_COLUMNS]:
+            set_shape(SOLVE_COLUMNS, 0, value)
             try:
-                times[cols][value] = _time(lambda: engine.lu_solve_inplace(fac, b.clone()), repeats)
+                times[cols][value] = sum(_time(lambda: engine.lu_solve_inplace(f, rhs.clone()), repeats)
+                                         for f, rhs in zip(facs, rhss))
             finally:
-                set_shape(TRSM_COLUMNS, 0, 0)
+                set_shape(SOLVE_COLUMNS, 0, 0)
     return times
 
 
@@ -143,15 +173,18 @@ def _table_from(times: Dict[int, Dict[int, float]]) -> List[List[int]]:
 
 
 def tune(far_sizes: Sequence[int] = (500, 1000, 2000, 4000, 8000),
-         angle_counts: Sequence[int] = (90, 360, 720, 1440), device=None, repeats: int = 5) -> dict:
+         angle_counts: Sequence[int] = (90, 360, 720, 1440), factor_n: int = 2000, device=None,
+         repeats: int = 5) -> dict:
     """Measure and return a launch-shape table for this device (does not apply it)."""
-    far = tune_far(far_sizes, device, repeats)
-    trsm = tune_trsm(angle_counts, device=device, repeats=repeats)
+    measured = {"far_columns": tune_far(far_sizes, device, repeats),
+                "trsm_columns": tune_trsm(factor_n, device=device, repeats=max(1, repeats // 2)),
+                "solve_columns": tune_solve(angle_counts, device=device, repeats=repeats)}
     name = torch.cuda.get_device_name(torch.device(device) if device is not None else None)
-    return {"device": name,
-            "far_columns": _table_from(far), "trsm_columns": _table_from(trsm),
-            "measured_ms": {"far_columns": {str(n): {str(v): t for v, t in d.items()} for n, d in far.items()},
-                            "trsm_columns": {str(c): {str(v): t for v, t in d.items()} for c, d in trsm.items()}}}
+    out = {"device": name, "measured_ms": {}}
+    for key, times in measured.items():
+        out[key] = _table_from(times)
+        out["measured_ms"][key] = {str(n): {str(v): t for v, t in d.items()} for n, d in times.items()}
+    return out
 
 
 def apply(table: dict) -> None:

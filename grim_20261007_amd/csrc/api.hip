@@ -81,7 +81,7 @@ std::mutex g_shape_lock;
 
 bool shape_allowed(int knob, int value) {
     if (knob == GRIM_SHAPE_FAR_COLUMNS) return value == 1 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32;
-    if (knob == GRIM_SHAPE_TRSM_COLUMNS) return value == 1 || value == 2 || value == 4;
+    if (knob == GRIM_SHAPE_TRSM_COLUMNS || knob == GRIM_SHAPE_SOLVE_COLUMNS) return value == 1 || value == 2 || value == 4;
     return false;
 }
 }  // namespace

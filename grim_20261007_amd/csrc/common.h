@@ -32,10 +32,17 @@ enum {
 };
 
 // Launch-shape knobs (grim_set_launch_shape): far-pass source columns per
-// wavefront, keyed by n; triangular-solve columns per wavefront, keyed by the
-// total right-hand-side columns of the call.  Shapes change only how work is
-// spread over wavefronts: every entry is computed by the same operations.
-enum { GRIM_SHAPE_FAR_COLUMNS = 0, GRIM_SHAPE_TRSM_COLUMNS = 1, GRIM_SHAPE_KNOBS = 2 };
+// wavefront, keyed by n; columns per wavefront of the factorization's 64-row
+// triangular solves, keyed by the total columns of the call; columns per
+// wavefront of the LU solve's 256-row block solves, keyed by the total
+// right-hand-side columns.  Shapes change only how work is spread over
+// wavefronts: every entry is computed by the same operations.
+enum {
+    GRIM_SHAPE_FAR_COLUMNS = 0,
+    GRIM_SHAPE_TRSM_COLUMNS = 1,
+    GRIM_SHAPE_SOLVE_COLUMNS = 2,
+    GRIM_SHAPE_KNOBS = 3
+};
 // The table's value for `size` (largest entry with min_size <= size), 0 = built-in rule.
 extern "C" int grim_shape_value(int knob, int64_t size);
 

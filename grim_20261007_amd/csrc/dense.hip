@@ -780,9 +780,8 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// X = T^{-1} B in place for a w x w triangle (w <= 64) and ncols columns:
-// LOWER: unit lower triangle; else upper with its diagonal, solved as the unit
-// upper triangle D^{-1} U after scaling B by D^{-1}.  Lane i of a wave holds
+// X = L^{-1} B in place for a w x w unit lower triangle (w <= 64) and ncols
+// columns (the factorization's U12 = L11^{-1} A12 steps).  Lane i of a wave holds
 // row i of CPW columns; step j broadcasts x_j from lane j with v_readlane
 // (wave-uniform, so it lands in SGPRs) and every lane subtracts T[i][j] x_j.
 // The strict triangle is staged transposed in LDS (zeros elsewhere, so the
@@ -791,42 +790,20 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 // to the top of B) each wave first applies the panel's row interchanges to its
 // columns of B: it reads every source row, writes the lower destinations and
 // solves on the permuted top rows, so no separate row-swap pass is needed.
-template <typename T, bool LOWER, int CPW>
+template <typename T, int CPW>
 __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __restrict__ tri, int64_t ldt, int64_t st,
                                                   T* __restrict__ b, int64_t ldb, int64_t sb,
                                                   const int* __restrict__ list, int64_t slist, int list_base) {
     __shared__ T tr[GRIM_LU_NB][GRIM_LU_NB], ti[GRIM_LU_NB][GRIM_LU_NB];   // [col j][row i]
-    __shared__ T dinv_r[GRIM_LU_NB], dinv_i[GRIM_LU_NB];
     tri += 2 * st * blockIdx.z;
     b += 2 * sb * blockIdx.z;
-    if (!LOWER) {
-        if (threadIdx.x < GRIM_LU_NB) {
-            const int j = threadIdx.x;
-            T ir = 0, ii = 0;
-            if (j < w) {
-                const T* p = tri + 2 * ((int64_t)j + (int64_t)j * ldt);
-                const T dr = p[0], di = p[1];
-                const T den = dr * dr + di * di;
-                ir = dr / den;
-                ii = -di / den;
-            }
-            dinv_r[j] = ir;
-            dinv_i[j] = ii;
-        }
-        __syncthreads();
-    }
     for (int idx = threadIdx.x; idx < GRIM_LU_NB * GRIM_LU_NB; idx += 256) {
         const int i = idx & 63, j = idx >> 6;
         T x = 0, y = 0;
-        if (i < w && j < w && (LOWER ? i > j : i < j)) {
+        if (i < w && j < w && i > j) {
             const T* p = tri + 2 * ((int64_t)i + (int64_t)j * ldt);
             x = p[0];
             y = p[1];
-            if (!LOWER) {   // row i of D^{-1} U
-                const T sr = x * dinv_r[i] - y * dinv_i[i], si = x * dinv_i[i] + y * dinv_r[i];
-                x = sr;
-                y = si;
-            }
         }
         tr[j][i] = x;
         ti[j][i] = y;
@@ -873,32 +850,13 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
         }
     }
     __syncthreads();
-    if (LOWER) {
-        for (int j = 0; j < w - 1; ++j) {
-            const T lr = tr[j][lane], li = ti[j][lane];
-#pragma unroll
-            for (int c = 0; c < CPW; ++c) {
-                const T ur = lane_value(xr[c], j), ui = lane_value(xi[c], j);
-                xr[c] -= lr * ur - li * ui;
-                xi[c] -= lr * ui + li * ur;
-            }
-        }
-    } else {
-        const T ir = dinv_r[lane], ii = dinv_i[lane];
+    for (int j = 0; j < w - 1; ++j) {
+        const T lr = tr[j][lane], li = ti[j][lane];
 #pragma unroll
         for (int c = 0; c < CPW; ++c) {
-            const T br = xr[c], bi = xi[c];
-            xr[c] = br * ir - bi * ii;
-            xi[c] = br * ii + bi * ir;
-        }
-        for (int j = w - 1; j > 0; --j) {
-            const T lr = tr[j][lane], li = ti[j][lane];
-#pragma unroll
-            for (int c = 0; c < CPW; ++c) {
-                const T ur = lane_value(xr[c], j), ui = lane_value(xi[c], j);
-                xr[c] -= lr * ur - li * ui;
-                xi[c] -= lr * ui + li * ur;
-            }
+            const T ur = lane_value(xr[c], j), ui = lane_value(xi[c], j);
+            xr[c] -= lr * ur - li * ui;
+            xi[c] -= lr * ui + li * ur;
         }
     }
 #pragma unroll
@@ -907,6 +865,194 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
             T* p = b + 2 * ((int64_t)lane + (int64_t)(col0 + c) * ldb);
             p[0] = xr[c];
             p[1] = xi[c];
+        }
+    }
+}
+
+// One SOLVE_CHUNK-column chunk of a step's coupling rows, spread over the workgroup: thread entry k
+// is e = threadIdx.x + 256 k, row e % CROWS of the coupling rows (from row c_lo of the block),
+// column j0 + e / CROWS of the step's sub-block (from column r0).  Zero outside nrow x w.
+#define SOLVE_CHUNK 16
+template <typename T, int CPT, int CROWS>
+__device__ __forceinline__ void fetch_coupling(const T* __restrict__ a, int64_t lda, int c_lo, int nrow, int r0,
+                                               int j0, int w, T (&pr)[CPT], T (&pi)[CPT]) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int e = threadIdx.x + 256 * k;
+        const int row = e % CROWS, col = j0 + e / CROWS;
+        T2 v = {0, 0};
+        if (row < nrow && col < w)
+            v = *reinterpret_cast<const T2*>(a + 2 * ((int64_t)(c_lo + row) + (int64_t)(r0 + col) * lda));
+        pr[k] = v.x;
+        pi[k] = v.y;
+    }
+}
+
+// Complex updates of the block solve with one fixed rounding sequence (explicit FMAs, no
+// compiler contraction), so that every launch shape gives bit-identical results.
+template <typename T>
+__device__ __forceinline__ void csub_mul(T& xr, T& xi, T ar, T ai, T ur, T ui) {   // x -= a u
+#pragma clang fp contract(off)
+    xr -= fma(ar, ur, -(ai * ui));
+    xi -= fma(ar, ui, ai * ur);
+}
+template <typename T>
+__device__ __forceinline__ void cmul_by(T& xr, T& xi, T ar, T ai) {   // x *= a
+#pragma clang fp contract(off)
+    const T br = xr, bi = xi;
+    xr = fma(br, ar, -(bi * ai));
+    xi = fma(br, ai, bi * ar);
+}
+
+// X = T^{-1} B in place for one diagonal block of up to GRIM_SOLVE_NB = 256 rows and ncols
+// columns, in one launch (the in-block steps of lu_solve).  Each wave owns CPW columns and all
+// rows of the block: lane l holds rows l, 64 + l, 128 + l and 192 + l.  Sub-block by sub-block
+// (top-down for the unit lower triangle, bottom-up for the upper one) the 64 x 64 diagonal
+// triangle is staged in LDS and shared by the four waves (the upper one as the unit triangle
+// D^{-1} U, with B scaled by D^{-1} first); each wave solves its columns' chain as trsm_lanes
+// does (x_j broadcast from lane j with v_readlane), then subtracts the coupling blocks of the
+// sub-blocks still to come times the new rows, broadcast the same way; the coupling blocks
+// pass through LDS in SOLVE_CHUNK-column chunks loaded by the whole workgroup.
+template <typename T, bool LOWER, int CPW>
+__global__ void __launch_bounds__(256) solve_block(int nb, int ncols, const T* __restrict__ a, int64_t lda,
+                                                   int64_t sa, T* __restrict__ b, int64_t ldb, int64_t sb) {
+    constexpr int NS = GRIM_SOLVE_NB / GRIM_LU_NB;
+    constexpr int CROWS = GRIM_SOLVE_NB - GRIM_LU_NB;              // coupling rows of one step, at most
+    constexpr int CPT = CROWS * SOLVE_CHUNK / 256;                 // chunk entries per thread
+    __shared__ T tr[GRIM_LU_NB][GRIM_LU_NB], ti[GRIM_LU_NB][GRIM_LU_NB];   // [col j][row i]
+    __shared__ T dinv_r[GRIM_LU_NB], dinv_i[GRIM_LU_NB];
+    __shared__ T cp_r[SOLVE_CHUNK][CROWS], cp_i[SOLVE_CHUNK][CROWS];       // coupling chunk [col][row]
+    a += 2 * sa * blockIdx.z;
+    b += 2 * sb * blockIdx.z;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col0 = (blockIdx.x * 4 + wave) * CPW;
+    const int nsub = (nb + 63) / 64;
+    T xr[NS][CPW], xi[NS][CPW];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        const int row = 64 * t + lane;
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            xr[t][c] = 0;
+            xi[t][c] = 0;
+            if (row < nb && col0 + c < ncols) {
+                const T* p = b + 2 * ((int64_t)row + (int64_t)(col0 + c) * ldb);
+                xr[t][c] = p[0];
+                xi[t][c] = p[1];
+            }
+        }
+    }
+#pragma unroll
+    for (int step = 0; step < NS; ++step) {
+        const int s = LOWER ? step : NS - 1 - step;
+        if (s >= nsub) continue;   // same for the whole workgroup
+        const int r0 = 64 * s;
+        const int w = min(64, nb - r0);
+        const T* d = a + 2 * ((int64_t)r0 + (int64_t)r0 * lda);
+        __syncthreads();   // every wave is done with the previous triangle
+        if (!LOWER) {
+            if (threadIdx.x < GRIM_LU_NB) {
+                const int j = threadIdx.x;
+                T ir = 0, ii = 0;
+                if (j < w) {
+                    const T* p = d + 2 * ((int64_t)j + (int64_t)j * lda);
+                    const T dr = p[0], di = p[1];
+                    const T den = fma(dr, dr, di * di);
+                    ir = dr / den;
+                    ii = -di / den;
+                }
+                dinv_r[j] = ir;
+                dinv_i[j] = ii;
+            }
+            __syncthreads();
+        }
+        for (int idx = threadIdx.x; idx < GRIM_LU_NB * GRIM_LU_NB; idx += 256) {
+            const int i = idx & 63, j = idx >> 6;
+            T x = 0, y = 0;
+            if (i < w && j < w && (LOWER ? i > j : i < j)) {
+                const T* p = d + 2 * ((int64_t)i + (int64_t)j * lda);
+                x = p[0];
+                y = p[1];
+                if (!LOWER) cmul_by(x, y, dinv_r[i], dinv_i[i]);   // row i of D^{-1} U
+            }
+            tr[j][i] = x;
+            ti[j][i] = y;
+        }
+        __syncthreads();
+        // coupling rows of this step: [r0 + 64, nb) below (lower) or [0, r0) above (upper); the
+        // first column chunk of them is fetched while the chain runs
+        const int c_lo = LOWER ? r0 + GRIM_LU_NB : 0;
+        const int nrow = LOWER ? nb - c_lo : r0;
+        const int nchunk = nrow > 0 ? (w + SOLVE_CHUNK - 1) / SOLVE_CHUNK : 0;
+        T pr[CPT], pi[CPT];
+        if (nchunk > 0) fetch_coupling<T, CPT, CROWS>(a, lda, c_lo, nrow, r0, 0, w, pr, pi);
+        // chain over the 64 x 64 triangle (entries from row or column w on are zero)
+        if (LOWER) {
+#pragma unroll 8
+            for (int j = 0; j < GRIM_LU_NB - 1; ++j) {
+                const T lr = tr[j][lane], li = ti[j][lane];
+#pragma unroll
+                for (int c = 0; c < CPW; ++c)
+                    csub_mul(xr[s][c], xi[s][c], lr, li, lane_value(xr[s][c], j), lane_value(xi[s][c], j));
+            }
+        } else {
+            const T ir = dinv_r[lane], ii = dinv_i[lane];
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) cmul_by(xr[s][c], xi[s][c], ir, ii);
+#pragma unroll 8
+            for (int j = GRIM_LU_NB - 1; j > 0; --j) {
+                const T lr = tr[j][lane], li = ti[j][lane];
+#pragma unroll
+                for (int c = 0; c < CPW; ++c)
+                    csub_mul(xr[s][c], xi[s][c], lr, li, lane_value(xr[s][c], j), lane_value(xi[s][c], j));
+            }
+        }
+        // coupling: x_t -= A_ts x_s for the sub-blocks t still to be solved, one LDS-staged
+        // chunk of columns at a time (the next chunk's loads in flight meanwhile)
+#pragma unroll
+        for (int q = 0; q < GRIM_LU_NB / SOLVE_CHUNK; ++q) {
+            if (q >= nchunk) break;   // same for the whole workgroup
+            __syncthreads();          // the previous chunk's readers are done
+#pragma unroll
+            for (int k = 0; k < CPT; ++k) {
+                const int e = threadIdx.x + 256 * k;
+                cp_r[e / CROWS][e % CROWS] = pr[k];
+                cp_i[e / CROWS][e % CROWS] = pi[k];
+            }
+            __syncthreads();
+            if (q + 1 < nchunk)
+                fetch_coupling<T, CPT, CROWS>(a, lda, c_lo, nrow, r0, (q + 1) * SOLVE_CHUNK, w, pr, pi);
+#pragma unroll 4
+            for (int jj = 0; jj < SOLVE_CHUNK; ++jj) {
+                const int j = q * SOLVE_CHUNK + jj;
+                T ur[CPW], ui[CPW];
+#pragma unroll
+                for (int c = 0; c < CPW; ++c) {
+                    ur[c] = lane_value(xr[s][c], j);
+                    ui[c] = lane_value(xi[s][c], j);
+                }
+#pragma unroll
+                for (int t = 0; t < NS; ++t) {
+                    if (LOWER ? t <= s : t >= s) continue;
+                    const int rl = GRIM_LU_NB * t + lane - c_lo;   // row within the coupling rows
+                    const T cr = cp_r[jj][rl], ci = cp_i[jj][rl];
+#pragma unroll
+                    for (int c = 0; c < CPW; ++c) csub_mul(xr[t][c], xi[t][c], cr, ci, ur[c], ui[c]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        const int row = 64 * t + lane;
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            if (row < nb && col0 + c < ncols) {
+                T* p = b + 2 * ((int64_t)row + (int64_t)(col0 + c) * ldb);
+                p[0] = xr[t][c];
+                p[1] = xi[t][c];
+            }
         }
     }
 }
@@ -1051,14 +1197,14 @@ static int leaf_width(int m) {
     return GRIM_TALL_TW;
 }
 
-template <typename T, bool LOWER>
+template <typename T>
 static hipError_t trsm(const Batch& bt, int w, int ncols, const T* tri, int64_t ldt, int64_t st, T* b, int64_t ldb,
                        int64_t sb, const int* list = nullptr, int64_t slist = 0, int list_base = 0) {
     if (w <= 0 || ncols <= 0) return hipSuccess;
     // columns per wave: enough waves to cover the chip first, then ILP
     const int64_t cols = (int64_t)ncols * bt.count;
 #define GRIM_TRSM(CPW)                                                                                            \
-    trsm_lanes<T, LOWER, CPW><<<dim3((ncols + 4 * CPW - 1) / (4 * CPW), 1, bt.count), 256, 0, bt.st>>>(        \
+    trsm_lanes<T, CPW><<<dim3((ncols + 4 * CPW - 1) / (4 * CPW), 1, bt.count), 256, 0, bt.st>>>(        \
         w, ncols, tri, ldt, st, b, ldb, sb, list, slist, list_base)
     int cpw = grim_shape_value(GRIM_SHAPE_TRSM_COLUMNS, cols);
     if (cpw == 0) cpw = cols >= 8192 ? 4 : cols >= 2048 ? 2 : 1;
@@ -1089,7 +1235,7 @@ static hipError_t panel_rec(const Batch& bt, int n, T* a, int64_t lda, int64_t s
     if (err != hipSuccess) return err;
     const int w2 = w - w1;
     T* right = a + 2 * ((int64_t)(j0 + w1) * lda);
-    if ((err = trsm<T, true>(bt, w1, w2, top, lda, sa, right + 2 * (int64_t)j0, lda, sa)) != hipSuccess) return err;
+    if ((err = trsm<T>(bt, w1, w2, top, lda, sa, right + 2 * (int64_t)j0, lda, sa)) != hipSuccess) return err;
     if ((err = gemm<T>(bt, m - w1, w2, w1, top + 2 * (int64_t)w1, lda, sa, right + 2 * (int64_t)j0, lda, sa,
                        right + 2 * (int64_t)(j0 + w1), lda, sa)) != hipSuccess)
         return err;
@@ -1132,7 +1278,7 @@ static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t s
             T* diag = a + 2 * ((int64_t)j + (int64_t)j * lda);
             T* a12 = a + 2 * ((int64_t)j + (int64_t)(j + jb) * lda);
             // interchanges on the right-hand columns fused with U12 = L11^{-1} A12, then A22 -= L21 U12
-            if ((err = trsm<T, true>(bt, jb, rest, diag, lda, sa, a12, lda, sa, list, sperm, j)) != hipSuccess)
+            if ((err = trsm<T>(bt, jb, rest, diag, lda, sa, a12, lda, sa, list, sperm, j)) != hipSuccess)
                 return err;
             if ((err = gemm<T>(bt, rest, rest, jb, diag + 2 * (int64_t)jb, lda, sa, a12, lda, sa,
                                a12 + 2 * (int64_t)jb, lda, sa)) != hipSuccess)
@@ -1143,6 +1289,30 @@ static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t s
 }
 
 // Solve in place: b <- A^{-1} b with work (n x nrhs per matrix, ld n).
+// Compute units of an MI355X (gfx950): the block-solve rule counts resident workgroups with it.
+#define GRIM_CUS 256
+template <typename T, bool LOWER>
+static hipError_t solve_diag_block(const Batch& bt, int nb, int ncols, const T* a, int64_t lda, int64_t sa, T* b,
+                                   int64_t ldb, int64_t sb) {
+    if (nb <= 0 || ncols <= 0) return hipSuccess;
+    const int64_t cols = (int64_t)ncols * bt.count;
+    int cpw = grim_shape_value(GRIM_SHAPE_SOLVE_COLUMNS, cols);
+    if (cpw == 0) {
+        // one column per wavefront while every workgroup fits at once (LDS: one complex128 or
+        // two complex64 workgroups per CU), else two
+        const int64_t resident = (sizeof(T) == 8 ? 1 : 2) * GRIM_CUS;
+        cpw = (cols + 3) / 4 <= resident ? 1 : 2;
+    }
+    const dim3 grid((ncols + 4 * cpw - 1) / (4 * cpw), 1, bt.count);
+    if (cpw == 4)
+        solve_block<T, LOWER, 4><<<grid, 256, 0, bt.st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
+    else if (cpw == 2)
+        solve_block<T, LOWER, 2><<<grid, 256, 0, bt.st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
+    else
+        solve_block<T, LOWER, 1><<<grid, 256, 0, bt.st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
+    return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t lu_solve(const Batch& bt, int n, int nrhs, const T* a, int64_t lda, int64_t sa, const void* auxp,
                            int64_t saux, T* b, int64_t ldb, int64_t sb, T* w, int64_t sw) {
@@ -1154,19 +1324,12 @@ static hipError_t lu_solve(const Batch& bt, int n, int nrhs, const T* a, int64_t
     // Blocks of GRIM_SOLVE_NB rows: inside a block, 64-row triangles (lanes) and
     // rank-64 updates; between blocks, one rank-GRIM_SOLVE_NB update, so most
     // of the solve's flops run as deep-k GEMMs.
-    // forward: w_J <- L_JJ^{-1} w_J ; w_{J+} -= L_{J+, J} w_J
+    // forward: w_J <- L_JJ^{-1} w_J (one solve_block launch) ; w_{J+} -= L_{J+, J} w_J
     for (int J = 0; J < n; J += GRIM_SOLVE_NB) {
         const int jend = min(n, J + GRIM_SOLVE_NB);
-        for (int j = J; j < jend; j += GRIM_LU_NB) {
-            const int jb = min(GRIM_LU_NB, jend - j);
-            const T* djj = a + 2 * ((int64_t)j + (int64_t)j * lda);
-            if ((err = trsm<T, true>(bt, jb, nrhs, djj, lda, sa, w + 2 * (int64_t)j, ldw, sw)) != hipSuccess)
-                return err;
-            if (j + jb < jend)
-                if ((err = gemm<T>(bt, jend - j - jb, nrhs, jb, djj + 2 * (int64_t)jb, lda, sa, w + 2 * (int64_t)j,
-                                   ldw, sw, w + 2 * (int64_t)(j + jb), ldw, sw)) != hipSuccess)
-                    return err;
-        }
+        if ((err = solve_diag_block<T, true>(bt, jend - J, nrhs, a + 2 * ((int64_t)J + (int64_t)J * lda), lda, sa,
+                                             w + 2 * (int64_t)J, ldw, sw)) != hipSuccess)
+            return err;
         if (jend < n)
             if ((err = gemm<T>(bt, n - jend, nrhs, jend - J, a + 2 * ((int64_t)jend + (int64_t)J * lda), lda, sa,
                                w + 2 * (int64_t)J, ldw, sw, w + 2 * (int64_t)jend, ldw, sw)) != hipSuccess)
@@ -1176,17 +1339,9 @@ static hipError_t lu_solve(const Batch& bt, int n, int nrhs, const T* a, int64_t
     const int lastJ = ((n - 1) / GRIM_SOLVE_NB) * GRIM_SOLVE_NB;
     for (int J = lastJ; J >= 0; J -= GRIM_SOLVE_NB) {
         const int jend = min(n, J + GRIM_SOLVE_NB);
-        const int lastj = J + ((jend - 1 - J) / GRIM_LU_NB) * GRIM_LU_NB;
-        for (int j = lastj; j >= J; j -= GRIM_LU_NB) {
-            const int jb = min(GRIM_LU_NB, jend - j);
-            const T* djj = a + 2 * ((int64_t)j + (int64_t)j * lda);
-            if ((err = trsm<T, false>(bt, jb, nrhs, djj, lda, sa, w + 2 * (int64_t)j, ldw, sw)) != hipSuccess)
-                return err;
-            if (j > J)
-                if ((err = gemm<T>(bt, j - J, nrhs, jb, a + 2 * ((int64_t)J + (int64_t)j * lda), lda, sa,
-                                   w + 2 * (int64_t)j, ldw, sw, w + 2 * (int64_t)J, ldw, sw)) != hipSuccess)
-                    return err;
-        }
+        if ((err = solve_diag_block<T, false>(bt, jend - J, nrhs, a + 2 * ((int64_t)J + (int64_t)J * lda), lda, sa,
+                                              w + 2 * (int64_t)J, ldw, sw)) != hipSuccess)
+            return err;
         if (J > 0)
             if ((err = gemm<T>(bt, J, nrhs, jend - J, a + 2 * (int64_t)J * lda, lda, sa, w + 2 * (int64_t)J, ldw, sw,
                                w, ldw, sw)) != hipSuccess)

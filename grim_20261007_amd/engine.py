@@ -164,7 +164,7 @@ class Factor:
     precision: str              # "double" or "mixed"
     lu: torch.Tensor            # (B, N, N) complex128 or complex64
     piv: torch.Tensor           # (B, N) int32
-    aux: torch.Tensor           # (B, aux_bytes) uint8: permutation + inverse diagonal blocks
+    aux: torch.Tensor           # (B, aux_bytes) uint8: row permutation (int32 x N) + factorization scratch
     z: torch.Tensor | None      # (B, N, N) complex128 systems (for residuals)
     n: int
     batch: int

@@ -163,9 +163,11 @@ int grim_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* 
 
 /* Launch shapes (no counterpart in the reference; for a project autotuner and
  * callers that pin their own shapes).  Knob 0: far-pass source columns per
- * wavefront (1, 2, 4, 8, 16 or 32), keyed by n; knob 1: triangular-solve columns per
- * wavefront (1, 2 or 4), keyed by the right-hand-side columns of one call times
- * its batch.  An entry (min_size, value) applies to every size >= min_size up
+ * wavefront (1, 2, 4, 8, 16 or 32), keyed by n; knob 1: columns per wavefront
+ * (1, 2 or 4) of the factorization's 64-row triangular solves, keyed by the
+ * columns of one call times its batch; knob 2: columns per wavefront (1, 2 or
+ * 4) of grim_lu_solve's 256-row block solves, keyed by the right-hand-side
+ * columns times the batch.  An entry (min_size, value) applies to every size >= min_size up
  * to the next entry; value 0 removes the entry; without an entry the library's
  * built-in rule applies.  Shapes never change results: every matrix entry and
  * solution component is computed by the same operations.  Process-wide; set

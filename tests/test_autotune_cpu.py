@@ -29,8 +29,9 @@ def test_table_lookup_ranges(lib):
 def test_table_rejects_bad_arguments(lib):
     assert lib.grim_set_launch_shape(0, 0, 5) == 3              # not a far-pass shape
     assert lib.grim_set_launch_shape(0, 0, 64) == 3
-    assert lib.grim_set_launch_shape(1, 0, 8) == 3              # not a solve shape
-    assert lib.grim_set_launch_shape(2, 0, 1) == 3              # unknown knob
+    assert lib.grim_set_launch_shape(1, 0, 8) == 3              # not a triangular-solve shape
+    assert lib.grim_set_launch_shape(2, 0, 8) == 3              # not a block-solve shape
+    assert lib.grim_set_launch_shape(3, 0, 1) == 3              # unknown knob
     assert lib.grim_set_launch_shape(0, -1, 4) == 3
     for i in range(16):
         assert lib.grim_set_launch_shape(1, 100 * i, 1 + (i % 2)) == 0
@@ -47,9 +48,13 @@ def test_table_from_measurements_merges_and_splits():
 
 
 def test_apply_loads_a_table(lib):
-    autotune.apply({"far_columns": [[0, 4], [4000, 16]], "trsm_columns": [[0, 1], [3000, 2]]})
+    autotune.apply({"far_columns": [[0, 4], [4000, 16]], "trsm_columns": [[0, 1], [3000, 2]],
+                    "solve_columns": [[0, 2], [1000, 1]]})
     assert autotune.shape(0, 100) == 4 and autotune.shape(0, 5000) == 16
     assert autotune.shape(1, 2999) == 1 and autotune.shape(1, 3000) == 2
+    assert autotune.shape(2, 999) == 2 and autotune.shape(2, 1000) == 1
+    autotune.apply({"far_columns": [[0, 8]]})                   # a table replaces every knob
+    assert autotune.shape(1, 3000) == 0 and autotune.shape(2, 0) == 0
 
 
 def test_shipped_table_is_valid(lib):

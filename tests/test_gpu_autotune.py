@@ -1,4 +1,4 @@
-"""Launch shapes on the GPU: every far-pass and triangular-solve shape gives bit-identical
+"""Launch shapes on the GPU: every far-pass, triangular-solve and block-solve shape gives bit-identical
 results; the autotuner measures a table and a server that applies it returns the same numbers."""
 import numpy as np
 import pytest
@@ -43,19 +43,44 @@ def test_far_shapes_are_bit_identical(clean_shapes, n):
             assert torch.equal(got, ref), value
 
 
-def test_trsm_shapes_are_bit_identical(clean_shapes):
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_trsm_shapes_are_bit_identical(clean_shapes, precision):
+    """Factorization triangular-solve shapes (knob 1) give the same factors bit for bit."""
     from grim_20261007_amd import engine
     autotune = clean_shapes
     dev = torch.device("cuda", 0)
     g = torch.Generator(device="cpu").manual_seed(5)
-    n = 300
-    z = torch.randn((2, n, n), dtype=torch.complex128, generator=g) + n * torch.eye(n, dtype=torch.complex128)
-    fac = engine.factor(z.to(dev), "double")
-    b = torch.randn((2, 77, n), dtype=torch.complex128, generator=g).to(dev)
+    n = 700
+    z = (torch.randn((2, n, n), dtype=torch.complex128, generator=g) + n * torch.eye(n, dtype=torch.complex128)).to(dev)
     ref = None
     for value in autotune.CHOICES[autotune.TRSM_COLUMNS]:
         autotune.set_shape(autotune.TRSM_COLUMNS, 0, value)
-        print(f"trsm columns per wavefront {value}", flush=True)
+        print(f"{precision} factor: trsm columns per wavefront {value}", flush=True)
+        f = engine.factor(z, precision)
+        # factors, pivots and the row permutation (the first 4 n bytes of aux; the rest is scratch)
+        got = (f.lu.cpu(), f.piv.cpu(), f.aux[:, :4 * n].cpu())
+        if ref is None:
+            ref = got
+        else:
+            assert all(torch.equal(x, y) for x, y in zip(got, ref)), value
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("n", [300, 700])
+def test_solve_shapes_are_bit_identical(clean_shapes, precision, n):
+    """Block-solve shapes (knob 2) give the same solutions bit for bit (n = 300: a ragged
+    second 256-row block)."""
+    from grim_20261007_amd import engine
+    autotune = clean_shapes
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    z = torch.randn((2, n, n), dtype=torch.complex128, generator=g) + n * torch.eye(n, dtype=torch.complex128)
+    fac = engine.factor(z.to(dev), precision)
+    b = torch.randn((2, 77, n), dtype=torch.complex128, generator=g).to(dev).to(fac.lu.dtype)
+    ref = None
+    for value in autotune.CHOICES[autotune.SOLVE_COLUMNS]:
+        autotune.set_shape(autotune.SOLVE_COLUMNS, 0, value)
+        print(f"{precision} n={n} solve columns per wavefront {value}", flush=True)
         x = engine.lu_solve_inplace(fac, b.clone()).cpu()
         if ref is None:
             ref = x
@@ -67,9 +92,9 @@ def test_tuned_server_matches_untuned(clean_shapes):
     from grim_20261007_amd.serving import Request, Server
     from tests.test_gpu_serving import _requests
     autotune = clean_shapes
-    table = autotune.tune(far_sizes=(200, 600), angle_counts=(40, 90), repeats=2)
-    print("table", table["far_columns"], table["trsm_columns"], flush=True)
-    for name in ("far_columns", "trsm_columns"):
+    table = autotune.tune(far_sizes=(200, 600), angle_counts=(40, 90), factor_n=600, repeats=2)
+    print("table", table["far_columns"], table["trsm_columns"], table["solve_columns"], flush=True)
+    for name in ("far_columns", "trsm_columns", "solve_columns"):
         assert table[name] and table[name][0][0] == 0
         assert all(v in autotune.CHOICES[k] for k, nm in autotune.KNOB_NAMES.items() if nm == name
                    for _, v in table[name])
