@@ -869,10 +869,9 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
     }
 }
 
-// One SOLVE_CHUNK-column chunk of a step's coupling rows, spread over the workgroup: thread entry k
+// One CH-column chunk of a step's coupling rows, spread over the workgroup: thread entry k
 // is e = threadIdx.x + 256 k, row e % CROWS of the coupling rows (from row c_lo of the block),
 // column j0 + e / CROWS of the step's sub-block (from column r0).  Zero outside nrow x w.
-#define SOLVE_CHUNK 16
 template <typename T, int CPT, int CROWS>
 __device__ __forceinline__ void fetch_coupling(const T* __restrict__ a, int64_t lda, int c_lo, int nrow, int r0,
                                                int j0, int w, T (&pr)[CPT], T (&pi)[CPT]) {
@@ -913,16 +912,17 @@ __device__ __forceinline__ void cmul_by(T& xr, T& xi, T ar, T ai) {   // x *= a
 // D^{-1} U, with B scaled by D^{-1} first); each wave solves its columns' chain as trsm_lanes
 // does (x_j broadcast from lane j with v_readlane), then subtracts the coupling blocks of the
 // sub-blocks still to come times the new rows, broadcast the same way; the coupling blocks
-// pass through LDS in SOLVE_CHUNK-column chunks loaded by the whole workgroup.
-template <typename T, bool LOWER, int CPW>
-__global__ void __launch_bounds__(256) solve_block(int nb, int ncols, const T* __restrict__ a, int64_t lda,
+// pass through LDS in CH-column chunks loaded by the whole workgroup (16 columns; 4 for
+// complex128 launches with more workgroups than CUs, so that two workgroups fit per CU).
+template <typename T, bool LOWER, int CPW, int CH>
+__global__ void __launch_bounds__(256, 2) solve_block(int nb, int ncols, const T* __restrict__ a, int64_t lda,
                                                    int64_t sa, T* __restrict__ b, int64_t ldb, int64_t sb) {
     constexpr int NS = GRIM_SOLVE_NB / GRIM_LU_NB;
     constexpr int CROWS = GRIM_SOLVE_NB - GRIM_LU_NB;              // coupling rows of one step, at most
-    constexpr int CPT = CROWS * SOLVE_CHUNK / 256;                 // chunk entries per thread
+    constexpr int CPT = CROWS * CH / 256;                          // chunk entries per thread
     __shared__ T tr[GRIM_LU_NB][GRIM_LU_NB], ti[GRIM_LU_NB][GRIM_LU_NB];   // [col j][row i]
     __shared__ T dinv_r[GRIM_LU_NB], dinv_i[GRIM_LU_NB];
-    __shared__ T cp_r[SOLVE_CHUNK][CROWS], cp_i[SOLVE_CHUNK][CROWS];       // coupling chunk [col][row]
+    __shared__ T cp_r[CH][CROWS], cp_i[CH][CROWS];                 // coupling chunk [col][row]
     a += 2 * sa * blockIdx.z;
     b += 2 * sb * blockIdx.z;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -984,7 +984,7 @@ __global__ void __launch_bounds__(256) solve_block(int nb, int ncols, const T* _
         // first column chunk of them is fetched while the chain runs
         const int c_lo = LOWER ? r0 + GRIM_LU_NB : 0;
         const int nrow = LOWER ? nb - c_lo : r0;
-        const int nchunk = nrow > 0 ? (w + SOLVE_CHUNK - 1) / SOLVE_CHUNK : 0;
+        const int nchunk = nrow > 0 ? (w + CH - 1) / CH : 0;
         T pr[CPT], pi[CPT];
         if (nchunk > 0) fetch_coupling<T, CPT, CROWS>(a, lda, c_lo, nrow, r0, 0, w, pr, pi);
         // chain over the 64 x 64 triangle (entries from row or column w on are zero)
@@ -1011,7 +1011,7 @@ __global__ void __launch_bounds__(256) solve_block(int nb, int ncols, const T* _
         // coupling: x_t -= A_ts x_s for the sub-blocks t still to be solved, one LDS-staged
         // chunk of columns at a time (the next chunk's loads in flight meanwhile)
 #pragma unroll
-        for (int q = 0; q < GRIM_LU_NB / SOLVE_CHUNK; ++q) {
+        for (int q = 0; q < GRIM_LU_NB / CH; ++q) {
             if (q >= nchunk) break;   // same for the whole workgroup
             __syncthreads();          // the previous chunk's readers are done
 #pragma unroll
@@ -1022,10 +1022,10 @@ __global__ void __launch_bounds__(256) solve_block(int nb, int ncols, const T* _
             }
             __syncthreads();
             if (q + 1 < nchunk)
-                fetch_coupling<T, CPT, CROWS>(a, lda, c_lo, nrow, r0, (q + 1) * SOLVE_CHUNK, w, pr, pi);
+                fetch_coupling<T, CPT, CROWS>(a, lda, c_lo, nrow, r0, (q + 1) * CH, w, pr, pi);
 #pragma unroll 4
-            for (int jj = 0; jj < SOLVE_CHUNK; ++jj) {
-                const int j = q * SOLVE_CHUNK + jj;
+            for (int jj = 0; jj < CH; ++jj) {
+                const int j = q * CH + jj;
                 T ur[CPW], ui[CPW];
 #pragma unroll
                 for (int c = 0; c < CPW; ++c) {
@@ -1291,25 +1291,32 @@ static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t s
 // Solve in place: b <- A^{-1} b with work (n x nrhs per matrix, ld n).
 // Compute units of an MI355X (gfx950): the block-solve rule counts resident workgroups with it.
 #define GRIM_CUS 256
+template <typename T, bool LOWER, int CPW>
+static void launch_solve_block(bool narrow, dim3 grid, hipStream_t st, int nb, int ncols, const T* a, int64_t lda,
+                               int64_t sa, T* b, int64_t ldb, int64_t sb) {
+    if constexpr (sizeof(T) == 8) {
+        if (narrow) {
+            solve_block<T, LOWER, CPW, 4><<<grid, 256, 0, st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
+            return;
+        }
+    }
+    solve_block<T, LOWER, CPW, 16><<<grid, 256, 0, st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
+}
 template <typename T, bool LOWER>
 static hipError_t solve_diag_block(const Batch& bt, int nb, int ncols, const T* a, int64_t lda, int64_t sa, T* b,
                                    int64_t ldb, int64_t sb) {
     if (nb <= 0 || ncols <= 0) return hipSuccess;
     const int64_t cols = (int64_t)ncols * bt.count;
     int cpw = grim_shape_value(GRIM_SHAPE_SOLVE_COLUMNS, cols);
-    if (cpw == 0) {
-        // one column per wavefront while every workgroup fits at once (LDS: one complex128 or
-        // two complex64 workgroups per CU), else two
-        const int64_t resident = (sizeof(T) == 8 ? 1 : 2) * GRIM_CUS;
-        cpw = (cols + 3) / 4 <= resident ? 1 : 2;
-    }
+    // one column per wavefront while every workgroup fits at once (two per CU), else two
+    if (cpw == 0) cpw = (cols + 3) / 4 <= 2 * GRIM_CUS ? 1 : 2;
     const dim3 grid((ncols + 4 * cpw - 1) / (4 * cpw), 1, bt.count);
-    if (cpw == 4)
-        solve_block<T, LOWER, 4><<<grid, 256, 0, bt.st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
-    else if (cpw == 2)
-        solve_block<T, LOWER, 2><<<grid, 256, 0, bt.st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
-    else
-        solve_block<T, LOWER, 1><<<grid, 256, 0, bt.st>>>(nb, ncols, a, lda, sa, b, ldb, sb);
+    // complex128 with 16-column coupling chunks takes 114 KB of LDS, one workgroup per CU; past
+    // one workgroup per CU, 4-column chunks (77 KB) let two share a CU
+    const bool narrow = sizeof(T) == 8 && (int64_t)grid.x * grid.z > GRIM_CUS;
+    if (cpw == 4) launch_solve_block<T, LOWER, 4>(narrow, grid, bt.st, nb, ncols, a, lda, sa, b, ldb, sb);
+    else if (cpw == 2) launch_solve_block<T, LOWER, 2>(narrow, grid, bt.st, nb, ncols, a, lda, sa, b, ldb, sb);
+    else launch_solve_block<T, LOWER, 1>(narrow, grid, bt.st, nb, ncols, a, lda, sa, b, ldb, sb);
     return hipGetLastError();
 }
 

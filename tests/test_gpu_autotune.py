@@ -66,24 +66,29 @@ def test_trsm_shapes_are_bit_identical(clean_shapes, precision):
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
-@pytest.mark.parametrize("n", [300, 700])
-def test_solve_shapes_are_bit_identical(clean_shapes, precision, n):
+@pytest.mark.parametrize("n,na", [(300, 77), (700, 77), (300, 600)])
+def test_solve_shapes_are_bit_identical(clean_shapes, precision, n, na):
     """Block-solve shapes (knob 2) give the same solutions bit for bit (n = 300: a ragged
-    second 256-row block)."""
+    second 256-row block; 1 200 columns at one column per wavefront: more complex128
+    workgroups than CUs, so the 4-column coupling chunks against the 16-column ones)."""
     from grim_20261007_amd import engine
     autotune = clean_shapes
     dev = torch.device("cuda", 0)
     g = torch.Generator(device="cpu").manual_seed(5)
     z = torch.randn((2, n, n), dtype=torch.complex128, generator=g) + n * torch.eye(n, dtype=torch.complex128)
     fac = engine.factor(z.to(dev), precision)
-    b = torch.randn((2, 77, n), dtype=torch.complex128, generator=g).to(dev).to(fac.lu.dtype)
+    b = torch.randn((2, na, n), dtype=torch.complex128, generator=g).to(dev).to(fac.lu.dtype)
     ref = None
     for value in autotune.CHOICES[autotune.SOLVE_COLUMNS]:
         autotune.set_shape(autotune.SOLVE_COLUMNS, 0, value)
-        print(f"{precision} n={n} solve columns per wavefront {value}", flush=True)
+        print(f"{precision} n={n} A={na} solve columns per wavefront {value}", flush=True)
         x = engine.lu_solve_inplace(fac, b.clone()).cpu()
         if ref is None:
             ref = x
+            # tensors hold column-major matrices: Z = z[i].T, B = b[i].T, X = x[i].T
+            want = torch.linalg.solve(z.transpose(1, 2), b.cpu().to(z.dtype).transpose(1, 2)).transpose(1, 2)
+            tol = 1e-11 if precision == "double" else 1e-4
+            assert float((x - want).abs().max() / want.abs().max()) < tol
         else:
             assert torch.equal(x, ref), value
 
