@@ -357,24 +357,73 @@ extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const 
     CHECK(grim_dense_add_c64(B, N, R, d, n, snr, x, ldx, stride_x, st));
     CHECK(hipStreamSynchronize(st));
     for (auto& v : hb) v = v > 0 ? v : 1.0;
-    double previous = INFINITY;
+    // Each system refines on its own, as linalg/refined_lu.py:41-61 does per matrix: a system
+    // whose error is within tol takes no further correction while the others continue, so a
+    // batched call returns what separate calls would.  Work on a subset of systems runs per
+    // system (batch 1, offset pointers).
+    std::vector<double> previous((size_t)B, INFINITY);
+    std::vector<char> done((size_t)B, 0);
+    int active = B;
+    double worst = 0.0;
+    auto for_active = [&](auto&& op) -> hipError_t {
+        if (active == B) return op(0, B);
+        for (int i = 0; i < B; ++i)
+            if (!done[(size_t)i]) {
+                hipError_t e = op(i, 1);
+                if (e != hipSuccess) return e;
+            }
+        return hipSuccess;
+    };
     for (int step = 0; step < max_steps; ++step) {
         // r = b - Z x (FP64)
-        CHECK(grim_dense_copy(B, N, R, b, ldb, stride_b, r, n, snr, st));
-        CHECK(grim_dense_gemm_minus(B, N, R, N, z, ldz, stride_z, x, ldx, stride_x, r, n, snr, 0, st));
-        CHECK(grim_dense_colmax(B, N, R, r, n, snr, rmax, st));
+        CHECK(for_active([&](int i, int cnt) -> hipError_t {
+            hipError_t e = grim_dense_copy(cnt, N, R, b + 2 * stride_b * i, ldb, stride_b, r + 2 * snr * i, n, snr,
+                                           st);
+            if (e == hipSuccess)
+                e = grim_dense_gemm_minus(cnt, N, R, N, z + 2 * stride_z * i, ldz, stride_z, x + 2 * stride_x * i,
+                                          ldx, stride_x, r + 2 * snr * i, n, snr, 0, st);
+            if (e == hipSuccess) e = grim_dense_colmax(cnt, N, R, r + 2 * snr * i, n, snr, rmax + (int64_t)R * i, st);
+            return e;
+        }));
         CHECK(hipMemcpyAsync(hr.data(), rmax, 8 * (size_t)B * R, hipMemcpyDeviceToHost, st));
         CHECK(hipStreamSynchronize(st));
-        double err = 0.0;
-        for (size_t c = 0; c < hr.size(); ++c) err = fmax(err, hr[c] / hb[c]);
-        if (error_out) *error_out = err;
         if (steps_out) *steps_out = step;
-        if (isfinite(err) && err <= tol) return GRIM_OK;
-        if (!isfinite(err) || err >= 0.98 * previous) return GRIM_ERR_NOT_CONVERGED;
-        previous = err;
-        CHECK(grim_dense_to_c64(B, N, R, r, n, snr, d, n, snr, st));
-        CHECK(grim_dense_lu_solve(B, N, R, lu, ldlu, stride_lu, 1, aux, stride_aux, d, n, snr, w, snr, st));
-        CHECK(grim_dense_add_c64(B, N, R, d, n, snr, x, ldx, stride_x, st));
+        for (int i = 0; i < B; ++i) {
+            if (done[(size_t)i]) continue;
+            double err = 0.0;
+            for (int c = 0; c < R; ++c) err = fmax(err, hr[(size_t)i * R + c] / hb[(size_t)i * R + c]);
+            if (isfinite(err) && err <= tol) {
+                done[(size_t)i] = 1;
+                --active;
+                worst = fmax(worst, err);
+                continue;
+            }
+            if (!isfinite(err) || err >= 0.98 * previous[(size_t)i]) {
+                if (error_out) *error_out = err;
+                return GRIM_ERR_NOT_CONVERGED;
+            }
+            previous[(size_t)i] = err;
+        }
+        if (active == 0) {
+            if (error_out) *error_out = worst;
+            return GRIM_OK;
+        }
+        if (error_out) {
+            double cur = worst;
+            for (int i = 0; i < B; ++i)
+                if (!done[(size_t)i]) cur = fmax(cur, previous[(size_t)i]);
+            *error_out = cur;
+        }
+        CHECK(for_active([&](int i, int cnt) -> hipError_t {
+            hipError_t e = grim_dense_to_c64(cnt, N, R, r + 2 * snr * i, n, snr, d + 2 * snr * i, n, snr, st);
+            if (e == hipSuccess)
+                e = grim_dense_lu_solve(cnt, N, R, lu + 2 * stride_lu * i, ldlu, stride_lu, 1,
+                                        (const char*)aux + stride_aux * i, stride_aux, d + 2 * snr * i, n, snr,
+                                        w + 2 * snr * i, snr, st);
+            if (e == hipSuccess)
+                e = grim_dense_add_c64(cnt, N, R, d + 2 * snr * i, n, snr, x + 2 * stride_x * i, ldx, stride_x, st);
+            return e;
+        }));
     }
     return GRIM_ERR_NOT_CONVERGED;
 }

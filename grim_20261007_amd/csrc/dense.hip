@@ -1288,7 +1288,6 @@ static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t s
     return hipSuccess;
 }
 
-// Solve in place: b <- A^{-1} b with work (n x nrhs per matrix, ld n).
 // Compute units of an MI355X (gfx950): the block-solve rule counts resident workgroups with it.
 #define GRIM_CUS 256
 template <typename T, bool LOWER, int CPW>
@@ -1320,6 +1319,7 @@ static hipError_t solve_diag_block(const Batch& bt, int nb, int ncols, const T* 
     return hipGetLastError();
 }
 
+// Solve in place: b <- A^{-1} b with work (n x nrhs per matrix, ld n).
 template <typename T>
 static hipError_t lu_solve(const Batch& bt, int n, int nrhs, const T* a, int64_t lda, int64_t sa, const void* auxp,
                            int64_t saux, T* b, int64_t ldb, int64_t sb, T* w, int64_t sw) {

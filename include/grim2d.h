@@ -125,9 +125,12 @@ int grim_gemm_minus(int64_t batch, int64_t m, int64_t n, int64_t k, const void* 
 int64_t grim_refine_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs);
 
 /* Mixed-precision solve (linalg/refined_lu.py:41-61): x = LU32^{-1} b, then
- * r = b - Z x in FP64, x += LU32^{-1} r until max_col(|r|_inf / |b|_inf) <= tol
- * over all matrices of the batch (at most max_steps corrections; a step that
- * does not reduce the error below 0.98 of the previous one returns 6).
+ * r = b - Z x in FP64, x += LU32^{-1} r until max_col(|r|_inf / |b|_inf) <= tol,
+ * for each matrix of the batch on its own: a matrix within tol takes no more
+ * corrections while the others continue, so a batched call returns what
+ * separate calls return (at most max_steps corrections; a step that does not
+ * reduce a matrix's error below 0.98 of its previous one returns 6).
+ * steps_out: corrections of the slowest matrix; error_out: largest final error.
  * z: complex128 systems; lu/aux: complex64 factors of the same matrices.
  * Synchronises the stream once per step.  steps_out / error_out: host
  * pointers (may be NULL).  On GRIM_OK the workspace starts with the final
