@@ -16,9 +16,11 @@
 // of the panel, and each panel's interchanges are composed into one list that
 // is applied as a gather to the columns left of the panel and, fused into the
 // U12 triangular solve, to the columns right of it; the same list updates the
-// permutation vector used by the solves.  Triangular solves of up to 64 rows
-// run lane-parallel (one row per lane, pivots broadcast with v_readlane);
-// everything else is MFMA matrix products.
+// permutation vector used by the solves.  Triangular solves run lane-parallel
+// (one row per lane, x_j broadcast with v_readlane): up to 64 rows per launch in
+// the factorization, and one 256-row diagonal block per launch in the LU solve,
+// whose four 64-row sub-blocks are chained inside the kernel; everything else is
+// MFMA matrix products.
 // All kernels take a batch of equally strided matrices (blockIdx.z or one
 // workgroup per matrix), e.g. the TE and TM systems of one request.
 #include "common.h"
