@@ -222,6 +222,23 @@ def far_traffic():
     return float(m.group(1)) if m else None
 
 
+def far_valu():
+    """VALUBusy and VALUUtilization (percent) of the far kernel from the committed counter
+    summary, or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "far_pass_naca0012_n2000.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    busy = re.search(r"valu_busy_percent:\s*([0-9.]+)", text)
+    util = re.search(r"valu_utilization_percent:\s*([0-9.]+)", text)
+    if not (busy and util):
+        return None
+    return {"valu_busy_percent": float(busy.group(1)), "valu_utilization_percent": float(util.group(1)),
+            "source": "profiles/far_pass_naca0012_n2000.md (rocprofv3 --pmc VALUBusy VALUUtilization)"}
+
+
 def serve(engine_mod, mesh, f, angles, precision, workers, nreq, warm):
     """Closed-loop serving: nreq distinct requests (no cache reuse) with `workers` in flight."""
     from grim_20261007_amd import serving
@@ -501,6 +518,18 @@ def main():
     del pair, xs, bs
     stage_ms = {name: round(tm.mean_ms(name), 3) for name in tm.acc}
     stage64 = {name: round(timers["double"].mean_ms(name), 3) for name in timers["double"].acc}
+    # assembly rate: TM + TE matrix entries per second of far + near assembly (FP64 path's timers)
+    asm_ms = timers["double"].mean_ms("assemble_far") + timers["double"].mean_ms("assemble_near")
+    assembly = {"entries_per_s": round(2.0 * mesh.n * mesh.n / (asm_ms * 1e-3), 1), "ms": round(asm_ms, 3),
+                "far_kernel_counters": far_valu()}
+    # LU rate of the batched TE + TM factorization, (8/3) N^3 real flops per complex system
+    lu_flops = 2 * (8.0 / 3.0) * mesh.n ** 3
+    lu = {}
+    for key, stage, peak, label in (("complex128", stage64.get("factor_double"), PEAK_FP64, "fp64"),
+                                    ("complex64", stage_ms.get("factor_mixed"), PEAK_FP32, "fp32")):
+        if stage:
+            tf = lu_flops / (stage * 1e-3) / 1e12
+            lu[key] = {"factor_ms": stage, "tflops": round(tf, 2), f"frac_of_{label}_peak": round(tf / peak, 4)}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mesh, f, angles, args.cpu_seconds)
@@ -537,6 +566,8 @@ def main():
                                "bound": "mfma", "achieved": round(gemm_tf, 2), "peak": PEAK_FP64,
                                "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4),
                                "ms": round(gemm_ms, 3)},
+        "assembly": assembly,
+        "lu_n2000": lu,
         "cpu_baseline": cpu,
         "launch_shapes": shapes,
     }
