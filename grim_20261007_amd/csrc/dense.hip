@@ -1330,9 +1330,9 @@ static hipError_t lu_solve(const Batch& bt, int n, int nrhs, const T* a, int64_t
     gather_rows<T><<<dim3((n + 255) / 256, nrhs, bt.count), 256, 0, bt.st>>>(n, (const int*)auxp, saux / 4, b, ldb,
                                                                               sb, w, ldw, sw);
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    // Blocks of GRIM_SOLVE_NB rows: inside a block, 64-row triangles (lanes) and
-    // rank-64 updates; between blocks, one rank-GRIM_SOLVE_NB update, so most
-    // of the solve's flops run as deep-k GEMMs.
+    // Blocks of GRIM_SOLVE_NB rows: inside a block, one solve_block launch (its 64-row
+    // triangles and their coupling in LDS and lanes); between blocks, one
+    // rank-GRIM_SOLVE_NB update, so most of the solve's flops run as deep-k GEMMs.
     // forward: w_J <- L_JJ^{-1} w_J (one solve_block launch) ; w_{J+} -= L_{J+, J} w_J
     for (int J = 0; J < n; J += GRIM_SOLVE_NB) {
         const int jend = min(n, J + GRIM_SOLVE_NB);
