@@ -44,7 +44,7 @@ FLOPS_PER_EVAL = 125   # FP64 flops per kernel evaluation (distance, Hankel pair
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32, help="requests in the timed serving region")
+    ap.add_argument("--steps", type=int, default=128, help="requests in the timed serving region")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workers", type=int, default=4, help="requests in flight in the serving form")
     ap.add_argument("--n-side", type=int, default=1000)
