@@ -521,6 +521,7 @@ def main():
     # assembly rate: TM + TE matrix entries per second of far + near assembly (FP64 path's timers)
     asm_ms = timers["double"].mean_ms("assemble_far") + timers["double"].mean_ms("assemble_near")
     assembly = {"entries_per_s": round(2.0 * mesh.n * mesh.n / (asm_ms * 1e-3), 1), "ms": round(asm_ms, 3),
+                "far_evaluations_per_s": round(pairs * q * q / (far_ms * 1e-3), 1),
                 "far_kernel_counters": far_valu()}
     # LU rate of the batched TE + TM factorization, (8/3) N^3 real flops per complex system
     lu_flops = 2 * (8.0 / 3.0) * mesh.n ** 3
@@ -546,6 +547,7 @@ def main():
                    "n": mesh.n, "angles": len(angles), "frequency_ghz": f, "workers": args.workers},
         "p50_ms": round(float(np.percentile(svc_m, 50)), 3), "p99_ms": round(float(np.percentile(svc_m, 99)), 3),
         "vs_plain_fp64": round(value / plain64, 3),
+        "angle_columns_per_s": round(value * len(angles), 1),
         "fp64_path": {"value": round(plain64, 3), "p50_ms": round(float(np.percentile(lat64, 50)), 3),
                       "p99_ms": round(float(np.percentile(lat64, 99)), 3), "stage_ms": stage64,
                       "form": "one request at a time, FP64 LU + backward-error gate"},
