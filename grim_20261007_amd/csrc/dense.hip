@@ -66,8 +66,11 @@ template <> struct Cplx2<double> { typedef double2 type; };
 template <> struct Cplx2<float> { typedef float2 type; };
 template <typename T> constexpr int gb_k() { return sizeof(T) == 8 ? 16 : 32; }
 
+// complex128 without the C read-ahead is held to 3 waves per SIMD (134 VGPRs instead of 210):
+// the 2000 x 720 x 2000 refinement residual's 768 workgroups then run in one round
+// (942 -> 845 us, measured); the other forms keep the compiler's choice.
 template <typename T, bool SET, bool PRE = false>
-__global__ void __launch_bounds__(256) cgemm(int m, int n, int k, const T* __restrict__ A, int64_t lda, int64_t sa,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && !PRE ? 3 : 1))) cgemm(int m, int n, int k, const T* __restrict__ A, int64_t lda, int64_t sa,
                                              const T* __restrict__ B, int64_t ldb, int64_t sb, T* __restrict__ C,
                                              int64_t ldc, int64_t sc) {
     typedef typename Mfma<T>::acc acc_t;
