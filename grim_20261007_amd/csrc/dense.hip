@@ -1186,14 +1186,22 @@ static hipError_t leaf(const Batch& bt, int m, int w, T* pan, int64_t lda, int64
     panel_leaf<T, R, W><<<b, 1024, 0, st>>>(m, w, pan, lda, sa, pw, lc0, row0, piv, spiv, status)
     if (m <= 1024) GRIM_LEAF(1, 16);
     else if (m <= 2048) {
-        // complex64 holds twice the columns in the same registers: half the leaves and in-panel
-        // updates (complex64 factor at N = 2000: 7.28 -> 6.88 ms, measured)
+        // complex64 holds twice the columns in the same registers (here and below): half the
+        // leaves and in-panel updates (complex64 factor, measured: N = 2000 7.28 -> 6.88 ms,
+        // N = 8000 87.4 -> 75.7 ms, N = 12 000 248 -> 203 ms)
         if constexpr (sizeof(T) == 4) GRIM_LEAF(2, 16);
         else GRIM_LEAF(2, 8);
     }
-    else if (m <= 4096) GRIM_LEAF(4, 4);
-    else if (m <= 8192) GRIM_LEAF(8, 2);
-    else GRIM_LEAF(16, 1);
+    else if (m <= 4096) {
+        if constexpr (sizeof(T) == 4) GRIM_LEAF(4, 8);
+        else GRIM_LEAF(4, 4);
+    } else if (m <= 8192) {
+        if constexpr (sizeof(T) == 4) GRIM_LEAF(8, 4);
+        else GRIM_LEAF(8, 2);
+    } else {
+        if constexpr (sizeof(T) == 4) GRIM_LEAF(16, 2);
+        else GRIM_LEAF(16, 1);
+    }
 #undef GRIM_LEAF
     return hipGetLastError();
 }
@@ -1202,9 +1210,9 @@ template <typename T>
 static int leaf_width(int m) {
     if (m <= 1024) return 16;
     if (m <= 2048) return sizeof(T) == 4 ? 16 : 8;
-    if (m <= 4096) return 4;
-    if (m <= 8192) return 2;
-    if (m <= GRIM_LEAF_MAX_M) return 1;
+    if (m <= 4096) return sizeof(T) == 4 ? 8 : 4;
+    if (m <= 8192) return sizeof(T) == 4 ? 4 : 2;
+    if (m <= GRIM_LEAF_MAX_M) return sizeof(T) == 4 ? 2 : 1;
     return GRIM_TALL_TW;
 }
 
