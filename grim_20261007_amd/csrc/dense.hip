@@ -874,17 +874,20 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
     }
 }
 
-// One CH-column chunk of a step's coupling rows, spread over the workgroup: thread entry k
-// is e = threadIdx.x + 256 k, row e % CROWS of the coupling rows (from row c_lo of the block),
-// column j0 + e / CROWS of the step's sub-block (from column r0).  Zero outside nrow x w.
+// One chunk of a step's coupling rows (CROWS = 3 x 64 rows, CPT / 3 x 4 columns), spread over
+// the workgroup: thread entry k is row lane + 64 (k % 3) of the coupling rows (from row c_lo of
+// the block) and column j0 + wave + 4 (k / 3) of the step's sub-block (from column r0), so every
+// offset but the lane's is a compile-time constant.  Zero outside nrow x w.
+__device__ __forceinline__ int chunk_row(int k) { return (int)(threadIdx.x & 63) + 64 * (k % 3); }
+__device__ __forceinline__ int chunk_col(int k) { return (int)(threadIdx.x >> 6) + 4 * (k / 3); }
 template <typename T, int CPT, int CROWS>
 __device__ __forceinline__ void fetch_coupling(const T* __restrict__ a, int64_t lda, int c_lo, int nrow, int r0,
                                                int j0, int w, T (&pr)[CPT], T (&pi)[CPT]) {
+    static_assert(CROWS == 3 * 64 && CPT % 3 == 0, "chunk map assumes 192 coupling rows");
     typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int e = threadIdx.x + 256 * k;
-        const int row = e % CROWS, col = j0 + e / CROWS;
+        const int row = chunk_row(k), col = j0 + chunk_col(k);
         T2 v = {0, 0};
         if (row < nrow && col < w)
             v = *reinterpret_cast<const T2*>(a + 2 * ((int64_t)(c_lo + row) + (int64_t)(r0 + col) * lda));
@@ -1021,9 +1024,8 @@ __global__ void __launch_bounds__(256, 2) solve_block(int nb, int ncols, const T
             __syncthreads();          // the previous chunk's readers are done
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
-                const int e = threadIdx.x + 256 * k;
-                cp_r[e / CROWS][e % CROWS] = pr[k];
-                cp_i[e / CROWS][e % CROWS] = pi[k];
+                cp_r[chunk_col(k)][chunk_row(k)] = pr[k];
+                cp_i[chunk_col(k)][chunk_row(k)] = pi[k];
             }
             __syncthreads();
             if (q + 1 < nchunk)
