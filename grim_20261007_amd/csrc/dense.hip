@@ -975,8 +975,9 @@ __global__ void __launch_bounds__(256, 2) solve_block(int nb, int ncols, const T
             }
             __syncthreads();
         }
-        for (int idx = threadIdx.x; idx < GRIM_LU_NB * GRIM_LU_NB; idx += 256) {
-            const int i = idx & 63, j = idx >> 6;
+#pragma unroll 4
+        for (int jm = 0; jm < GRIM_LU_NB; jm += 4) {   // lane: row i; wave: columns j = wave + 4 m
+            const int i = lane, j = wave + jm;
             T x = 0, y = 0;
             if (i < w && j < w && (LOWER ? i > j : i < j)) {
                 const T* p = d + 2 * ((int64_t)i + (int64_t)j * lda);
