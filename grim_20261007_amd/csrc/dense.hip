@@ -802,8 +802,9 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
     __shared__ T tr[GRIM_LU_NB][GRIM_LU_NB], ti[GRIM_LU_NB][GRIM_LU_NB];   // [col j][row i]
     tri += 2 * st * blockIdx.z;
     b += 2 * sb * blockIdx.z;
-    for (int idx = threadIdx.x; idx < GRIM_LU_NB * GRIM_LU_NB; idx += 256) {
-        const int i = idx & 63, j = idx >> 6;
+#pragma unroll 4
+    for (int jm = 0; jm < GRIM_LU_NB; jm += 4) {   // lane: row i; wave: columns j = wave + 4 m
+        const int i = threadIdx.x & 63, j = (threadIdx.x >> 6) + jm;
         T x = 0, y = 0;
         if (i < w && j < w && i > j) {
             const T* p = tri + 2 * ((int64_t)i + (int64_t)j * ldt);
