@@ -273,34 +273,31 @@ struct Composite {
 __device__ Composite wave_composite(int r0, int cnt, const int* piv) {
     const int lane = threadIdx.x & 63;
     const int pl = lane < cnt ? piv[lane] : 0;   // lane i holds piv[i]
+    // Every lane traces one final position back through the interchanges in
+    // reverse order, so the cnt steps carry no cross-lane dependence: lane l
+    // traces top slot r0 + l and lower position piv[l].
+    int top = r0 + lane, src = pl;
+    bool repeat = false;   // an earlier interchange already targets piv[l]
+    for (int i = cnt - 1; i >= 0; --i) {
+        const int p = __builtin_amdgcn_readlane(pl, i);
+        const int t = r0 + i;
+        top = top == t ? p : top == p ? t : top;
+        src = src == t ? p : src == p ? t : src;
+        repeat = repeat || (i < lane && p == pl);
+    }
     Composite x;
-    x.top = r0 + lane;
+    x.top = lane < cnt ? top : r0 + lane;
     x.ldst = -1;
     x.lsrc = -1;
-    int nlow = 0;
-    for (int i = 0; i < cnt; ++i) {
-        const int p = __builtin_amdgcn_readlane(pl, i);
-        const int vi = __builtin_amdgcn_readlane(x.top, i);
-        if (p - r0 < cnt) {
-            const int q = p - r0;
-            const int vq = __builtin_amdgcn_readlane(x.top, q);
-            if (lane == i) x.top = vq;
-            if (lane == q) x.top = vi;
-        } else {
-            const unsigned long long hit = __ballot(x.ldst == p);
-            int q;
-            if (hit) {
-                q = __ffsll((long long)hit) - 1;
-            } else {
-                q = nlow++;
-                if (lane == q) { x.ldst = p; x.lsrc = p; }
-            }
-            const int vq = __builtin_amdgcn_readlane(x.lsrc, q);
-            if (lane == i) x.top = vq;
-            if (lane == q) x.lsrc = vi;
-        }
+    // lower entries in order of first occurrence: the k-th moves to lane k
+    unsigned long long first = __ballot(lane < cnt && pl - r0 >= cnt && !repeat);
+    x.nlow = __popcll(first);
+    for (int k = 0; first; ++k) {
+        const int i = __ffsll((long long)first) - 1;
+        first &= first - 1;
+        const int d = __builtin_amdgcn_readlane(pl, i), s = __builtin_amdgcn_readlane(src, i);
+        if (lane == k) { x.ldst = d; x.lsrc = s; }
     }
-    x.nlow = nlow;
     return x;
 }
 
