@@ -270,9 +270,8 @@ struct Composite {
     int top, ldst, lsrc, nlow;
 };
 
-__device__ Composite wave_composite(int r0, int cnt, const int* piv) {
+__device__ Composite wave_composite_of(int r0, int cnt, int pl) {   // lane i passes piv[i]
     const int lane = threadIdx.x & 63;
-    const int pl = lane < cnt ? piv[lane] : 0;   // lane i holds piv[i]
     // Every lane traces one final position back through the interchanges in
     // reverse order, so the cnt steps carry no cross-lane dependence: lane l
     // traces top slot r0 + l and lower position piv[l].
@@ -301,10 +300,31 @@ __device__ Composite wave_composite(int r0, int cnt, const int* piv) {
     return x;
 }
 
+__device__ Composite wave_composite(int r0, int cnt, const int* piv) {
+    const int lane = threadIdx.x & 63;
+    return wave_composite_of(r0, cnt, lane < cnt ? piv[lane] : 0);
+}
+
 // Interchange list of one 64-column panel in the auxiliary storage:
 // [0] = nlow, [1 + l] = top source of slot l, [65 + l] / [129 + l] = lower
 // destination / source of entry l (absolute rows).
 #define GRIM_LIST_INTS 256
+
+// One wave: store the interchange list of the panel [pj, pj + pw) (lane i
+// passes the panel's pivot i) and apply it to the running row permutation
+// (perm[i] = original row now at position i).
+__device__ void panel_list(int pj, int pw, int pl, int* __restrict__ perm, int* __restrict__ list) {
+    const int lane = threadIdx.x & 63;
+    const Composite x = wave_composite_of(pj, pw, pl);
+    if (lane == 0) list[0] = x.nlow;
+    list[1 + lane] = x.top;
+    list[65 + lane] = x.ldst;
+    list[129 + lane] = x.lsrc;
+    const int vt = lane < pw ? perm[x.top] : 0;
+    const int vl = lane < x.nlow ? perm[x.lsrc] : 0;
+    if (lane < pw) perm[pj + lane] = vt;
+    if (lane < x.nlow) perm[x.ldst] = vl;
+}
 
 // ---------------------------------------------------------------------------
 // Panel leaf: factor leaf columns [lc0, lc0 + w) of the panel whose first
@@ -321,7 +341,8 @@ __device__ Composite wave_composite(int r0, int cnt, const int* piv) {
 template <typename T, int R, int W>
 __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__ pan, int64_t lda, int64_t sa,
                                                    int pw, int lc0, int row0, int* __restrict__ piv, int64_t spiv,
-                                                   unsigned* __restrict__ status) {
+                                                   unsigned* __restrict__ status, int* __restrict__ perm,
+                                                   int64_t sperm, int* __restrict__ list) {
     __shared__ T cand_v[2][16];
     __shared__ int cand_i[2][16];
     __shared__ T prow_re[2][W], prow_im[2][W], crow_re[2][W], crow_im[2][W];
@@ -439,6 +460,13 @@ __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__
     __syncthreads();
     if (tid < w) piv[tid] = row0 + lpiv[tid];
     if (tid == 0 && zero_pivot) atomicOr(status, (unsigned)GRIM_DEV_ZERO_PIVOT);
+    // The panel's last leaf (perm given) also builds the panel's interchange
+    // list, on wave 1 while wave 0 builds the leaf's own.  Pivots of earlier
+    // leaves come from memory (earlier launches), this leaf's from LDS.
+    if (perm != nullptr && wave == 1) {
+        const int pl = lane < lc0 ? piv[lane - lc0] : lane < lc0 + w ? row0 + lpiv[lane - lc0] : 0;
+        panel_list(row0 - lc0, pw, pl, perm + sperm * blockIdx.x, list + sperm * blockIdx.x);
+    }
     if (pw == w) return;
     // the leaf's interchanges on the panel's other columns (rows local to r0)
     if (wave == 0) {
@@ -730,18 +758,8 @@ __global__ void __launch_bounds__(64) panel_perm(int pj, int pw, const int* __re
                                                  int* __restrict__ perm, int64_t sperm, int* __restrict__ list,
                                                  int64_t slist) {
     piv += spiv * blockIdx.x;
-    perm += sperm * blockIdx.x;
-    list += slist * blockIdx.x;
     const int lane = threadIdx.x;
-    const Composite x = wave_composite(pj, pw, piv + pj);
-    if (lane == 0) list[0] = x.nlow;
-    list[1 + lane] = x.top;
-    list[65 + lane] = x.ldst;
-    list[129 + lane] = x.lsrc;
-    const int vt = lane < pw ? perm[x.top] : 0;
-    const int vl = lane < x.nlow ? perm[x.lsrc] : 0;
-    if (lane < pw) perm[pj + lane] = vt;
-    if (lane < x.nlow) perm[x.ldst] = vl;
+    panel_list(pj, pw, lane < pw ? piv[pj + lane] : 0, perm + sperm * blockIdx.x, list + slist * blockIdx.x);
 }
 
 __global__ void __launch_bounds__(256) perm_init(int n, int* __restrict__ perm, int64_t sperm,
@@ -1166,7 +1184,8 @@ static hipError_t gemm(const Batch& bt, int m, int n, int k, const T* A, int64_t
 
 template <typename T>
 static hipError_t leaf(const Batch& bt, int m, int w, T* pan, int64_t lda, int64_t sa, int pw, int lc0, int row0,
-                       int* piv, int64_t spiv, unsigned* status, char* tall, int64_t stall, unsigned& arrivals) {
+                       int* piv, int64_t spiv, unsigned* status, char* tall, int64_t stall, unsigned& arrivals,
+                       int* perm, int* list, bool& listed) {
     hipStream_t st = bt.st;
     const int b = bt.count;
     if (m > GRIM_LEAF_MAX_M) {
@@ -1183,8 +1202,12 @@ static hipError_t leaf(const Batch& bt, int m, int w, T* pan, int64_t lda, int64
         arrivals += (unsigned)(w * G);
         return hipGetLastError();
     }
+    // the panel's last leaf also builds the panel's interchange list
+    int* lperm = lc0 + w == pw ? perm : nullptr;
+    if (lperm != nullptr) listed = true;
 #define GRIM_LEAF(R, W) \
-    panel_leaf<T, R, W><<<b, 1024, 0, st>>>(m, w, pan, lda, sa, pw, lc0, row0, piv, spiv, status)
+    panel_leaf<T, R, W><<<b, 1024, 0, st>>>(m, w, pan, lda, sa, pw, lc0, row0, piv, spiv, status, lperm, stall / 4, \
+                                            list)
     if (m <= 1024) GRIM_LEAF(1, 16);
     else if (m <= 2048) {
         // complex64 holds twice the columns in the same registers (here and below): half the
@@ -1242,16 +1265,17 @@ static hipError_t trsm(const Batch& bt, int w, int ncols, const T* tri, int64_t 
 template <typename T>
 static hipError_t panel_rec(const Batch& bt, int n, T* a, int64_t lda, int64_t sa, int pj, int pw, int j0, int w,
                             int* piv, int64_t spiv, unsigned* status, char* tall, int64_t stall,
-                            unsigned& arrivals) {
+                            unsigned& arrivals, int* perm, int* list, bool& listed) {
     const int m = n - j0;
     const int lw = leaf_width<T>(m);
     T* top = a + 2 * ((int64_t)j0 + (int64_t)j0 * lda);
     if (w <= lw)
         return leaf<T>(bt, m, w, a + 2 * ((int64_t)j0 + (int64_t)pj * lda), lda, sa, pw, j0 - pj, j0, piv + j0,
-                       spiv, status, tall, stall, arrivals);
+                       spiv, status, tall, stall, arrivals, perm, list, listed);
     int w1 = w / 2;
     if (w1 > lw) w1 = (w1 / lw) * lw;
-    hipError_t err = panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0, w1, piv, spiv, status, tall, stall, arrivals);
+    hipError_t err =
+        panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0, w1, piv, spiv, status, tall, stall, arrivals, perm, list, listed);
     if (err != hipSuccess) return err;
     const int w2 = w - w1;
     T* right = a + 2 * ((int64_t)(j0 + w1) * lda);
@@ -1259,7 +1283,8 @@ static hipError_t panel_rec(const Batch& bt, int n, T* a, int64_t lda, int64_t s
     if ((err = gemm<T>(bt, m - w1, w2, w1, top + 2 * (int64_t)w1, lda, sa, right + 2 * (int64_t)j0, lda, sa,
                        right + 2 * (int64_t)(j0 + w1), lda, sa)) != hipSuccess)
         return err;
-    return panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0 + w1, w2, piv, spiv, status, tall, stall, arrivals);
+    return panel_rec<T>(bt, n, a, lda, sa, pj, pw, j0 + w1, w2, piv, spiv, status, tall, stall, arrivals, perm, list,
+                        listed);
 }
 
 // Factor-side auxiliary storage per matrix: the row permutation the pivots
@@ -1283,11 +1308,14 @@ static hipError_t lu_factor(const Batch& bt, int n, T* a, int64_t lda, int64_t s
     unsigned arrivals = 0;
     for (int j = 0; j < n; j += GRIM_LU_NB) {
         const int jb = min(GRIM_LU_NB, n - j);
-        if ((err = panel_rec<T>(bt, n, a, lda, sa, j, jb, j, jb, piv, spiv, status, tall, saux, arrivals)) !=
-            hipSuccess)
+        bool listed = false;
+        if ((err = panel_rec<T>(bt, n, a, lda, sa, j, jb, j, jb, piv, spiv, status, tall, saux, arrivals, perm, list,
+                                listed)) != hipSuccess)
             return err;
-        panel_perm<<<bt.count, 64, 0, bt.st>>>(j, jb, piv, spiv, perm, sperm, list, sperm);
-        if ((err = hipGetLastError()) != hipSuccess) return err;
+        if (!listed) {   // the last leaf was a tall one
+            panel_perm<<<bt.count, 64, 0, bt.st>>>(j, jb, piv, spiv, perm, sperm, list, sperm);
+            if ((err = hipGetLastError()) != hipSuccess) return err;
+        }
         // the panel's interchanges on the columns left of it (LAPACK keeps L permuted)
         if (j > 0) {
             apply_list<T><<<dim3((j + 3) / 4, 1, bt.count), 256, 0, bt.st>>>(j, j, jb, a, lda, sa, list, sperm);
