@@ -51,15 +51,14 @@ template <> struct Mfma<float> {
 
 // ---------------------------------------------------------------------------
 // C[m x n] (-)= A[m x k] * B[k x n]   (complex, column-major, interleaved)
-// SET: C = A B, else C -= A B.  256 threads, 64 x 64 output tile, 4 waves of
-// 32 x 32 (2 x 2 MFMA tiles); k staged through LDS GB_K<T> at a time (16 for
+// SET: C = A B, else C -= A B.  256 threads, (32 MI) x 64 output tile, 4 waves of
+// (16 MI) x 32 (MI x 2 MFMA tiles, MI = 1 or 2); k staged through LDS GB_K<T> at a time (16 for
 // complex128, 32 for complex64) with the next k-slab prefetched into
 // registers while the current one is multiplied.  The MFMA computes the
 // transposed tile (B^T A^T), so a lane's outputs run down a column of C and
 // the read-modify-write of C is coalesced.  blockIdx.z selects the matrix of a
 // strided batch (strides in complex elements).
 // ---------------------------------------------------------------------------
-#define GB_M 64
 #define GB_N 64
 template <typename T> struct Cplx2;
 template <> struct Cplx2<double> { typedef double2 type; };
@@ -69,41 +68,42 @@ template <typename T> constexpr int gb_k() { return sizeof(T) == 8 ? 16 : 32; }
 // complex128 without the C read-ahead is held to 3 waves per SIMD (134 VGPRs instead of 210):
 // the 2000 x 720 x 2000 refinement residual's 768 workgroups then run in one round
 // (942 -> 845 us, measured); the other forms keep the compiler's choice.
-template <typename T, bool SET, bool PRE = false>
+template <typename T, bool SET, bool PRE = false, int MI = 2>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && !PRE ? 3 : 1))) cgemm(int m, int n, int k, const T* __restrict__ A, int64_t lda, int64_t sa,
                                              const T* __restrict__ B, int64_t ldb, int64_t sb, T* __restrict__ C,
                                              int64_t ldc, int64_t sc) {
     typedef typename Mfma<T>::acc acc_t;
     typedef typename Cplx2<T>::type v2;
     constexpr int BK = gb_k<T>();
-    constexpr int PA = BK / 4;   // complex A elements each thread stages per k-slab (64 x BK / 256)
+    constexpr int BM = 32 * MI;   // tile rows: each wave covers MI 16-row MFMA tiles
+    constexpr int PA = BK * BM / 256;   // complex A elements each thread stages per k-slab
     constexpr int PB = BK / 4;   // complex B elements (BK x 64 / 256)
-    __shared__ T a_re[BK][GB_M + 1], a_im[BK][GB_M + 1];
+    __shared__ T a_re[BK][BM + 1], a_im[BK][BM + 1];
     __shared__ T b_re[BK][GB_N + 1], b_im[BK][GB_N + 1];
     A += 2 * sa * blockIdx.z;
     B += 2 * sb * blockIdx.z;
     C += 2 * sc * blockIdx.z;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.x * GB_M, n0 = blockIdx.y * GB_N;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    acc_t cr[2][2], ci[2][2];
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * GB_N;
+    const int wm = (wave & 1) * 16 * MI, wn = (wave >> 1) * 32;
+    acc_t cr[MI][2], ci[MI][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             cr[i][j] = acc_t{0, 0, 0, 0};
             ci[i][j] = acc_t{0, 0, 0, 0};
         }
     const int fr = lane & 15, fk = lane >> 4;
-    // staging map: A rows tid & 63, columns (tid >> 6) + 4 i;  B rows tid % BK, columns tid / BK + (256 / BK) i
-    const int ar0 = tid & 63, ac0 = tid >> 6;
+    // staging map: A rows tid % BM, columns tid / BM + (256 / BM) i;  B rows tid % BK, columns tid / BK + (256 / BK) i
+    const int ar0 = tid % BM, ac0 = tid / BM;
     const int br0 = tid % BK, bc0 = tid / BK;
     v2 pa[PA], pb[PB];
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < PA; ++i) {
-            const int gr = m0 + ar0, gc = k0 + ac0 + 4 * i;
+            const int gr = m0 + ar0, gc = k0 + ac0 + (256 / BM) * i;
             pa[i] = (gr < m && gc < k) ? *(const v2*)(A + 2 * ((int64_t)gr + (int64_t)gc * lda)) : v2{0, 0};
         }
 #pragma unroll
@@ -114,10 +114,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     };
     // PRE: this thread's C values are read before the k-loop, so a short update (k <= 64, the
     // LU's rank-64 steps) overlaps their latency with the MFMA work instead of the epilogue
-    v2 cpre[2][2][4];
+    v2 cpre[MI][2][4];
     if (PRE && !SET) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < MI; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -132,8 +132,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     for (int k0 = 0; k0 < k; k0 += BK) {
 #pragma unroll
         for (int i = 0; i < PA; ++i) {
-            a_re[ac0 + 4 * i][ar0] = pa[i].x;
-            a_im[ac0 + 4 * i][ar0] = pa[i].y;
+            a_re[ac0 + (256 / BM) * i][ar0] = pa[i].x;
+            a_im[ac0 + (256 / BM) * i][ar0] = pa[i].y;
         }
 #pragma unroll
         for (int i = 0; i < PB; ++i) {
@@ -144,9 +144,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
         if (k0 + BK < k) fetch(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            T ar[2], ai[2], br[2], bi[2], nbi[2];
+            T ar[MI], ai[MI], br[2], bi[2], nbi[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < MI; ++i) {
                 ar[i] = a_re[kk + fk][wm + 16 * i + fr];
                 ai[i] = a_im[kk + fk][wm + 16 * i + fr];
             }
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
             }
             // transposed tile: D[j-col][i-row] = sum_k B[k][j] A[i][k]
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < MI; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     cr[i][j] = Mfma<T>::run(br[j], ar[i], cr[i][j]);
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     // lane holds D[row(lane / 16, v)][lane % 16] of each transposed 16 x 16 tile:
     // C row = lane % 16 (consecutive lanes, consecutive rows), C column = row(lane / 16, v)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -1166,15 +1166,29 @@ struct Batch {
     hipStream_t st;
 };
 
+// 32-row output tiles for short updates (k <= 256: the LU's rank-64 steps and the solve's
+// between-block products), twice the workgroups of 64-row tiles.  Measured at N = 2000, two
+// systems, 720 columns: the seven between-block products 338 -> 311 us (complex64) and
+// 529 -> 457 us (complex128), the complex128 factor 9.13 -> 8.70 ms.  Long products
+// (k = 2000 residual: 444 -> 496 us complex64) keep 64-row tiles and their operand reuse.
+static bool gemm_narrow_rows(int m, int k) { return k <= 256 && m <= 65535 * 32; }
+
 template <typename T, bool SET = false>
 static hipError_t gemm(const Batch& bt, int m, int n, int k, const T* A, int64_t lda, int64_t sa, const T* B,
                        int64_t ldb, int64_t sb, T* C, int64_t ldc, int64_t sc) {
     if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
-    dim3 grid((m + GB_M - 1) / GB_M, (n + GB_N - 1) / GB_N, bt.count);
+    const bool narrow = gemm_narrow_rows(m, k);
+    const int bm = narrow ? 32 : 64;
+    dim3 grid((m + bm - 1) / bm, (n + GB_N - 1) / GB_N, bt.count);
     // complex64 updates with k <= 64 read C ahead (measured: 1000^2 x 64 19 -> 16 us); complex128
     // keeps the epilogue read (the prefetch registers cost it occupancy)
-    if (sizeof(T) == 4 && !SET && k <= 64)
+    const bool pre = sizeof(T) == 4 && !SET && k <= 64;
+    if (pre && narrow)
+        cgemm<T, SET, true, 1><<<grid, 256, 0, bt.st>>>(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    else if (pre)
         cgemm<T, SET, true><<<grid, 256, 0, bt.st>>>(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    else if (narrow)
+        cgemm<T, SET, false, 1><<<grid, 256, 0, bt.st>>>(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc);
     else
         cgemm<T, SET><<<grid, 256, 0, bt.st>>>(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc);
     return hipGetLastError();
