@@ -27,7 +27,10 @@ def _lib():
 
 
 @pytest.mark.parametrize("prec,tol", [(64, 1e-13), (32, 2e-5)])
-@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (130, 77, 35), (200, 300, 129), (1, 5, 3)])
+# k <= 256 runs on 32-row output tiles, longer products on 64-row tiles: both sides of that
+# threshold, with row counts that are not multiples of either tile
+@pytest.mark.parametrize("m,n,k", [(64, 64, 64), (130, 77, 35), (200, 300, 129), (1, 5, 3), (97, 131, 256),
+                                   (161, 70, 257), (300, 200, 300)])
 def test_gemm_minus(prec, tol, m, n, k):
     dev = _dev()
     native = _lib()
