@@ -425,6 +425,7 @@ extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const 
             return e;
         }));
     }
+    if (steps_out) *steps_out = max_steps;   // the correction limit, as distinct from a stall
     return GRIM_ERR_NOT_CONVERGED;
 }
 

@@ -168,6 +168,8 @@ class Factor:
     z: torch.Tensor | None      # (B, N, N) complex128 systems (for residuals)
     n: int
     batch: int
+    fallback_reason: str = ""   # why a mixed factorization was replaced by the FP64 one
+    parts: list | None = None   # per-system factors once some systems of a batch fell back
 
     @property
     def prec(self) -> int:
@@ -205,10 +207,57 @@ def factor(z: torch.Tensor, precision: str = "double", keep_matrix: bool = True,
                                                                n, aux.data_ptr(), aux_bytes, word.data_ptr(), st))
     if check:
         code = native.lib().grim_workspace_status(word.data_ptr(), st)
+        if code == 8 and prec == 32 and b > 1:
+            # only the systems whose complex64 LU is singular fall back (linalg/dense.py:86-91)
+            del lu
+            parts = [factor(zb[i:i + 1], "mixed", True, status, True) for i in range(b)]
+            return Factor(precision, None, None, None, zb, n, b,
+                          next((p.fallback_reason for p in parts if p.fallback_reason), ""), parts=parts)
+        if code == 8 and prec == 32:
+            del lu
+            f = factor(zb, "double", keep_matrix, status, True)
+            f.fallback_reason = MIXED_FALLBACK + "zero pivot in the single-precision LU."
+            return f
         if code == 8:
             raise np.linalg.LinAlgError("zero pivot in the LU factorization")
         native.check("grim_lu_factor (device)", code)
     return Factor(precision, lu, piv, aux, zb if keep_matrix else None, n, b)
+
+
+MIXED_FALLBACK = "Mixed precision fell back to double LU: "   # linalg/dense.py:90, 229
+
+
+def system(f: Factor, i: int) -> Factor:
+    """Factor of system i of a batch (views, no copies)."""
+    if f.parts is not None:
+        return f.parts[i]
+    return Factor(f.precision, f.lu[i:i + 1], f.piv[i:i + 1], f.aux[i:i + 1],
+                  None if f.z is None else f.z[i:i + 1], f.n, 1, f.fallback_reason)
+
+
+def _solve_parts(parts, bb: torch.Tensor, gate: bool):
+    """Solve each system with its own factor and merge the results into one batch."""
+    xs, infos = [], []
+    for i, part in enumerate(parts):
+        xi, inf = solve(part, bb[i:i + 1], gate)
+        xs.append(xi)
+        infos.append(inf)
+    new_parts = [inf.pop("factor", part) for inf, part in zip(infos, parts)]
+    info = {"refinement_steps": max(inf.get("refinement_steps", 0) for inf in infos),
+            "residual": torch.cat([inf["residual"] for inf in infos]),
+            "precisions": [p.precision for p in new_parts]}
+    reasons = [p.fallback_reason for p in new_parts if p.fallback_reason]
+    if reasons:
+        info["fallback_reason"] = reasons[0]
+        info["fallback_reasons"] = reasons
+    errs = [inf["refinement_error"] for inf in infos if "refinement_error" in inf]
+    if errs:
+        info["refinement_error"] = max(errs)
+    if any(a is not b for a, b in zip(new_parts, parts)):
+        head = parts[0]
+        info["factor"] = Factor("mixed", None, None, None, torch.cat([p.z for p in new_parts]), head.n,
+                                len(new_parts), reasons[0] if reasons else "", parts=new_parts)
+    return torch.cat(xs), info
 
 
 def lu_solve_inplace(f: Factor, x: torch.Tensor) -> torch.Tensor:
@@ -241,10 +290,15 @@ def solve(f: Factor, b: torch.Tensor, gate: bool = True):
     dev = b.device
     st = _stream(dev)
     info = {"refinement_steps": 0}
+    if f.parts is not None:
+        x, info = _solve_parts(f.parts, bb, gate)
+        return (x if b.dim() == 3 else x[0]), info
     if f.precision == "double":
         x = lu_solve_inplace(f, bb.clone())
         if gate and f.z is not None:
             x, info = _backward_error_gate(f, x, bb)
+        if f.fallback_reason:
+            info["fallback_reason"] = f.fallback_reason
         return (x if b.dim() == 3 else x[0]), info
     x = torch.empty_like(bb)
     nbytes = native.lib().grim_refine_workspace_bytes(nb, n, na)
@@ -258,7 +312,20 @@ def solve(f: Factor, b: torch.Tensor, gate: bool = True):
     info["refinement_steps"] = int(steps.value)
     info["refinement_error"] = float(err.value)
     if code == 6:
-        raise np.linalg.LinAlgError("Mixed-precision refinement stalled; double LU required.")
+        # RefinedLU raised; DenseFactor refactors in complex128 (linalg/dense.py:223-230).  Each
+        # system of a batch falls back on its own, so a batched call still returns what
+        # separate calls return.
+        if nb > 1:
+            x, info = _solve_parts([system(f, i) for i in range(nb)], bb, gate)
+            return (x if b.dim() == 3 else x[0]), info
+        why = ("Mixed-precision refinement exceeded its correction limit." if steps.value >= REFINE_MAX_STEPS
+               else "Mixed-precision refinement stalled; double LU required.")
+        fb = factor(f.z, "double")
+        fb.fallback_reason = MIXED_FALLBACK + why
+        x, info = solve(fb, bb, True)
+        info["fallback_reason"] = fb.fallback_reason
+        info["factor"] = fb
+        return (x if b.dim() == 3 else x[0]), info
     native.check("grim_refined_solve", code)
     # the workspace starts with the final residual b - Z x of the returned x
     info["residual"] = ws[: 16 * nb * na * n].view(torch.complex128).view(nb, na, n)

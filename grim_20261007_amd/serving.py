@@ -39,7 +39,7 @@ class Request:
     frequency_ghz: float
     angles_deg: np.ndarray
     pols: Tuple[str, ...] = ("TE", "TM")
-    precision: str = "mixed"          # "mixed", "double" or "auto" (choose_precision)
+    precision: str = "auto"           # "auto" (choose_precision), "mixed" or "double"
     arrival: float = 0.0
     rid: int = 0
     key: Optional[str] = None          # cache key; None -> geometry digest
@@ -116,6 +116,9 @@ class SystemCache:
 
     def put(self, key, item, nbytes):
         with self.lock:
+            old = self.items.pop(key, None)           # a replaced entry (e.g. after a fallback)
+            if old is not None:
+                self.used -= old["bytes"]
             if nbytes > self.budget:
                 return False
             while self.items and self.used + nbytes > self.budget:
@@ -140,6 +143,11 @@ class Server:
         self.cache = SystemCache(cache_bytes if cache_bytes is not None else int(0.25 * total))
         self.in_use = 0
         self.cv = threading.Condition()
+        # single flight: a key being assembled and factored maps to an event that the request
+        # doing the work sets once the factors are in the cache (or it failed)
+        self.inflight: Dict[tuple, threading.Event] = {}
+        self.inflight_lock = threading.Lock()
+        self.waited = 0
         self.queue: "queue.Queue" = queue.Queue()
         self.results: List[Result] = []
         self.workers = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(workers)]
@@ -198,13 +206,40 @@ class Server:
             req.precision = choose_precision(mesh.n, len(req.angles_deg))
         k = wavenumber(req.frequency_ghz)
         key = (req.key or geometry_digest(mesh), float(k), req.precision)
-        cached = self.cache.get(key)
+        while True:
+            with self.inflight_lock:
+                cached = self.cache.get(key)
+                pending = None if cached is not None else self.inflight.get(key)
+                if pending is not None and pending.is_set():
+                    pending = None      # its factors were stored and already evicted: redo them
+                owner = cached is None and pending is None
+                if owner:
+                    self.inflight[key] = threading.Event()
+            if pending is None:
+                break
+            # another worker is assembling and factoring this system: wait for its factors
+            # instead of repeating the work, then look again (if it failed or the system did
+            # not fit the cache, this request becomes the owner)
+            self.waited += 1
+            pending.wait()
+        done = self.inflight.get(key) if owner else None
+
+        def store(item, nb):
+            if self.cache.put(key, item, nb) and done is not None:
+                done.set()
+
         nbytes = request_bytes(mesh.n, len(req.angles_deg), len(req.pols), req.precision, cached is not None)
-        self._admit(nbytes)
         try:
-            return execute(req, self.device, ws, cached, lambda item, nb: self.cache.put(key, item, nb))
+            self._admit(nbytes)
+            try:
+                return execute(req, self.device, ws, cached, store)
+            finally:
+                self._release(nbytes)
         finally:
-            self._release(nbytes)
+            if owner:
+                with self.inflight_lock:
+                    self.inflight.pop(key, None)
+                done.set()
 
 
 def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store=None) -> Result:
@@ -240,6 +275,10 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     rhs = (torch.stack((-bdn, -bu)) if mesh.impedance is None     # [TE, TM]
            else engine.impedance_rhs(contour, k, bu, bdn))
     x, inf = engine.solve(fac, rhs, gate=(req.precision == "double"))
+    if inf.get("factor") is not None and store is not None:
+        # the mixed factorization fell back to FP64: later requests reuse the FP64 factors
+        fac = inf.pop("factor")
+        store({"contour": contour, "factor": fac}, 2 * 32 * mesh.n * mesh.n)
     rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
     amp, sig, resid, info = {}, {}, {}, {}
     pending = []

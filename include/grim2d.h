@@ -130,7 +130,8 @@ int64_t grim_refine_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs);
  * corrections while the others continue, so a batched call returns what
  * separate calls return (at most max_steps corrections; a step that does not
  * reduce a matrix's error below 0.98 of its previous one returns 6).
- * steps_out: corrections of the slowest matrix; error_out: largest final error.
+ * steps_out: corrections of the slowest matrix (max_steps when a matrix used every
+ * correction without reaching tol, as distinct from a stall); error_out: largest final error.
  * z: complex128 systems; lu/aux: complex64 factors of the same matrices.
  * Synchronises the stream once per step.  steps_out / error_out: host
  * pointers (may be NULL).  On GRIM_OK the workspace starts with the final

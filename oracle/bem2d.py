@@ -446,9 +446,19 @@ def _far_rows(mesh: Mesh, k: float, rows: np.ndarray, far: np.ndarray, q: int):
     return s * scale * m, kp * scale * m, d * scale * m
 
 
-def assemble(mesh: Mesh, k: float, rows_per_chunk: int = 16):
-    """TM and TE system matrices of one closed PEC contour (complex128, node order)."""
+def assemble(mesh: Mesh, k: float, rows_per_chunk: int = 16, node_rows=None):
+    """TM and TE system matrices of one closed PEC contour (complex128, node order).
+
+    With ``node_rows`` only the element rows that touch those nodes are traversed, so the
+    returned matrices are exact on those rows (and partial elsewhere): a row sample of a
+    system too large for the full CPU assembly."""
     E = len(mesh.length)
+    if node_rows is None:
+        erows = np.arange(E)
+    else:
+        want = np.zeros(len(mesh.nodes), bool)
+        want[np.asarray(node_rows)] = True
+        erows = np.flatnonzero(want[mesh.elements[:, 0]] | want[mesh.elements[:, 1]])
     N = len(mesh.nodes)
     gamma = 1.0 / (1j * k)
     q = far_order(mesh, k)
@@ -472,18 +482,23 @@ def assemble(mesh: Mesh, k: float, rows_per_chunk: int = 16):
                 np.add.at(z_tm, (r[:, None], c[None, :]), tm[:, :, a, b])
                 np.add.at(z_te, (r[:, None], c[None, :]), te[:, :, a, b])
 
-    for start in range(0, E, rows_per_chunk):
-        rows = np.arange(start, min(E, start + rows_per_chunk))
+    for start in range(0, len(erows), rows_per_chunk):
+        rows = erows[start:start + rows_per_chunk]
         far = far_mask(mesh, rows)
         s, kp, d = _far_rows(mesh, k, rows, far, q)
         scatter(rows, s, kp, d)
 
     # near pairs: S and K' per ordered pair; D from the reverse pair's K'.
-    near = np.argwhere(~far_mask(mesh))
+    near = np.argwhere(~far_mask(mesh, erows))
+    near[:, 0] = erows[near[:, 0]]
     blocks = {}
     for e, f in near:
-        blocks[(int(e), int(f))] = near_blocks(mesh, k, int(e), int(f))
-    for (e, f), (s, kp) in blocks.items():
+        for pair in ((int(e), int(f)), (int(f), int(e))):
+            if pair not in blocks:
+                blocks[pair] = near_blocks(mesh, k, *pair)
+    for e, f in near:
+        e, f = int(e), int(f)
+        s, kp = blocks[(e, f)]
         d = blocks[(f, e)][1].T
         w = maue(s, k, mesh.normal[e], mesh.normal[f], mesh.length[e], mesh.length[f])
         tm = s + gamma * d
@@ -493,7 +508,7 @@ def assemble(mesh: Mesh, k: float, rows_per_chunk: int = 16):
                 z_tm[ids[e, a], ids[f, b]] += tm[a, b]
                 z_te[ids[e, a], ids[f, b]] += te[a, b]
     # mass / jump terms
-    for e in range(E):
+    for e in erows:
         m = mesh.length[e] * np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
         for a in range(2):
             for b in range(2):

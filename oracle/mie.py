@@ -1,4 +1,4 @@
-"""CPU ORACLE -- test infrastructure only.
+"""CPU ORACLE -- test infrastructure only, never part of the product path.
 
 Closed-form Mie (cylindrical-wave) series for the monostatic 2-D scattering of
 a PEC circular cylinder, in the solver's amplitude convention: the stored
@@ -7,8 +7,21 @@ with a_n = -J_n(ka) / H_n^(2)(ka) (TM, Dirichlet) or
 a_n = -J_n'(ka) / H_n^(2)'(ka) (TE, Neumann), n from -M to M,
 M = ceil(ka + 4.05 (ka)^(1/3) + 2) + 10 (Wiscombe's rule plus a pad).
 Follows tools/GHOST/ghost_backend/validation/cylinder.py:30-90.
-Pinned in tests/test_oracle_mie.py by the small-ka Rayleigh limit and by the
-reference solver's own cylinder fixtures.
+
+Pinned in tests/test_oracle_mie.py, without running the reference's code:
+* the small-ka (Rayleigh) limits derived independently of the series' sum:
+  TM B -> -4j a_0 with a_0 = -1 / (1 - (2j/pi)(ln(ka/2) + gamma_E)), and
+  TE B -> 3 pi (ka)^2 (the n = 0 and n = +-1 terms, so a wrong sign of the
+  (-1)^n factor or of the n = +-1 coefficients fails);
+* losslessness of every PEC coefficient, |1 + 2 a_n| = 1;
+* the reference's own cylinder gates (tests/test_rcs_physics_regression.py:
+  648-669 and 671-719) re-run with the oracle's BEM solve, itself pinned to
+  the reference's golden vectors: the 96-chord ka = 1 circle within 0.01 dB
+  in both polarizations, and the complex amplitude of the 64-chord circle at
+  the first zero of J_0 within 5e-3;
+* the reference-written golden amplitudes of the 64-chord ka = 1 and the
+  400-chord ka = 10 circles (tests/golden), which agree with the series to
+  the discretisation error (1.7e-3 and 4.3e-4 relative).
 """
 import math
 
@@ -23,8 +36,8 @@ def terms_for_ka(ka: float, pad: int = 10) -> int:
     return max(10, n + pad)
 
 
-def pec_backscatter_amplitude(radius: float, k: float, pol: str) -> complex:
-    ka = k * radius
+def pec_coefficients(ka: float, pol: str):
+    """(n, a_n) for n = -M..M."""
     m = terms_for_ka(ka)
     n = np.arange(-m, m + 1)
     if pol.upper() == "TM":
@@ -33,6 +46,11 @@ def pec_backscatter_amplitude(radius: float, k: float, pol: str) -> complex:
         a = -sp.jvp(n, ka, 1) / sp.h2vp(n, ka, 1)
     else:
         raise ValueError(pol)
+    return n, a
+
+
+def pec_backscatter_amplitude(radius: float, k: float, pol: str) -> complex:
+    n, a = pec_coefficients(k * radius, pol)
     return complex(-4.0j * np.sum(a * (-1.0) ** n))
 
 
