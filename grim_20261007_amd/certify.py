@@ -317,10 +317,12 @@ def solve_monostatic_rcs_2d_certified(
                               "mesh_convergence_certified": True, "certified_entry_point": True,
                               "published_mesh": "fine"}
         return merged
+    # the certified entry always estimates the condition number, so its strict gate applies the
+    # condition limit too (twod/solver.py:3734)
     kwargs = dict(frequencies_ghz=frequencies_ghz, elevations_deg=elevations_deg,
                   geometry_units=geometry_units, material_base_dir=material_base_dir,
-                  quality_thresholds=quality_thresholds, strict_quality_gate=True, max_panels=max_panels,
-                  mesh_reference_ghz=mesh_reference_ghz, rcs_normalization_mode=rcs_normalization_mode,
+                  quality_thresholds=quality_thresholds, strict_quality_gate=True, compute_condition_number=True,
+                  max_panels=max_panels, mesh_reference_ghz=mesh_reference_ghz, rcs_normalization_mode=rcs_normalization_mode,
                   abort_event=abort_event, solver_method=solver_method, precision=precision)
     fine_snapshot = certification_fine_snapshot(geometry_snapshot, policy["fine_factor"])
     base, fine = _run_pair(solve_monostatic_rcs_2d, geometry_snapshot, fine_snapshot, kwargs, device)

@@ -44,6 +44,15 @@ extern "C" hipError_t grim_dense_add_c64(int batch, int m, int n, const float* d
 extern "C" hipError_t grim_dense_colmax(int batch, int m, int n, const double* v, int64_t ld, int64_t sv,
                                         double* out, hipStream_t st);
 extern "C" hipError_t grim_dense_colnorm(int m, int n, const double* v, int64_t ld, double* out, hipStream_t st);
+extern "C" int64_t grim_dense_trans_block_elems(void);
+extern "C" hipError_t grim_dense_lu_solve_trans(int batch, int n, int nrhs, const void* a, int64_t lda, int64_t sa,
+                                                int single, const void* aux, int64_t saux, void* b, int64_t ldb,
+                                                int64_t sb, void* w, int64_t sw, void* blk, hipStream_t st);
+extern "C" hipError_t grim_dense_gemm_minus_ta(int batch, int m, int n, int k, const void* a, int64_t lda,
+                                               int64_t sa, const void* b, int64_t ldb, int64_t sb, void* c,
+                                               int64_t ldc, int64_t sc, int single, hipStream_t st);
+extern "C" hipError_t grim_dense_equilibration(int batch, int n, const double* a, int64_t lda, int64_t sa,
+                                               double* row_scale, double* col_scale, double* norm, hipStream_t st);
 
 #define CHECK(x)                                   \
     do {                                           \
@@ -323,35 +332,110 @@ extern "C" int grim_gemm_minus(int64_t batch, int64_t m, int64_t n, int64_t k, c
     return GRIM_OK;
 }
 
+extern "C" int64_t grim_lu_solve_ex_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs, int precision,
+                                                    int trans) {
+    if (trans == 0) return grim_lu_solve_workspace_bytes(batch, n, nrhs, precision);
+    if (trans != 2 || !batch_ok(batch) || n < 1 || nrhs < 1) return -1;
+    const int64_t es = precision == 32 ? 8 : 16;
+    return align_up(batch * n * nrhs * es) + batch * grim_dense_trans_block_elems() * es;
+}
+
+extern "C" int grim_lu_solve_ex(int64_t batch, int64_t n, int64_t nrhs, const void* lu, int64_t ldlu,
+                                int64_t stride_lu, int precision, const void* aux, int64_t stride_aux, int trans,
+                                void* b, int64_t ldb, int64_t stride_b, void* work, int64_t work_bytes, void* stream) {
+    if (trans == 0)
+        return grim_lu_solve(batch, n, nrhs, lu, ldlu, stride_lu, precision, aux, stride_aux, b, ldb, stride_b, work,
+                             work_bytes, stream);
+    if (trans != 2 || !batch_ok(batch) || n < 1 || nrhs < 0 || !lu || !aux || !b || ldlu < n || ldb < n ||
+        nrhs > 65535 || (precision != 64 && precision != 32))
+        return GRIM_ERR_ARGS;
+    if (nrhs == 0) return GRIM_OK;
+    if (!work || work_bytes < grim_lu_solve_ex_workspace_bytes(batch, n, nrhs, precision, trans)) return GRIM_ERR_RANGE;
+    const int64_t es = precision == 32 ? 8 : 16;
+    char* blk = (char*)work + align_up(batch * n * nrhs * es);
+    CHECK(grim_dense_lu_solve_trans((int)batch, (int)n, (int)nrhs, lu, ldlu, stride_lu, precision == 32, aux,
+                                    stride_aux, b, ldb, stride_b, work, n * nrhs, blk, (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+extern "C" int grim_gemm_minus_ex(int64_t batch, int64_t m, int64_t n, int64_t k, int trans_a, const void* a,
+                                  int64_t lda, int64_t stride_a, const void* b, int64_t ldb, int64_t stride_b, void* c,
+                                  int64_t ldc, int64_t stride_c, int precision, void* stream) {
+    if (trans_a == 0)
+        return grim_gemm_minus(batch, m, n, k, a, lda, stride_a, b, ldb, stride_b, c, ldc, stride_c, precision,
+                               stream);
+    if (trans_a != 2 || !batch_ok(batch) || m < 0 || n < 0 || k < 0 || !a || !b || !c || lda < k || ldb < k ||
+        ldc < m || (precision != 64 && precision != 32) || n > 65535 * 64 || m > 65535 * 64)
+        return GRIM_ERR_ARGS;
+    CHECK(grim_dense_gemm_minus_ta((int)batch, (int)m, (int)n, (int)k, a, lda, stride_a, b, ldb, stride_b, c, ldc,
+                                   stride_c, precision == 32, (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+extern "C" int grim_equilibration(int64_t batch, int64_t n, const double* z, int64_t ldz, int64_t stride_z,
+                                  double* row_scale, double* col_scale, double* norm, void* stream) {
+    if (!batch_ok(batch) || n < 1 || n > (1 << 30) || !z || !row_scale || !col_scale || !norm || ldz < n ||
+        (batch > 1 && stride_z < ldz * n))
+        return GRIM_ERR_ARGS;
+    CHECK(grim_dense_equilibration((int)batch, (int)n, z, ldz, stride_z, row_scale, col_scale, norm,
+                                   (hipStream_t)stream));
+    return GRIM_OK;
+}
+
 extern "C" int64_t grim_refine_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs) {
     if (!batch_ok(batch) || n < 1 || nrhs < 1) return -1;
     return align_up(batch * 16 * n * nrhs) + 2 * align_up(batch * 8 * n * nrhs) + 2 * align_up(batch * 8 * nrhs);
 }
 
-extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz,
-                                  int64_t stride_z, const float* lu, int64_t ldlu, int64_t stride_lu, const void* aux,
-                                  int64_t stride_aux, const double* b, int64_t ldb, int64_t stride_b, double* x,
-                                  int64_t ldx, int64_t stride_x, int max_steps, double tol, void* workspace,
-                                  int64_t workspace_bytes, int* steps_out, double* error_out, void* stream) {
+extern "C" int64_t grim_refine_ex_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs, int trans) {
+    const int64_t base = grim_refine_workspace_bytes(batch, n, nrhs);
+    if (base < 0 || (trans != 0 && trans != 2)) return -1;
+    return trans == 0 ? base : base + align_up(batch * 8 * grim_dense_trans_block_elems());
+}
+
+// complex64 solve with the factors (trans 0) or their conjugate transpose (trans 2), and the
+// FP64 residual b - op(Z) x
+static hipError_t refine_lu_solve(int trans, int cnt, int N, int R, const float* lu, int64_t ldlu, int64_t stride_lu,
+                                  const void* aux, int64_t stride_aux, float* d, int64_t snr, float* w, float* blk,
+                                  hipStream_t st) {
+    if (trans == 0)
+        return grim_dense_lu_solve(cnt, N, R, lu, ldlu, stride_lu, 1, aux, stride_aux, d, N, snr, w, snr, st);
+    return grim_dense_lu_solve_trans(cnt, N, R, lu, ldlu, stride_lu, 1, aux, stride_aux, d, N, snr, w, snr, blk, st);
+}
+
+static hipError_t refine_residual(int trans, int cnt, int N, int R, const double* z, int64_t ldz, int64_t stride_z,
+                                  const double* x, int64_t ldx, int64_t stride_x, double* r, int64_t snr,
+                                  hipStream_t st) {
+    if (trans == 0) return grim_dense_gemm_minus(cnt, N, R, N, z, ldz, stride_z, x, ldx, stride_x, r, N, snr, 0, st);
+    return grim_dense_gemm_minus_ta(cnt, N, R, N, z, ldz, stride_z, x, ldx, stride_x, r, N, snr, 0, st);
+}
+
+static int refined_solve_impl(int trans, int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz,
+                              int64_t stride_z, const float* lu, int64_t ldlu, int64_t stride_lu, const void* aux,
+                              int64_t stride_aux, const double* b, int64_t ldb, int64_t stride_b, double* x,
+                              int64_t ldx, int64_t stride_x, int max_steps, double tol, void* workspace,
+                              int64_t workspace_bytes, int* steps_out, double* error_out, void* stream) {
     if (!batch_ok(batch) || n < 1 || nrhs < 1 || !z || !lu || !aux || !b || !x || ldz < n || ldlu < n || ldb < n ||
-        ldx < n || max_steps < 0 || !(tol > 0) || !workspace || nrhs > 65535)
+        ldx < n || max_steps < 0 || !(tol > 0) || !workspace || nrhs > 65535 || (trans != 0 && trans != 2))
         return GRIM_ERR_ARGS;
-    if (grim_refine_workspace_bytes(batch, n, nrhs) > workspace_bytes) return GRIM_ERR_RANGE;
+    if (grim_refine_ex_workspace_bytes(batch, n, nrhs, trans) > workspace_bytes) return GRIM_ERR_RANGE;
     hipStream_t st = (hipStream_t)stream;
     const int B = (int)batch, N = (int)n, R = (int)nrhs;
     const int64_t snr = n * nrhs;   // per-matrix stride of the n x nrhs work arrays
+    const int64_t sblk = 2 * grim_dense_trans_block_elems();   // floats per matrix
     char* p = (char*)workspace;
     double* r = (double*)p; p += align_up(batch * 16 * n * nrhs);
     float* d = (float*)p; p += align_up(batch * 8 * n * nrhs);
     float* w = (float*)p; p += align_up(batch * 8 * n * nrhs);
     double* bmax = (double*)p; p += align_up(batch * 8 * nrhs);
-    double* rmax = (double*)p;
+    double* rmax = (double*)p; p += align_up(batch * 8 * nrhs);
+    float* blk = (float*)p;
     std::vector<double> hb((size_t)B * R), hr((size_t)B * R);
     CHECK(grim_dense_colmax(B, N, R, b, ldb, stride_b, bmax, st));
     CHECK(hipMemcpyAsync(hb.data(), bmax, 8 * (size_t)B * R, hipMemcpyDeviceToHost, st));
     // x = LU32^-1 b
     CHECK(grim_dense_to_c64(B, N, R, b, ldb, stride_b, d, n, snr, st));
-    CHECK(grim_dense_lu_solve(B, N, R, lu, ldlu, stride_lu, 1, aux, stride_aux, d, n, snr, w, snr, st));
+    CHECK(refine_lu_solve(trans, B, N, R, lu, ldlu, stride_lu, aux, stride_aux, d, snr, w, blk, st));
     zero_c128<<<dim3((N + 255) / 256, R, B), 256, 0, st>>>(N, R, x, ldx, stride_x);
     CHECK(hipGetLastError());
     CHECK(grim_dense_add_c64(B, N, R, d, n, snr, x, ldx, stride_x, st));
@@ -380,8 +464,8 @@ extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const 
             hipError_t e = grim_dense_copy(cnt, N, R, b + 2 * stride_b * i, ldb, stride_b, r + 2 * snr * i, n, snr,
                                            st);
             if (e == hipSuccess)
-                e = grim_dense_gemm_minus(cnt, N, R, N, z + 2 * stride_z * i, ldz, stride_z, x + 2 * stride_x * i,
-                                          ldx, stride_x, r + 2 * snr * i, n, snr, 0, st);
+                e = refine_residual(trans, cnt, N, R, z + 2 * stride_z * i, ldz, stride_z, x + 2 * stride_x * i, ldx,
+                                    stride_x, r + 2 * snr * i, snr, st);
             if (e == hipSuccess) e = grim_dense_colmax(cnt, N, R, r + 2 * snr * i, n, snr, rmax + (int64_t)R * i, st);
             return e;
         }));
@@ -417,9 +501,9 @@ extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const 
         CHECK(for_active([&](int i, int cnt) -> hipError_t {
             hipError_t e = grim_dense_to_c64(cnt, N, R, r + 2 * snr * i, n, snr, d + 2 * snr * i, n, snr, st);
             if (e == hipSuccess)
-                e = grim_dense_lu_solve(cnt, N, R, lu + 2 * stride_lu * i, ldlu, stride_lu, 1,
-                                        (const char*)aux + stride_aux * i, stride_aux, d + 2 * snr * i, n, snr,
-                                        w + 2 * snr * i, snr, st);
+                e = refine_lu_solve(trans, cnt, N, R, lu + 2 * stride_lu * i, ldlu, stride_lu,
+                                    (const char*)aux + stride_aux * i, stride_aux, d + 2 * snr * i, snr,
+                                    w + 2 * snr * i, blk + sblk * i, st);
             if (e == hipSuccess)
                 e = grim_dense_add_c64(cnt, N, R, d + 2 * snr * i, n, snr, x + 2 * stride_x * i, ldx, stride_x, st);
             return e;
@@ -427,6 +511,27 @@ extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const 
     }
     if (steps_out) *steps_out = max_steps;   // the correction limit, as distinct from a stall
     return GRIM_ERR_NOT_CONVERGED;
+}
+
+extern "C" int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz,
+                                  int64_t stride_z, const float* lu, int64_t ldlu, int64_t stride_lu, const void* aux,
+                                  int64_t stride_aux, const double* b, int64_t ldb, int64_t stride_b, double* x,
+                                  int64_t ldx, int64_t stride_x, int max_steps, double tol, void* workspace,
+                                  int64_t workspace_bytes, int* steps_out, double* error_out, void* stream) {
+    return refined_solve_impl(0, batch, n, nrhs, z, ldz, stride_z, lu, ldlu, stride_lu, aux, stride_aux, b, ldb,
+                              stride_b, x, ldx, stride_x, max_steps, tol, workspace, workspace_bytes, steps_out,
+                              error_out, stream);
+}
+
+extern "C" int grim_refined_solve_ex(int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz,
+                                     int64_t stride_z, const float* lu, int64_t ldlu, int64_t stride_lu,
+                                     const void* aux, int64_t stride_aux, int trans, const double* b, int64_t ldb,
+                                     int64_t stride_b, double* x, int64_t ldx, int64_t stride_x, int max_steps,
+                                     double tol, void* workspace, int64_t workspace_bytes, int* steps_out,
+                                     double* error_out, void* stream) {
+    return refined_solve_impl(trans, batch, n, nrhs, z, ldz, stride_z, lu, ldlu, stride_lu, aux, stride_aux, b, ldb,
+                              stride_b, x, ldx, stride_x, max_steps, tol, workspace, workspace_bytes, steps_out,
+                              error_out, stream);
 }
 
 extern "C" int grim_to_c64(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds, int64_t stride_src,

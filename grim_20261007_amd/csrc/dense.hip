@@ -68,7 +68,9 @@ template <typename T> constexpr int gb_k() { return sizeof(T) == 8 ? 16 : 32; }
 // complex128 without the C read-ahead is held to 3 waves per SIMD (134 VGPRs instead of 210):
 // the 2000 x 720 x 2000 refinement residual's 768 workgroups then run in one round
 // (942 -> 845 us, measured); the other forms keep the compiler's choice.
-template <typename T, bool SET, bool PRE = false, int MI = 2>
+// TA: A is used conjugate-transposed, C (-)= A^H B with A stored k x m (lda >= k); its k-slab is
+// staged with the k index running along the workgroup's threads, so those reads stay coalesced.
+template <typename T, bool SET, bool PRE = false, int MI = 2, bool TA = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 && !PRE ? 3 : 1))) cgemm(int m, int n, int k, const T* __restrict__ A, int64_t lda, int64_t sa,
                                              const T* __restrict__ B, int64_t ldb, int64_t sb, T* __restrict__ C,
                                              int64_t ldc, int64_t sc) {
@@ -103,8 +105,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     auto fetch = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < PA; ++i) {
-            const int gr = m0 + ar0, gc = k0 + ac0 + (256 / BM) * i;
-            pa[i] = (gr < m && gc < k) ? *(const v2*)(A + 2 * ((int64_t)gr + (int64_t)gc * lda)) : v2{0, 0};
+            if constexpr (TA) {
+                const int gk = k0 + tid % BK, gm = m0 + tid / BK + (256 / BK) * i;
+                v2 v = (gm < m && gk < k) ? *(const v2*)(A + 2 * ((int64_t)gk + (int64_t)gm * lda)) : v2{0, 0};
+                v.y = -v.y;
+                pa[i] = v;
+            } else {
+                const int gr = m0 + ar0, gc = k0 + ac0 + (256 / BM) * i;
+                pa[i] = (gr < m && gc < k) ? *(const v2*)(A + 2 * ((int64_t)gr + (int64_t)gc * lda)) : v2{0, 0};
+            }
         }
 #pragma unroll
         for (int i = 0; i < PB; ++i) {
@@ -132,8 +141,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof
     for (int k0 = 0; k0 < k; k0 += BK) {
 #pragma unroll
         for (int i = 0; i < PA; ++i) {
-            a_re[ac0 + (256 / BM) * i][ar0] = pa[i].x;
-            a_im[ac0 + (256 / BM) * i][ar0] = pa[i].y;
+            if constexpr (TA) {
+                a_re[tid % BK][tid / BK + (256 / BK) * i] = pa[i].x;
+                a_im[tid % BK][tid / BK + (256 / BK) * i] = pa[i].y;
+            } else {
+                a_re[ac0 + (256 / BM) * i][ar0] = pa[i].x;
+                a_im[ac0 + (256 / BM) * i][ar0] = pa[i].y;
+            }
         }
 #pragma unroll
         for (int i = 0; i < PB; ++i) {
@@ -1194,6 +1208,22 @@ static hipError_t gemm(const Batch& bt, int m, int n, int k, const T* A, int64_t
     return hipGetLastError();
 }
 
+// C -= A^H B with A stored k x m (the off-diagonal updates of the conjugate-transposed solve and
+// the residual b - Z^H x).
+template <typename T>
+static hipError_t gemm_ta(const Batch& bt, int m, int n, int k, const T* A, int64_t lda, int64_t sa, const T* B,
+                          int64_t ldb, int64_t sb, T* C, int64_t ldc, int64_t sc) {
+    if (m <= 0 || n <= 0 || k <= 0) return hipSuccess;
+    const bool narrow = gemm_narrow_rows(m, k);
+    const int bm = narrow ? 32 : 64;
+    dim3 grid((m + bm - 1) / bm, (n + GB_N - 1) / GB_N, bt.count);
+    if (narrow)
+        cgemm<T, false, false, 1, true><<<grid, 256, 0, bt.st>>>(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    else
+        cgemm<T, false, false, 2, true><<<grid, 256, 0, bt.st>>>(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    return hipGetLastError();
+}
+
 #define GRIM_LEAF_MAX_M 16384
 
 template <typename T>
@@ -1417,6 +1447,263 @@ static hipError_t lu_solve(const Batch& bt, int n, int nrhs, const T* a, int64_t
                 return err;
     }
     copy_cast<T, T><<<dim3((n + 255) / 256, nrhs, bt.count), 256, 0, bt.st>>>(n, w, ldw, sw, b, ldb, sb);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Conjugate-transposed solve A^H x = b with the factors of P A = L U (scipy lu_solve with
+// trans=2, LAPACK getrs 'C', as the reference's condition estimate and RefinedLU call it):
+// A^H = U^H L^H P, so U^H z = b (lower, forward), L^H w = z (unit upper, backward), x = P^T w.
+// Each 256-row diagonal block is staged conjugate-transposed into a 256 x 256 workspace and
+// solved by solve_block: U_JJ^H as the unit lower triangle U_JJ^H D^{-1} (D = diag(conj U_jj),
+// the rows then scaled by D^{-1}), L_JJ^H as a unit upper triangle (diagonal staged as 1).  The
+// off-diagonal updates are conjugate-transposed GEMMs on the factors in place.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void cdiv(T xr, T xi, T dr, T di, T& qr, T& qi) {   // q = x / d
+    const T den = fma(dr, dr, di * di);
+    qr = fma(xr, dr, xi * di) / den;
+    qi = fma(xi, dr, -(xr * di)) / den;
+}
+
+// W[i, j] (i, j < jw, ld GRIM_SOLVE_NB) from the diagonal block D = a[j0.., j0..]:
+// FROM_U: i > j: conj(D[j, i]) / conj(D[j, j]) (strict lower of U^H D^{-1}), else 0;
+// else:   i < j: conj(D[j, i]) (strict upper of L^H), i == j: 1, else 0.
+// 64 x 64 tiles through LDS: reads down columns of D, writes down columns of W.
+template <typename T, bool FROM_U>
+__global__ void __launch_bounds__(256) stage_trans_block(int jw, const T* __restrict__ d, int64_t lda, int64_t sa,
+                                                         T* __restrict__ w, int64_t sw) {
+    __shared__ T tr[64][65], ti[64][65];   // [column i of the tile of D][row j]
+    __shared__ T gr[64], gi[64];            // diagonal of D for the tile's rows j
+    d += 2 * sa * blockIdx.z;
+    w += 2 * sw * blockIdx.z;
+    const int i0 = 64 * blockIdx.x, j0 = 64 * blockIdx.y;   // W tile rows i0.., columns j0..
+    const int t = threadIdx.x, l = t & 63, q = t >> 6;
+    for (int c = q; c < 64; c += 4) {            // D rows j0 + l, columns i0 + c
+        const int j = j0 + l, i = i0 + c;
+        T x = 0, y = 0;
+        if (j < jw && i < jw) {
+            const T* p = d + 2 * ((int64_t)j + (int64_t)i * lda);
+            x = p[0];
+            y = p[1];
+        }
+        tr[c][l] = x;
+        ti[c][l] = y;
+    }
+    if (FROM_U && t < 64) {
+        const int j = j0 + t;
+        T x = 1, y = 0;
+        if (j < jw) {
+            const T* p = d + 2 * ((int64_t)j + (int64_t)j * lda);
+            x = p[0];
+            y = p[1];
+        }
+        gr[t] = x;
+        gi[t] = y;
+    }
+    __syncthreads();
+    for (int c = q; c < 64; c += 4) {            // W rows i0 + l, columns j0 + c
+        const int i = i0 + l, j = j0 + c;
+        if (i >= jw || j >= jw) continue;
+        const T x = tr[l][c], y = -ti[l][c];     // conj(D[j, i])
+        T vr = 0, vi = 0;
+        if (FROM_U) {
+            if (i > j) cdiv(x, y, gr[c], -gi[c], vr, vi);
+        } else {
+            if (i < j) {
+                vr = x;
+                vi = y;
+            } else if (i == j) {
+                vr = 1;
+            }
+        }
+        T* p = w + 2 * ((int64_t)i + (int64_t)j * GRIM_SOLVE_NB);
+        p[0] = vr;
+        p[1] = vi;
+    }
+}
+
+// x[r, c] /= conj(D[r, r]) for the jw rows of a block (D: diagonal block of U).
+template <typename T>
+__global__ void __launch_bounds__(256) scale_inv_conj_diag(int jw, int ncols, const T* __restrict__ d, int64_t lda,
+                                                           int64_t sa, T* __restrict__ x, int64_t ldx, int64_t sx) {
+    const int r = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+    if (r >= jw || c >= ncols) return;
+    d += 2 * sa * blockIdx.z;
+    x += 2 * sx * blockIdx.z;
+    const T* g = d + 2 * ((int64_t)r + (int64_t)r * lda);
+    T* p = x + 2 * ((int64_t)r + (int64_t)c * ldx);
+    T qr, qi;
+    cdiv(p[0], p[1], g[0], -g[1], qr, qi);
+    p[0] = qr;
+    p[1] = qi;
+}
+
+// b[perm[i], col] = y[i, col]
+template <typename T>
+__global__ void __launch_bounds__(256) scatter_rows(int n, const int* __restrict__ perm, int64_t sperm,
+                                                    const T* __restrict__ y, int64_t ldy, int64_t sy,
+                                                    T* __restrict__ b, int64_t ldb, int64_t sb) {
+    const int i = blockIdx.x * 256 + threadIdx.x, col = blockIdx.y;
+    if (i >= n) return;
+    perm += sperm * blockIdx.z;
+    b += 2 * sb * blockIdx.z;
+    y += 2 * sy * blockIdx.z;
+    const T* s = y + 2 * ((int64_t)i + (int64_t)col * ldy);
+    T* d = b + 2 * ((int64_t)perm[i] + (int64_t)col * ldb);
+    d[0] = s[0];
+    d[1] = s[1];
+}
+
+// Solve A^H x = b in place.  w: n x nrhs per matrix (stride sw); blk: GRIM_SOLVE_NB^2 per matrix.
+template <typename T>
+static hipError_t lu_solve_trans(const Batch& bt, int n, int nrhs, const T* a, int64_t lda, int64_t sa,
+                                 const void* auxp, int64_t saux, T* b, int64_t ldb, int64_t sb, T* w, int64_t sw,
+                                 T* blk) {
+    const int64_t ldw = n, sblk = (int64_t)GRIM_SOLVE_NB * GRIM_SOLVE_NB;
+    const int tiles = GRIM_SOLVE_NB / 64;
+    hipError_t err;
+    copy_cast<T, T><<<dim3((n + 255) / 256, nrhs, bt.count), 256, 0, bt.st>>>(n, b, ldb, sb, w, ldw, sw);
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    // forward: U^H z = b
+    for (int J = 0; J < n; J += GRIM_SOLVE_NB) {
+        const int jw = min(GRIM_SOLVE_NB, n - J);
+        const T* djj = a + 2 * ((int64_t)J + (int64_t)J * lda);
+        stage_trans_block<T, true><<<dim3(tiles, tiles, bt.count), 256, 0, bt.st>>>(jw, djj, lda, sa, blk, sblk);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+        if ((err = solve_diag_block<T, true>(bt, jw, nrhs, blk, GRIM_SOLVE_NB, sblk, w + 2 * (int64_t)J, ldw, sw)) !=
+            hipSuccess)
+            return err;
+        scale_inv_conj_diag<T><<<dim3((jw + 255) / 256, nrhs, bt.count), 256, 0, bt.st>>>(
+            jw, nrhs, djj, lda, sa, w + 2 * (int64_t)J, ldw, sw);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+        const int jend = J + jw;
+        if (jend < n)   // z_{J+} -= (U_{J, J+})^H z_J
+            if ((err = gemm_ta<T>(bt, n - jend, nrhs, jw, a + 2 * ((int64_t)J + (int64_t)jend * lda), lda, sa,
+                                  w + 2 * (int64_t)J, ldw, sw, w + 2 * (int64_t)jend, ldw, sw)) != hipSuccess)
+                return err;
+    }
+    // backward: L^H w = z
+    const int lastJ = ((n - 1) / GRIM_SOLVE_NB) * GRIM_SOLVE_NB;
+    for (int J = lastJ; J >= 0; J -= GRIM_SOLVE_NB) {
+        const int jw = min(GRIM_SOLVE_NB, n - J);
+        const T* djj = a + 2 * ((int64_t)J + (int64_t)J * lda);
+        stage_trans_block<T, false><<<dim3(tiles, tiles, bt.count), 256, 0, bt.st>>>(jw, djj, lda, sa, blk, sblk);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+        if ((err = solve_diag_block<T, false>(bt, jw, nrhs, blk, GRIM_SOLVE_NB, sblk, w + 2 * (int64_t)J, ldw,
+                                              sw)) != hipSuccess)
+            return err;
+        if (J > 0)      // w_{<J} -= (L_{J, <J})^H w_J
+            if ((err = gemm_ta<T>(bt, J, nrhs, jw, a + 2 * (int64_t)J, lda, sa, w + 2 * (int64_t)J, ldw, sw, w, ldw,
+                                  sw)) != hipSuccess)
+                return err;
+    }
+    scatter_rows<T><<<dim3((n + 255) / 256, nrhs, bt.count), 256, 0, bt.st>>>(n, (const int*)auxp, saux / 4, w, ldw,
+                                                                               sw, b, ldb, sb);
+    return hipGetLastError();
+}
+
+extern "C" int64_t grim_dense_trans_block_elems(void) { return (int64_t)GRIM_SOLVE_NB * GRIM_SOLVE_NB; }
+
+extern "C" hipError_t grim_dense_lu_solve_trans(int batch, int n, int nrhs, const void* a, int64_t lda, int64_t sa,
+                                                int single, const void* aux, int64_t saux, void* b, int64_t ldb,
+                                                int64_t sb, void* w, int64_t sw, void* blk, hipStream_t st) {
+    Batch bt{batch, st};
+    return single ? lu_solve_trans<float>(bt, n, nrhs, (const float*)a, lda, sa, aux, saux, (float*)b, ldb, sb,
+                                          (float*)w, sw, (float*)blk)
+                  : lu_solve_trans<double>(bt, n, nrhs, (const double*)a, lda, sa, aux, saux, (double*)b, ldb, sb,
+                                           (double*)w, sw, (double*)blk);
+}
+
+extern "C" hipError_t grim_dense_gemm_minus_ta(int batch, int m, int n, int k, const void* a, int64_t lda,
+                                               int64_t sa, const void* b, int64_t ldb, int64_t sb, void* c,
+                                               int64_t ldc, int64_t sc, int single, hipStream_t st) {
+    Batch bt{batch, st};
+    return single ? gemm_ta<float>(bt, m, n, k, (const float*)a, lda, sa, (const float*)b, ldb, sb, (float*)c, ldc,
+                                   sc)
+                  : gemm_ta<double>(bt, m, n, k, (const double*)a, lda, sa, (const double*)b, ldb, sb, (double*)c,
+                                    ldc, sc);
+}
+
+// ---------------------------------------------------------------------------
+// Row/column equilibration of a complex128 matrix and the equilibrated 1-norm
+// (twod/solver.py:472-525): r_i = max_j |a_ij| (1 if 0), c_j = max_i |a_ij| / r_i (1 if 0),
+// norm = max_j sum_i |a_ij| / r_i / c_j.  The maxima are taken with atomicMax on the bit
+// patterns of non-negative doubles (which order as the values), so they are exact and do not
+// depend on the order of arrival; each column sum is a fixed tree over one workgroup.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) fill_u64(int64_t count, unsigned long long* p) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) p[i] = 0ull;
+}
+
+// rows i of one thread, 64 columns per workgroup column-chunk (grid.y), batch grid.z
+__global__ void __launch_bounds__(256) row_absmax(int n, const double* __restrict__ a, int64_t lda, int64_t sa,
+                                                  unsigned long long* __restrict__ rmax) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    a += 2 * sa * blockIdx.z;
+    rmax += (int64_t)n * blockIdx.z;
+    const int c0 = 64 * blockIdx.y, c1 = min(n, c0 + 64);
+    double m = 0.0;
+    for (int c = c0; c < c1; ++c) {
+        const double* p = a + 2 * ((int64_t)i + (int64_t)c * lda);
+        m = fmax(m, hypot(p[0], p[1]));
+    }
+    atomicMax(rmax + i, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void __launch_bounds__(256) finish_row_scale(int n, int batch, double* __restrict__ r) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)n * batch) return;
+    const double v = r[i];   // the bit pattern written by row_absmax
+    r[i] = v > 0.0 ? v : 1.0;
+}
+
+__device__ __forceinline__ double block_reduce(double v, bool is_max, double* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) sh[t] = is_max ? fmax(sh[t], sh[t + s]) : sh[t] + sh[t + s];
+        __syncthreads();
+    }
+    const double out = sh[0];
+    __syncthreads();
+    return out;
+}
+
+// one workgroup per column j (grid.x), batch grid.z
+__global__ void __launch_bounds__(256) col_equilibrate(int n, const double* __restrict__ a, int64_t lda, int64_t sa,
+                                                       const double* __restrict__ rs, double* __restrict__ cs,
+                                                       unsigned long long* __restrict__ norm) {
+    __shared__ double sh[256];
+    const int j = blockIdx.x;
+    a += 2 * sa * blockIdx.z + 2 * (int64_t)j * lda;
+    rs += (int64_t)n * blockIdx.z;
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, hypot(a[2 * i], a[2 * i + 1]) / rs[i]);
+    double c = block_reduce(m, true, sh);
+    c = c > 0.0 ? c : 1.0;
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) sum += hypot(a[2 * i], a[2 * i + 1]) / rs[i] / c;
+    sum = block_reduce(sum, false, sh);
+    if (threadIdx.x == 0) {
+        cs[(int64_t)n * blockIdx.z + j] = c;
+        atomicMax(norm + blockIdx.z, (unsigned long long)__double_as_longlong(sum));
+    }
+}
+
+extern "C" hipError_t grim_dense_equilibration(int batch, int n, const double* a, int64_t lda, int64_t sa,
+                                               double* row_scale, double* col_scale, double* norm, hipStream_t st) {
+    unsigned long long* rbits = (unsigned long long*)row_scale;
+    unsigned long long* nbits = (unsigned long long*)norm;
+    const int64_t rcount = (int64_t)n * batch;
+    fill_u64<<<(unsigned)((rcount + 255) / 256), 256, 0, st>>>(rcount, rbits);
+    fill_u64<<<1, 256, 0, st>>>(batch, nbits);
+    row_absmax<<<dim3((n + 255) / 256, (n + 63) / 64, batch), 256, 0, st>>>(n, a, lda, sa, rbits);
+    finish_row_scale<<<(unsigned)((rcount + 255) / 256), 256, 0, st>>>(n, batch, row_scale);
+    col_equilibrate<<<dim3(n, 1, batch), 256, 0, st>>>(n, a, lda, sa, row_scale, col_scale, nbits);
     return hipGetLastError();
 }
 

@@ -22,6 +22,20 @@ REFINE_MAX_STEPS = 10        # linalg/refined_lu.py:49
 BACKWARD_ERROR_MAX = 1e-12   # twod/constants.py:24
 
 
+def request_device_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool = False) -> int:
+    """Device bytes one co-polarized request allocates: the two assembled systems and their
+    factors (unless cached), the loads, right-hand sides, solutions and residuals, refinement
+    and assembly workspace."""
+    z = 16 * n * n
+    lu = (8 if precision == "mixed" else 16) * n * n
+    vec = 16 * n * nangles
+    work = 0 if cached else NEAR_PER_ELEMENT * n * 160 + 64 * n
+    refine = (24 * n * nangles) if precision == "mixed" else 0
+    per_pol = vec * 3 + refine                       # rhs, solution, residual
+    systems = 0 if cached else 2 * (z + lu)          # both polarizations assembled together
+    return systems + 2 * vec + npols * per_pol + work
+
+
 def _stream(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 

@@ -41,6 +41,16 @@ SIGNATURES = {
     "grim_refined_solve": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _i64,
                                   _vp, _i64, _i64, _i32, _dbl, _vp, _i64, ctypes.POINTER(_i32),
                                   ctypes.POINTER(_dbl), _vp]),
+    "grim_lu_solve_ex_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32, _i32]),
+    "grim_lu_solve_ex": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _i32, _vp, _i64, _i32, _vp, _i64, _i64, _vp,
+                                _i64, _vp]),
+    "grim_gemm_minus_ex": (_i32, [_i64, _i64, _i64, _i64, _i32, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64,
+                                  _i32, _vp]),
+    "grim_equilibration": (_i32, [_i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "grim_refine_ex_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32]),
+    "grim_refined_solve_ex": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i32, _vp, _i64,
+                                     _i64, _vp, _i64, _i64, _i32, _dbl, _vp, _i64, ctypes.POINTER(_i32),
+                                     ctypes.POINTER(_dbl), _vp]),
     "grim_to_c64": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "grim_copy_c128": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "grim_column_norms": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp]),

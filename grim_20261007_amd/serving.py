@@ -84,14 +84,7 @@ def choose_precision(n: int, nangles: int, corrections: Optional[int] = None) ->
 
 def request_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool) -> int:
     """Device bytes a request allocates (systems, factors, loads, RHS, solutions, workspace)."""
-    z = 16 * n * n
-    lu = (8 if precision == "mixed" else 16) * n * n
-    vec = 16 * n * nangles
-    work = 0 if cached else engine.NEAR_PER_ELEMENT * n * 160 + 64 * n
-    refine = (24 * n * nangles) if precision == "mixed" else 0
-    per_pol = vec * 3 + refine                       # rhs, solution, residual
-    systems = 0 if cached else 2 * (z + lu)          # both polarizations assembled together
-    return systems + 2 * vec + npols * per_pol + work
+    return engine.request_device_bytes(n, nangles, npols, precision, cached)
 
 
 class SystemCache:

@@ -31,6 +31,13 @@ RCS_DB_FLOOR = 1e-12
 AMPLITUDE_CONVENTION = "A_physical_asymptotic = +j * B_stored"
 CO_POLARIZED_CHANNELS = (("VV", "TE"), ("HH", "TM"))
 MAX_PANELS_DEFAULT = 20_000
+MEMORY_LIMIT_FRACTION = 0.9        # twod/solver.py:1416
+FORMULATION = "2D combined regional potential S + gamma D (linear Galerkin)"
+DENSITY_FORMULATION = ("Multi-region indirect (exterior potential density; closed contours include a DLP term)")
+SOLVER_BACKEND = "grim_20261007_amd (HIP, gfx950)"
+# twod/solver.py:953-1080
+DEFAULT_QUALITY_THRESHOLDS = {"residual_norm_max": 1.0e-6, "constraint_residual_norm_max": 1.0e-8,
+                              "condition_est_max": 1.0e6, "warnings_max": 10}
 
 
 def wavenumber(frequency_ghz: float) -> float:
@@ -56,12 +63,137 @@ def _validate(frequencies_ghz, elevations_deg):
     return freqs, angs
 
 
+def _validate_cfie_alpha(value) -> float:
+    """The exact disabled value only (twod/solver.py:814-836)."""
+    try:
+        alpha = float(value)
+    except (TypeError, ValueError, OverflowError) as exc:
+        raise ValueError("cfie_alpha is not implemented by any active 2-D formulation; "
+                         "use the exact disabled value cfie_alpha=0.") from exc
+    if not math.isfinite(alpha) or alpha != 0.0:
+        raise ValueError("cfie_alpha is not implemented by any active 2-D formulation; "
+                         "use the exact disabled value cfie_alpha=0. No unchanged field "
+                         "was returned under a different solver setting.")
+    return 0.0
+
+
 def _device(device):
     if device is None:
         if not torch.cuda.is_available():
             raise RuntimeError("The GRIM-2D solver needs a ROCm GPU (torch.cuda.is_available() is False).")
         return torch.device("cuda", torch.cuda.current_device())
     return torch.device(device)
+
+
+def evaluate_quality_gate(metadata: Dict[str, Any], thresholds: Optional[Dict[str, Any]] = None) -> Dict[str, Any]:
+    """The reference's numeric quality gate on solver metadata (twod/solver.py:953-1100): residual,
+    non-finite residuals, junction constraints, the condition estimate when it was computed, and
+    the warning count, against the default or supplied thresholds (unknown keys are refused)."""
+    merged = dict(DEFAULT_QUALITY_THRESHOLDS)
+    if thresholds:
+        supplied = dict(thresholds)
+        unknown = sorted(set(supplied) - set(DEFAULT_QUALITY_THRESHOLDS))
+        if unknown:
+            raise ValueError("Unknown 2-D quality threshold field(s): " + ", ".join(str(k) for k in unknown))
+        merged.update(supplied)
+    residual_limit = float(merged["residual_norm_max"])
+    constraint_limit = float(merged["constraint_residual_norm_max"])
+    condition_limit = float(merged["condition_est_max"])
+    try:
+        warnings_float = float(merged["warnings_max"])
+    except (TypeError, ValueError, OverflowError) as exc:
+        raise ValueError("warnings_max must be a finite non-negative integer.") from exc
+    if any(not math.isfinite(v) or v < 0.0 for v in (residual_limit, constraint_limit, condition_limit)):
+        raise ValueError("2-D residual and condition quality thresholds must be finite non-negative values.")
+    if not math.isfinite(warnings_float) or warnings_float < 0.0 or not warnings_float.is_integer():
+        raise ValueError("warnings_max must be a finite non-negative integer.")
+    warnings_limit = int(warnings_float)
+
+    def as_float(raw):
+        try:
+            return float(raw) if raw is not None else float("nan")
+        except (TypeError, ValueError):
+            return float("nan")
+
+    residual_value = as_float(metadata.get("residual_norm_max"))
+    try:
+        nonfinite = int(metadata.get("residual_nonfinite_count", 0))
+    except (TypeError, ValueError, OverflowError):
+        nonfinite = -1
+    constraint_value = float(metadata.get("constraint_residual_norm_max", 0.0) or 0.0)
+    condition_value = as_float(metadata.get("condition_est_max"))
+    condition_computed = (bool(metadata.get("condition_est_computed")) if "condition_est_computed" in metadata
+                          else math.isfinite(condition_value))
+    warnings_count = len(list(metadata.get("warnings", []) or []))
+    violations: List[str] = []
+    if not math.isfinite(residual_value) or residual_value > residual_limit:
+        violations.append(f"residual_norm_max={residual_value:.6g} exceeds limit {residual_limit:.6g}")
+    if nonfinite > 0:
+        violations.append(f"residual_nonfinite_count={nonfinite} must be zero")
+    elif nonfinite < 0:
+        violations.append("residual_nonfinite_count is missing a valid non-negative integer value")
+    if bool(metadata.get("junction_constraints_applied", False)) and (
+            not math.isfinite(constraint_value) or constraint_value > constraint_limit):
+        violations.append(f"constraint_residual_norm_max={constraint_value:.6g} exceeds limit {constraint_limit:.6g}")
+    if condition_computed and (not math.isfinite(condition_value) or condition_value > condition_limit):
+        violations.append(f"condition_est_max={condition_value:.6g} exceeds limit {condition_limit:.6g}")
+    if warnings_count > warnings_limit:
+        violations.append(f"warnings_count={warnings_count} exceeds limit {warnings_limit}")
+    return {
+        "passed": not violations,
+        "thresholds": {"residual_norm_max": residual_limit, "constraint_residual_norm_max": constraint_limit,
+                       "condition_est_max": condition_limit, "warnings_max": warnings_limit},
+        "values": {"residual_norm_max": residual_value, "residual_nonfinite_count": nonfinite,
+                   "constraint_residual_norm_max": constraint_value, "condition_est_max": condition_value,
+                   "condition_est_computed": condition_computed, "warnings_count": warnings_count},
+        "violations": violations,
+        "certification_scope": ("discrete_linear_system_residual_and_condition" if condition_computed
+                                else "discrete_linear_system_residual_only"),
+        "mesh_convergence_certified": bool(metadata.get("mesh_convergence_certified", False)),
+        "reason": ("; ".join(violations) if violations else
+                   "discrete linear-system quality thresholds satisfied; "
+                   + ("condition number was not requested; " if not condition_computed else "")
+                   + "mesh convergence is separately certified by the production workflow"),
+    }
+
+
+def _apply_quality_gate(metadata, quality_thresholds, strict_quality_gate):
+    gate = evaluate_quality_gate(metadata, quality_thresholds)
+    metadata["quality_gate"] = gate
+    if strict_quality_gate and not gate["passed"]:
+        raise ValueError(f"Quality gate failed: {gate.get('reason', 'quality gate failed')}")
+    return gate
+
+
+def _summarize_residuals(values):
+    """Finite max and mean, and the count of non-finite residuals (twod/solver.py:457-468)."""
+    r = np.asarray(values, dtype=float).reshape(-1)
+    finite = r[np.isfinite(r)]
+    if finite.size == 0:
+        return float("nan"), float("nan"), int(r.size)
+    return float(np.max(finite)), float(np.mean(finite)), int(r.size - finite.size)
+
+
+def admit_device_memory(n: int, nangles: int, npols: int, precision: str, device, context: str,
+                        condition: bool = False) -> float:
+    """Refuse a solve whose device footprint exceeds 0.9 of the GPU memory it can use now (free
+    memory plus what PyTorch's allocator holds unused), before anything is assembled, as the
+    reference refuses host memory (twod/solver.py:1419-1554, 2342-2400).  Returns the estimate
+    in GiB."""
+    need = engine.request_device_bytes(n, nangles, npols, precision)
+    if condition:   # equilibration scales, two probe columns and a 256 x 256 staging block per system
+        need += 2 * (3 * 8 * n + 16 * 2 * 2 * n + 16 * 256 * 256)
+    free, _total = torch.cuda.mem_get_info(device)
+    available = free + torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    limit = MEMORY_LIMIT_FRACTION * available
+    gib = 2.0 ** 30
+    if need > limit:
+        raise MemoryError(
+            f"{context} requires an estimated {need / gib:.2f} GiB, but {available / gib:.2f} GiB is currently "
+            f"available on the GPU; the safe allocation limit is {limit / gib:.2f} GiB. Planned system: {n} DOFs "
+            "across 1 region(s). Reduce panel count or frequency, set an appropriate mesh_reference_ghz, or reduce "
+            "the angle batch.")
+    return need / gib
 
 
 def _systems(contour, k, z_te, z_tm, bu, bdn):
@@ -76,10 +208,15 @@ def _systems(contour, k, z_te, z_tm, bu, bdn):
 
 def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("TE", "TM"),
                     precision: str = "double", device=None, return_density: bool = False,
-                    abort_event=None) -> Dict[str, Dict[str, Any]]:
-    """Assemble both systems once, then factor, solve and project each requested polarization."""
+                    abort_event=None, compute_condition_number: bool = False,
+                    context: str = "The combined_potential 2-D solve") -> Dict[str, Dict[str, Any]]:
+    """Assemble both systems once, then factor, solve and project each requested polarization.
+    Per polarization: amplitude, sigma, relative residual per angle, refinement info, the
+    condition estimate when requested, and the precision the factor ended in."""
     dev = _device(device)
     k = wavenumber(frequency_ghz)
+    order = [p for p in ("TE", "TM") if p in pols]
+    admit_device_memory(mesh.n, len(angles_deg), len(order), precision, dev, context, compute_condition_number)
     contour = engine.DeviceContour(mesh, dev)
     z_tm, z_te = engine.assemble(contour, k)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
@@ -87,70 +224,35 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
     pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn)
-    order = [p for p in ("TE", "TM") if p in pols]
     sel = [("TE", "TM").index(p) for p in order]
     systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
     rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
     fac = engine.factor(systems.contiguous(), precision)
+    conds = methods = None
+    if compute_condition_number:
+        from . import condition
+        conds, methods, replacement = condition.condition_estimates(fac)
+        if replacement is not None:
+            fac = replacement
     x, info = engine.solve(fac, rhs)
     rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
+    final = info.get("factor", fac)
     out = {}
     for i, pol in enumerate(order):
+        part = engine.system(final, i) if final.parts is not None else final
         amp, sigma = engine.farfield(x[i], bu, bdn, k)
         res = {"amplitude": amp.cpu().numpy(), "sigma": sigma.cpu().numpy(),
-               "residual": rel[i].cpu().numpy(), "info": {kk: v for kk, v in info.items() if kk != "residual"},
-               "k": k, "n": mesh.n}
+               "residual": rel[i].cpu().numpy(),
+               "info": {kk: v for kk, v in info.items() if kk not in ("residual", "factor")},
+               "k": k, "n": mesh.n, "precision_used": part.precision,
+               "fallback_reason": part.fallback_reason}
+        if conds is not None:
+            res["condition"] = float(conds[i])
+            res["condition_method"] = methods[i]
         if return_density:
             res["density"] = x[i].cpu().numpy().T      # (N, A) nodal densities
         out[pol] = res
     return out
-
-
-def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[float], observation_deg: List[float],
-                             pols=("TE", "TM"), precision: str = "double", device=None,
-                             abort_event=None) -> Dict[str, Dict[str, Any]]:
-    """Both systems from one assembly; solve for every incidence, then project every solution
-    on every observation direction (twod/solver.py:3476-3517 with observation angles)."""
-    dev = _device(device)
-    k = wavenumber(frequency_ghz)
-    contour = engine.DeviceContour(mesh, dev)
-    z_tm, z_te = engine.assemble(contour, k)
-    inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)
-    obs = torch.tensor(observation_deg, dtype=torch.float64, device=dev)
-    bu, bdn = engine.loads(contour, k, inc)
-    if abort_event is not None and abort_event.is_set():
-        raise InterruptedError("Solve canceled by user.")
-    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn)
-    order = [p for p in ("TE", "TM") if p in pols]
-    sel = [("TE", "TM").index(p) for p in order]
-    systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
-    rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
-    fac = engine.factor(systems.contiguous(), precision)
-    x, info = engine.solve(fac, rhs)
-    rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
-    bu_o, bdn_o = engine.loads(contour, k, obs)
-    amp, sigma = engine.bistatic_farfield(x, bu_o, bdn_o, k)
-    out = {}
-    for i, pol in enumerate(order):
-        out[pol] = {"amplitude": amp[i].cpu().numpy(), "sigma": sigma[i].cpu().numpy(),
-                    "residual": rel[i].cpu().numpy(),
-                    "info": {kk: v for kk, v in info.items() if kk != "residual"}, "k": k, "n": mesh.n}
-    return out
-
-
-def _bistatic_samples(frequency_ghz, incidence, observation, amp, sigma, residual):
-    """Rows in request order: incidence, then observation (twod/solver.py:3519-3537)."""
-    rows = []
-    dbs = rcs_db(sigma)
-    for i, a in enumerate(incidence):
-        for j, o in enumerate(observation):
-            z = complex(amp[i, j])
-            rows.append({"frequency_ghz": float(frequency_ghz), "theta_inc_deg": float(a),
-                         "theta_scat_deg": float(o), "rcs_linear": float(sigma[i, j]), "rcs_db": float(dbs[i, j]),
-                         "rcs_amp_real": float(z.real), "rcs_amp_imag": float(z.imag),
-                         "rcs_amp_phase_deg": float(math.degrees(cmath.phase(z))),
-                         "linear_residual": float(residual[i])})
-    return rows
 
 
 def _samples(frequency_ghz, angles, amp, sigma, residual):
@@ -173,10 +275,88 @@ def _samples(frequency_ghz, angles, amp, sigma, residual):
 
 
 def _normalize_pol(polarization: str) -> str:
-    pol = str(polarization).strip().upper()
-    if pol not in ("TE", "TM"):
-        raise ValueError(f"Unsupported polarization '{polarization}'. Use TE or TM.")
-    return pol
+    """TM, HH, H, HORIZONTAL -> TM; TE, VV, V, VERTICAL -> TE (twod/solver.py:395-422)."""
+    pol = str(polarization or "").strip().upper()
+    if pol in {"TM", "HH", "H", "HORIZONTAL"}:
+        return "TM"
+    if pol in {"TE", "VV", "V", "VERTICAL"}:
+        return "TE"
+    raise ValueError(f"Unsupported polarization '{polarization}'. Use TM/TE or VV/HH.")
+
+
+class _Record:
+    """What a run of solves reports in the reference's metadata layout."""
+
+    def __init__(self, snapshot):
+        self.snapshot = snapshot
+        self.residuals: List[float] = []
+        self.conditions: List[float] = []
+        self.methods: List[str] = []
+        self.panel_counts: List[int] = []
+        self.lengths: List[float] = []
+        self.mesh_ghz: List[float] = []
+        self.reasons: List[str] = []
+        self.precisions: List[str] = []
+        self.columns = 0
+        self.batches = 0
+        self.factorizations = 0
+
+    def add_mesh(self, mesh, mesh_ghz):
+        self.panel_counts.append(mesh.n)
+        self.lengths += [float(np.min(mesh.lengths)), float(np.max(mesh.lengths))]
+        self.mesh_ghz.append(float(mesh_ghz))
+
+    def add_solve(self, res, nangles):
+        self.residuals += [float(v) for v in np.asarray(res["residual"]).reshape(-1)]
+        if "condition" in res:
+            self.conditions.append(res["condition"])
+            if res["condition_method"] not in self.methods:
+                self.methods.append(res["condition_method"])
+        if res["fallback_reason"] and res["fallback_reason"] not in self.reasons:
+            self.reasons.append(res["fallback_reason"])
+        self.precisions.append(res["precision_used"])
+        self.columns += nangles
+        self.batches += 1
+
+    def metadata(self, compute_condition_number, precision, solver_method):
+        rmax, rmean, nonfinite = _summarize_residuals(self.residuals)
+        ghz = self.mesh_ghz
+        wl = [C0 / (g * 1e9) for g in ghz]
+        mixed_solves = sum(p == "mixed" for p in self.precisions)
+        return {
+            "source_path": str(self.snapshot.get("source_path", "") or ""),
+            "segment_count": int(len(self.snapshot.get("segments", []) or [])),
+            "panel_count": max(self.panel_counts), "panel_count_min": min(self.panel_counts),
+            "panel_count_max": max(self.panel_counts), "panel_counts": list(self.panel_counts),
+            "panel_length_min_m": min(self.lengths), "panel_length_max_m": max(self.lengths),
+            "mesh_reference_ghz": ghz[0] if len(set(round(v, 12) for v in ghz)) == 1 else None,
+            "mesh_reference_ghz_min": min(ghz), "mesh_reference_ghz_max": max(ghz),
+            "mesh_wavelength_m": wl[0] if len(set(round(v, 15) for v in wl)) == 1 else None,
+            "mesh_wavelength_min_m": min(wl), "mesh_wavelength_max_m": max(wl),
+            "mesh_max_refractive_index": 1.0, "mesh_material_flags": [],
+            "formulation": FORMULATION, "solver_method": "dense_lu", "solver_method_requested": str(solver_method),
+            "residual_norm_max": rmax, "residual_norm_mean": rmean, "residual_nonfinite_count": nonfinite,
+            "constraint_residual_norm_max": 0.0, "constraint_residual_norm_mean": 0.0,
+            "constraint_residual_applicable": False,
+            "condition_est_max": float(np.max(self.conditions)) if self.conditions else float("nan"),
+            "condition_est_mean": float(np.mean(self.conditions)) if self.conditions else float("nan"),
+            "condition_est_computed": bool(compute_condition_number),
+            "condition_estimator": (self.methods[0] if self.methods else "equilibrated_1norm_lu_onenormest")
+            if compute_condition_number else "not_requested",
+            "dense_condition_methods": sorted(self.methods),
+            "warnings": [], "information": [], "warning_count": 0,
+            "junction_constraints_applied": False, "junction_constraints": 0, "junction_nodes": 0,
+            "linear_node_count": int(sum(self.panel_counts)), "linear_element_count": int(sum(self.panel_counts)),
+            "precision": precision, "solver_backend": SOLVER_BACKEND,
+            "linear_backend": "gpu_hip_mixed_lu" if mixed_solves == len(self.precisions) else
+            ("gpu_hip" if mixed_solves == 0 else "mixed"),
+            "dense_mixed_precision_solve_count": mixed_solves,
+            "dense_gpu_solve_count": len(self.precisions), "dense_cpu_solve_count": 0,
+            "dense_fallback_reasons": list(self.reasons), "dense_gpu_fallback_reasons": list(self.reasons),
+            "dense_rhs_batch_count": self.batches, "dense_rhs_column_count": self.columns,
+            "dense_largest_system": max(self.panel_counts),
+            "amplitude_version": 2,
+        }
 
 
 def solve_monostatic_rcs_2d_single_polarization(
@@ -189,45 +369,42 @@ def solve_monostatic_rcs_2d_single_polarization(
         cfie_alpha: float = 0.0, abort_event=None, solver_method: str = "auto", *,
         execution_options=None, precision: str = "double", device=None) -> Dict[str, Any]:
     freqs, angs = _validate(frequencies_ghz, elevations_deg)
-    if float(cfie_alpha) != 0.0:
-        raise ValueError("cfie_alpha is not implemented by any active 2-D formulation; use cfie_alpha=0.")
+    cfie_alpha = _validate_cfie_alpha(cfie_alpha)
     if str(rcs_normalization_mode).strip().lower() != "physical":
         raise ValueError("Only the physical RCS normalization is supported.")
     pol = _normalize_pol(polarization)
     unit_scale(geometry_units)
+    evaluate_quality_gate({}, quality_thresholds)            # refuse unknown thresholds up front
     samples = []
-    residuals = []
+    record = _Record(geometry_snapshot)
     total = len(freqs) * (len(angs) + 1)
     done = 0
-    n_used = []
     for f in freqs:
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve canceled by user.")
-        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_reference_ghz or f, max_panels)
-        res = solve_frequency(mesh, f, angs, (pol,), precision, device, abort_event=abort_event)[pol]
+        mesh_ghz = mesh_reference_ghz or f
+        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+        record.add_mesh(mesh, mesh_ghz)
+        res = solve_frequency(mesh, f, angs, (pol,), precision, device, abort_event=abort_event,
+                              compute_condition_number=compute_condition_number,
+                              context=f"The combined_potential 2-D {pol} solve")[pol]
         samples += _samples(f, angs, res["amplitude"], res["sigma"], res["residual"])
-        residuals.append(float(np.max(res["residual"])))
-        n_used.append(mesh.n)
+        record.add_solve(res, len(angs))
         done += len(angs) + 1
         if progress_callback is not None:
             try:
                 progress_callback(done, total, f"Solved {f:g} GHz")
             except Exception:
                 pass
-    max_res = max(residuals)
-    gate_limit = float((quality_thresholds or {}).get("max_linear_residual", 1e-6))
-    passed = max_res <= gate_limit
-    if strict_quality_gate and not passed:
-        raise ValueError(f"Quality gate failed: linear residual {max_res:.3e} exceeds {gate_limit:.1e}.")
-    metadata = {"solver_backend": "grim_20261007_amd (HIP, gfx950)", "precision": precision,
-                "formulation": "2D combined regional potential S + gamma D (linear Galerkin)",
-                "panel_counts": n_used, "residual_max": max_res,
-                "quality_gate": {"passed": passed, "max_linear_residual": gate_limit},
-                "solver_method": solver_method}
+    metadata = record.metadata(compute_condition_number, precision, solver_method)
+    alias = "HH" if pol == "TM" else "VV"
+    metadata.update({"polarization_internal": pol, "polarization_user": pol, "polarization_aliases": [pol],
+                     "polarization_export": pol, "polarization_export_alias": alias,
+                     "rcs_normalization_mode": "physical", "cfie_alpha": cfie_alpha})
+    _apply_quality_gate(metadata, quality_thresholds, strict_quality_gate)
     return {"solver": "2d_bie_mom_rcs", "scattering_mode": "monostatic",
             "amplitude_convention": AMPLITUDE_CONVENTION, "amplitude_version": 2,
-            "rcs_log_unit": "dBke", "rcs_linear_quantity": "sigma_2d", "polarization": pol,
-            "samples": samples, "metadata": metadata}
+            "polarization": pol, "polarization_export": pol, "samples": samples, "metadata": metadata}
 
 
 def solve_monostatic_rcs_2d(
@@ -244,26 +421,30 @@ def solve_monostatic_rcs_2d(
     if str(rcs_normalization_mode).strip().lower() != "physical":
         raise ValueError("Only the physical RCS normalization is supported.")
     unit_scale(geometry_units)
+    evaluate_quality_gate({}, quality_thresholds)
     co = {"VV": [], "HH": []}
     flat = []
-    residual_max = 0.0
-    counts = []
+    record = _Record(geometry_snapshot)
     total = 2 * len(freqs) * (len(angs) + 1)
     done = 0
     for f in freqs:
-        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_reference_ghz or f, max_panels)
-        counts.append(mesh.n)
-        res = solve_frequency(mesh, f, angs, ("TE", "TM"), precision, device, abort_event=abort_event)
+        if abort_event is not None and abort_event.is_set():
+            raise InterruptedError("Solve canceled by user.")
+        mesh_ghz = mesh_reference_ghz or f
+        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+        record.add_mesh(mesh, mesh_ghz)
+        res = solve_frequency(mesh, f, angs, ("TE", "TM"), precision, device, abort_event=abort_event,
+                              compute_condition_number=compute_condition_number)
         per = {}
         for export, internal in CO_POLARIZED_CHANNELS:
             r = res[internal]
+            record.add_solve(r, len(angs))
             rows = _samples(f, angs, r["amplitude"], r["sigma"], r["residual"])
             for row in rows:
                 row["polarization"] = export
                 row["polarization_internal"] = internal
             per[export] = rows
             co[export] += rows
-            residual_max = max(residual_max, float(np.max(r["residual"])))
         for vv, hh in zip(per["VV"], per["HH"]):
             flat += [vv, hh]
         done += 2 * (len(angs) + 1)
@@ -272,19 +453,76 @@ def solve_monostatic_rcs_2d(
                 progress_callback(done, total, f"Solved {f:g} GHz")
             except Exception:
                 pass
-    gate_limit = float((quality_thresholds or {}).get("max_linear_residual", 1e-6))
-    passed = residual_max <= gate_limit
-    if strict_quality_gate and not passed:
-        raise ValueError(f"Quality gate failed: linear residual {residual_max:.3e} exceeds {gate_limit:.1e}.")
-    metadata = {"solver_backend": "grim_20261007_amd (HIP, gfx950)", "precision": precision,
-                "panel_counts": counts, "residual_max": residual_max,
-                "quality_gate": {"passed": passed, "max_linear_residual": gate_limit},
-                "solver_method": solver_method}
+    metadata = record.metadata(compute_condition_number, precision, solver_method)
+    metadata.update({"polarizations": ["VV", "HH"], "co_solve_shared_discretization": True,
+                     "rcs_normalization_mode": "physical"})
+    _apply_quality_gate(metadata, quality_thresholds, strict_quality_gate)
     return {"solver": "2d_bie_mom_rcs", "scattering_mode": "monostatic",
             "amplitude_convention": AMPLITUDE_CONVENTION, "amplitude_version": 2,
             "rcs_log_unit": "dBke", "rcs_linear_quantity": "sigma_2d",
             "polarizations": ["VV", "HH"], "polarization_mapping": {"VV": "TE", "HH": "TM"},
             "samples": flat, "co_solved_samples": co, "metadata": metadata}
+
+
+def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[float], observation_deg: List[float],
+                             pols=("TE", "TM"), precision: str = "double", device=None,
+                             abort_event=None, compute_condition_number: bool = False) -> Dict[str, Dict[str, Any]]:
+    """Both systems from one assembly; solve for every incidence, then project every solution
+    on every observation direction (twod/solver.py:3476-3517 with observation angles)."""
+    dev = _device(device)
+    k = wavenumber(frequency_ghz)
+    order = [p for p in ("TE", "TM") if p in pols]
+    admit_device_memory(mesh.n, len(incidence_deg) + len(observation_deg), len(order), precision, dev,
+                        "The bistatic 2-D solve", compute_condition_number)
+    contour = engine.DeviceContour(mesh, dev)
+    z_tm, z_te = engine.assemble(contour, k)
+    inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)
+    obs = torch.tensor(observation_deg, dtype=torch.float64, device=dev)
+    bu, bdn = engine.loads(contour, k, inc)
+    if abort_event is not None and abort_event.is_set():
+        raise InterruptedError("Solve canceled by user.")
+    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn)
+    sel = [("TE", "TM").index(p) for p in order]
+    systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
+    rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
+    fac = engine.factor(systems.contiguous(), precision)
+    conds = methods = None
+    if compute_condition_number:
+        from . import condition
+        conds, methods, replacement = condition.condition_estimates(fac)
+        if replacement is not None:
+            fac = replacement
+    x, info = engine.solve(fac, rhs)
+    rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
+    final = info.get("factor", fac)
+    bu_o, bdn_o = engine.loads(contour, k, obs)
+    amp, sigma = engine.bistatic_farfield(x, bu_o, bdn_o, k)
+    out = {}
+    for i, pol in enumerate(order):
+        part = engine.system(final, i) if final.parts is not None else final
+        out[pol] = {"amplitude": amp[i].cpu().numpy(), "sigma": sigma[i].cpu().numpy(),
+                    "residual": rel[i].cpu().numpy(),
+                    "info": {kk: v for kk, v in info.items() if kk not in ("residual", "factor")}, "k": k,
+                    "n": mesh.n, "precision_used": part.precision, "fallback_reason": part.fallback_reason}
+        if conds is not None:
+            out[pol]["condition"] = float(conds[i])
+            out[pol]["condition_method"] = methods[i]
+    return out
+
+
+def _bistatic_samples(frequency_ghz, incidence, observation, amp, sigma, residual):
+    """Rows in request order: incidence, then observation (twod/solver.py:3519-3537)."""
+    rows = []
+    dbs = rcs_db(sigma)
+    for i, a in enumerate(incidence):
+        for j, o in enumerate(observation):
+            z = complex(amp[i, j])
+            rows.append({"frequency_ghz": float(frequency_ghz), "theta_inc_deg": float(a),
+                         "theta_scat_deg": float(o), "rcs_linear": float(sigma[i, j]), "rcs_db": float(dbs[i, j]),
+                         "rcs_amp_real": float(z.real), "rcs_amp_imag": float(z.imag),
+                         "rcs_amp_phase_deg": float(math.degrees(cmath.phase(z))),
+                         "linear_residual": float(residual[i])})
+    return rows
 
 
 def _validate_bistatic(frequencies_ghz, incidence_angles_deg, observation_angles_deg):
@@ -308,37 +546,34 @@ def _validate_bistatic(frequencies_ghz, incidence_angles_deg, observation_angles
 
 def _bistatic_run(geometry_snapshot, frequencies_ghz, incidence_angles_deg, observation_angles_deg, pols,
                   geometry_units, progress_callback, quality_thresholds, strict_quality_gate, max_panels,
-                  mesh_reference_ghz, abort_event, precision, device):
+                  mesh_reference_ghz, abort_event, precision, device, compute_condition_number=False):
     freqs, inc, obs = _validate_bistatic(frequencies_ghz, incidence_angles_deg, observation_angles_deg)
     unit_scale(geometry_units)
+    evaluate_quality_gate({}, quality_thresholds)
     rows = {p: [] for p in pols}
-    counts, residual_max = [], 0.0
+    record = _Record(geometry_snapshot)
     total, done = len(freqs) * len(inc), 0
     for f in freqs:
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve canceled by user.")
-        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_reference_ghz or f, max_panels)
-        counts.append(mesh.n)
-        res = solve_bistatic_frequency(mesh, f, inc, obs, pols, precision, device, abort_event=abort_event)
+        mesh_ghz = mesh_reference_ghz or f
+        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+        record.add_mesh(mesh, mesh_ghz)
+        res = solve_bistatic_frequency(mesh, f, inc, obs, pols, precision, device, abort_event=abort_event,
+                                       compute_condition_number=compute_condition_number)
         for p in pols:
             r = res[p]
             rows[p] += _bistatic_samples(f, inc, obs, r["amplitude"], r["sigma"], r["residual"])
-            residual_max = max(residual_max, float(np.max(r["residual"])))
+            record.add_solve(r, len(inc))
         done += len(inc)
         if progress_callback is not None:
             try:
                 progress_callback(done, total, f"Bistatic {f:g} GHz")
             except Exception:
                 pass
-    gate_limit = float((quality_thresholds or {}).get("max_linear_residual", 1e-6))
-    passed = residual_max <= gate_limit
-    if strict_quality_gate and not passed:
-        raise ValueError(f"Quality gate failed: linear residual {residual_max:.3e} exceeds {gate_limit:.1e}.")
-    metadata = {"solver_backend": "grim_20261007_amd (HIP, gfx950)", "precision": precision,
-                "formulation": "bistatic 2D BIE/MoM", "panel_count": max(counts), "panel_count_min": min(counts),
-                "panel_count_max": max(counts), "panel_counts": counts, "residual_norm_max": residual_max,
-                "quality_gate": {"passed": passed, "max_linear_residual": gate_limit}, "solver_method": "dense_lu",
-                "amplitude_version": 2}
+    metadata = record.metadata(compute_condition_number, precision, "dense_lu")
+    metadata["formulation"] = "bistatic 2D BIE/MoM"
+    _apply_quality_gate(metadata, quality_thresholds, strict_quality_gate)
     return rows, metadata
 
 
@@ -352,13 +587,12 @@ def solve_bistatic_rcs_2d_single_polarization(
         solver_method: str = "auto", *, precision: str = "double", device=None) -> Dict[str, Any]:
     """One channel on an incidence x observation grid (twod/solver.py:3262-3600): samples in
     request order (frequency, incidence, observation), theta_scat_deg = observation angle."""
-    if float(cfie_alpha) != 0.0:
-        raise ValueError("cfie_alpha is not implemented by any active 2-D formulation; use cfie_alpha=0.")
+    _validate_cfie_alpha(cfie_alpha)
     pol = _normalize_pol(polarization)
     rows, metadata = _bistatic_run(geometry_snapshot, frequencies_ghz, incidence_angles_deg,
                                    observation_angles_deg, (pol,), geometry_units, progress_callback,
                                    quality_thresholds, strict_quality_gate, max_panels, mesh_reference_ghz,
-                                   abort_event, precision, device)
+                                   abort_event, precision, device, compute_condition_number)
     metadata["solver_method_requested"] = str(solver_method)
     return {"solver": "2d_bie_mom_rcs", "scattering_mode": "bistatic", "amplitude_convention": AMPLITUDE_CONVENTION,
             "amplitude_version": 2, "polarization": pol, "polarization_export": pol, "samples": rows[pol],
@@ -378,7 +612,7 @@ def solve_bistatic_rcs_2d(
     rows, metadata = _bistatic_run(geometry_snapshot, frequencies_ghz, incidence_angles_deg,
                                    observation_angles_deg, ("TE", "TM"), geometry_units, progress_callback,
                                    quality_thresholds, strict_quality_gate, max_panels, mesh_reference_ghz,
-                                   abort_event, precision, device)
+                                   abort_event, precision, device, compute_condition_number)
     co = {}
     for export, internal in CO_POLARIZED_CHANNELS:
         for row in rows[internal]:
@@ -395,22 +629,66 @@ def solve_bistatic_rcs_2d(
 
 def compute_boundary_densities(geometry_snapshot: Dict[str, Any], frequency_ghz: float, elevation_deg: float,
                                polarization: str, geometry_units: str = "inches",
-                               material_base_dir: Optional[str] = None, max_panels: int = MAX_PANELS_DEFAULT,
-                               mesh_reference_ghz: Optional[float] = None, *, precision: str = "double",
+                               material_base_dir: Optional[str] = None, cfie_alpha: float = 0.0,
+                               max_panels: int = MAX_PANELS_DEFAULT, abort_event=None, *,
+                               mesh_reference_ghz: Optional[float] = None, precision: str = "double",
                                device=None) -> Dict[str, Any]:
-    """Layer-potential density of one incidence: nodal values and element-centre samples."""
+    """Layer-potential density of one incidence at the element centres (twod/solver.py:4150-4346):
+    the same arguments in the same positions, the same result keys.  ``mesh_reference_ghz``
+    (keyword only) meshes at another frequency; ``precision`` and ``device`` choose the LU
+    precision and the GPU."""
+
+    def check_abort():
+        if abort_event is not None and abort_event.is_set():
+            raise InterruptedError("Boundary-density calculation canceled by user.")
+
+    check_abort()
+    _validate_cfie_alpha(cfie_alpha)
     pol = _normalize_pol(polarization)
-    mesh = build_mesh(geometry_snapshot, geometry_units, mesh_reference_ghz or frequency_ghz, max_panels)
-    res = solve_frequency(mesh, float(frequency_ghz), [float(elevation_deg)], (pol,), precision, device,
-                          return_density=True)[pol]
+    unit_scale(geometry_units)
+    frequency_ghz = float(frequency_ghz)
+    if not math.isfinite(frequency_ghz) or frequency_ghz <= 0.0:
+        raise ValueError("frequency_ghz must be a positive finite value.")
+    mesh_ghz = float(mesh_reference_ghz or frequency_ghz)
+    mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+    check_abort()
+    res = solve_frequency(mesh, frequency_ghz, [float(elevation_deg)], (pol,), precision, device,
+                          return_density=True, abort_event=abort_event,
+                          context="Boundary-density diagnostics")[pol]
+    check_abort()
     nodal = res["density"][:, 0]
-    centre = 0.5 * (nodal + nodal[mesh.next])
+    ids = mesh.elements
+    density = 0.5 * (nodal[ids[:, 0]] + nodal[ids[:, 1]])      # linear basis at t = 1/2
     centers = 0.5 * (mesh.p0 + mesh.p1)
     normals = mesh.normals
-    return {"polarization": pol, "frequency_ghz": float(frequency_ghz), "elevation_deg": float(elevation_deg),
-            "centers_x": centers[:, 0].tolist(), "centers_y": centers[:, 1].tolist(),
-            "normals_x": normals[:, 0].tolist(), "normals_y": normals[:, 1].tolist(),
-            "lengths": mesh.lengths.tolist(),
-            "density_real": centre.real.tolist(), "density_imag": centre.imag.tolist(),
-            "density_abs": np.abs(centre).tolist(), "density_phase_deg": np.degrees(np.angle(centre)).tolist(),
-            "nodal_density": nodal, "linear_residual": float(res["residual"][0])}
+    return {
+        "quantity": "boundary_integral_layer_density",
+        "is_physical_surface_current": False,
+        "interpretation": ("Formulation-specific SLP/DLP representation density; do not interpret as "
+                           "electric or magnetic surface current without a formulation- and "
+                           "polarization-specific trace conversion."),
+        "formulation": DENSITY_FORMULATION,
+        "frequency_ghz": frequency_ghz,
+        "elevation_deg": float(elevation_deg),
+        "polarization": pol,
+        "coordinate_units": "meters",
+        "length_units": "meters",
+        "mesh_wavelength_m": float(C0 / (mesh_ghz * 1e9)),
+        "mesh_max_refractive_index": 1.0,
+        "mesh_material_flags": [],
+        "element_count": int(mesh.n),
+        "node_count": int(mesh.n),
+        "centers_x": centers[:, 0].tolist(),
+        "centers_y": centers[:, 1].tolist(),
+        "normals_x": normals[:, 0].tolist(),
+        "normals_y": normals[:, 1].tolist(),
+        "lengths": mesh.lengths.tolist(),
+        "density_real": np.real(density).tolist(),
+        "density_imag": np.imag(density).tolist(),
+        "density_abs": np.abs(density).tolist(),
+        "density_phase_deg": np.degrees(np.angle(density)).tolist(),
+        "amplitude_version": 2,
+        "metadata": {"solver_backend": SOLVER_BACKEND, "precision": precision,
+                     "precision_used": res["precision_used"], "fallback_reason": res["fallback_reason"],
+                     "linear_residual": float(res["residual"][0])},
+    }

@@ -116,6 +116,32 @@ int grim_lu_solve(int64_t batch, int64_t n, int64_t nrhs, const void* lu, int64_
                   int precision, const void* aux, int64_t stride_aux, void* b, int64_t ldb, int64_t stride_b,
                   void* work, int64_t work_bytes, void* stream);
 
+/* Bytes of workspace grim_lu_solve_ex needs (trans 0: as grim_lu_solve; trans 2: also one
+ * 256 x 256 staging block per matrix). */
+int64_t grim_lu_solve_ex_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs, int precision, int trans);
+
+/* grim_lu_solve with a transpose option: trans 0 solves A X = B, trans 2 solves A^H X = B
+ * (conjugate transpose) with the same factors.  Replaces scipy.linalg.lu_solve(..., trans=2)
+ * as twod/solver.py:546-565 (_equilibrated_condition_from_lu, the adjoint products of the
+ * condition estimate) and linalg/refined_lu.py:41-61 (RefinedLU.solve with trans) call it. */
+int grim_lu_solve_ex(int64_t batch, int64_t n, int64_t nrhs, const void* lu, int64_t ldlu, int64_t stride_lu,
+                     int precision, const void* aux, int64_t stride_aux, int trans, void* b, int64_t ldb,
+                     int64_t stride_b, void* work, int64_t work_bytes, void* stream);
+
+/* C -= op(A) B, op(A) = A (trans_a 0, A m x k) or A^H (trans_a 2, A stored k x m, lda >= k):
+ * the FP64 residual b - Z^H x of linalg/refined_lu.py:50-51 with trans=2. */
+int grim_gemm_minus_ex(int64_t batch, int64_t m, int64_t n, int64_t k, int trans_a, const void* a, int64_t lda,
+                       int64_t stride_a, const void* b, int64_t ldb, int64_t stride_b, void* c, int64_t ldc,
+                       int64_t stride_c, int precision, void* stream);
+
+/* Row/column equilibration of complex128 matrices and the equilibrated 1-norm: row_scale[i] =
+ * max_j |z_ij| (1 if 0), col_scale[j] = max_i |z_ij| / row_scale[i] (1 if 0), norm =
+ * max_j sum_i |z_ij| / row_scale[i] / col_scale[j].  row_scale, col_scale: double [n] per
+ * matrix (stride n); norm: double [batch].  Replaces _equilibrated_scaling_and_norm_1
+ * (twod/solver.py:472-525). */
+int grim_equilibration(int64_t batch, int64_t n, const double* z, int64_t ldz, int64_t stride_z, double* row_scale,
+                       double* col_scale, double* norm, void* stream);
+
 /* C -= A B (complex; precision 64 or 32); the FP64 refinement residual. */
 int grim_gemm_minus(int64_t batch, int64_t m, int64_t n, int64_t k, const void* a, int64_t lda, int64_t stride_a,
                     const void* b, int64_t ldb, int64_t stride_b, void* c, int64_t ldc, int64_t stride_c,
@@ -143,6 +169,18 @@ int grim_refined_solve(int64_t batch, int64_t n, int64_t nrhs, const double* z, 
                        const double* b, int64_t ldb, int64_t stride_b, double* x, int64_t ldx, int64_t stride_x,
                        int max_steps, double tol, void* workspace, int64_t workspace_bytes, int* steps_out,
                        double* error_out, void* stream);
+
+/* Bytes of workspace grim_refined_solve_ex needs (trans 0: as grim_refined_solve). */
+int64_t grim_refine_ex_workspace_bytes(int64_t batch, int64_t n, int64_t nrhs, int trans);
+
+/* grim_refined_solve for op(Z) x = b, op = identity (trans 0) or conjugate transpose (trans 2):
+ * linalg/refined_lu.py:41-61 with its trans argument (the refined inverse of the condition
+ * estimate of mixed-precision factors, linalg/dense.py:166-172). */
+int grim_refined_solve_ex(int64_t batch, int64_t n, int64_t nrhs, const double* z, int64_t ldz, int64_t stride_z,
+                          const float* lu, int64_t ldlu, int64_t stride_lu, const void* aux, int64_t stride_aux,
+                          int trans, const double* b, int64_t ldb, int64_t stride_b, double* x, int64_t ldx,
+                          int64_t stride_x, int max_steps, double tol, void* workspace, int64_t workspace_bytes,
+                          int* steps_out, double* error_out, void* stream);
 
 /* complex128 -> complex64 copy (m x n) and complex128 copy, batched. */
 int grim_to_c64(int64_t batch, int64_t m, int64_t n, const double* src, int64_t lds, int64_t stride_src, float* dst,

@@ -651,3 +651,63 @@ def impedance_system(z_tm, z_te, bu, bdn, node_impedance, k: float, pol: str):
         flux = np.where(dirichlet[:, None], z_tm, flux)
         load = np.where(dirichlet[:, None], -bu, load)
     return flux, load
+
+
+# ----------------------------------------------------------------------------
+# Condition estimate
+# ----------------------------------------------------------------------------
+CONDITION_SEED = 0x2D5EED
+
+
+def condition_estimate(z, precision: str = "double"):
+    """Equilibrated 1-norm condition estimate (twod/solver.py:472-612): row/column
+    equilibration R, C of z, the exact 1-norm of R z C and SciPy's onenormest of its inverse
+    (t = 2, itmax = 5) with the legacy NumPy generator seeded 0x2D5EED, the inverse applied
+    with LAPACK getrs (trans 0 and 2) on the LU of z, or with refined complex64 solves
+    (linalg/refined_lu.py:41-61) for precision 'mixed'."""
+    import scipy.linalg as sl
+    import scipy.sparse.linalg as spl
+    a = np.asarray(z, np.complex128)
+    n = a.shape[0]
+    row = np.max(np.abs(a), axis=1)
+    row = np.where(row > 0, row, 1.0)
+    eq = np.abs(a) / row[:, None]
+    col = np.max(eq, axis=0)
+    col = np.where(col > 0, col, 1.0)
+    norm_a = float(np.max(np.sum(eq / col[None, :], axis=0)))
+    if precision == "double":
+        lu = sl.lu_factor(a)
+        inv = lambda b, trans=0: sl.lu_solve(lu, b, trans=trans)
+    else:
+        lu = sl.lu_factor(a.astype(np.complex64))
+
+        def inv(b, trans=0):
+            b = np.asarray(b, np.complex128)
+            op = a if trans == 0 else a.conj().T
+            x = sl.lu_solve(lu, b.astype(np.complex64), trans=trans).astype(np.complex128)
+            scale = np.max(np.abs(b), axis=0)
+            scale = np.where(scale > 0, scale, 1.0)
+            previous = math.inf
+            for _ in range(10):
+                r = b - op @ x
+                err = float(np.max(np.max(np.abs(r), axis=0) / scale))
+                if err <= 2e-12:
+                    return x
+                if not np.isfinite(err) or err >= 0.98 * previous:
+                    raise np.linalg.LinAlgError("refinement stalled")
+                previous = err
+                x = x + sl.lu_solve(lu, r.astype(np.complex64), trans=trans).astype(np.complex128)
+            raise np.linalg.LinAlgError("refinement exceeded its correction limit")
+
+    op = spl.LinearOperator((n, n), dtype=np.complex128,
+                            matvec=lambda v: col * inv(row * np.asarray(v).reshape(-1)),
+                            rmatvec=lambda v: row * inv(col * np.asarray(v).reshape(-1), 2),
+                            matmat=lambda m: col[:, None] * inv(row[:, None] * np.asarray(m)),
+                            rmatmat=lambda m: row[:, None] * inv(col[:, None] * np.asarray(m), 2))
+    saved = np.random.get_state()
+    try:
+        np.random.seed(CONDITION_SEED)
+        est = norm_a * float(spl.onenormest(op))
+    finally:
+        np.random.set_state(saved)
+    return est if math.isfinite(est) else float("inf")

@@ -15,7 +15,8 @@ reference factors it.  Outputs are small ``.npz`` fixtures next to this file:
 inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
-Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc]
+Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
+                                               --condition]
 """
 from __future__ import annotations
 
@@ -393,8 +394,72 @@ def main_ibc():
                  workloads.frequency_for_segments_per_wavelength(pts, 10.0), [0.0, 45.0, 180.0, 333.0])
 
 
+def run_condition_case(name, points, frequency_ghz, angles):
+    """The reference's single-polarization entry (twod/solver.py:1984-2712) with
+    compute_condition_number=True in both LU precisions: per channel the condition estimate and
+    method, the quality-gate dict, the metadata keys; the strict gate's refusal text for a
+    residual limit no solve meets; compute_boundary_densities (:4150-4346) per polarization."""
+    import ghost_backend.twod.solver as rcs
+    from ghost_backend.linalg.refined_lu import linear_precision
+    snapshot = workloads._snapshot(points, name)
+    out = {"points": np.asarray(points, float), "frequency_ghz": float(frequency_ghz),
+           "angles_deg": np.asarray(angles, float)}
+    for precision in ("double", "mixed"):
+        for pol in ("TE", "TM"):
+            with linear_precision(precision):
+                res = rcs.solve_monostatic_rcs_2d_single_polarization(
+                    snapshot, [frequency_ghz], list(map(float, angles)), pol, geometry_units="meters",
+                    strict_quality_gate=False, compute_condition_number=True, max_panels=100000,
+                    solver_method="direct", execution_options=EXECUTION)
+            meta = res["metadata"]
+            tag = f"{precision}_{pol}"
+            out[f"cond_{tag}"] = float(meta["condition_est_max"])
+            out[f"cond_method_{tag}"] = np.array(meta["condition_estimator"])
+            out[f"gate_{tag}"] = np.array(json.dumps(meta["quality_gate"], sort_keys=True, default=str))
+            out[f"residual_{tag}"] = float(meta["residual_norm_max"])
+            if precision == "double":
+                out[f"meta_keys_{pol}"] = np.array(sorted(meta.keys()))
+                out[f"result_keys_{pol}"] = np.array(sorted(res.keys()))
+    try:
+        rcs.solve_monostatic_rcs_2d_single_polarization(
+            snapshot, [frequency_ghz], [0.0], "TM", geometry_units="meters", max_panels=100000,
+            quality_thresholds={"residual_norm_max": 1e-30}, solver_method="direct", execution_options=EXECUTION)
+        out["strict_error"] = np.array("")
+    except ValueError as exc:
+        out["strict_error"] = np.array(str(exc))
+    try:
+        rcs.evaluate_quality_gate({}, {"max_linear_residual": 1.0})
+    except ValueError as exc:
+        out["unknown_key_error"] = np.array(str(exc))
+    for pol in ("TE", "TM"):
+        dens = rcs.compute_boundary_densities(snapshot, frequency_ghz, float(angles[0]), pol,
+                                              geometry_units="meters", max_panels=100000)
+        arrays = {k: v for k, v in dens.items() if isinstance(v, list) and v and isinstance(v[0], float)}
+        scalars = {k: v for k, v in dens.items() if k not in arrays}
+        for k, v in arrays.items():
+            out[f"dens_{pol}_{k}"] = np.asarray(v, float)
+        out[f"dens_{pol}_scalars"] = np.array(json.dumps(scalars, sort_keys=True, default=str))
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print(f"{name}: conditions " + ", ".join(f"{k[5:]}={out[k]:.6g}" for k in out if k.startswith("cond_")
+                                             and not k.startswith("cond_method")), flush=True)
+
+
+def main_condition():
+    r = 0.1
+    run_condition_case("cond_circle64_ka1", workloads.circle_points(r, 64), workloads.frequency_for_ka(1.0, r),
+                       [0.0, 90.0])
+    j0 = 2.404825557695773
+    for tag, ka in (("below", j0 - 0.02), ("at", j0), ("above", j0 + 0.02)):
+        run_condition_case(f"cond_circle64_j0_{tag}", workloads.circle_points(r, 64),
+                           workloads.frequency_for_ka(ka, r), [0.0])
+    pts = workloads.naca0012_points(40)
+    run_condition_case("cond_naca80", pts, workloads.frequency_for_segments_per_wavelength(pts, 10.0), [0.0, 30.0])
+
+
 if __name__ == "__main__":
-    if "--certified" in sys.argv:
+    if "--condition" in sys.argv:
+        main_condition()
+    elif "--certified" in sys.argv:
         main_certified()
     elif "--meshes" in sys.argv:
         main_meshes()
