@@ -106,7 +106,8 @@ def test_equilibration_matches_definition():
         eq = np.abs(a[i]) / row[:, None]
         col = np.max(eq, axis=0)
         col = np.where(col > 0, col, 1.0)
-        assert np.array_equal(rs[i].cpu().numpy(), row)
+        # the device modulus (hypot) and NumPy's may differ in the last bit
+        assert np.max(np.abs(rs[i].cpu().numpy() / row - 1)) < 4e-16
         assert np.max(np.abs(cs[i].cpu().numpy() / col - 1)) < 1e-15
         want = np.max(np.sum(eq / col[None, :], axis=0))
         assert abs(float(norm[i]) / want - 1) < 1e-13
