@@ -143,33 +143,6 @@ extern "C" int grim_shape_value(int knob, int64_t size) {
 
 extern "C" int grim_launch_shape(int knob, int64_t size) { return grim_shape_value(knob, size); }
 
-// Gauss-Legendre rules (numpy.polynomial.legendre.leggauss mapped to [0, 1],
-// the reference's _quadrature_nodes, twod/operators.py:3319-3323) and the
-// 20-point t^4 graded corner rule (twod/polynomial_quadrature.py:319-335).
-extern "C" int grim_init(int device) {
-    if (hipSetDevice(device) != hipSuccess) return GRIM_ERR_HIP;
-    const int n = 20;
-    std::vector<double> tx(800), ty(800), tw(800);
-    for (int i = 0; i < n; ++i) {
-        const double t = GRIM_GL20_T[i], w = GRIM_GL20_W[i];
-        const double u = t * t * t * t;
-        for (int j = 0; j < n; ++j) {
-            const double v = GRIM_GL20_T[j];
-            const double wt = 4.0 * (t * t * t * t * t * t * t) * w * GRIM_GL20_W[j];
-            const int a = i * n + j;
-            tx[a] = u;
-            ty[a] = u * v;
-            tw[a] = wt;
-            tx[400 + a] = u * v;
-            ty[400 + a] = u;
-            tw[400 + a] = wt;
-        }
-    }
-    CHECK(grim_upload_rules(GRIM_GL8_T, GRIM_GL8_W, GRIM_GL16_T, GRIM_GL16_W, tx.data(), ty.data(), tw.data()));
-    CHECK(hipDeviceSynchronize());
-    return GRIM_OK;
-}
-
 struct AssemblyWork {
     unsigned* status;
     int *counts, *offsets, *pair_e, *pair_f, *kind;
@@ -192,26 +165,97 @@ static AssemblyWork carve(void* base, int64_t n, int64_t cap) {
     return w;
 }
 
+// ---------------------------------------------------------------------------
+// Static scratch of grim_assemble_noalloc: the near-pair lists and blocks that grim_assemble
+// takes from its workspace argument.  Each device the library runs on holds its own copy for the
+// life of the process.  Host code never uses the array's name as an address: grim_init (or the
+// entry point's first call on a device) asks the runtime for the device's address once, with
+// that device current, and keeps it per device.
+// ---------------------------------------------------------------------------
+#define GRIM_NOALLOC_SCRATCH_BYTES (64ll << 20)
+static __device__ __align__(256) unsigned char scratch[GRIM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices];
+std::mutex g_scratch_lock;
+
+// The current device's address of `scratch` (looked up once per device), or null.
+void* device_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch[device] = ptr;
+    }
+    return g_scratch[device];
+}
+
+// The near-pass arrays of carve() without the status word, which the caller owns here.
+AssemblyWork carve_scratch(void* base, int64_t n, int64_t cap, unsigned* status) {
+    AssemblyWork w;
+    char* p = (char*)base;
+    int64_t off = 0;
+    w.status = status;
+    w.counts = (int*)(p + off); off += align_up(4 * n);
+    w.offsets = (int*)(p + off); off += align_up(4 * (n + 1));
+    w.pair_e = (int*)(p + off); off += align_up(4 * cap);
+    w.pair_f = (int*)(p + off); off += align_up(4 * cap);
+    w.kind = (int*)(p + off); off += align_up(4 * cap);
+    w.blocks = (double*)(p + off); off += align_up(16 * 8 * cap);
+    w.bytes = off;
+    return w;
+}
+}  // namespace
+
+// Gauss-Legendre rules (numpy.polynomial.legendre.leggauss mapped to [0, 1],
+// the reference's _quadrature_nodes, twod/operators.py:3319-3323) and the
+// 20-point t^4 graded corner rule (twod/polynomial_quadrature.py:319-335).
+extern "C" int grim_init(int device) {
+    if (hipSetDevice(device) != hipSuccess) return GRIM_ERR_HIP;
+    if (!device_scratch(device)) return GRIM_ERR_HIP;
+    const int n = 20;
+    std::vector<double> tx(800), ty(800), tw(800);
+    for (int i = 0; i < n; ++i) {
+        const double t = GRIM_GL20_T[i], w = GRIM_GL20_W[i];
+        const double u = t * t * t * t;
+        for (int j = 0; j < n; ++j) {
+            const double v = GRIM_GL20_T[j];
+            const double wt = 4.0 * (t * t * t * t * t * t * t) * w * GRIM_GL20_W[j];
+            const int a = i * n + j;
+            tx[a] = u;
+            ty[a] = u * v;
+            tw[a] = wt;
+            tx[400 + a] = u * v;
+            ty[400 + a] = u;
+            tw[400 + a] = wt;
+        }
+    }
+    CHECK(grim_upload_rules(GRIM_GL8_T, GRIM_GL8_W, GRIM_GL16_T, GRIM_GL16_W, tx.data(), ty.data(), tw.data()));
+    CHECK(hipDeviceSynchronize());
+    return GRIM_OK;
+}
+
 extern "C" int64_t grim_assemble_workspace_bytes(int64_t n, int64_t near_per_element) {
     if (n <= 0 || near_per_element <= 0) return -1;
     return carve(nullptr, n, n * near_per_element).bytes;
 }
 
 
-static int assemble_pass(int passes, int64_t n, const double* p0, const double* p1, const int32_t* next,
-                         const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
-                         double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, void* workspace,
-                         int64_t workspace_bytes, int64_t near_per_element, void* stream) {
-    if (n < 2 || n > (1 << 30) || !p0 || !p1 || !next || !prev || !z_tm || !z_te || !workspace ||
-        ld_tm < n || ld_te < n || !(k > 0) || !isfinite(k) || near_per_element <= 0 || nfix < 0 ||
-        (nfix > 0 && !fix_rows))
-        return GRIM_ERR_ARGS;
-    if (far_order != 8 && far_order != 16) return GRIM_ERR_ARGS;
-    const int64_t cap = n * near_per_element;
-    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
-    AssemblyWork w = carve(workspace, n, cap);
-    if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
-    hipStream_t st = (hipStream_t)stream;
+static bool assemble_args_ok(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                             const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
+                             double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, int64_t near_per_element) {
+    return !(n < 2 || n > (1 << 30) || !p0 || !p1 || !next || !prev || !z_tm || !z_te || ld_tm < n || ld_te < n ||
+             !(k > 0) || !isfinite(k) || near_per_element <= 0 || nfix < 0 || (nfix > 0 && !fix_rows) ||
+             (far_order != 8 && far_order != 16));
+}
+
+// The far and near passes with the near-pair lists and blocks in `w` (status word included).
+static int assemble_launch(int passes, int64_t n, const double* p0, const double* p1, const int32_t* next,
+                           const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
+                           double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, const AssemblyWork& w,
+                           int64_t cap, hipStream_t st) {
     if (passes & 1) {
         zero_words<<<1, 64, 0, st>>>(w.status, 1);
         CHECK(hipGetLastError());
@@ -222,6 +266,21 @@ static int assemble_pass(int passes, int64_t n, const double* p0, const double* 
         CHECK(grim_launch_near((int)n, p0, p1, next, prev, k, w.counts, w.offsets, (int)cap, w.pair_e, w.pair_f,
                                w.blocks, w.kind, w.status, z_tm, ld_tm, z_te, ld_te, st));
     return GRIM_OK;
+}
+
+static int assemble_pass(int passes, int64_t n, const double* p0, const double* p1, const int32_t* next,
+                         const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
+                         double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, void* workspace,
+                         int64_t workspace_bytes, int64_t near_per_element, void* stream) {
+    if (!workspace || !assemble_args_ok(n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te,
+                                        ld_te, near_per_element))
+        return GRIM_ERR_ARGS;
+    const int64_t cap = n * near_per_element;
+    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
+    AssemblyWork w = carve(workspace, n, cap);
+    if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
+    return assemble_launch(passes, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te, w,
+                           cap, (hipStream_t)stream);
 }
 
 extern "C" int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* next,
@@ -246,6 +305,25 @@ extern "C" int grim_assemble_near(int64_t n, const double* p0, const double* p1,
                                   void* workspace, int64_t workspace_bytes, int64_t near_per_element, void* stream) {
     return assemble_pass(2, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te,
                          workspace, workspace_bytes, near_per_element, stream);
+}
+
+extern "C" int grim_assemble_noalloc(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                     const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k,
+                                     int far_order, double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te,
+                                     int64_t near_per_element, void* status, void* stream) {
+    if (!status || !assemble_args_ok(n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te,
+                                     near_per_element))
+        return GRIM_ERR_ARGS;
+    const int64_t cap = n * near_per_element;
+    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
+    if (carve_scratch(nullptr, n, cap, nullptr).bytes > GRIM_NOALLOC_SCRATCH_BYTES) return GRIM_ERR_RANGE;
+    int device = -1;
+    CHECK(hipGetDevice(&device));
+    void* base = device_scratch(device);
+    if (!base) return GRIM_ERR_HIP;
+    const AssemblyWork w = carve_scratch(base, n, cap, (unsigned*)status);
+    return assemble_launch(3, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te, w, cap,
+                           (hipStream_t)stream);
 }
 
 extern "C" int grim_workspace_status(void* workspace, void* stream) {

@@ -28,6 +28,8 @@ SIGNATURES = {
                                  _i64, _i64, _vp]),
     "grim_assemble_near": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _i32, _vp, _i64, _vp, _i64, _vp,
                                   _i64, _i64, _vp]),
+    "grim_assemble_noalloc": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _i32, _vp, _i64, _vp, _i64, _i64,
+                                     _vp, _vp]),
     "grim_workspace_status": (_i32, [_vp, _vp]),
     "grim_loads": (_i32, [_i64, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _i64, _vp]),
     "grim_farfield": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp, _i64, _dbl, _vp, _vp, _vp]),

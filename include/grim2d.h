@@ -54,6 +54,22 @@ int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* 
                   double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
                   int64_t near_per_element, void* stream);
 
+/* grim_assemble without a workspace argument, for callers that cannot allocate: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation.  No counterpart in the
+ * reference.  It allocates nothing: the near-pair lists and blocks that grim_assemble keeps in
+ * its workspace live in a static device array of 64 MiB, one per device the library runs on,
+ * held until the process ends.  That array must hold
+ * n * 8 + near_per_element * n * 140 bytes (each of six arrays rounded up to 256 bytes):
+ * n <= 9974 at 48 near pairs per element, 14952 at 32, 29852 at 16; larger requests return 1
+ * and launch nothing.  status: caller-owned device word (>= 4 bytes), cleared and then set by
+ * the kernels; read it with grim_workspace_status(status, stream).  Same results as
+ * grim_assemble, bit for bit.  Every call on one GPU uses that GPU's one array: calls that
+ * overlap on one GPU (on different streams) race. */
+int grim_assemble_noalloc(int64_t n, const double* p0, const double* p1, const int32_t* next, const int32_t* prev,
+                          const int32_t* fix_rows, int64_t nfix, double k, int far_order, double* z_tm,
+                          int64_t ld_tm, double* z_te, int64_t ld_te, int64_t near_per_element, void* status,
+                          void* stream);
+
 /* The two passes of grim_assemble, for callers that time or overlap them:
  * grim_assemble_far writes every entry of z_tm / z_te (far pairs; zero for
  * near pairs) and must run first; grim_assemble_near adds the near-pair
