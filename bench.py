@@ -15,7 +15,11 @@ Measured forms:
     refinement), per-stage timers and its accuracy against the FP64 path;
   * serving form (the headline value): --steps distinct requests (no cache
     reuse, so every request assembles and factors) with --workers requests in
-    flight on their own streams, for mixed and for FP64.
+    flight on their own streams, each in the precision the flop model picks
+    for it (the server's default); forced mixed and forced FP64 serving of
+    the same requests beside it;
+  * a 32-request Poisson stream of the serving configuration (N up to about
+    12 000, cache reuse): plain FP64 against the serving form.
 Each request uploads its contour (a few tens of KB) as a serving system does;
 everything else is produced and consumed on the GPU.
 
@@ -58,7 +62,8 @@ def parse():
                     help="naca: the headline NACA 0012 N=2000 request (default); sweep: 64-frequency star-curve "
                          "sweep; stream: Poisson serving stream with cache reuse; large: one N=40000 request")
     ap.add_argument("--large-n", type=int, default=40000)
-    ap.add_argument("--stream-count", type=int, default=64)
+    ap.add_argument("--stream-count", type=int, default=64,
+                    help="requests of the stream workload; the default run reports a 32-request stream (0: skip)")
     ap.add_argument("--stream-rate", type=float, default=200.0,
                     help="Poisson arrival rate (requests/s); the default saturates the server")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -240,7 +245,9 @@ def far_valu():
 
 
 def serve(engine_mod, mesh, f, angles, precision, workers, nreq, warm):
-    """Closed-loop serving: nreq distinct requests (no cache reuse) with `workers` in flight."""
+    """Closed-loop serving: nreq distinct requests (no cache reuse) with `workers` in flight.
+    Returns the results, the elapsed time, service times (ms) and the precision each request
+    ran in ('auto' resolves per request by the flop model)."""
     from grim_20261007_amd import serving
     srv = serving.Server(workers=workers)
     try:
@@ -251,7 +258,48 @@ def serve(engine_mod, mesh, f, angles, precision, workers, nreq, warm):
     finally:
         srv.close()
     svc = np.array([r.finished - r.started for r in res]) * 1e3
-    return res, elapsed, svc
+    chosen = {p: sum(r.precision == p for r in reqs) for p in ("double", "mixed")}
+    return res, elapsed, svc, chosen
+
+
+def stream_summary(workers: int, count: int = 32, seed: int = 11, rate_hz: float = 200.0):
+    """The Poisson serving stream (star-shaped curves, N log-uniform in [200, 12000], 90 / 360 /
+    1440 angles, 30 % reusing an earlier system): plain FP64 back to back against the serving
+    form in the cost-model precision ('auto') and in forced mixed precision, on the same seeded
+    requests and arrivals."""
+    from grim_20261007_amd import engine, serving, workloads
+    from grim_20261007_amd.mesh import build_mesh
+    stream = workloads.serving_stream(count, seed=seed)
+    meshes = {}
+    for it in stream:
+        if it["key"] not in meshes:
+            meshes[it["key"]] = build_mesh(workloads._snapshot(it["points"], "star"), "meters", it["frequency_ghz"])
+    ms = [meshes[it["key"]] for it in stream]
+    freqs = [it["frequency_ghz"] for it in stream]
+    angs = [it["angles"] for it in stream]
+    _requests_plain(engine, ms[:1], freqs[:1], angs[:1], "double")
+    t64, lat64 = _requests_plain(engine, ms, freqs, angs, "double")
+    out = {"requests": len(stream), "n_min": min(m.n for m in ms), "n_max": max(m.n for m in ms),
+           "reused": int(sum(it["reused"] for it in stream)), "arrival_rate_hz": rate_hz,
+           "fp64_plain": {"value": round(2 * len(stream) / t64, 3), "p99_ms": round(float(np.percentile(lat64, 99)), 3),
+                          "form": "back to back, no arrivals, no reuse"}}
+    for precision in ("auto", "mixed"):
+        srv = serving.Server(workers=workers)
+        try:
+            serving.run_stream(srv, [serving.Request(ms[0], freqs[0], angs[0], key="warm")])
+            reqs = [serving.Request(m, f, a, precision=precision, key=it["key"])
+                    for m, f, a, it in zip(ms, freqs, angs, stream)]
+            res, tm = serving.run_stream(srv, reqs, rate_hz=rate_hz, rng=np.random.default_rng(seed))
+            hits = srv.cache.hits
+        finally:
+            srv.close()
+        lat = np.array([r.latency_s for r in res]) * 1e3
+        out[f"{precision}_serving"] = {
+            "value": round(2 * len(stream) / tm, 3), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "cache_hits": hits,
+            "requests_mixed": int(sum(r.precision == "mixed" for r in reqs)),
+            "vs_fp64_plain": round(t64 / tm, 3), "max_residual": max(max(r.residual.values()) for r in res)}
+    return out
 
 
 def _requests_plain(engine, meshes, freqs, angle_sets, precision):
@@ -488,16 +536,19 @@ def main():
     plain64 = 2 * seq_steps / t64
     lat64 = np.array(lat["double"]) * 1e3
     lat32 = np.array(lat["mixed"]) * 1e3
-    # 2. the serving form: args.steps distinct requests, args.workers in flight (timed region)
+    # 2. the serving form: args.steps distinct requests, args.workers in flight.  The headline is
+    # the server's default, the cost-model precision per request ('auto'); forced mixed and
+    # forced FP64 serving run on the same requests beside it.
     torch.cuda.synchronize()
-    res_m, t_m, svc_m = serve(engine, mesh, f, angles, "mixed", args.workers, args.steps, args.workers)
-    res_d, t_d, svc_d = serve(engine, mesh, f, angles, "double", args.workers, args.steps, args.workers)
-    value = 2 * args.steps / t_m
+    res_a, t_a, svc_a, chosen = serve(engine, mesh, f, angles, "auto", args.workers, args.steps, args.workers)
+    res_m, t_m, svc_m, _ = serve(engine, mesh, f, angles, "mixed", args.workers, args.steps, args.workers)
+    res_d, t_d, svc_d, _ = serve(engine, mesh, f, angles, "double", args.workers, args.steps, args.workers)
+    value = 2 * args.steps / t_a
     # every served mixed result within the tolerance of the FP64 path
     ref_sig = {pol: results["double"][0][pol][1].cpu().numpy() for pol in ("TE", "TM")}
     serve_db = max(float(np.max(np.abs(10 * np.log10(r.sigma[pol]) - 10 * np.log10(ref_sig[pol]))))
-                   for r in res_m for pol in ("TE", "TM"))
-    serve_res = max(max(r.residual.values()) for r in res_m)
+                   for r in res_m + res_a for pol in ("TE", "TM"))
+    serve_res = max(max(r.residual.values()) for r in res_m + res_a)
     # roofline of the dominant single launch (far-pair assembly pass)
     tm = timers["mixed"]
     far_ms = 0.5 * (tm.mean_ms("assemble_far") + timers["double"].mean_ms("assemble_far"))
@@ -531,22 +582,34 @@ def main():
         if stage:
             tf = lu_flops / (stage * 1e-3) / 1e12
             lu[key] = {"factor_ms": stage, "tflops": round(tf, 2), f"frac_of_{label}_peak": round(tf / peak, 4)}
+    stream = stream_summary(args.workers) if args.stream_count > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mesh, f, angles, args.cpu_seconds)
+    plain32 = 2 * seq_steps / results["mixed"][1]
+    if chosen["mixed"] == 0:
+        dtype = "c128 (f64) LU + f64 backward-error gate"
+    elif chosen["double"] == 0:
+        dtype = "c64 LU + f64 refinement"
+    else:
+        dtype = "per request: c128 LU or c64 LU + f64 refinement"
     line = {
         "metric": "RCS solves/sec + p99 latency, mixed-prec vs FP64 GPU at <=0.05 dB RCS error",
         "value": round(value, 3), "unit": "RCS solves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * t_m / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "c64 LU + f64 refinement",
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_a / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": dtype,
         "data": "synthetic NACA 0012 geometry",
         "config": {"workload": f"NACA 0012, chord 1 m, N={mesh.n} linear Galerkin unknowns, f={f:.4f} GHz "
                                f"(longest panel lambda/10), {len(angles)} monostatic angles, TE+TM co-polarized "
-                               f"requests; mixed-precision serving form with {args.workers} requests in flight "
-                               "(distinct requests, no cache reuse)",
-                   "n": mesh.n, "angles": len(angles), "frequency_ghz": f, "workers": args.workers},
-        "p50_ms": round(float(np.percentile(svc_m, 50)), 3), "p99_ms": round(float(np.percentile(svc_m, 99)), 3),
-        "vs_plain_fp64": round(value / plain64, 3),
+                               f"requests; serving form with {args.workers} requests in flight (distinct requests, "
+                               "no cache reuse), each request in the precision the flop model picks for it",
+                   "n": mesh.n, "angles": len(angles), "frequency_ghz": f, "workers": args.workers,
+                   "precision_choice": chosen},
+        "p50_ms": round(float(np.percentile(svc_a, 50)), 3), "p99_ms": round(float(np.percentile(svc_a, 99)), 3),
+        "serving_vs_plain_fp64": round(value / plain64, 3),
+        "mixed_vs_fp64": {"serving": round(t_d / t_m, 3), "sequential": round(plain32 / plain64, 3),
+                          "note": "same form both sides: mixed serving / FP64 serving, mixed one at a time / "
+                                  "FP64 one at a time"},
         "angle_columns_per_s": round(value * len(angles), 1),
         "fp64_path": {"value": round(plain64, 3), "p50_ms": round(float(np.percentile(lat64, 50)), 3),
                       "p99_ms": round(float(np.percentile(lat64, 99)), 3), "stage_ms": stage64,
@@ -554,16 +617,20 @@ def main():
         "fp64_serving": {"value": round(2 * args.steps / t_d, 3),
                          "p50_ms": round(float(np.percentile(svc_d, 50)), 3),
                          "p99_ms": round(float(np.percentile(svc_d, 99)), 3)},
+        "mixed_serving": {"value": round(2 * args.steps / t_m, 3),
+                          "p50_ms": round(float(np.percentile(svc_m, 50)), 3),
+                          "p99_ms": round(float(np.percentile(svc_m, 99)), 3)},
         "mixed_sequential": {"value": round(2 * seq_steps / results["mixed"][1], 3),
                              "p50_ms": round(float(np.percentile(lat32, 50)), 3),
                              "p99_ms": round(float(np.percentile(lat32, 99)), 3), "stage_ms": stage_ms},
         "accuracy": {"max_rcs_db_error_vs_fp64": max(db_err, serve_db), "max_amplitude_rel_error": amp_err,
                      "max_refined_residual": max(resid, serve_res), "refinement_steps": steps_mixed},
-        "roofline": {"kernel": "far pass (far_kernel + far_fixup, grim_assemble_far)", "bound": "mfma",
+        "roofline": {"kernel": "far pass (far_kernel + far_fixup, grim_assemble_far)", "bound": "valu",
                      "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64, 4), "traffic": far_traffic(),
                      "model": f"{pairs} unordered far pairs x {q}x{q} points x {FLOPS_PER_EVAL} FP64 flops "
-                              f"per launch; FP64 VALU-bound (MI355X FP64 vector peak = FP64 matrix peak)"},
+                              f"per launch; FP64 VALU-bound: peak is the FP64 vector rate, which equals the FP64 "
+                              f"matrix rate on MI355X"},
         "roofline_fp64_gemm": {"kernel": "cgemm<double> (refinement residual b - Z x, 2 x N x A x N)",
                                "bound": "mfma", "achieved": round(gemm_tf, 2), "peak": PEAK_FP64,
                                "unit": "TFLOP/s", "frac": round(gemm_tf / PEAK_FP64, 4),
@@ -571,6 +638,7 @@ def main():
         "assembly": assembly,
         "lu_n2000": lu,
         "cpu_baseline": cpu,
+        "stream": stream,
         "launch_shapes": shapes,
     }
     print(json.dumps(line))

@@ -53,6 +53,9 @@ extern "C" hipError_t grim_dense_gemm_minus_ta(int batch, int m, int n, int k, c
                                                int64_t ldc, int64_t sc, int single, hipStream_t st);
 extern "C" hipError_t grim_dense_equilibration(int batch, int n, const double* a, int64_t lda, int64_t sa,
                                                double* row_scale, double* col_scale, double* norm, hipStream_t st);
+extern "C" int64_t grim_dense_norm_inf_work(int n);
+extern "C" hipError_t grim_dense_norm_inf(int batch, int n, const double* a, int64_t lda, int64_t sa, double* work,
+                                          double* out, hipStream_t st);
 
 #define CHECK(x)                                   \
     do {                                           \
@@ -457,6 +460,21 @@ extern "C" int grim_equilibration(int64_t batch, int64_t n, const double* z, int
         return GRIM_ERR_ARGS;
     CHECK(grim_dense_equilibration((int)batch, (int)n, z, ldz, stride_z, row_scale, col_scale, norm,
                                    (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+extern "C" int64_t grim_norm_inf_workspace_bytes(int64_t batch, int64_t n) {
+    if (!batch_ok(batch) || n < 1 || n > (1 << 30)) return -1;
+    return batch * grim_dense_norm_inf_work((int)n);
+}
+
+extern "C" int grim_norm_inf(int64_t batch, int64_t n, const double* z, int64_t ldz, int64_t stride_z, void* work,
+                             int64_t work_bytes, double* out, void* stream) {
+    if (!batch_ok(batch) || n < 1 || n > (1 << 30) || !z || !out || !work || ldz < n ||
+        (batch > 1 && stride_z < ldz * n))
+        return GRIM_ERR_ARGS;
+    if (work_bytes < grim_norm_inf_workspace_bytes(batch, n)) return GRIM_ERR_RANGE;
+    CHECK(grim_dense_norm_inf((int)batch, (int)n, z, ldz, stride_z, (double*)work, out, (hipStream_t)stream));
     return GRIM_OK;
 }
 

@@ -1707,6 +1707,50 @@ extern "C" hipError_t grim_dense_equilibration(int batch, int n, const double* a
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// ||A||_inf = max_i sum_j |a_ij| of complex128 matrices (linalg/dense.py, the backward-error
+// gate's matrix_inf_norm): partial sums of 64 columns per thread and row (coalesced down the
+// columns), then each row adds its partials in chunk order, so the result is deterministic.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) row_abs_partial(int n, const double* __restrict__ a, int64_t lda, int64_t sa,
+                                                       double* __restrict__ part) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    a += 2 * sa * blockIdx.z;
+    const int chunks = (n + 63) / 64;
+    part += (int64_t)n * chunks * blockIdx.z;
+    const int c0 = 64 * blockIdx.y, c1 = min(n, c0 + 64);
+    double s = 0.0;
+    for (int c = c0; c < c1; ++c) {
+        const double* p = a + 2 * ((int64_t)i + (int64_t)c * lda);
+        s += hypot(p[0], p[1]);
+    }
+    part[(int64_t)blockIdx.y * n + i] = s;
+}
+
+__global__ void __launch_bounds__(256) row_abs_finish(int n, const double* __restrict__ part,
+                                                      unsigned long long* __restrict__ out) {
+    __shared__ double sh[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int chunks = (n + 63) / 64;
+    part += (int64_t)n * chunks * blockIdx.z;
+    double s = 0.0;
+    if (i < n)
+        for (int c = 0; c < chunks; ++c) s += part[(int64_t)c * n + i];
+    const double m = block_reduce(s, true, sh);
+    if (threadIdx.x == 0) atomicMax(out + blockIdx.z, (unsigned long long)__double_as_longlong(m));
+}
+
+extern "C" int64_t grim_dense_norm_inf_work(int n) { return (int64_t)n * ((n + 63) / 64) * 8; }
+
+extern "C" hipError_t grim_dense_norm_inf(int batch, int n, const double* a, int64_t lda, int64_t sa, double* work,
+                                          double* out, hipStream_t st) {
+    fill_u64<<<1, 256, 0, st>>>(batch, (unsigned long long*)out);
+    row_abs_partial<<<dim3((n + 255) / 256, (n + 63) / 64, batch), 256, 0, st>>>(n, a, lda, sa, work);
+    row_abs_finish<<<dim3((n + 255) / 256, 1, batch), 256, 0, st>>>(n, work, (unsigned long long*)out);
+    return hipGetLastError();
+}
+
 extern "C" int64_t grim_dense_aux_bytes(int n, int single) {
     (void)single;
     return aux_bytes(n);

@@ -184,6 +184,7 @@ class Factor:
     batch: int
     fallback_reason: str = ""   # why a mixed factorization was replaced by the FP64 one
     parts: list | None = None   # per-system factors once some systems of a batch fell back
+    norm_inf: torch.Tensor | None = None   # (B,) ||Z||_inf of the systems, once computed
 
     @property
     def prec(self) -> int:
@@ -359,8 +360,9 @@ def ctypes_double():
 def _backward_error_gate(f: Factor, x: torch.Tensor, b: torch.Tensor):
     """Normwise backward error <= 1e-12 with at most two corrections (linalg/dense.py:257-280)."""
     z = f.z
-    # column-major storage: row sums of |Z| are sums over dim 1 of each (N, N) tensor
-    a_inf = torch.amax(torch.sum(torch.abs(z), dim=1), dim=1)          # (B,)
+    if f.norm_inf is None:
+        f.norm_inf = norm_inf(z)
+    a_inf = f.norm_inf                                                  # (B,)
     steps = 0
 
     def metrics(xx):
@@ -382,6 +384,19 @@ def _backward_error_gate(f: Factor, x: torch.Tensor, b: torch.Tensor):
     if not math.isfinite(err) or err > BACKWARD_ERROR_MAX:
         raise RuntimeError(f"normwise backward error {err} exceeds the release limit {BACKWARD_ERROR_MAX}.")
     return x, {"refinement_steps": steps, "backward_error": err, "residual": r}
+
+
+def norm_inf(z: torch.Tensor) -> torch.Tensor:
+    """||Z||_inf (max row sum of moduli) of each system of a batch (B, N, N), on the device."""
+    zb = _as_batch(z)
+    nb, n = zb.shape[0], zb.shape[-1]
+    lib = native.lib()
+    nbytes = int(lib.grim_norm_inf_workspace_bytes(nb, n))
+    work = torch.empty(nbytes, dtype=torch.uint8, device=z.device)
+    out = torch.empty(nb, dtype=torch.float64, device=z.device)
+    native.check("grim_norm_inf", lib.grim_norm_inf(nb, n, zb.data_ptr(), n, n * n, work.data_ptr(), nbytes,
+                                                    out.data_ptr(), _stream(z.device)))
+    return out
 
 
 def relative_residual(z: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor | None = None):

@@ -49,6 +49,8 @@ SIGNATURES = {
     "grim_gemm_minus_ex": (_i32, [_i64, _i64, _i64, _i64, _i32, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64,
                                   _i32, _vp]),
     "grim_equilibration": (_i32, [_i64, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "grim_norm_inf_workspace_bytes": (_i64, [_i64, _i64]),
+    "grim_norm_inf": (_i32, [_i64, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _vp]),
     "grim_refine_ex_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32]),
     "grim_refined_solve_ex": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i32, _vp, _i64,
                                      _i64, _vp, _i64, _i64, _i32, _dbl, _vp, _i64, ctypes.POINTER(_i32),

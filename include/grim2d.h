@@ -158,6 +158,14 @@ int grim_gemm_minus_ex(int64_t batch, int64_t m, int64_t n, int64_t k, int trans
 int grim_equilibration(int64_t batch, int64_t n, const double* z, int64_t ldz, int64_t stride_z, double* row_scale,
                        double* col_scale, double* norm, void* stream);
 
+/* Infinity norms max_i sum_j |z_ij| of complex128 matrices into out[batch] (device), with a
+ * caller-owned workspace of grim_norm_inf_workspace_bytes: the matrix norm of the normwise
+ * backward-error gate (linalg/workspace.py:19-98, matrix_inf_norm / checked_row_norms, as
+ * linalg/dense.py:36 and :257-280 use it).  Deterministic (fixed summation order). */
+int64_t grim_norm_inf_workspace_bytes(int64_t batch, int64_t n);
+int grim_norm_inf(int64_t batch, int64_t n, const double* z, int64_t ldz, int64_t stride_z, void* work,
+                  int64_t work_bytes, double* out, void* stream);
+
 /* C -= A B (complex; precision 64 or 32); the FP64 refinement residual. */
 int grim_gemm_minus(int64_t batch, int64_t m, int64_t n, int64_t k, const void* a, int64_t lda, int64_t stride_a,
                     const void* b, int64_t ldb, int64_t stride_b, void* c, int64_t ldc, int64_t stride_c,

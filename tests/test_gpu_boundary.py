@@ -255,3 +255,15 @@ def test_memory_admission_before_assembly():
         solve_monostatic_rcs_2d(workloads.circle(50.0, 200000), [1.0], [0.0], geometry_units="meters",
                                 max_panels=250000)
     assert torch.cuda.memory_allocated() == before
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 700, 2000])
+def test_norm_inf_matches_definition(n):
+    """||Z||_inf of the backward-error gate (linalg/workspace.py:19-98) on the device, batch of two."""
+    from grim_20261007_amd import engine
+    _dev()
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((2, n, n)) * np.exp(rng.standard_normal((2, n, 1))) + 1j * rng.standard_normal((2, n, n))
+    got = engine.norm_inf(_cm(a)).cpu().numpy()
+    want = np.max(np.sum(np.abs(a), axis=2), axis=1)
+    assert np.max(np.abs(got / want - 1)) < 1e-13
