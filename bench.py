@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--tuning", default="mi355x",
                     help="launch-shape table: 'mi355x' (shipped, measured by the autotuner), a JSON path, "
                          "'tune' (measure now) or 'none' (built-in rules)")
-    ap.add_argument("--cpu-seconds", type=float, default=40.0)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
     ap.add_argument("--workload", choices=("naca", "sweep", "stream", "large"), default="naca",
                     help="naca: the headline NACA 0012 N=2000 request (default); sweep: 64-frequency star-curve "
                          "sweep; stream: Poisson serving stream with cache reuse; large: one N=40000 request")
@@ -164,6 +164,35 @@ def far_pair_count(mesh):
 
 
 def cpu_baseline(mesh, f, angles, seconds):
+    """Whole co-polarized requests on the host cores, timed end to end: the far pass with the
+    reference's own far.c (built from its source into oracle/_ref, symmetric tiles on one thread
+    per core, as its tile scheduler runs them), near pairs and mass terms with the oracle's
+    restatement of the reference's NumPy rules, SciPy/LAPACK LU and solves of both systems, the
+    matched far field.  Without oracle/_ref, the older bounded sample of the oracle alone."""
+    from oracle import bem2d, ref_far
+    if ref_far.available():
+        om = bem2d.Mesh(mesh.nodes, mesh.elements, mesh.p0, mesh.p1, 0.5 * (mesh.p0 + mesh.p1), mesh.lengths,
+                        mesh.normals)
+        cores = len(os.sched_getaffinity(0))
+        times, stages = [], {}
+        t_start = time.perf_counter()
+        while not times or (len(times) < 3 and time.perf_counter() - t_start < 0.5 * seconds):
+            t0 = time.perf_counter()
+            _, st = ref_far.full_request(om, f, angles, cores)
+            times.append(time.perf_counter() - t0)
+            for key, v in st.items():
+                stages[key] = stages.get(key, 0.0) + v
+        per = float(np.mean(times))
+        stage_txt = ", ".join(f"{key} {v / len(times):.2f} s" for key, v in stages.items())
+        return {"value": 2.0 / per, "unit": "RCS solves/s", "cores": cores, "kind": "reference",
+                "extrapolated": False,
+                "sample": (f"{len(times)} whole co-polarized requests (N={mesh.n}, {len(angles)} angles, TE + TM), "
+                           f"{per:.2f} s each: far pass by the reference's own far.c on {cores} threads, near pairs "
+                           f"by the oracle (one thread), LAPACK with {cores} threads; per request {stage_txt}")}
+    return _cpu_baseline_sampled(mesh, f, angles, seconds)
+
+
+def _cpu_baseline_sampled(mesh, f, angles, seconds):
     """The oracle (NumPy/SciPy restatement) on a bounded sample of one request, scaled to solves/s."""
     import scipy.linalg as sl
     from oracle import bem2d
@@ -209,7 +238,7 @@ def cpu_baseline(mesh, f, angles, seconds):
     t_ff = time.perf_counter() - t0
     per_request = t_far + t_near + 2 * (t_lu + t_solve + t_ff)
     cores = len(os.sched_getaffinity(0))
-    return {"value": 2.0 / per_request, "unit": "RCS solves/s", "cores": cores, "kind": "port",
+    return {"value": 2.0 / per_request, "unit": "RCS solves/s", "cores": cores, "kind": "port", "extrapolated": True,
             "sample": (f"oracle far rows {rows_done}/{n} and {done}/{int(total_near)} near pairs timed and "
                        f"scaled; full-size LAPACK LU ({t_lu:.2f} s), {len(angles)}-column solve "
                        f"({t_solve:.2f} s) and far field; per co-polarized request {per_request:.1f} s")}
@@ -540,9 +569,12 @@ def main():
     # the server's default, the cost-model precision per request ('auto'); forced mixed and
     # forced FP64 serving run on the same requests beside it.
     torch.cuda.synchronize()
-    res_a, t_a, svc_a, chosen = serve(engine, mesh, f, angles, "auto", args.workers, args.steps, args.workers)
-    res_m, t_m, svc_m, _ = serve(engine, mesh, f, angles, "mixed", args.workers, args.steps, args.workers)
-    res_d, t_d, svc_d, _ = serve(engine, mesh, f, angles, "double", args.workers, args.steps, args.workers)
+    # each form warms its own server with two rounds of requests (first-use allocations of every
+    # worker's stream pool fall outside the timed region)
+    warm = max(args.warmup, 2 * args.workers)
+    res_a, t_a, svc_a, chosen = serve(engine, mesh, f, angles, "auto", args.workers, args.steps, warm)
+    res_m, t_m, svc_m, _ = serve(engine, mesh, f, angles, "mixed", args.workers, args.steps, warm)
+    res_d, t_d, svc_d, _ = serve(engine, mesh, f, angles, "double", args.workers, args.steps, warm)
     value = 2 * args.steps / t_a
     # every served mixed result within the tolerance of the FP64 path
     ref_sig = {pol: results["double"][0][pol][1].cpu().numpy() for pol in ("TE", "TM")}

@@ -487,7 +487,18 @@ def assemble(mesh: Mesh, k: float, rows_per_chunk: int = 16, node_rows=None):
         far = far_mask(mesh, rows)
         s, kp, d = _far_rows(mesh, k, rows, far, q)
         scatter(rows, s, kp, d)
+    add_near_and_mass(mesh, k, z_tm, z_te, erows)
+    return z_tm, z_te
 
+
+def add_near_and_mass(mesh: Mesh, k: float, z_tm, z_te, erows=None):
+    """Add the near-pair blocks (exact self series, touching, adaptive and fixed rules; D from
+    the reverse pair's K', Maue W) and the mass / jump terms of element rows ``erows`` (all by
+    default) to z_tm, z_te in place."""
+    E = len(mesh.length)
+    erows = np.arange(E) if erows is None else np.asarray(erows)
+    gamma = 1.0 / (1j * k)
+    ids = mesh.elements
     # near pairs: S and K' per ordered pair; D from the reverse pair's K'.
     near = np.argwhere(~far_mask(mesh, erows))
     near[:, 0] = erows[near[:, 0]]
