@@ -344,7 +344,19 @@ def solve(f: Factor, b: torch.Tensor, gate: bool = True):
     native.check("grim_refined_solve", code)
     # the workspace starts with the final residual b - Z x of the returned x
     info["residual"] = ws[: 16 * nb * na * n].view(torch.complex128).view(nb, na, n)
+    # the FP64 gate's acceptance metric for the refined answer (linalg/dense.py:257-268), from
+    # that residual: a per-result statement of its accuracy next to the refinement error
+    if f.norm_inf is None:
+        f.norm_inf = norm_inf(f.z)
+    info["backward_error"] = backward_error(f.norm_inf, x, bb, info["residual"])
     return (x if b.dim() == 3 else x[0]), info
+
+
+def backward_error(a_inf: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor) -> float:
+    """max over columns of max|r| / (||Z||_inf max|x| + max|b|), batched (B, A, N)."""
+    ri = torch.amax(torch.abs(r), dim=2)
+    den = a_inf[:, None] * torch.amax(torch.abs(x), dim=2) + torch.amax(torch.abs(b), dim=2)
+    return float(torch.max(ri / den))
 
 
 def ctypes_int():
@@ -367,9 +379,7 @@ def _backward_error_gate(f: Factor, x: torch.Tensor, b: torch.Tensor):
 
     def metrics(xx):
         r = residual(z, xx, b)
-        ri = torch.amax(torch.abs(r), dim=2)                            # (B, A)
-        den = a_inf[:, None] * torch.amax(torch.abs(xx), dim=2) + torch.amax(torch.abs(b), dim=2)
-        return r, float(torch.max(ri / den))
+        return r, backward_error(a_inf, xx, b, r)
 
     r, err = metrics(x)
     for _ in range(2):

@@ -280,7 +280,10 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
             continue
         a, s = engine.farfield(x[i], bu, bdn, k)
         pending.append((pol, a, s, rel[i]))
-        info[pol] = {kk: v for kk, v in inf.items() if kk != "residual"}
+        # per result: refinement steps and error, the normwise backward error (the FP64 gate's
+        # metric) and the precision the factor ended in
+        info[pol] = {kk: v for kk, v in inf.items() if kk not in ("residual", "factor")}
+        info[pol]["precision_used"] = engine.system(fac, i).precision if fac.parts is not None else fac.precision
     torch.cuda.current_stream().synchronize()
     for pol, a, s, r in pending:
         amp[pol] = a.cpu().numpy()
