@@ -803,7 +803,10 @@ __global__ void __launch_bounds__(256) apply_list(int ncols, int pj, int pw, T* 
 }
 
 #define GRIM_LU_NB 64
+#ifndef GRIM_SOLVE_NB
 #define GRIM_SOLVE_NB 256
+#endif
+static_assert(GRIM_SOLVE_NB % 128 == 0 && GRIM_SOLVE_NB <= 256, "block solve: 128 or 256 rows");
 
 __device__ __forceinline__ double lane_value(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -904,20 +907,21 @@ __global__ void __launch_bounds__(256) trsm_lanes(int w, int ncols, const T* __r
     }
 }
 
-// One chunk of a step's coupling rows (CROWS = 3 x 64 rows, CPT / 3 x 4 columns), spread over
-// the workgroup: thread entry k is row lane + 64 (k % 3) of the coupling rows (from row c_lo of
-// the block) and column j0 + wave + 4 (k / 3) of the step's sub-block (from column r0), so every
+// One chunk of a step's coupling rows (CROWS = RB x 64 rows, CPT / RB x 4 columns), spread over
+// the workgroup: thread entry k is row lane + 64 (k % RB) of the coupling rows (from row c_lo of
+// the block) and column j0 + wave + 4 (k / RB) of the step's sub-block (from column r0), so every
 // offset but the lane's is a compile-time constant.  Zero outside nrow x w.
-__device__ __forceinline__ int chunk_row(int k) { return (int)(threadIdx.x & 63) + 64 * (k % 3); }
-__device__ __forceinline__ int chunk_col(int k) { return (int)(threadIdx.x >> 6) + 4 * (k / 3); }
+template <int RB> __device__ __forceinline__ int chunk_row(int k) { return (int)(threadIdx.x & 63) + 64 * (k % RB); }
+template <int RB> __device__ __forceinline__ int chunk_col(int k) { return (int)(threadIdx.x >> 6) + 4 * (k / RB); }
 template <typename T, int CPT, int CROWS>
 __device__ __forceinline__ void fetch_coupling(const T* __restrict__ a, int64_t lda, int c_lo, int nrow, int r0,
                                                int j0, int w, T (&pr)[CPT], T (&pi)[CPT]) {
-    static_assert(CROWS == 3 * 64 && CPT % 3 == 0, "chunk map assumes 192 coupling rows");
+    constexpr int RB = CROWS / 64;
+    static_assert(CROWS % 64 == 0 && CPT % RB == 0, "chunk map: whole 64-row groups");
     typedef T T2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
-        const int row = chunk_row(k), col = j0 + chunk_col(k);
+        const int row = chunk_row<RB>(k), col = j0 + chunk_col<RB>(k);
         T2 v = {0, 0};
         if (row < nrow && col < w)
             v = *reinterpret_cast<const T2*>(a + 2 * ((int64_t)(c_lo + row) + (int64_t)(r0 + col) * lda));
@@ -1055,8 +1059,8 @@ __global__ void __launch_bounds__(256, 2) solve_block(int nb, int ncols, const T
             __syncthreads();          // the previous chunk's readers are done
 #pragma unroll
             for (int k = 0; k < CPT; ++k) {
-                cp_r[chunk_col(k)][chunk_row(k)] = pr[k];
-                cp_i[chunk_col(k)][chunk_row(k)] = pi[k];
+                cp_r[chunk_col<CROWS / 64>(k)][chunk_row<CROWS / 64>(k)] = pr[k];
+                cp_i[chunk_col<CROWS / 64>(k)][chunk_row<CROWS / 64>(k)] = pi[k];
             }
             __syncthreads();
             if (q + 1 < nchunk)
