@@ -26,6 +26,14 @@ extern "C" hipError_t grim_launch_farfield(int n, int na, const double* x, int64
 extern "C" hipError_t grim_launch_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* y,
                                              int64_t ldy, const double* ax, const double* ay, const int* dirichlet,
                                              hipStream_t st);
+extern "C" hipError_t grim_launch_robin_split(int q, int n, const double* p0, const double* p1, const int* next,
+                                              const int* prev, double k, int nblocks, const int* walk,
+                                              const int* offsets, const int* pair_f, const double* blocks,
+                                              const double* c_te, const double* c_tm, const int* flags, double* zte,
+                                              int64_t ldte, double* ztm, int64_t ldtm, hipStream_t st);
+extern "C" hipError_t grim_launch_robin_rhs(int n, int na, const double* p0, const double* p1, const int* prev,
+                                            double k, const double* angles_deg, const double* alpha_e,
+                                            const int* dirichlet, double* out, int64_t ld, hipStream_t st);
 extern "C" int64_t grim_dense_aux_bytes(int n, int single);
 extern "C" hipError_t grim_dense_lu_factor(int batch, int n, void* a, int64_t lda, int64_t sa, int single, int* piv,
                                            int64_t spiv, void* aux, int64_t saux, unsigned* status, hipStream_t st);
@@ -649,6 +657,38 @@ extern "C" int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double*
 extern "C" int grim_column_norms(int64_t m, int64_t n, const double* v, int64_t ld, double* out, void* stream) {
     if (m < 1 || n < 0 || !v || !out || ld < m) return GRIM_ERR_ARGS;
     CHECK(grim_dense_colnorm((int)m, (int)n, v, ld, out, (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+extern "C" int grim_robin_split(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                const int32_t* prev, double k, int far_order, int64_t nblocks, const int32_t* walk,
+                                const double* c_te, const double* c_tm, const int32_t* flags, const void* workspace,
+                                int64_t workspace_bytes, int64_t near_per_element, double* z_te, int64_t ld_te,
+                                double* z_tm, int64_t ld_tm, void* stream) {
+    if (n < 2 || n > (1 << 30) || nblocks < 0 || nblocks > (int64_t)1 << 30 || !(k > 0) || !isfinite(k) ||
+        (far_order != 8 && far_order != 16) || near_per_element <= 0 || ld_te < n || ld_tm < n ||
+        (n + 31) / 32 > 65535)
+        return GRIM_ERR_ARGS;
+    if (nblocks == 0) return GRIM_OK;
+    if (!p0 || !p1 || !next || !prev || !walk || !c_te || !c_tm || !flags || !workspace || !z_te || !z_tm)
+        return GRIM_ERR_ARGS;
+    const int64_t cap = n * near_per_element;
+    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
+    const AssemblyWork w = carve(const_cast<void*>(workspace), n, cap);
+    if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
+    CHECK(grim_launch_robin_split(far_order, (int)n, p0, p1, next, prev, k, (int)nblocks, walk, w.offsets, w.pair_f,
+                                  w.blocks, c_te, c_tm, flags, z_te, ld_te, z_tm, ld_tm, (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+extern "C" int grim_robin_rhs(int64_t n, int64_t na, const double* p0, const double* p1, const int32_t* prev,
+                              double k, const double* angles_deg, const double* alpha_e, const int32_t* dirichlet,
+                              double* out, int64_t ld, void* stream) {
+    if (n < 1 || n > (1 << 30) || na < 0 || na > 65535 || ld < n || !(k > 0) || !isfinite(k)) return GRIM_ERR_ARGS;
+    if (na == 0) return GRIM_OK;
+    if (!p0 || !p1 || !prev || !angles_deg || !alpha_e || !dirichlet || !out) return GRIM_ERR_ARGS;
+    CHECK(grim_launch_robin_rhs((int)n, (int)na, p0, p1, prev, k, angles_deg, alpha_e, dirichlet, out, ld,
+                                (hipStream_t)stream));
     return GRIM_OK;
 }
 

@@ -887,6 +887,210 @@ __global__ void __launch_bounds__(256) robin_rows_kernel(int64_t rows, int64_t c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Surface impedance that varies along a contour (zones, tapers, tables)
+// ---------------------------------------------------------------------------
+// The Robin row of node p weights S by each observation element's own coefficient:
+//   alpha_e0 S0[p, :] + alpha_e1 S1[p, :] = alpha_p S[p, :] + delta_p (S0 - S1)[p, :],
+// where e0 = p (basis 1 - t), e1 = prev p (basis t), alpha_p = (alpha_e0 + alpha_e1) / 2 and
+// delta_p = (alpha_e0 - alpha_e1) / 2; its trace jump uses the element-weighted mass, which
+// differs from alpha_p M by delta_p m_p, m_p = (L_e0/6 at next p, (L_e0 - L_e1)/3 at p,
+// -L_e1/6 at prev p) (twod/formulations/regions.py:274-304, combined_regions.py:65-83).
+// robin_rows forms Z_TE + alpha_p Z_TM; robin_split adds delta_p ((S0 - S1)[p, :] + gamma/2 m_p).
+
+// S block of a far ordered pair (obs o, src s), the Q x Q rule of far_pair.
+template <int Q>
+__device__ __forceinline__ void far_pair_s(const Elem& o, const Elem& s, double k, cplx (&S)[2][2]) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) S[a][b] = {0.0, 0.0};
+    for (int qi = 0; qi < Q; ++qi) {
+        const double ti = Rule<Q>::t(qi), wi = Rule<Q>::w(qi);
+        const double xo = fma(ti, o.sx, o.x0), yo = fma(ti, o.sy, o.y0);
+        cplx ps0 = {0, 0}, ps1 = {0, 0};
+        for (int qj = 0; qj < Q; ++qj) {
+            const double tj = Rule<Q>::t(qj), wj = Rule<Q>::w(qj);
+            const double dx = xo - fma(tj, s.sx, s.x0);
+            const double dy = yo - fma(tj, s.sy, s.y0);
+            const double r2 = fmax(fma(dx, dx, dy * dy), GRIM_EPS * GRIM_EPS);
+            double inv = __builtin_amdgcn_rsq(r2);
+            inv = inv * fma(-0.5 * r2, inv * inv, 1.5);
+            inv = inv * fma(-0.5 * r2, inv * inv, 1.5);
+            double r = r2 * inv;
+            r = fma(0.5 * inv, fma(-r, r, r2), r);
+            cplx g, h;
+            green_kernels(k, r, g, h);
+            const double w0 = wj * (1.0 - tj), w1 = wj * tj;
+            ps0.re = fma(w0, g.re, ps0.re); ps0.im = fma(w0, g.im, ps0.im);
+            ps1.re = fma(w1, g.re, ps1.re); ps1.im = fma(w1, g.im, ps1.im);
+        }
+        const double wo0 = wi * (1.0 - ti), wo1 = wi * ti;
+        S[0][0] = cadd(S[0][0], cscale(ps0, wo0)); S[0][1] = cadd(S[0][1], cscale(ps1, wo0));
+        S[1][0] = cadd(S[1][0], cscale(ps0, wo1)); S[1][1] = cadd(S[1][1], cscale(ps1, wo1));
+    }
+    const double scale = o.len * s.len;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) S[a][b] = cscale(S[a][b], scale);
+}
+
+// S block of the ordered pair (e, f): far rule, or the near block the near pass stored for it.
+template <int Q>
+__device__ __forceinline__ void pair_s(const Elem& o, int e, int ne, const double* __restrict__ p0,
+                                       const double* __restrict__ p1, const int* __restrict__ next, int f, double k,
+                                       const int* __restrict__ offsets, const int* __restrict__ pair_f,
+                                       const double* __restrict__ blocks, cplx (&S)[2][2]) {
+    const Elem s = load_elem(p0, p1, f);
+    if (is_far(o, e, ne, s, f, next[f])) {
+        far_pair_s<Q>(o, s, k, S);
+        return;
+    }
+    const int idx = find_pair(offsets, pair_f, e, f);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int at = 16 * idx + 2 * (2 * a + b);
+            S[a][b] = idx < 0 ? cplx{0.0, 0.0} : cplx{blocks[at], blocks[at + 1]};
+        }
+}
+
+// One wavefront per (walk block, column chunk).  walk[64 b + l] is an element (-1: none): lane 0
+// holds the predecessor of lane 1's row, lane l >= 1 a row p whose previous element is lane
+// l - 1's, so (S1)[p, :] comes from lane l - 1 by a shuffle.  Along the chunk's columns q the
+// block S(e, prev q) is carried from the previous column (re-evaluated at contour starts).
+// Row p of each system gets c[p] ((S0 - S1)[p, q] + gamma/2 m_p(q)), the mass part only on rows
+// whose flag bit is set (bit 0: TE system, bit 1: TM system).  Every entry has one writer.
+template <int Q>
+__global__ void __launch_bounds__(64) robin_split(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                  const int* __restrict__ next, const int* __restrict__ prev,
+                                                  double k, const int* __restrict__ walk, int chunk,
+                                                  const int* __restrict__ offsets, const int* __restrict__ pair_f,
+                                                  const double* __restrict__ blocks, const double* __restrict__ c_te,
+                                                  const double* __restrict__ c_tm, const int* __restrict__ flags,
+                                                  double* __restrict__ zte, int64_t ldte, double* __restrict__ ztm,
+                                                  int64_t ldtm) {
+    const int lane = threadIdx.x;
+    const int w = walk[64 * (int64_t)blockIdx.x + lane];
+    const bool active = w >= 0 && w < n;
+    const int e = active ? w : 0;
+    const Elem o = load_elem(p0, p1, e);
+    const int ne = next[e];
+    const bool row = active && lane > 0;
+    cplx cte = {0.0, 0.0}, ctm = {0.0, 0.0};
+    int flag = 0, pe = 0;
+    double l0 = 0.0, l1 = 0.0;
+    if (row) {
+        cte = {c_te[2 * e], c_te[2 * e + 1]};
+        ctm = {c_tm[2 * e], c_tm[2 * e + 1]};
+        flag = flags[e];
+        pe = prev[e];
+        l0 = o.len;
+        l1 = load_elem(p0, p1, pe).len;
+    }
+    const double half_g = -0.5 / k;   // gamma / 2 = (0, -1 / (2k))
+    const int c0 = blockIdx.y * chunk;
+    const int c1 = min(n, c0 + chunk);
+    cplx carry[2][2], cur[2][2];
+    for (int q = c0; q < c1; ++q) {
+        const int pq = prev[q];
+        if (q == c0 || pq != q - 1) {
+            if (active) pair_s<Q>(o, e, ne, p0, p1, next, pq, k, offsets, pair_f, blocks, carry);
+            else
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) carry[a][b] = {0.0, 0.0};
+        }
+        if (active) pair_s<Q>(o, e, ne, p0, p1, next, q, k, offsets, pair_f, blocks, cur);
+        else
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) cur[a][b] = {0.0, 0.0};
+        const cplx own = cadd(cur[0][0], carry[0][1]);
+        const cplx up = cadd(cur[1][0], carry[1][1]);
+        const double ur = __shfl_up(up.re, 1, 64), ui = __shfl_up(up.im, 1, 64);
+        if (row) {
+            const cplx d = {own.re - ur, own.im - ui};
+            double m = 0.0;
+            if (q == ne) m += l0 / 6.0;
+            if (q == e) m += (l0 - l1) / 3.0;
+            if (q == pe) m -= l1 / 6.0;
+            const cplx dm = {d.re, d.im + half_g * m};
+            const cplx vte = cmul(cte, (flag & 1) ? dm : d);
+            const cplx vtm = cmul(ctm, (flag & 2) ? dm : d);
+            double* t = zte + 2 * ((int64_t)e + (int64_t)q * ldte);
+            t[0] += vte.re;
+            t[1] += vte.im;
+            double* u = ztm + 2 * ((int64_t)e + (int64_t)q * ldtm);
+            u[0] += vtm.re;
+            u[1] += vtm.im;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) carry[a][b] = cur[a][b];
+    }
+}
+
+// Right-hand sides of one polarization's impedance rows: out[p, a] = -b_u[p, a] on Dirichlet
+// rows, else -sum_e i_e(p) (j k d.n_e + alpha_e): the trace load weighted by each element's own
+// Robin coefficient (twod/assembly/kernels.py:222-294, observation coefficients;
+// twod/formulations/regions.py:557-590).
+__global__ void __launch_bounds__(256) robin_rhs_kernel(int n, int na, const double* __restrict__ p0,
+                                                        const double* __restrict__ p1, const int* __restrict__ prev,
+                                                        double k, const double* __restrict__ angles_deg,
+                                                        const double* __restrict__ alpha_e,
+                                                        const int* __restrict__ dirichlet, double* __restrict__ out,
+                                                        int64_t ld) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (p >= n) return;
+    const double phi = angles_deg[a] * (GRIM_PI / 180.0);
+    double dy, dx;
+    sincos(phi, &dy, &dx);
+    const int pp = prev[p];
+    const Elem A = load_elem(p0, p1, p), B = load_elem(p0, p1, pp);
+    cplx i0, i1, dnA, j0_, j1_, dnB;
+    elem_moments(A, k, dx, dy, i0, i1, dnA);
+    elem_moments(B, k, dx, dy, j0_, j1_, dnB);
+    cplx v;
+    if (dirichlet[p]) {
+        v = cadd(i0, j1_);
+    } else {
+        const cplx wa = cadd(dnA, cplx{alpha_e[2 * p], alpha_e[2 * p + 1]});
+        const cplx wb = cadd(dnB, cplx{alpha_e[2 * pp], alpha_e[2 * pp + 1]});
+        v = cadd(cmul(i0, wa), cmul(j1_, wb));
+    }
+    const int64_t at = 2 * ((int64_t)p + (int64_t)a * ld);
+    out[at] = -v.re;
+    out[at + 1] = -v.im;
+}
+
+extern "C" hipError_t grim_launch_robin_split(int q, int n, const double* p0, const double* p1, const int* next,
+                                              const int* prev, double k, int nblocks, const int* walk,
+                                              const int* offsets, const int* pair_f, const double* blocks,
+                                              const double* c_te, const double* c_tm, const int* flags, double* zte,
+                                              int64_t ldte, double* ztm, int64_t ldtm, hipStream_t st) {
+    if (nblocks == 0) return hipSuccess;
+    const int chunk = 32;
+    dim3 grid((unsigned)nblocks, (unsigned)((n + chunk - 1) / chunk));
+    if (q == 16)
+        robin_split<16><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, walk, chunk, offsets, pair_f, blocks, c_te,
+                                             c_tm, flags, zte, ldte, ztm, ldtm);
+    else
+        robin_split<8><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, walk, chunk, offsets, pair_f, blocks, c_te,
+                                            c_tm, flags, zte, ldte, ztm, ldtm);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t grim_launch_robin_rhs(int n, int na, const double* p0, const double* p1, const int* prev,
+                                            double k, const double* angles_deg, const double* alpha_e,
+                                            const int* dirichlet, double* out, int64_t ld, hipStream_t st) {
+    dim3 grid((n + 255) / 256, na);
+    robin_rhs_kernel<<<grid, 256, 0, st>>>(n, na, p0, p1, prev, k, angles_deg, alpha_e, dirichlet, out, ld);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t grim_launch_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* y,
                                              int64_t ldy, const double* ax, const double* ay, const int* dirichlet,
                                              hipStream_t st) {

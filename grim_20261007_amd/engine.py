@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import native
-from .mesh import ContourMesh
+from .mesh import EPS, ContourMesh
 
 NEAR_PER_ELEMENT = 48
 REFINE_TOL = 2e-12           # linalg/refined_lu.py:54
@@ -112,45 +112,119 @@ def farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
 ETA0 = 376.730313668      # free-space wave impedance (twod/constants.py)
 
 
-def robin_coefficients(mesh: ContourMesh, k: float):
-    """Per-node Robin coefficients of a surface impedance Zs in air (twod/geometry.py:2488-2521):
-    alpha_TE = j k Zs / eta0, alpha_TM = j k eta0 / Zs; 0 and a Dirichlet TM row on PEC contours.
-    Node p takes the law of element p (one law per contour)."""
+@dataclass
+class RobinPlan:
+    """Coefficients of a contour's impedance rows at one wavenumber (host arrays, node order)."""
+    alpha_te_e: np.ndarray      # (N,) element coefficients j k Zs / eta0 (0 on PEC elements)
+    alpha_tm_e: np.ndarray      # (N,) element coefficients j k eta0 / Zs (0 on PEC elements)
+    alpha_te: np.ndarray        # (N,) nodal: mean of the node's two elements
+    alpha_tm: np.ndarray        # (N,) nodal, 0 on every node of a PEC element
+    dirichlet: np.ndarray       # (N,) int32: TM rows with |alpha| <= EPS keep the Dirichlet row
+    delta_te: np.ndarray        # (N,) (alpha_e(p) - alpha_e(prev p)) / 2
+    delta_tm: np.ndarray        # (N,) the same in TM, 0 on Dirichlet rows
+    flags: np.ndarray           # (N,) int32: bit 0 / bit 1 = TE / TM row with |alpha| > EPS
+    walk: np.ndarray            # (64 B,) int32 walk blocks of the rows with delta != 0
+
+
+def robin_plan(mesh: ContourMesh, k: float) -> RobinPlan:
+    """Element and nodal Robin coefficients of the contour's surface impedances in air
+    (twod/geometry.py:2488-2521: alpha_TE = j k Zs/eta0, alpha_TM = j k eta0/Zs; 0 on PEC),
+    nodal values as the mean over the node's two elements with every node of a PEC element at 0
+    in TM (twod/formulations/regions.py:66-119), and the walk of the rows whose two elements
+    carry different coefficients (zones, tapers): per block, lane 0 the element before lane 1's
+    row, lanes 1..63 consecutive rows along the contour."""
     z = np.asarray(mesh.impedance, complex)
-    ibc = np.abs(z) > 1e-12
-    a_te = np.where(ibc, 1j * k * z / ETA0, 0.0)
-    a_tm = np.where(ibc, 1j * k * ETA0 / np.where(ibc, z, 1.0), 0.0)
-    return a_te.astype(np.complex128), a_tm.astype(np.complex128), (~ibc).astype(np.int32)
+    ibc = np.abs(z) > EPS
+    safe = np.where(ibc, z, 1.0)
+    a_te = np.where(ibc, 1j * k * safe / ETA0, 0.0).astype(np.complex128)
+    a_tm = np.where(ibc, 1j * k * ETA0 / safe, 0.0).astype(np.complex128)
+    prev = np.asarray(mesh.prev, np.int64)
+    n_te = (a_te + a_te[prev]) / 2
+    n_tm = (a_tm + a_tm[prev]) / 2
+    pec_e = np.abs(a_tm) <= EPS
+    n_tm = np.where(pec_e | pec_e[prev], 0.0, n_tm)
+    dirichlet = np.abs(n_tm) <= EPS
+    d_te = (a_te - a_te[prev]) / 2
+    d_tm = np.where(dirichlet, 0.0, (a_tm - a_tm[prev]) / 2)
+    flags = ((np.abs(n_te) > EPS).astype(np.int32) | 2 * (~dirichlet).astype(np.int32))
+    need = (d_te != 0) | (d_tm != 0)
+    blocks = []
+    seen = np.zeros(mesh.n, bool)
+    nxt = np.asarray(mesh.next, np.int64)
+    for start in range(mesh.n):
+        if seen[start]:
+            continue
+        cycle, e = [], start
+        while not seen[e]:
+            seen[e] = True
+            cycle.append(e)
+            e = int(nxt[e])
+        flags_c = need[cycle]
+        # rotate so that a run of needed rows never wraps past the end of the list
+        if flags_c.all():
+            order = cycle
+        else:
+            first_gap = int(np.argmin(flags_c))
+            order = cycle[first_gap:] + cycle[:first_gap]
+        run = []
+        for e in order + [-1]:
+            if e >= 0 and need[e]:
+                run.append(e)
+                continue
+            for i in range(0, len(run), 63):
+                part = run[i:i + 63]
+                blocks.append([int(prev[part[0]])] + part + [-1] * (63 - len(part)))
+            run = []
+    walk = np.array(blocks, np.int32).reshape(-1) if blocks else np.zeros(0, np.int32)
+    return RobinPlan(a_te, a_tm, n_te, n_tm, dirichlet.astype(np.int32), d_te.astype(np.complex128),
+                     d_tm.astype(np.complex128), flags, walk)
 
 
-def _robin_device(contour: DeviceContour, k: float, dev):
-    a_te, a_tm, dirichlet = robin_coefficients(contour.mesh, k)
-    return (torch.as_tensor(a_te, device=dev), torch.as_tensor(a_tm, device=dev),
-            torch.as_tensor(dirichlet, device=dev))
+def _c128(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(a, np.complex128), device=dev)
 
 
-def impedance_systems(contour: DeviceContour, k: float, pair: torch.Tensor) -> torch.Tensor:
+def impedance_systems(contour: DeviceContour, k: float, pair: torch.Tensor, workspace: torch.Tensor) -> torch.Tensor:
     """Turn the [TE, TM] PEC systems `pair` (2, N, N) in place into the systems of the contours'
-    impedance laws: flux rows Z_TE + alpha Z_TM for both polarizations, TM rows of PEC contours
-    unchanged (grim_robin_rows)."""
-    n = contour.n
-    ax, ay, dm = _robin_device(contour, k, pair.device)
+    impedance laws: flux rows Z_TE + alpha_p Z_TM for both polarizations, TM Dirichlet rows
+    unchanged (grim_robin_rows); then, on rows whose two elements carry different coefficients,
+    the element-weighted alpha S and trace-jump mass (grim_robin_split, which reads the near
+    blocks that `assemble` left in `workspace`)."""
+    n, dev = contour.n, pair.device
+    plan = robin_plan(contour.mesh, k)
+    st = _stream(dev)
+    ax, ay, dm = _c128(plan.alpha_te, dev), _c128(plan.alpha_tm, dev), torch.as_tensor(plan.dirichlet, device=dev)
     native.check("grim_robin_rows", native.lib().grim_robin_rows(n, n, pair[0].data_ptr(), n, pair[1].data_ptr(), n,
-                                                                 ax.data_ptr(), ay.data_ptr(), dm.data_ptr(),
-                                                                 _stream(pair.device)))
+                                                                 ax.data_ptr(), ay.data_ptr(), dm.data_ptr(), st))
+    nblocks = plan.walk.size // 64
+    if nblocks:
+        walk = torch.as_tensor(plan.walk, device=dev)
+        cte, ctm = _c128(plan.delta_te, dev), _c128(plan.delta_tm, dev)
+        flags = torch.as_tensor(plan.flags, device=dev)
+        code = native.lib().grim_robin_split(
+            n, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.next.data_ptr(), contour.prev.data_ptr(),
+            float(k), contour.mesh.far_order(k), nblocks, walk.data_ptr(), cte.data_ptr(), ctm.data_ptr(),
+            flags.data_ptr(), workspace.data_ptr(), workspace.numel(), NEAR_PER_ELEMENT, pair[0].data_ptr(), n,
+            pair[1].data_ptr(), n, st)
+        native.check("grim_robin_split", code)
     return pair
 
 
-def impedance_rhs(contour: DeviceContour, k: float, bu: torch.Tensor, bdn: torch.Tensor) -> torch.Tensor:
-    """Right-hand sides (2, A, N) [TE, TM] of impedance_systems: -(b_dn + alpha b_u) on flux rows,
-    -b_u on TM rows of PEC contours."""
-    n, na = contour.n, int(bu.shape[0])
-    ax, ay, dm = _robin_device(contour, k, bu.device)
-    x, y = bdn.clone(), bu.clone()
-    native.check("grim_robin_rows", native.lib().grim_robin_rows(n, na, x.data_ptr(), n, y.data_ptr(), n,
-                                                                 ax.data_ptr(), ay.data_ptr(), dm.data_ptr(),
-                                                                 _stream(bu.device)))
-    return torch.stack((-x, -y))
+def impedance_rhs(contour: DeviceContour, k: float, angles: torch.Tensor) -> torch.Tensor:
+    """Right-hand sides (2, A, N) [TE, TM] of impedance_systems: -(b_dn with each element's own
+    Robin coefficient on its trace) on flux rows, -b_u on TM Dirichlet rows (grim_robin_rhs)."""
+    n, na, dev = contour.n, int(angles.numel()), angles.device
+    plan = robin_plan(contour.mesh, k)
+    out = torch.empty((2, na, n), dtype=torch.complex128, device=dev)
+    st = _stream(dev)
+    zero = torch.zeros(n, dtype=torch.int32, device=dev)
+    dirichlet = torch.as_tensor(plan.dirichlet, device=dev)
+    for i, (alpha, dm) in enumerate(((_c128(plan.alpha_te_e, dev), zero), (_c128(plan.alpha_tm_e, dev), dirichlet))):
+        code = native.lib().grim_robin_rhs(n, na, contour.p0.data_ptr(), contour.p1.data_ptr(),
+                                           contour.prev.data_ptr(), float(k), angles.data_ptr(), alpha.data_ptr(),
+                                           dm.data_ptr(), out[i].data_ptr(), n, st)
+        native.check("grim_robin_rhs", code)
+    return out
 
 
 def bistatic_farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):

@@ -36,6 +36,7 @@ class ContourMesh:
     prev: np.ndarray         # (N,) element ending at node e
     contour_starts: list = field(default_factory=list)
     impedance: np.ndarray | None = None   # (N,) complex surface impedance per element (0: PEC), or None
+    ref_node: np.ndarray | None = None    # (N,) the reference's node number of node p (None: p itself)
 
     @property
     def n(self) -> int:
@@ -176,47 +177,430 @@ def refined_chain_counts(base_counts, lengths, factor: float) -> list:
     return out
 
 
-def impedance_laws(snapshot: dict) -> dict:
-    """Inline constant surface impedances of the snapshot's ``ibcs`` rows,
-    ``[flag, "constant", R, X, R_end, X_end]`` (twod/geometry.py:261-312: inline rows are
-    ``<kind> R_start X_start R_end X_end``).  Tapered, file-backed and thin-layer laws are not
-    handled by this path and are refused when a segment uses them."""
+# ----------------------------------------------------------------------------
+# Surface-impedance laws of the snapshot's ``ibcs`` rows (twod/geometry.py:128-262, 261-312,
+# 369-398, 603-680)
+# ----------------------------------------------------------------------------
+ETA0 = 376.730313668
+TAPER_KINDS = ("constant", "linear", "cosine", "exp")
+# A frequency this close (relative) to a table's first or last row samples that row.
+TABLE_RANGE_RELATIVE_TOLERANCE = 1.0e-12
+IMPEDANCE_CSV_HEADER = ["frequency_hz", "resistance_ohm", "reactance_ohm"]
+
+
+def _passive(z: complex, context: str) -> complex:
+    """A finite Leontovich impedance with Re(Zs) >= 0 up to 64 ulp (twod/geometry.py:513-526)."""
+    z = complex(z)
+    if not (math.isfinite(z.real) and math.isfinite(z.imag)):
+        raise ValueError(f"{context} contains non-finite value {z!r}.")
+    if z.real < -64.0 * np.finfo(float).eps * max(1.0, abs(z)):
+        raise ValueError(f"{context} has negative resistance Re(Zs)={z.real:g} ohm. "
+                         "Active/gain surface impedances are not supported.")
+    return z
+
+
+@dataclass
+class ImpedanceLaw:
+    """One IBC flag: an inline law along its segment (``constant``, ``linear``, ``cosine``, ``exp``
+    between z_start at arc_s = 0 and z_end at arc_s = 1 in the drawing direction), a frequency
+    table (``table``, linear interpolation in GHz) or a thin dielectric layer (``thin_dielectric``,
+    a transmitting sheet that an opaque contour cannot carry)."""
+    kind: str
+    z_start: complex = 0j
+    z_end: complex = 0j
+    freqs_ghz: np.ndarray | None = None
+    values: np.ndarray | None = None
+
+    @property
+    def tapered(self) -> bool:
+        return self.kind in ("linear", "cosine", "exp")
+
+    def _taper(self, arc_s: float) -> complex:
+        s = float(max(0.0, min(1.0, arc_s)))
+        if self.kind == "constant":
+            return self.z_start
+        if self.kind in ("linear", "cosine"):
+            w = s if self.kind == "linear" else 0.5 * (1.0 - math.cos(math.pi * s))
+            return (1.0 - w) * self.z_start + w * self.z_end
+        z1 = self.z_start if abs(self.z_start) > EPS else complex(EPS, 0.0)
+        z2 = self.z_end if abs(self.z_end) > EPS else complex(EPS, 0.0)
+        return complex(np.exp((1.0 - s) * np.log(z1) + s * np.log(z2)))
+
+    def _sample(self, frequency_ghz: float) -> complex:
+        f = float(frequency_ghz)
+        if not math.isfinite(f):
+            raise ValueError("Material-table sample frequency must be finite.")
+        lo, hi = float(self.freqs_ghz[0]), float(self.freqs_ghz[-1])
+        if lo - TABLE_RANGE_RELATIVE_TOLERANCE * abs(lo) <= f < lo:
+            f = lo
+        elif hi < f <= hi + TABLE_RANGE_RELATIVE_TOLERANCE * abs(hi):
+            f = hi
+        elif f < lo or f > hi:
+            raise ValueError(f"Material-table sample frequency {f:.17g} GHz is outside the characterized range "
+                             f"[{lo:.17g}, {hi:.17g}] GHz.")
+        if len(self.freqs_ghz) == 1:
+            return complex(self.values[0])
+        return complex(np.interp(f, self.freqs_ghz, self.values.real), np.interp(f, self.freqs_ghz, self.values.imag))
+
+    def impedance(self, flag: int, frequency_ghz: float | None, arc_s: float | None = None) -> complex:
+        """Zs of this law at ``frequency_ghz`` and the arc position ``arc_s`` (twod/geometry.py:369-390)."""
+        if self.kind == "thin_dielectric":
+            raise ValueError("A thin dielectric layer is a transmitting sheet, not an opaque IBC. "
+                             "Assign it only to TYPE 1.")
+        if self.kind == "table":
+            return _passive(self._sample(frequency_ghz), f"IBC flag {flag} impedance sampled at {frequency_ghz:g} GHz")
+        if self.tapered:
+            s = 0.5 if arc_s is None else float(arc_s)
+            return _passive(self._taper(s), f"IBC flag {flag} tapered impedance at s={s:g}")
+        return _passive(self.z_start, f"IBC flag {flag} impedance")
+
+
+def _material_flag(token) -> int:
+    text = str(token).strip()
+    if text.lower().startswith("mat."):
+        text = text[4:]
+    try:
+        value = float(text)
+    except (TypeError, ValueError):
+        raise ValueError(f"IBC material definition flag must be an integer; got {token!r}.") from None
+    if not math.isfinite(value) or not value.is_integer():
+        raise ValueError(f"IBC material definition flag must be an integer; got {token!r}.")
+    if value <= 0:
+        raise ValueError(f"IBC material definition flag must be a positive integer; got {token!r}.")
+    return int(value)
+
+
+def _material_float(token, context: str) -> float:
+    try:
+        value = float(token)
+    except (TypeError, ValueError):
+        raise ValueError(f"{context} must be a finite numeric value; got {token!r}.") from None
+    if not math.isfinite(value):
+        raise ValueError(f"{context} must be finite; got {token!r}.")
+    return value
+
+
+def material_base_dir(snapshot: dict, base_dir: str | None) -> str:
+    """The one directory material tables are read from: the explicit one, else the folder of the
+    snapshot's ``source_path``, else the working directory (twod/geometry.py:577-600)."""
+    import os
+    if base_dir is not None and str(base_dir).strip():
+        return os.path.abspath(os.path.expanduser(str(base_dir)))
+    source = str(snapshot.get("source_path", "") or "").strip()
+    if source:
+        return os.path.dirname(os.path.abspath(os.path.expanduser(source)))
+    return os.path.abspath(os.getcwd())
+
+
+def _table_file(folder: str, flag: int, filename) -> str:
+    """A same-directory .csv sidecar name, resolved in ``folder`` only
+    (ghost_backend/geometry/io.py:34-61, twod/geometry.py:566-575)."""
+    import os
+    raw = str(filename)
+    name = raw.strip()
+    context = f"Material flag {flag} CSV reference"
+    if not name or not name.lower().endswith(".csv"):
+        raise ValueError(f"{context} must name a .csv file; got {filename!r}.")
+    if any(ch.isspace() for ch in raw):
+        raise ValueError(f"{context} cannot contain whitespace because .geo material rows use unquoted fields; "
+                         f"got {filename!r}. Rename the CSV and try again.")
+    if name in (".", "..") or os.path.isabs(name) or os.path.basename(name) != name or "/" in name or "\\" in name:
+        raise ValueError(f"{context} must be a filename in the same directory as the geometry file (no directory "
+                         f"components); got {filename!r}.")
+    path = os.path.join(os.path.abspath(folder), name)
+    if not os.path.isfile(path):
+        raise FileNotFoundError(f"Could not locate material file {name} in declared material directory "
+                                f"{os.path.abspath(folder)}. Material tables are never searched in the process "
+                                "working directory.")
+    return path
+
+
+def load_impedance_csv(path: str) -> ImpedanceLaw:
+    """``frequency_hz,resistance_ohm,reactance_ohm`` rows (UTF-8, optional BOM, # comments and blank
+    lines skipped, frequencies positive and unique, sorted) -> a table law
+    (twod/geometry.py:603-680)."""
+    import csv
+    if not str(path).lower().endswith(".csv"):
+        raise ValueError(f"Material/IBC file must use the .csv extension: {path}")
+    rows, header = [], False
+    with open(path, "r", encoding="utf-8-sig", newline="") as handle:
+        for lineno, raw in enumerate(handle, start=1):
+            stripped = raw.strip()
+            if not stripped or stripped.startswith("#"):
+                continue
+            try:
+                parts = [part.strip() for part in next(csv.reader([raw], strict=True))]
+            except csv.Error as exc:
+                raise ValueError(f"{path}: line {lineno} contains invalid CSV: {exc}") from exc
+            if not header:
+                if parts != IMPEDANCE_CSV_HEADER:
+                    raise ValueError(f"{path}: line {lineno} must have comma-separated header "
+                                     f"{','.join(IMPEDANCE_CSV_HEADER)} (frequency in Hz); found {','.join(parts)}.")
+                header = True
+                continue
+            if len(parts) != len(IMPEDANCE_CSV_HEADER):
+                raise ValueError(f"{path}: line {lineno} must contain exactly {len(IMPEDANCE_CSV_HEADER)} "
+                                 f"comma-separated columns; found {len(parts)}.")
+            try:
+                values = [float(part) for part in parts]
+            except ValueError:
+                raise ValueError(f"{path}: line {lineno} contains a non-numeric value.") from None
+            if not all(math.isfinite(v) for v in values):
+                raise ValueError(f"{path}: line {lineno} contains a non-finite value.")
+            if values[0] <= 0:
+                raise ValueError(f"{path}: line {lineno} has non-positive frequency {values[0]:g} Hz.")
+            rows.append(values)
+    if not header:
+        raise ValueError(f"{path}: missing required header {','.join(IMPEDANCE_CSV_HEADER)}.")
+    if not rows:
+        raise ValueError(f"{path}: no material data rows found after the header.")
+    rows.sort(key=lambda r: r[0])
+    for a, b in zip(rows, rows[1:]):
+        if a[0] == b[0]:
+            raise ValueError(f"{path}: duplicate frequency {b[0]:g} Hz.")
+    values = np.array([complex(r[1], r[2]) for r in rows], dtype=np.complex128)
+    for i, v in enumerate(values, start=1):
+        _passive(v, f"Impedance CSV '{path}' data row {i}")
+    return ImpedanceLaw("table", freqs_ghz=np.array([r[0] / 1.0e9 for r in rows]), values=values)
+
+
+def impedance_laws(snapshot: dict, base_dir: str | None = None) -> dict:
+    """flag -> ImpedanceLaw for every ``ibcs`` row of the snapshot (twod/geometry.py:261-312):
+    ``[flag, kind, R_start, X_start, R_end, X_end]`` inline laws (a constant law ignores its end
+    values), ``[flag, "file.csv"]`` frequency tables read from the material directory, and
+    ``[flag, "thin_dielectric", ...]`` layers."""
     laws = {}
+    folder = None
     for row in snapshot.get("ibcs", []) or []:
         if not row:
             continue
-        try:
-            flag = int(round(float(row[0])))
-        except (TypeError, ValueError):
-            raise ValueError(f"Invalid IBC material definition flag {row[0]!r}.")
+        flag = _material_flag(row[0])
         if flag in laws:
             raise ValueError(f"Duplicate IBC material flag {flag}.")
-        tokens = [str(t).strip() for t in row[1:] if str(t).strip() != ""]
-        if len(tokens) != 5 or tokens[0].lower() != "constant":
-            laws[flag] = None            # a law this path does not evaluate
+        if len(row) > 1 and str(row[1]).lower() == "thin_dielectric":
+            laws[flag] = ImpedanceLaw("thin_dielectric")
             continue
-        try:
-            z = complex(float(tokens[1]), float(tokens[2]))
-        except ValueError:
-            raise ValueError(f"IBC flag {flag}: impedance values must be numeric.")
-        if not (math.isfinite(z.real) and math.isfinite(z.imag)):
-            raise ValueError(f"IBC flag {flag}: impedance must be finite.")
-        if z.real < -1e-12 * max(1.0, abs(z)):
-            raise ValueError(f"IBC flag {flag} has negative resistance Re(Zs)={z.real:g} ohm. Active/gain "
-                             "surface impedances are not supported.")
-        laws[flag] = 0j if abs(z) <= EPS else z
+        if len(row) == 2:
+            if not str(row[1]).strip().lower().endswith(".csv"):
+                raise ValueError(f"IBC flag {flag}: file-backed material definitions must use 'flag filename.csv'.")
+            folder = folder or material_base_dir(snapshot, base_dir)
+            laws[flag] = load_impedance_csv(_table_file(folder, flag, row[1]))
+            continue
+        tokens = [str(t).strip() for t in row[1:] if str(t).strip() != ""]
+        if len(tokens) != 5:
+            raise ValueError(f"IBC flag {flag}: inline impedance requires '<kind> R_start X_start R_end X_end' "
+                             f"(got {len(tokens)} data tokens after the flag).")
+        kind = tokens[0].lower()
+        r0 = _material_float(tokens[1], f"IBC flag {flag} R_start")
+        x0 = _material_float(tokens[2], f"IBC flag {flag} X_start")
+        r1 = _material_float(tokens[3], f"IBC flag {flag} R_end")
+        x1 = _material_float(tokens[4], f"IBC flag {flag} X_end")
+        if kind not in TAPER_KINDS:
+            raise ValueError(f"Unknown impedance taper kind '{kind}'. Expected one of {TAPER_KINDS}.")
+        z0 = _passive(complex(r0, x0), "taper z_start")
+        z1 = z0 if kind == "constant" else _passive(complex(r1, x1), "taper z_end")
+        laws[flag] = ImpedanceLaw(kind, z0, z1)
     return laws
 
 
-def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int = 20000) -> ContourMesh:
-    """Panels and welded nodes of the PEC contours of ``snapshot`` at ``frequency_ghz``.
+# ----------------------------------------------------------------------------
+# Bound surface waves and junction grading (twod/geometry.py:737-905, 1468-1545)
+# ----------------------------------------------------------------------------
+SURFACE_WAVE_MAX_DENSITY_FACTOR = 4.0
+SURFACE_WAVE_MAX_UNRESOLVED_FACTOR = 12.0
+SURFACE_WAVE_TAPER_SAMPLES = 9
+JUNCTION_GRADING_LEVELS = 4
+JUNCTION_GRADING_RATIO = 0.5
+JUNCTION_GRADING_CONTRAST = 4.0
+JUNCTION_GRADING_DENSITY = 32.0
+
+
+def bound_surface_wave_index(z: complex) -> float:
+    """Phase index of the wave bound to a flat surface of impedance ``z`` in air: with
+    kappa/k = -j Zs/eta0 (TE) or -j eta0/Zs (TM), a wave counts when Re kappa > 0 and
+    |Im beta| <= Re beta for beta/k = sqrt(1 + (kappa/k)^2); the larger index of the two."""
+    z = complex(z)
+    if abs(z) <= EPS:
+        return 0.0
+    best = 0.0
+    for ratio in (z / ETA0, ETA0 / z):
+        if (-1j * ratio).real <= 0.0:
+            continue
+        beta = complex(np.sqrt(complex(1.0 - ratio * ratio)))
+        if abs(beta.imag) > beta.real:
+            continue
+        best = max(best, abs(beta.real))
+    return best
+
+
+def surface_wave_mesh_wavelength(material_wavelength: float, law: ImpedanceLaw | None, flag: int,
+                                 frequencies) -> float:
+    """Mesh wavelength of an impedance segment: the shortest bound-wave wavelength over the mesh
+    frequencies when the air mesh would leave it unresolved by at most 12x, never below a quarter
+    of the air wavelength (twod/geometry.py:1517-1545)."""
+    if law is None or not frequencies or law.kind == "thin_dielectric":
+        return material_wavelength
+    shortest = material_wavelength
+    for f in frequencies:
+        positions = np.linspace(0.0, 1.0, SURFACE_WAVE_TAPER_SAMPLES) if law.tapered else (0.5,)
+        try:
+            index = max(bound_surface_wave_index(law.impedance(flag, f, float(s))) for s in positions)
+        except ValueError:
+            index = 0.0
+        if index > 0.0:
+            bound = C0 / (float(f) * 1.0e9) / index
+            if material_wavelength <= SURFACE_WAVE_MAX_UNRESOLVED_FACTOR * bound:
+                shortest = min(shortest, bound)
+    return max(shortest, material_wavelength / SURFACE_WAVE_MAX_DENSITY_FACTOR)
+
+
+def junction_grading_levels(panel_length: float, wavelength: float) -> int:
+    """Geometric levels toward a junction: 4, one more per doubling of the density beyond 32 panels
+    per wavelength, the innermost panel kept above 1e-7 m."""
+    levels = JUNCTION_GRADING_LEVELS
+    if not panel_length > 0.0:
+        return levels
+    if math.isfinite(wavelength) and wavelength > 0.0:
+        density = wavelength / panel_length
+        if density > JUNCTION_GRADING_DENSITY:
+            levels += int(math.ceil(math.log2(density / JUNCTION_GRADING_DENSITY) - 1.0e-9))
+    return max(0, min(levels, int(math.floor(math.log2(max(panel_length / 1.0e-7, 1.0))))))
+
+
+def impedance_jump_is_graded(first: complex, second: complex) -> bool:
+    """Two evaluated laws (0 = PEC) differ by more than 3/4 of the larger magnitude."""
+    larger = max(abs(first), abs(second))
+    return larger > 0.0 and abs(first - second) > (1.0 - 1.0 / JUNCTION_GRADING_CONTRAST) * larger
+
+
+class _Welder:
+    """Vertex identity on the 1e-9 m snap: the snap key of a registered vertex, or the nearest
+    registered vertex within the tolerance (twod/geometry.py:1994-2044)."""
+
+    def __init__(self, tol: float = NODE_SNAP):
+        self.tol = tol
+        self.cell = 2.0 * max(tol, EPS)
+        self.alias, self.grid = {}, {}
+
+    def _raw(self, xy):
+        return int(round(float(xy[0]) / self.tol)), int(round(float(xy[1]) / self.tol))
+
+    def lookup(self, xy):
+        known = self.alias.get(self._raw(xy))
+        if known is not None:
+            return known
+        x, y = float(xy[0]), float(xy[1])
+        cx, cy = math.floor(x / self.cell), math.floor(y / self.cell)
+        best = None
+        for i in (cx - 1, cx, cx + 1):
+            for j in (cy - 1, cy, cy + 1):
+                for px, py, key in self.grid.get((i, j), ()):
+                    d = math.hypot(px - x, py - y)
+                    if d <= self.tol and (best is None or d < best[0]):
+                        best = (d, key)
+        return None if best is None else best[1]
+
+    def key(self, xy):
+        raw = self._raw(xy)
+        known = self.lookup(xy)
+        if known is None:
+            known = raw
+            x, y = float(xy[0]), float(xy[1])
+            self.grid.setdefault((math.floor(x / self.cell), math.floor(y / self.cell)), []).append((x, y, raw))
+        self.alias.setdefault(raw, known)
+        return known
+
+
+def boundary_condition_junctions(segments, scale: float, laws: dict, frequencies):
+    """Welded vertices where the evaluated law of the conductor segments ending there jumps at
+    some mesh frequency (segment ends at arc_s 0 and 1 in the drawing direction; an unevaluable
+    law next to PEC counts as a jump) (twod/geometry.py:825-866).  Returns (welder, keys)."""
+    welder = _Welder()
+    ends = {}
+    freqs = list(frequencies) or [None]
+    for seg in segments:
+        props = list(seg.get("properties", []) or [])
+        seg_type = _parse_int(props[0]) if props and str(props[0]).strip() else int(seg.get("seg_type", 2))
+        pairs = list(seg.get("point_pairs", []) or [])
+        if seg_type not in (2, 4) or not pairs:
+            continue
+        flag = _parse_int(props[2]) if len(props) > 2 else 0
+        for pair, (kx, ky), arc in ((pairs[0], ("x1", "y1"), 0.0), (pairs[-1], ("x2", "y2"), 1.0)):
+            key = welder.key((float(pair.get(kx, 0.0)) * scale, float(pair.get(ky, 0.0)) * scale))
+            values = []
+            for f in freqs:
+                if flag <= 0:
+                    values.append(0j)
+                    continue
+                law = laws.get(flag)
+                if law is None or (f is None and law.kind == "table"):
+                    values.append(None)
+                    continue
+                try:
+                    z = law.impedance(flag, f, arc)
+                except ValueError:
+                    values.append(None)
+                    continue
+                values.append(0j if abs(z) <= EPS else z)
+            ends.setdefault(key, []).append(values)
+    junctions = set()
+    for key, per_end in ends.items():
+        for at_f in zip(*per_end):
+            known = [v for v in at_f if v is not None]
+            jump = len(known) < len(at_f) and any(v == 0.0 for v in known)
+            if jump or any(impedance_jump_is_graded(a, b) for i, a in enumerate(known) for b in known[i + 1:]):
+                junctions.add(key)
+                break
+    return welder, junctions
+
+
+def grade_toward_junctions(pts, welder, junctions, wavelength: float):
+    """Split the end panels of a primitive geometrically (ratio 1/2) toward junction vertices;
+    a one-panel primitive with junctions at both ends is halved first (twod/geometry.py:869-891)."""
+    if len(pts) < 2 or not junctions:
+        return pts
+
+    def at(xy):
+        key = welder.lookup(xy)
+        return key is not None and key in junctions
+
+    at_start, at_end = at(pts[0]), at(pts[-1])
+    if not (at_start or at_end):
+        return pts
+    if len(pts) == 2 and at_start and at_end:
+        pts = [pts[0], 0.5 * (pts[0] + pts[1]), pts[1]]
+
+    def fractions(first, second):
+        levels = junction_grading_levels(float(np.linalg.norm(second - first)), wavelength)
+        return [JUNCTION_GRADING_RATIO ** level for level in range(levels, 0, -1)]
+
+    if at_start:
+        pts = [pts[0]] + [pts[0] + (pts[1] - pts[0]) * f for f in fractions(pts[0], pts[1])] + pts[1:]
+    if at_end:
+        pts = pts[:-1] + [pts[-1] + (pts[-2] - pts[-1]) * f for f in reversed(fractions(pts[-1], pts[-2]))] + [pts[-1]]
+    return pts
+
+
+def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int = 20000, *,
+               material_base_dir: str | None = None, law_ghz: float | None = None,
+               mesh_frequencies=None) -> ContourMesh:
+    """Panels and welded nodes of the conductor contours of ``snapshot`` meshed at
+    ``frequency_ghz``, with each element's surface impedance evaluated at ``law_ghz`` (default:
+    the mesh frequency).  ``mesh_frequencies`` (default: the mesh frequency) are the frequencies the
+    mesh serves: junction grading and the bound-surface-wave mesh wavelength use them.
+
+    Impedance segments take their law's value at each panel's arc-length centre along the
+    segment in the drawing direction (tapers), at ``law_ghz`` (tables); junctions where the law
+    jumps are graded (twod/geometry.py:1800-1981, 2600-2625).
 
     A snapshot that carries ``_2d_certification_refinement_factor`` (> 1) and
     ``_2d_certification_base_segment_n`` (each segment's base N) is the fine mesh of a
     certification pair: its chains get ``refined_chain_counts`` of the base counts
     (twod/geometry.py:1788-1850)."""
     scale = unit_scale(units)
-    wavelength = C0 / (float(frequency_ghz) * 1e9)
+    wavelength0 = C0 / (float(frequency_ghz) * 1e9)
+    law_ghz = float(frequency_ghz if law_ghz is None else law_ghz)
+    freqs = [float(frequency_ghz)] if mesh_frequencies is None else sorted({float(f) for f in mesh_frequencies})
     segments = list(snapshot.get("segments", []) or [])
     if not segments:
         raise ValueError("Geometry does not contain any segments.")
@@ -227,24 +611,22 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     if refine > 1.0 and len(base_n) != len(segments):
         raise ValueError("Internal 2-D certification refinement metadata does not match the geometry "
                          "segment count.")
-    laws = impedance_laws(snapshot)
+    laws = impedance_laws(snapshot, material_base_dir)
+    welder, junctions = boundary_condition_junctions(segments, scale, laws, freqs)
     panels0, panels1, zs = [], [], []
     for s, seg in enumerate(segments):
         seg_type, n_prop, ibc = _parse_props(seg)
         name = str(seg.get("name", "segment"))
         if seg_type != 2:
             raise ValueError(f"Segment '{name}': only conductor (TYPE 2) contours are supported by the GPU path.")
-        z_seg = 0j
+        law = None
         if ibc > 0:
             if ibc not in laws:
-                raise ValueError(f"Segment '{name}' uses IBC flag {ibc}, which the snapshot does not define.")
-            if laws[ibc] is None:
-                raise ValueError(f"Segment '{name}': only constant inline surface impedances are supported by "
-                                 "the GPU path (no tapers, tables or thin layers).")
-            z_seg = laws[ibc]
-            if z_seg != 0 and n_prop <= 0:
-                raise ValueError(f"Segment '{name}': impedance segments need an explicit N > 0 here (the automatic "
-                                 "density follows the bound surface wave).")
+                raise ValueError(f"Undefined IBC flag {ibc}.")
+            law = laws[ibc]
+            if law.kind == "thin_dielectric":
+                raise ValueError("Thin dielectric layers can be assigned only to a free TYPE 1 sheet.")
+        wavelength = surface_wave_mesh_wavelength(wavelength0, law, ibc, freqs)
         pairs = list(seg.get("point_pairs", []) or [])
         # TYPE 2: the user draws the normal into air; reverse the chain.
         pairs = [{"x1": p.get("x2", 0.0), "y1": p.get("y2", 0.0), "x2": p.get("x1", 0.0), "y2": p.get("y1", 0.0)}
@@ -267,12 +649,13 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
             counts = refined_chain_counts(base_counts, [e[2] for e in ends], refine) if ends else []
         else:
             counts = [_panel_count(n_prop, length, wavelength) for _, _, length in ends]
+        seg_start = len(panels0)
         for (a, b, length), count in zip(ends, counts):
             if n_prop > 0:
-                minimum = max(1, int(math.ceil(length * MIN_EXPLICIT_PANELS_PER_WAVELENGTH / wavelength)))
+                minimum = max(1, int(math.ceil(length * MIN_EXPLICIT_PANELS_PER_WAVELENGTH / wavelength0)))
                 if count < minimum:
                     raise ValueError(f"Segment '{name}' explicit N={n_prop} under-resolves a {length:.6g} m "
-                                     f"primitive at wavelength {wavelength:.6g} m: at least N={minimum} is "
+                                     f"primitive at wavelength {wavelength0:.6g} m: at least N={minimum} is "
                                      "required.")
             if len(panels0) + count > max_panels:
                 raise ValueError("Discretization exceeds the configured panel limit.")
@@ -281,12 +664,22 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
                                  "panels no longer than the mesh node tolerance. Reduce N.")
             pts = [a + (b - a) * (i / count) for i in range(count + 1)]
             pts[0], pts[-1] = a.copy(), b.copy()
-            for i in range(count):
+            if junctions:
+                pts = grade_toward_junctions(pts, welder, junctions, wavelength)
+                if len(panels0) + len(pts) - 1 > max_panels:
+                    raise ValueError("Discretization exceeds the configured panel limit.")
+            for i in range(len(pts) - 1):
                 if np.linalg.norm(pts[i + 1] - pts[i]) <= EPS:
                     continue
                 panels0.append(pts[i])
                 panels1.append(pts[i + 1])
-                zs.append(z_seg)
+        # arc-length centre of each panel along the segment, in the drawing direction
+        seg_len = [float(np.linalg.norm(panels1[i] - panels0[i])) for i in range(seg_start, len(panels0))]
+        total, cum = sum(seg_len), 0.0
+        for length in seg_len:
+            arc = (cum + 0.5 * length) / total if total > EPS else 0.5
+            cum += length
+            zs.append(0j if law is None else law.impedance(ibc, law_ghz, 1.0 - arc))
     if not panels0:
         raise ValueError("Geometry does not contain any valid discretized panels.")
     p0 = np.array(panels0)
@@ -314,38 +707,26 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
             keys.setdefault(key, found)
             ids[e, c] = found
     n = len(p0)
-    if len(nodes) != n or not np.array_equal(ids[:, 0], np.arange(n)):
-        raise ValueError("The GPU path needs closed PEC contours drawn as chained primitives "
+    first = ids[:, 0]
+    if len(nodes) != n or sorted(first.tolist()) != list(range(n)):
+        raise ValueError("The GPU path needs closed conductor contours drawn as chained primitives "
                          "(every node shared by exactly two panels).")
-    nxt = ids[:, 1].astype(np.int32)
-    if sorted(nxt.tolist()) != list(range(n)):
-        raise ValueError("Every PEC contour must be closed.")
+    if sorted(ids[:, 1].tolist()) != list(range(n)):
+        raise ValueError("Every conductor contour must be closed.")
+    # Element e starts at node e here; the reference numbers nodes by first appearance, which
+    # differs where a contour is drawn as several segments.  ref_node keeps that numbering.
+    ours = np.empty(n, np.int64)
+    ours[first] = np.arange(n)
+    nxt = ours[ids[:, 1]].astype(np.int32)
     prev = np.empty(n, np.int32)
     prev[nxt] = np.arange(n, dtype=np.int32)
     starts = [int(e) for e in range(n) if prev[e] != e - 1]
     impedance = np.array(zs, complex)
-    mesh = ContourMesh(np.array(nodes), p0, p1, nxt, prev, starts,
-                       impedance if np.any(impedance != 0) else None)
+    mesh = ContourMesh(np.array(nodes)[first], p0, p1, nxt, prev, starts,
+                       impedance if np.any(np.abs(impedance) > EPS) else None,
+                       None if np.array_equal(first, np.arange(n)) else first.copy())
     _check_orientation(mesh)
-    if mesh.impedance is not None:
-        _check_uniform_law(mesh)
     return mesh
-
-
-def _check_uniform_law(mesh: ContourMesh) -> None:
-    """Each closed contour carries one law (PEC or one constant impedance): the flux rows
-    Z_TE + alpha Z_TM hold exactly then; zones along a contour need element-weighted terms."""
-    seen = np.zeros(mesh.n, bool)
-    for s in range(mesh.n):
-        if seen[s]:
-            continue
-        e, law = s, mesh.impedance[s]
-        while not seen[e]:
-            seen[e] = True
-            if mesh.impedance[e] != law:
-                raise ValueError("Impedance zones along one closed contour are not supported by the GPU path: "
-                                 "each contour must be PEC or carry one constant surface impedance.")
-            e = int(mesh.next[e])
 
 
 def _check_orientation(mesh: ContourMesh) -> None:

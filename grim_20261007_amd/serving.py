@@ -251,7 +251,7 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
         z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
         pair = z_tm._base                                         # batch [TE, TM]
         if mesh.impedance is not None:
-            engine.impedance_systems(contour, k, pair)
+            engine.impedance_systems(contour, k, pair, ws[mesh.n])
         fac = engine.factor(pair, req.precision)
         if store is not None:
             sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * mesh.n * mesh.n
@@ -266,7 +266,7 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, angles)
     rhs = (torch.stack((-bdn, -bu)) if mesh.impedance is None     # [TE, TM]
-           else engine.impedance_rhs(contour, k, bu, bdn))
+           else engine.impedance_rhs(contour, k, angles))
     x, inf = engine.solve(fac, rhs, gate=(req.precision == "double"))
     if inf.get("factor") is not None and store is not None:
         # the mixed factorization fell back to FP64: later requests reuse the FP64 factors

@@ -196,14 +196,30 @@ def admit_device_memory(n: int, nangles: int, npols: int, precision: str, device
     return need / gib
 
 
-def _systems(contour, k, z_te, z_tm, bu, bdn):
+def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, material_base_dir):
+    """(mesh frequency, mesh) that frequency ``f`` solves on.  Without a reference frequency the mesh
+    is built at f for f alone; with one, every frequency shares the mesh of the shortest wavelength
+    over the sweep and the reference (twod/solver.py:2195-2215, geometry.py:2454-2471), whose junction
+    grading and bound-surface-wave sizing see all of them.  Impedance laws are evaluated at f."""
+    if mesh_reference_ghz:
+        served = sorted(set(float(v) for v in freqs) | {float(mesh_reference_ghz)})
+        mesh_ghz = max(served)
+    else:
+        served, mesh_ghz = [float(f)], float(f)
+    return mesh_ghz, build_mesh(snapshot, units, mesh_ghz, max_panels, material_base_dir=material_base_dir,
+                                law_ghz=float(f), mesh_frequencies=served)
+
+
+def _systems(contour, k, z_te, z_tm, bu, bdn, angles, workspace):
     """[TE, TM] systems (2, N, N) and right-hand sides (2, A, N): the PEC combined potential
-    (TM Dirichlet rows -b_u, TE flux rows -b_dn), or, for contours with a surface impedance,
-    the flux rows dn(u) + alpha u of both polarizations (engine.impedance_rows)."""
+    (TM Dirichlet rows -b_u, TE flux rows -b_dn), or, for contours with surface impedances,
+    the flux rows dn(u) + alpha u of both polarizations (engine.impedance_systems, which reads
+    the near blocks the assembly left in ``workspace``)."""
     pair = z_tm._base if z_tm._base is not None else torch.stack((z_te, z_tm))   # [TE, TM]
     if contour.mesh.impedance is None:
         return pair, torch.stack((-bdn, -bu))
-    return engine.impedance_systems(contour, k, pair), engine.impedance_rhs(contour, k, bu, bdn)
+    return (engine.impedance_systems(contour, k, pair, workspace),
+            engine.impedance_rhs(contour, k, angles))
 
 
 def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("TE", "TM"),
@@ -218,12 +234,14 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     order = [p for p in ("TE", "TM") if p in pols]
     admit_device_memory(mesh.n, len(angles_deg), len(order), precision, dev, context, compute_condition_number)
     contour = engine.DeviceContour(mesh, dev)
-    z_tm, z_te = engine.assemble(contour, k)
+    ws = engine.assembly_workspace(mesh.n, dev)
+    z_tm, z_te = engine.assemble(contour, k, ws)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, angles)
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
-    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn)
+    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn, angles, ws)
+    del ws
     sel = [("TE", "TM").index(p) for p in order]
     systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
     rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
@@ -382,8 +400,8 @@ def solve_monostatic_rcs_2d_single_polarization(
     for f in freqs:
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve canceled by user.")
-        mesh_ghz = mesh_reference_ghz or f
-        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+        mesh_ghz, mesh = _frequency_mesh(geometry_snapshot, geometry_units, f, freqs, mesh_reference_ghz, max_panels,
+                                         material_base_dir)
         record.add_mesh(mesh, mesh_ghz)
         res = solve_frequency(mesh, f, angs, (pol,), precision, device, abort_event=abort_event,
                               compute_condition_number=compute_condition_number,
@@ -430,8 +448,8 @@ def solve_monostatic_rcs_2d(
     for f in freqs:
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve canceled by user.")
-        mesh_ghz = mesh_reference_ghz or f
-        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+        mesh_ghz, mesh = _frequency_mesh(geometry_snapshot, geometry_units, f, freqs, mesh_reference_ghz, max_panels,
+                                         material_base_dir)
         record.add_mesh(mesh, mesh_ghz)
         res = solve_frequency(mesh, f, angs, ("TE", "TM"), precision, device, abort_event=abort_event,
                               compute_condition_number=compute_condition_number)
@@ -475,13 +493,15 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
     admit_device_memory(mesh.n, len(incidence_deg) + len(observation_deg), len(order), precision, dev,
                         "The bistatic 2-D solve", compute_condition_number)
     contour = engine.DeviceContour(mesh, dev)
-    z_tm, z_te = engine.assemble(contour, k)
+    ws = engine.assembly_workspace(mesh.n, dev)
+    z_tm, z_te = engine.assemble(contour, k, ws)
     inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)
     obs = torch.tensor(observation_deg, dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, inc)
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
-    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn)
+    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn, inc, ws)
+    del ws
     sel = [("TE", "TM").index(p) for p in order]
     systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
     rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
@@ -546,7 +566,8 @@ def _validate_bistatic(frequencies_ghz, incidence_angles_deg, observation_angles
 
 def _bistatic_run(geometry_snapshot, frequencies_ghz, incidence_angles_deg, observation_angles_deg, pols,
                   geometry_units, progress_callback, quality_thresholds, strict_quality_gate, max_panels,
-                  mesh_reference_ghz, abort_event, precision, device, compute_condition_number=False):
+                  mesh_reference_ghz, abort_event, precision, device, compute_condition_number=False,
+                  material_base_dir=None):
     freqs, inc, obs = _validate_bistatic(frequencies_ghz, incidence_angles_deg, observation_angles_deg)
     unit_scale(geometry_units)
     evaluate_quality_gate({}, quality_thresholds)
@@ -556,8 +577,8 @@ def _bistatic_run(geometry_snapshot, frequencies_ghz, incidence_angles_deg, obse
     for f in freqs:
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve canceled by user.")
-        mesh_ghz = mesh_reference_ghz or f
-        mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+        mesh_ghz, mesh = _frequency_mesh(geometry_snapshot, geometry_units, f, freqs, mesh_reference_ghz, max_panels,
+                                         material_base_dir)
         record.add_mesh(mesh, mesh_ghz)
         res = solve_bistatic_frequency(mesh, f, inc, obs, pols, precision, device, abort_event=abort_event,
                                        compute_condition_number=compute_condition_number)
@@ -592,7 +613,7 @@ def solve_bistatic_rcs_2d_single_polarization(
     rows, metadata = _bistatic_run(geometry_snapshot, frequencies_ghz, incidence_angles_deg,
                                    observation_angles_deg, (pol,), geometry_units, progress_callback,
                                    quality_thresholds, strict_quality_gate, max_panels, mesh_reference_ghz,
-                                   abort_event, precision, device, compute_condition_number)
+                                   abort_event, precision, device, compute_condition_number, material_base_dir)
     metadata["solver_method_requested"] = str(solver_method)
     return {"solver": "2d_bie_mom_rcs", "scattering_mode": "bistatic", "amplitude_convention": AMPLITUDE_CONVENTION,
             "amplitude_version": 2, "polarization": pol, "polarization_export": pol, "samples": rows[pol],
@@ -612,7 +633,7 @@ def solve_bistatic_rcs_2d(
     rows, metadata = _bistatic_run(geometry_snapshot, frequencies_ghz, incidence_angles_deg,
                                    observation_angles_deg, ("TE", "TM"), geometry_units, progress_callback,
                                    quality_thresholds, strict_quality_gate, max_panels, mesh_reference_ghz,
-                                   abort_event, precision, device, compute_condition_number)
+                                   abort_event, precision, device, compute_condition_number, material_base_dir)
     co = {}
     for export, internal in CO_POLARIZED_CHANNELS:
         for row in rows[internal]:
@@ -649,8 +670,8 @@ def compute_boundary_densities(geometry_snapshot: Dict[str, Any], frequency_ghz:
     frequency_ghz = float(frequency_ghz)
     if not math.isfinite(frequency_ghz) or frequency_ghz <= 0.0:
         raise ValueError("frequency_ghz must be a positive finite value.")
-    mesh_ghz = float(mesh_reference_ghz or frequency_ghz)
-    mesh = build_mesh(geometry_snapshot, geometry_units, mesh_ghz, max_panels)
+    mesh_ghz, mesh = _frequency_mesh(geometry_snapshot, geometry_units, frequency_ghz, [frequency_ghz],
+                                     mesh_reference_ghz, max_panels, material_base_dir)
     check_abort()
     res = solve_frequency(mesh, frequency_ghz, [float(elevation_deg)], (pol,), precision, device,
                           return_density=True, abort_event=abort_event,

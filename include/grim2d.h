@@ -227,6 +227,40 @@ int grim_copy_c128(int64_t batch, int64_t m, int64_t n, const double* src, int64
 int grim_robin_rows(int64_t rows, int64_t cols, double* x, int64_t ldx, double* y, int64_t ldy,
                     const double* alpha_x, const double* alpha_y, const int32_t* dirichlet, void* stream);
 
+/* Impedance rows of a law that varies along a contour (PEC and impedance zones,
+ * tapers, tables), in place, after grim_robin_rows with the nodal coefficients
+ * alpha_p = (alpha_e(p) + alpha_e(prev p)) / 2: for each row p listed in `walk`
+ *   Z_TE[p, :] += c_te[p] ((S0 - S1)[p, :] + (gamma/2) m_p)   (mass part if flags[p] & 1)
+ *   Z_TM[p, :] += c_tm[p] ((S0 - S1)[p, :] + (gamma/2) m_p)   (mass part if flags[p] & 2)
+ * with S0 (S1) the rows of S that element p (prev p) contributes through its
+ * basis function at node p, m_p = (L_p/6 at next p, (L_p - L_prev p)/3 at p,
+ * -L_prev p/6 at prev p), and c = (alpha_e(p) - alpha_e(prev p)) / 2.  With
+ * these the rows carry the element-weighted alpha S and trace-jump mass of the
+ * reference: twod/formulations/regions.py:274-304 (single-layer observation
+ * coefficients), combined_regions.py:43-83 (routing, _trace_jump_mass),
+ * regions.py:66-119 (element and nodal alpha).  walk: int32 [64 nblocks]:
+ * per block, lane 0 the element before lane 1's row, lanes 1..63 rows p whose
+ * previous element is the lane before's, -1 for none (entries in [-1, n)).
+ * c_te, c_tm: complex128 [n]; flags: int32 [n].  The far rule is far_order;
+ * near pairs are read from `workspace`, which must hold the near lists and
+ * blocks of a grim_assemble call on the same contour and wavenumber (same
+ * near_per_element), unchanged since. */
+int grim_robin_split(int64_t n, const double* p0, const double* p1, const int32_t* next, const int32_t* prev,
+                     double k, int far_order, int64_t nblocks, const int32_t* walk, const double* c_te,
+                     const double* c_tm, const int32_t* flags, const void* workspace, int64_t workspace_bytes,
+                     int64_t near_per_element, double* z_te, int64_t ld_te, double* z_tm, int64_t ld_tm,
+                     void* stream);
+
+/* Right-hand sides of one polarization's impedance rows: out[p, a] = -b_u[p, a]
+ * on rows with dirichlet[p] != 0, else -sum_e i_e(p, a) (j k d_a.n_e + alpha_e[e])
+ * over the two elements of node p -- the derivative load with each element's own
+ * Robin coefficient.  Replaces twod/formulations/regions.py:557-590 (rhs_many)
+ * with twod/assembly/kernels.py:222-294 (incident_loads, observation
+ * coefficients).  alpha_e: complex128 [n]; out: complex128 n x na (ld >= n). */
+int grim_robin_rhs(int64_t n, int64_t na, const double* p0, const double* p1, const int32_t* prev, double k,
+                   const double* angles_deg, const double* alpha_e, const int32_t* dirichlet, double* out,
+                   int64_t ld, void* stream);
+
 /* Launch shapes (no counterpart in the reference; for a project autotuner and
  * callers that pin their own shapes).  Knob 0: far-pass source columns per
  * wavefront (1, 2, 4, 8, 16 or 32), keyed by n; knob 1: columns per wavefront

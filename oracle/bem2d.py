@@ -45,6 +45,13 @@ What each function follows in the reference (paths under ``tools/GHOST/ghost_bac
                           (alpha_TM = j k eta/Zs, alpha_TE = j k Zs/eta) and
                           twod/solver.py:1810-1825 (RHS -(b_dn + alpha b_u); -b_u on
                           TM PEC rows), for one constant law per contour.
+* ``robin_system``        the same rows for a law that varies along a contour
+                          (zones, tapers, tables): element-weighted alpha S
+                          (``weighted_single_layer``), nodal alpha on D and the
+                          element-weighted trace-jump mass (combined_regions.py:43-83),
+                          element-weighted trace loads (kernels.py:222-294,
+                          regions.py:557-590); ``robin_alpha`` follows
+                          regions.py:66-119 (TM nodes of PEC elements: alpha 0).
 
 Pinning (see tests/test_oracle_*.py): every function above is checked against
 the reference's own outputs (tests/golden/*.npz, written by
@@ -661,6 +668,124 @@ def impedance_system(z_tm, z_te, bu, bdn, node_impedance, k: float, pol: str):
         dirichlet = ~ibc
         flux = np.where(dirichlet[:, None], z_tm, flux)
         load = np.where(dirichlet[:, None], -bu, load)
+    return flux, load
+
+
+def mesh_from_nodes(nodes, elements) -> Mesh:
+    """Mesh of given node coordinates and element node pairs (element e runs from elements[e, 0]
+    to elements[e, 1])."""
+    nodes = np.asarray(nodes, float)
+    ids = np.asarray(elements, np.int64)
+    p0, p1 = nodes[ids[:, 0]], nodes[ids[:, 1]]
+    seg = p1 - p0
+    length = np.linalg.norm(seg, axis=1)
+    tangent = seg / length[:, None]
+    normal = np.stack((-tangent[:, 1], tangent[:, 0]), axis=1)
+    return Mesh(nodes.copy(), ids, p0, p1, 0.5 * (p0 + p1), length, normal)
+
+
+def weighted_single_layer(mesh: Mesh, k: float, coefficients, rows_per_chunk: int = 16):
+    """For each per-element coefficient vector c: sum_e c_e S_e, S_e being the rows of S that
+    observation element e contributes (its two basis functions), with the far and near rules of
+    ``assemble`` -- the single layer with the reference's element-weighted observation
+    coefficients (twod/operators.py:2087-2607, ``single_layer_observation_coefficients``)."""
+    E, N = len(mesh.length), len(mesh.nodes)
+    coefficients = [np.asarray(c, complex) for c in coefficients]
+    out = [np.zeros((N, N), complex) for _ in coefficients]
+    ids = mesh.elements
+    q = far_order(mesh, k)
+    for start in range(0, E, rows_per_chunk):
+        rows = np.arange(start, min(E, start + rows_per_chunk))
+        far = far_mask(mesh, rows)
+        s = _far_rows(mesh, k, rows, far, q)[0]
+        for c, z in zip(coefficients, out):
+            for a in range(2):
+                for b in range(2):
+                    np.add.at(z, (ids[rows, a][:, None], ids[None, :, b]), c[rows][:, None] * s[:, :, a, b])
+    for e, f in np.argwhere(~far_mask(mesh)):
+        sb = near_blocks(mesh, k, int(e), int(f))[0]
+        for c, z in zip(coefficients, out):
+            for a in range(2):
+                for b in range(2):
+                    z[ids[e, a], ids[f, b]] += c[e] * sb[a, b]
+    return out
+
+
+def weighted_mass(mesh: Mesh, coefficient):
+    """sum_e c_e M_e with M_e = L_e [[1/3, 1/6], [1/6, 1/3]] (twod/assembly/mass.py:6-17)."""
+    N = len(mesh.nodes)
+    m = np.zeros((N, N), complex)
+    ids = mesh.elements
+    for e in range(len(mesh.length)):
+        block = mesh.length[e] * np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
+        for a in range(2):
+            for b in range(2):
+                m[ids[e, a], ids[e, b]] += coefficient[e] * block[a, b]
+    return m
+
+
+def robin_alpha(mesh: Mesh, element_impedance, k: float, pol: str):
+    """Element and nodal Robin coefficients of surface impedances in air (twod/geometry.py:2488-2521,
+    twod/formulations/regions.py:66-119): alpha_e = j k eta0/Zs (TM) or j k Zs/eta0 (TE), 0 on PEC
+    elements; nodal alpha = the mean over the node's elements; in TM every node of a PEC element
+    gets 0 (its Dirichlet row)."""
+    z = np.asarray(element_impedance, complex)
+    ibc = np.abs(z) > EPS
+    safe = np.where(ibc, z, 1.0)
+    alpha_e = np.where(ibc, 1j * k * (ETA0 / safe if pol == "TM" else safe / ETA0), 0.0)
+    N = len(mesh.nodes)
+    alpha = np.zeros(N, complex)
+    count = np.zeros(N)
+    for e, (i, j) in enumerate(mesh.elements):
+        for node in (i, j):
+            alpha[node] += alpha_e[e]
+            count[node] += 1
+    alpha /= np.maximum(count, 1)
+    if pol == "TM":
+        for e, (i, j) in enumerate(mesh.elements):
+            if abs(alpha_e[e]) <= EPS:
+                alpha[i] = alpha[j] = 0.0
+    return alpha_e, alpha
+
+
+def weighted_derivative_load(mesh: Mesh, k: float, angles_deg, coefficient):
+    """b_dn with the element-weighted trace: sum over elements of the moments times
+    (j k d.n_e + c_e) (twod/assembly/kernels.py:222-294, ``observation_coefficients``)."""
+    dirs = directions(angles_deg)
+    z = k * (mesh.seg @ dirs.T)
+    f, fp = sinc_and_derivative(0.5 * z)
+    phase = mesh.length[:, None] * np.exp(1j * k * (mesh.center @ dirs.T))
+    i0 = phase * (0.5 * f + 1j * fp)
+    i1 = phase * (0.5 * f - 1j * fp)
+    dn = 1j * k * (mesh.normal @ dirs.T) + np.asarray(coefficient, complex)[:, None]
+    out = np.zeros((len(mesh.nodes), len(dirs)), complex)
+    np.add.at(out, mesh.elements[:, 0], i0 * dn)
+    np.add.at(out, mesh.elements[:, 1], i1 * dn)
+    return out
+
+
+def robin_system(mesh: Mesh, k: float, element_impedance, pol: str, z_tm, z_te, s, s_weighted, angles_deg):
+    """System and right-hand sides of closed contours whose surface impedance varies along them
+    (PEC and impedance zones, tapers, tables): the reference's combined-potential rows
+    (twod/formulations/regions.py:274-304, combined_regions.py:1-83,149-213) --
+    flux rows -M/2 + K' - gamma W + (alpha S)_e, and on nodes with |alpha| > EPS also
+    gamma alpha_node D + (gamma/2) M_alpha (the trace jump with the element-weighted mass);
+    TM rows of nodes with alpha 0 keep the Dirichlet row S + gamma (D + M/2).  Loads:
+    -(b_dn + element-weighted b_u) on flux rows, -b_u on Dirichlet rows
+    (regions.py:557-590).  ``s`` is S and ``s_weighted`` (alpha S)_e of this polarization."""
+    gamma = 1.0 / (1j * k)
+    alpha_e, alpha = robin_alpha(mesh, element_impedance, k, pol)
+    m = weighted_mass(mesh, np.ones(len(mesh.length)))
+    m_alpha = weighted_mass(mesh, alpha_e)
+    gamma_d = z_tm - s - 0.5 * gamma * m
+    robin = np.abs(alpha) > EPS
+    flux = z_te + s_weighted
+    flux[robin] += alpha[robin, None] * gamma_d[robin] + 0.5 * gamma * m_alpha[robin]
+    bu, _ = incident_loads(mesh, k, angles_deg)
+    load = -weighted_derivative_load(mesh, k, angles_deg, alpha_e)
+    if pol == "TM":
+        flux[~robin] = z_tm[~robin]
+        load[~robin] = -bu[~robin]
     return flux, load
 
 

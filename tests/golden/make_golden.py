@@ -16,7 +16,7 @@ inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
 Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
-                                               --condition]
+                                               --ibc-zones | --condition]
 """
 from __future__ import annotations
 
@@ -394,6 +394,119 @@ def main_ibc():
                  workloads.frequency_for_segments_per_wavelength(pts, 10.0), [0.0, 45.0, 180.0, 333.0])
 
 
+def _zone_snapshot(points, bounds, flags, ibcs, n_prop=1):
+    """One closed polygon drawn as consecutive segments: segment s holds chords
+    bounds[s]..bounds[s+1]-1 and carries IBC flag flags[s] (0: PEC)."""
+    pts = np.asarray(points, float)
+    n = len(pts)
+    segments = []
+    for s, flag in enumerate(flags):
+        pairs = []
+        for i in range(bounds[s], bounds[s + 1]):
+            a, b = pts[i], pts[(i + 1) % n]
+            pairs.append({"x1": float(a[0]), "y1": float(a[1]), "x2": float(b[0]), "y2": float(b[1])})
+        segments.append({"name": f"zone{s}", "seg_type": 2, "properties": ["2", str(n_prop), str(flag), "0", "0"],
+                         "point_pairs": pairs})
+    return {"segments": segments, "ibcs": ibcs, "dielectrics": []}
+
+
+def run_zone_case(name, snapshot, frequency_ghz, angles, keep_matrix=False, material_base_dir=None):
+    """The reference's co-polarized monostatic solve of a contour whose surface impedance varies
+    along it (zones, tapers, tables): per channel the welded mesh, right-hand sides, nodal
+    densities and amplitudes (and the system matrix for small cases), recorded at the solver's
+    _solve_fields hook, plus the sample amplitudes of the public result."""
+    import ghost_backend.twod.solver as rcs
+
+    captured = []
+    original = rcs._solve_fields
+
+    def recording(mesh, matrix, k0, angs, rhs_builder, diagnostics, label, **kwargs):
+        entry = {"k0": float(k0), "label": str(label)}
+        if keep_matrix:
+            entry["matrix"] = np.array(matrix, dtype=np.complex128, copy=True)
+        entry["rhs"] = np.array(rhs_builder(np.asarray(angs, float)), dtype=np.complex128)
+        entry["nodes"] = np.array([n.xy for n in mesh.nodes], dtype=float)
+        entry["elements"] = np.array([e.node_ids for e in mesh.elements], dtype=np.int64)
+        kwargs = dict(kwargs)
+        kwargs["return_density"] = True
+        out = original(mesh, matrix, k0, angs, rhs_builder, diagnostics, label, **kwargs)
+        entry["density"] = np.array(out[3], dtype=np.complex128)
+        entry["amplitude"] = np.array(out[1], dtype=np.complex128)
+        captured.append(entry)
+        return out
+
+    rcs._solve_fields = recording
+    try:
+        result = rcs.solve_monostatic_rcs_2d(
+            snapshot, frequencies_ghz=[frequency_ghz], elevations_deg=list(map(float, angles)),
+            geometry_units="meters", material_base_dir=material_base_dir, strict_quality_gate=False,
+            max_panels=100000, solver_method="direct", execution_options=EXECUTION)
+    finally:
+        rcs._solve_fields = original
+    if len(captured) != 2:
+        raise RuntimeError(f"{name}: expected two captured systems, got {len(captured)}")
+    te, tm = captured
+    out = {"snapshot_json": np.array(json.dumps(snapshot)), "frequency_ghz": float(frequency_ghz),
+           "angles_deg": np.asarray(angles, float), "k0": te["k0"], "nodes": te["nodes"],
+           "elements": te["elements"], "label": np.array(te["label"])}
+    for pol, entry in (("TE", te), ("TM", tm)):
+        if not np.array_equal(entry["elements"], te["elements"]) or not np.array_equal(entry["nodes"], te["nodes"]):
+            raise RuntimeError("polarizations used different meshes")
+        if keep_matrix:
+            out[f"Z_{pol}"] = entry["matrix"]
+        out[f"rhs_{pol}"] = entry["rhs"]
+        out[f"density_{pol}"] = entry["density"]
+        out[f"amp_{pol}"] = entry["amplitude"]
+    for export, pol in (("VV", "TE"), ("HH", "TM")):
+        rows = result["co_solved_samples"][export]
+        out[f"sample_amp_{pol}"] = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+        out[f"sample_db_{pol}"] = np.array([r["rcs_db"] for r in rows])
+    out["formulation"] = np.array(str(result["metadata"].get("formulation")))
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print(f"{name}: {te['label']}; {result['metadata'].get('formulation')} N={len(te['nodes'])}", flush=True)
+
+
+IBC_TABLE_CSV = "ibcz_table.csv"
+
+
+def main_ibc_zones():
+    """Surface impedances that vary along a closed contour: PEC and impedance zones (graded
+    junctions), several laws, linear / cosine / exponential tapers, and a frequency table."""
+    r = 0.1
+    f = workloads.frequency_for_ka(1.0, r)
+    c48 = workloads.circle_points(r, 48)
+    run_zone_case("ibcz_pec_half", _zone_snapshot(c48, [0, 24, 48], [0, 1],
+                                                  [["1", "constant", "75", "-20", "0", "0"]]),
+                  f, [0.0, 90.0, 180.0, 250.0], keep_matrix=True)
+    c64 = workloads.circle_points(r, 64)
+    laws = [["1", "constant", "150", "-80", "0", "0"], ["2", "constant", "20", "200", "0", "0"],
+            ["3", "constant", "376", "0", "0", "0"]]
+    run_zone_case("ibcz_three_laws", _zone_snapshot(c64, [0, 20, 44, 64], [1, 2, 3], laws),
+                  workloads.frequency_for_ka(2.0, r), [0.0, 60.0, 135.0, 300.0])
+    run_zone_case("ibcz_taper_linear", _zone_snapshot(c64, [0, 64], [1], [["1", "linear", "20", "0", "300", "-100"]]),
+                  f, [0.0, 45.0, 200.0], keep_matrix=True)
+    run_zone_case("ibcz_taper_cosine_exp",
+                  _zone_snapshot(c64, [0, 32, 64], [1, 2], [["1", "cosine", "50", "10", "200", "-30"],
+                                                            ["2", "exp", "200", "-30", "50", "10"]]),
+                  f, [0.0, 120.0, 240.0])
+    # automatic panel counts (N = 0) sized for the bound surface wave of a reactive law
+    c16 = workloads.circle_points(r, 16)
+    run_zone_case("ibcz_auto_inductive", _zone_snapshot(c16, [0, 16], [1], [["1", "constant", "0", "600", "0", "0"]],
+                                                        n_prop=0), f, [0.0, 90.0])
+    run_zone_case("ibcz_auto_capacitive_taper",
+                  _zone_snapshot(c16, [0, 8, 16], [1, 0], [["1", "cosine", "0", "-60", "100", "-30"]], n_prop=0),
+                  f, [0.0, 200.0])
+    # a frequency table on the upper surface of the airfoil, PEC below
+    table = HERE / IBC_TABLE_CSV
+    table.write_text("frequency_hz,resistance_ohm,reactance_ohm\n"
+                     "# synthetic absorber, linear in frequency between rows\n"
+                     "0.5e9,30,150\n0.7e9,45,120\n0.9e9,80,60\n1.1e9,120,10\n")
+    pts = workloads.naca0012_points(40)
+    fn = workloads.frequency_for_segments_per_wavelength(pts, 10.0)
+    run_zone_case("ibcz_naca_table", _zone_snapshot(pts, [0, 40, 80], [0, 1], [["1", IBC_TABLE_CSV]]),
+                  fn, [0.0, 30.0, 180.0, 290.0], material_base_dir=str(HERE))
+
+
 def run_condition_case(name, points, frequency_ghz, angles):
     """The reference's single-polarization entry (twod/solver.py:1984-2712) with
     compute_condition_number=True in both LU precisions: per channel the condition estimate and
@@ -467,6 +580,8 @@ if __name__ == "__main__":
         main_grim()
     elif "--bistatic" in sys.argv:
         main_bistatic()
+    elif "--ibc-zones" in sys.argv:
+        main_ibc_zones()
     elif "--ibc" in sys.argv:
         main_ibc()
     else:
