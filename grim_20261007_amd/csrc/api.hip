@@ -12,11 +12,11 @@ extern "C" hipError_t grim_upload_rules(const double*, const double*, const doub
                                         const double*, const double*);
 extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
                                       const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
-                                      int64_t ldte, const int* fix_rows, int nfix, hipStream_t st);
+                                      int64_t ldte, const int* fix_rows, int nfix, int plain, hipStream_t st);
 extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1, const int* next, const int* prev,
                                        double k, int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
                                        double* blocks, int* kind, unsigned* status, double* ztm, int64_t ldtm,
-                                       double* zte, int64_t ldte, hipStream_t st);
+                                       double* zte, int64_t ldte, int plain, double plain_mass, hipStream_t st);
 extern "C" hipError_t grim_launch_loads(int n, int na, const double* p0, const double* p1, const int* prev, double k,
                                         const double* angles_deg, double* bu, double* bdn, int64_t ld,
                                         hipStream_t st);
@@ -34,6 +34,7 @@ extern "C" hipError_t grim_launch_robin_split(int q, int n, const double* p0, co
 extern "C" hipError_t grim_launch_robin_rhs(int n, int na, const double* p0, const double* p1, const int* prev,
                                             double k, const double* angles_deg, const double* alpha_e,
                                             const int* dirichlet, double* out, int64_t ld, hipStream_t st);
+extern "C" hipError_t grim_launch_scale_blocks(int64_t n, double* z, int64_t ld, const double* s, hipStream_t st);
 extern "C" int64_t grim_dense_aux_bytes(int n, int single);
 extern "C" hipError_t grim_dense_lu_factor(int batch, int n, void* a, int64_t lda, int64_t sa, int single, int* piv,
                                            int64_t spiv, void* aux, int64_t saux, unsigned* status, hipStream_t st);
@@ -266,16 +267,16 @@ static bool assemble_args_ok(int64_t n, const double* p0, const double* p1, cons
 static int assemble_launch(int passes, int64_t n, const double* p0, const double* p1, const int32_t* next,
                            const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
                            double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, const AssemblyWork& w,
-                           int64_t cap, hipStream_t st) {
+                           int64_t cap, hipStream_t st, int plain = 0, double plain_mass = 0.0) {
     if (passes & 1) {
         zero_words<<<1, 64, 0, st>>>(w.status, 1);
         CHECK(hipGetLastError());
         CHECK(grim_launch_far(far_order, (int)n, p0, p1, next, prev, k, z_tm, ld_tm, z_te, ld_te, fix_rows,
-                              (int)nfix, st));
+                              (int)nfix, plain, st));
     }
     if (passes & 2)
         CHECK(grim_launch_near((int)n, p0, p1, next, prev, k, w.counts, w.offsets, (int)cap, w.pair_e, w.pair_f,
-                               w.blocks, w.kind, w.status, z_tm, ld_tm, z_te, ld_te, st));
+                               w.blocks, w.kind, w.status, z_tm, ld_tm, z_te, ld_te, plain, plain_mass, st));
     return GRIM_OK;
 }
 
@@ -292,6 +293,22 @@ static int assemble_pass(int passes, int64_t n, const double* p0, const double* 
     if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
     return assemble_launch(passes, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, z_tm, ld_tm, z_te, ld_te, w,
                            cap, (hipStream_t)stream);
+}
+
+extern "C" int grim_assemble_plain(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                   const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k,
+                                   int far_order, double* s, int64_t ld_s, double* kp, int64_t ld_kp,
+                                   double kp_mass, void* workspace, int64_t workspace_bytes,
+                                   int64_t near_per_element, void* stream) {
+    if (!workspace || !isfinite(kp_mass) ||
+        !assemble_args_ok(n, p0, p1, next, prev, fix_rows, nfix, k, far_order, s, ld_s, kp, ld_kp, near_per_element))
+        return GRIM_ERR_ARGS;
+    const int64_t cap = n * near_per_element;
+    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
+    AssemblyWork w = carve(workspace, n, cap);
+    if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
+    return assemble_launch(3, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, s, ld_s, kp, ld_kp, w, cap,
+                           (hipStream_t)stream, 1, kp_mass);
 }
 
 extern "C" int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* next,
@@ -689,6 +706,14 @@ extern "C" int grim_robin_rhs(int64_t n, int64_t na, const double* p0, const dou
     if (!p0 || !p1 || !prev || !angles_deg || !alpha_e || !dirichlet || !out) return GRIM_ERR_ARGS;
     CHECK(grim_launch_robin_rhs((int)n, (int)na, p0, p1, prev, k, angles_deg, alpha_e, dirichlet, out, ld,
                                 (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+extern "C" int grim_scale_blocks(int64_t n, double* z, int64_t ld, const double* scale, void* stream) {
+    if (n < 0 || 2 * n > 65535 || ld < 2 * n || (n > 0 && (!z || !scale))) return GRIM_ERR_ARGS;
+    for (int i = 0; i < 8; ++i)
+        if (!isfinite(scale[i])) return GRIM_ERR_ARGS;
+    CHECK(grim_launch_scale_blocks(n, z, ld, scale, (hipStream_t)stream));
     return GRIM_OK;
 }
 

@@ -79,8 +79,10 @@ __device__ __forceinline__ void zero_blk2(Blk2& b) {
 }
 
 // Far element-pair quadrature with the Q x Q Gauss rule (far.c:100-191 semantics,
-// kernels evaluated directly instead of through an interpolation table).
-template <int Q>
+// kernels evaluated directly instead of through an interpolation table).  PLAIN gives the
+// operators themselves instead of the combined potential: tm = S, te = K', tmt = S^T and
+// tet = D^T (the K' of the reversed pair).
+template <int Q, bool PLAIN = false>
 __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k, Blk2& out) {
     cplx S[2][2], K[2][2], D[2][2];
 #pragma unroll
@@ -135,6 +137,18 @@ __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k,
             K[a][b] = cscale(K[a][b], scale);
             D[a][b] = cscale(D[a][b], scale);
         }
+    if (PLAIN) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                out.tm[a][b] = S[a][b];
+                out.te[a][b] = K[a][b];
+                out.tmt[b][a] = S[a][b];
+                out.tet[b][a] = D[a][b];
+            }
+        return;
+    }
     const double g = -1.0 / k;   // gamma = (0, g)
     const double nn = o.nx * s.nx + o.ny * s.ny;
     double denom = scale < GRIM_EPS * GRIM_EPS ? GRIM_EPS * GRIM_EPS : scale;
@@ -154,10 +168,10 @@ __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k,
         }
 }
 
-template <int Q>
+template <int Q, bool PLAIN>
 __device__ __forceinline__ void far_or_zero(const Elem& o, int e, int ne, const Elem& s, int f, int nf,
                                             double k, bool active, Blk2& out) {
-    if (active && is_far(o, e, ne, s, f, nf)) far_pair<Q>(o, s, k, out);
+    if (active && is_far(o, e, ne, s, f, nf)) far_pair<Q, PLAIN>(o, s, k, out);
     else zero_blk2(out);
 }
 
@@ -176,8 +190,9 @@ __device__ __forceinline__ void store_add(double* z, int64_t ld, int p, int q, c
 // same evaluations also give the transposed tile (rows in chunk, columns in
 // block I), written with the roles exchanged.  Every entry is written exactly
 // once, in a fixed summation order.  Entries whose observation-side neighbour
-// is not lane - 1 (lane 0, contour starts) are completed by far_fixup.
-template <int Q>
+// is not lane - 1 (lane 0, contour starts) are completed by far_fixup.  With PLAIN the two
+// outputs are S and K' (see far_pair).
+template <int Q, bool PLAIN>
 __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                  const int* __restrict__ next, const int* __restrict__ prev,
                                                  double k, int chunk, double* __restrict__ ztm, int64_t ldtm,
@@ -207,7 +222,7 @@ __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict
         cplx ltm0, ltm1, lte0, lte1, mtm0, mtm1, mte0, mte1;
         if (f == c0 || pf != f - 1) {
             Blk2 b;
-            far_or_zero<Q>(o, ee, ne, load_elem(p0, p1, pf), pf, next[pf], k, active, b);
+            far_or_zero<Q, PLAIN>(o, ee, ne, load_elem(p0, p1, pf), pf, next[pf], k, active, b);
             ltm0 = b.tm[0][1]; ltm1 = b.tm[1][1]; lte0 = b.te[0][1]; lte1 = b.te[1][1];
             mtm0 = b.tmt[1][0]; mtm1 = b.tmt[1][1]; mte0 = b.tet[1][0]; mte1 = b.tet[1][1];
         } else {
@@ -215,7 +230,7 @@ __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict
             mtm0 = pt_tm0; mtm1 = pt_tm1; mte0 = pt_te0; mte1 = pt_te1;
         }
         Blk2 b;
-        far_or_zero<Q>(o, ee, ne, load_elem(p0, p1, f), f, next[f], k, active, b);
+        far_or_zero<Q, PLAIN>(o, ee, ne, load_elem(p0, p1, f), f, next[f], k, active, b);
         // direct tile: column f, rows e (own, local 0) and next(e) (local 1 -> lane + 1)
         const cplx ctm0 = cadd(ltm0, b.tm[0][0]), ctm1 = cadd(ltm1, b.tm[1][0]);
         const cplx cte0 = cadd(lte0, b.te[0][0]), cte1 = cadd(lte1, b.te[1][0]);
@@ -246,7 +261,7 @@ __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict
 //       Z[c, q] += blk(e', prev q).tm[1][1] + blk(e', q).tm[1][0]
 //   column c (entries written by a mirror tile,  block(q) >  block(c)):
 //       Z[q, c] += blk(e', prev q).tmt[1][1] + blk(e', q).tmt[0][1]
-template <int Q>
+template <int Q, bool PLAIN>
 __global__ void __launch_bounds__(64) far_fixup(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                 const int* __restrict__ next, const int* __restrict__ prev,
                                                 double k, const int* __restrict__ rows, double* __restrict__ ztm,
@@ -265,7 +280,7 @@ __global__ void __launch_bounds__(64) far_fixup(int n, const double* __restrict_
     const int qc = q < n ? q : n - 1;
     const int col = lane == 0 ? prev[qb] : qc;
     Blk2 b;
-    far_or_zero<Q>(o, e, ne, load_elem(p0, p1, col), col, next[col], k, lane == 0 || q < n, b);
+    far_or_zero<Q, PLAIN>(o, e, ne, load_elem(p0, p1, col), col, next[col], k, lane == 0 || q < n, b);
     cplx ltm, lte, lmt, lme;
     ltm.re = __shfl_up(b.tm[1][1].re, 1, 64); ltm.im = __shfl_up(b.tm[1][1].im, 1, 64);
     lte.re = __shfl_up(b.te[1][1].re, 1, 64); lte.im = __shfl_up(b.te[1][1].im, 1, 64);
@@ -274,7 +289,7 @@ __global__ void __launch_bounds__(64) far_fixup(int n, const double* __restrict_
     const int pq = prev[qc];
     if (lane >= 2 && q < n && pq != q - 1) {
         Blk2 b1;
-        far_or_zero<Q>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
+        far_or_zero<Q, PLAIN>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
         ltm = b1.tm[1][1]; lte = b1.te[1][1]; lmt = b1.tmt[1][1]; lme = b1.tet[1][1];
     }
     if (lane == 0 || q >= n) return;
@@ -679,12 +694,13 @@ __device__ __forceinline__ void add_entry(double* z, int64_t ld, int p, int q, c
 }
 
 // Row-owner gather of near blocks and mass terms: thread p adds everything of node row p.
+// With plain != 0 the outputs are S and K' + plain_mass M instead (no D, W or jump terms).
 __global__ void __launch_bounds__(256) near_gather(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                    const int* __restrict__ next, const int* __restrict__ prev,
                                                    const int* __restrict__ offsets, const int* __restrict__ pair_f,
-                                                   const double* __restrict__ blocks, double k,
-                                                   double* __restrict__ ztm, int64_t ldtm, double* __restrict__ zte,
-                                                   int64_t ldte) {
+                                                   const double* __restrict__ blocks, double k, int plain,
+                                                   double plain_mass, double* __restrict__ ztm, int64_t ldtm,
+                                                   double* __restrict__ zte, int64_t ldte) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const double g = -1.0 / k;
@@ -707,7 +723,15 @@ __global__ void __launch_bounds__(256) near_gather(int n, const double* __restri
                     D[a][b] = rev < 0 ? cplx{0.0, 0.0} : cplx{R[8 + 2 * (2 * b + a)], R[8 + 2 * (2 * b + a) + 1]};
                 }
             Blk blk;
-            combine(S, K, D, o, s, k, blk);
+            if (plain) {
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) {
+                        blk.tm[a][b] = S[a][b];
+                        blk.te[a][b] = K[a][b];
+                    }
+            } else {
+                combine(S, K, D, o, s, k, blk);
+            }
             const int cols[2] = {f, next[f]};
             for (int b = 0; b < 2; ++b) {
                 add_entry(ztm, ldtm, p, cols[b], blk.tm[loc][b]);
@@ -718,8 +742,12 @@ __global__ void __launch_bounds__(256) near_gather(int n, const double* __restri
         const int cols[2] = {e, next[e]};
         for (int b = 0; b < 2; ++b) {
             const double m = o.len * (loc == b ? 1.0 / 3.0 : 1.0 / 6.0);
-            add_entry(ztm, ldtm, p, cols[b], cplx{0.0, 0.5 * m * g});
-            add_entry(zte, ldte, p, cols[b], cplx{-0.5 * m, 0.0});
+            if (plain) {
+                add_entry(zte, ldte, p, cols[b], cplx{plain_mass * m, 0.0});
+            } else {
+                add_entry(ztm, ldtm, p, cols[b], cplx{0.0, 0.5 * m * g});
+                add_entry(zte, ldte, p, cols[b], cplx{-0.5 * m, 0.0});
+            }
         }
     }
 }
@@ -824,9 +852,10 @@ extern "C" hipError_t grim_upload_rules(const double* gl8t, const double* gl8w, 
 }
 
 // Far pass launcher: one wavefront per (row block I, column chunk j) with j in block J >= I.
-extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
-                                      const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
-                                      int64_t ldte, const int* fix_rows, int nfix, hipStream_t st) {
+template <bool PLAIN>
+static hipError_t launch_far(int q, int n, const double* p0, const double* p1, const int* next, const int* prev,
+                             double k, double* ztm, int64_t ldtm, double* zte, int64_t ldte, const int* fix_rows,
+                             int nfix, hipStream_t st) {
     // columns per wavefront: enough wavefronts (about 8-16 K) to fill 1024 SIMDs
     // with their tails overlapping; each chunk re-evaluates one column pair.
     int chunk = grim_shape_value(GRIM_SHAPE_FAR_COLUMNS, n);
@@ -837,27 +866,37 @@ extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const doub
     const int nc = (n + chunk - 1) / chunk, cpb = 64 / chunk, nb = (n + 63) / 64;
     long ntiles = 0;
     for (int I = 0; I < nb; ++I) ntiles += nc - I * cpb;
-    if (q == 8) far_kernel<8><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
-    else far_kernel<16><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+    if (q == 8)
+        far_kernel<8, PLAIN><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+    else
+        far_kernel<16, PLAIN><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || nfix == 0) return err;
     dim3 g2((n + 62) / 63, nfix);
-    if (q == 8) far_fixup<8><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
-    else far_fixup<16><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    if (q == 8) far_fixup<8, PLAIN><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    else far_fixup<16, PLAIN><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
     return hipGetLastError();
+}
+
+// plain = 0: the combined-potential pair (TM, TE); plain = 1: S and K' (far_pair).
+extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
+                                      const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
+                                      int64_t ldte, const int* fix_rows, int nfix, int plain, hipStream_t st) {
+    return plain ? launch_far<true>(q, n, p0, p1, next, prev, k, ztm, ldtm, zte, ldte, fix_rows, nfix, st)
+                 : launch_far<false>(q, n, p0, p1, next, prev, k, ztm, ldtm, zte, ldte, fix_rows, nfix, st);
 }
 
 extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1, const int* next, const int* prev,
                                        double k, int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
                                        double* blocks, int* kind, unsigned* status, double* ztm, int64_t ldtm,
-                                       double* zte, int64_t ldte, hipStream_t st) {
+                                       double* zte, int64_t ldte, int plain, double plain_mass, hipStream_t st) {
     near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 0, counts, offsets, capacity, pair_e, pair_f, status);
     scan_counts<<<1, 1024, 0, st>>>(n, counts, offsets);
     near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 1, counts, offsets, capacity, pair_e, pair_f, status);
     near_blocks<<<capacity, 64, 0, st>>>(n, p0, p1, next, offsets, pair_e, pair_f, k, blocks, kind, status);
     near_adaptive<<<capacity, 64, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, k, blocks, kind, status);
-    near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, blocks, k, ztm, ldtm, zte,
-                                                 ldte);
+    near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, blocks, k, plain,
+                                                 plain_mass, ztm, ldtm, zte, ldte);
     return hipGetLastError();
 }
 
@@ -1088,6 +1127,29 @@ extern "C" hipError_t grim_launch_robin_rhs(int n, int na, const double* p0, con
                                             const int* dirichlet, double* out, int64_t ld, hipStream_t st) {
     dim3 grid((n + 255) / 256, na);
     robin_rhs_kernel<<<grid, 256, 0, st>>>(n, na, p0, p1, prev, k, angles_deg, alpha_e, dirichlet, out, ld);
+    return hipGetLastError();
+}
+
+// z(block i, block j) *= scale[i + 2 j] over the four n x n blocks of a 2n x 2n matrix.
+__global__ void __launch_bounds__(256) scale_blocks_kernel(int64_t n, double* __restrict__ z, int64_t ld, double s00r,
+                                                           double s00i, double s10r, double s10i, double s01r,
+                                                           double s01i, double s11r, double s11i) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (r >= 2 * n || c >= 2 * n) return;
+    const bool lo_r = r < n, lo_c = c < n;
+    const double sr = lo_c ? (lo_r ? s00r : s10r) : (lo_r ? s01r : s11r);
+    const double si = lo_c ? (lo_r ? s00i : s10i) : (lo_r ? s01i : s11i);
+    double* t = z + 2 * (r + c * ld);
+    const double xr = t[0], xi = t[1];
+    t[0] = sr * xr - si * xi;
+    t[1] = sr * xi + si * xr;
+}
+
+extern "C" hipError_t grim_launch_scale_blocks(int64_t n, double* z, int64_t ld, const double* s, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    dim3 grid((unsigned)((2 * n + 255) / 256), (unsigned)(2 * n));
+    scale_blocks_kernel<<<grid, 256, 0, st>>>(n, z, ld, s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7]);
     return hipGetLastError();
 }
 

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import native
-from .mesh import EPS, ContourMesh
+from .mesh import EPS, ContourMesh, causal_index
 
 NEAR_PER_ELEMENT = 48
 REFINE_TOL = 2e-12           # linalg/refined_lu.py:54
@@ -225,6 +225,79 @@ def impedance_rhs(contour: DeviceContour, k: float, angles: torch.Tensor) -> tor
                                            dm.data_ptr(), out[i].data_ptr(), n, st)
         native.check("grim_robin_rhs", code)
     return out
+
+
+def dielectric_wavenumber(mesh: ContourMesh, k0: float) -> float:
+    """Interior wavenumber k0 n of a homogeneous dielectric body.  Only lossless positive-index media
+    are evaluated here: their kernels are the real-argument Hankel functions of the rest of the
+    assembly; a lossy or negative-index medium needs complex-argument Hankel kernels."""
+    flag, eps, mu = mesh.medium
+    index = causal_index(eps, mu)
+    tol = 64.0 * np.finfo(float).eps * max(1.0, abs(index))
+    if abs(index.imag) > tol or not index.real > 0.0:
+        raise ValueError(f"Dielectric flag {flag} (eps={eps!r}, mu={mu!r}) has refractive index {index!r}: lossy and "
+                         "negative-index media need complex-argument Hankel kernels, which the GPU path does not "
+                         "evaluate. Use a lossless medium with a positive index or the reference's CPU path.")
+    return float(k0 * index.real)
+
+
+def dielectric_systems(contour: DeviceContour, k0: float, workspace: torch.Tensor) -> torch.Tensor:
+    """[TE, TM] two-density systems (2, 2N, 2N) of a homogeneous dielectric body in air
+    (twod/formulations/dielectric.py:17-81):
+        [ eta Z_TM(k0)    -S(k1)            ]
+        [ -eta Z_TE(k0)   factor (K'(k1) + M/2) ]
+    with eta = j k0 (so that gamma = 1/eta in the PEC pair of k0), k1 = k0 n and factor =
+    1/eps (TE) or 1/mu (TM).  The four blocks are assembled in place (grim_assemble at k0 into the
+    left block column, grim_assemble_plain at k1 into the right one), the system is copied for the
+    second polarization, and grim_scale_blocks applies each polarization's block factors."""
+    flag, eps, mu = contour.mesh.medium
+    k1 = dielectric_wavenumber(contour.mesh, k0)
+    n, dev = contour.n, contour.device
+    ld = 2 * n
+    pair = torch.empty((2, ld, ld), dtype=torch.complex128, device=dev)
+    base = pair[0].data_ptr()
+
+    def block(r, c):          # column-major: entry (r, c) at r + c ld
+        return base + 16 * (r + c * ld)
+
+    st = _stream(dev)
+    lib = native.lib()
+    args = (n, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.next.data_ptr(), contour.prev.data_ptr(),
+            contour.fix_rows.data_ptr(), int(contour.fix_rows.numel()))
+    native.check("grim_assemble", lib.grim_assemble(*args, float(k0), contour.mesh.far_order(k0), block(0, 0), ld,
+                                                     block(n, 0), ld, workspace.data_ptr(), workspace.numel(),
+                                                     NEAR_PER_ELEMENT, st))
+    native.check("grim_assemble (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
+    native.check("grim_assemble_plain", lib.grim_assemble_plain(*args, float(k1), contour.mesh.far_order(k1),
+                                                                 block(0, n), ld, block(n, n), ld, 0.5,
+                                                                 workspace.data_ptr(), workspace.numel(),
+                                                                 NEAR_PER_ELEMENT, st))
+    native.check("grim_assemble_plain (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
+    pair[1].copy_(pair[0])
+    eta = 1j * float(k0)
+    one = 1.0 + 0.0j
+    for i, factor in enumerate((one / complex(eps), one / complex(mu))):
+        scale = np.array([eta, -eta, -1.0, factor], dtype=np.complex128).view(np.float64)
+        native.check("grim_scale_blocks", lib.grim_scale_blocks(n, pair[i].data_ptr(), ld,
+                                                                scale.ctypes.data, st))
+    return pair
+
+
+def dielectric_rhs(bu: torch.Tensor, bdn: torch.Tensor) -> torch.Tensor:
+    """Right-hand sides (2, A, 2N) [TE, TM] of dielectric_systems: [-b_u; b_dn]
+    (twod/formulations/dielectric.py:84-91)."""
+    both = torch.cat((-bu, bdn), dim=1)
+    return torch.stack((both, both))
+
+
+def radiating_density(contour: DeviceContour, k0: float, x: torch.Tensor) -> torch.Tensor:
+    """The single-layer density whose matched far field (b_u + gamma b_dn weights, gamma = 1/(j k0))
+    is the solution's: x for conductor contours; eta psi for a dielectric body, whose exterior
+    field (D + eta S) psi has DLP density psi and SLP density eta psi (the reference's DLP plus
+    second SLP potential, twod/solver.py:1672-1681)."""
+    if contour.mesh.medium is None:
+        return x
+    return (1j * float(k0) * x[..., :contour.n]).contiguous()
 
 
 def bistatic_farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):

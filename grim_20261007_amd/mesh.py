@@ -37,6 +37,7 @@ class ContourMesh:
     contour_starts: list = field(default_factory=list)
     impedance: np.ndarray | None = None   # (N,) complex surface impedance per element (0: PEC), or None
     ref_node: np.ndarray | None = None    # (N,) the reference's node number of node p (None: p itself)
+    medium: tuple | None = None           # (flag, eps, mu) of a dielectric body (TYPE 3), or None (conductor)
 
     @property
     def n(self) -> int:
@@ -255,18 +256,18 @@ class ImpedanceLaw:
         return _passive(self.z_start, f"IBC flag {flag} impedance")
 
 
-def _material_flag(token) -> int:
+def _material_flag(token, context: str = "IBC material definition flag") -> int:
     text = str(token).strip()
     if text.lower().startswith("mat."):
         text = text[4:]
     try:
         value = float(text)
     except (TypeError, ValueError):
-        raise ValueError(f"IBC material definition flag must be an integer; got {token!r}.") from None
+        raise ValueError(f"{context} must be an integer; got {token!r}.") from None
     if not math.isfinite(value) or not value.is_integer():
-        raise ValueError(f"IBC material definition flag must be an integer; got {token!r}.")
+        raise ValueError(f"{context} must be an integer; got {token!r}.")
     if value <= 0:
-        raise ValueError(f"IBC material definition flag must be a positive integer; got {token!r}.")
+        raise ValueError(f"{context} must be a positive integer; got {token!r}.")
     return int(value)
 
 
@@ -315,11 +316,11 @@ def _table_file(folder: str, flag: int, filename) -> str:
     return path
 
 
-def load_impedance_csv(path: str) -> ImpedanceLaw:
-    """``frequency_hz,resistance_ohm,reactance_ohm`` rows (UTF-8, optional BOM, # comments and blank
-    lines skipped, frequencies positive and unique, sorted) -> a table law
-    (twod/geometry.py:603-680)."""
+def _read_csv_rows(path: str, header_names) -> list:
+    """Numeric rows of a headered material CSV (UTF-8, optional BOM, # comments and blank lines
+    skipped, frequencies in Hz positive and unique), sorted by frequency (twod/geometry.py:603-658)."""
     import csv
+    names = list(header_names)
     if not str(path).lower().endswith(".csv"):
         raise ValueError(f"Material/IBC file must use the .csv extension: {path}")
     rows, header = [], False
@@ -333,13 +334,13 @@ def load_impedance_csv(path: str) -> ImpedanceLaw:
             except csv.Error as exc:
                 raise ValueError(f"{path}: line {lineno} contains invalid CSV: {exc}") from exc
             if not header:
-                if parts != IMPEDANCE_CSV_HEADER:
+                if parts != names:
                     raise ValueError(f"{path}: line {lineno} must have comma-separated header "
-                                     f"{','.join(IMPEDANCE_CSV_HEADER)} (frequency in Hz); found {','.join(parts)}.")
+                                     f"{','.join(names)} (frequency in Hz); found {','.join(parts)}.")
                 header = True
                 continue
-            if len(parts) != len(IMPEDANCE_CSV_HEADER):
-                raise ValueError(f"{path}: line {lineno} must contain exactly {len(IMPEDANCE_CSV_HEADER)} "
+            if len(parts) != len(names):
+                raise ValueError(f"{path}: line {lineno} must contain exactly {len(names)} "
                                  f"comma-separated columns; found {len(parts)}.")
             try:
                 values = [float(part) for part in parts]
@@ -351,13 +352,19 @@ def load_impedance_csv(path: str) -> ImpedanceLaw:
                 raise ValueError(f"{path}: line {lineno} has non-positive frequency {values[0]:g} Hz.")
             rows.append(values)
     if not header:
-        raise ValueError(f"{path}: missing required header {','.join(IMPEDANCE_CSV_HEADER)}.")
+        raise ValueError(f"{path}: missing required header {','.join(names)}.")
     if not rows:
         raise ValueError(f"{path}: no material data rows found after the header.")
     rows.sort(key=lambda r: r[0])
     for a, b in zip(rows, rows[1:]):
         if a[0] == b[0]:
             raise ValueError(f"{path}: duplicate frequency {b[0]:g} Hz.")
+    return rows
+
+
+def load_impedance_csv(path: str) -> ImpedanceLaw:
+    """``frequency_hz,resistance_ohm,reactance_ohm`` rows -> a table law (twod/geometry.py:661-680)."""
+    rows = _read_csv_rows(path, IMPEDANCE_CSV_HEADER)
     values = np.array([complex(r[1], r[2]) for r in rows], dtype=np.complex128)
     for i, v in enumerate(values, start=1):
         _passive(v, f"Impedance CSV '{path}' data row {i}")
@@ -401,6 +408,137 @@ def impedance_laws(snapshot: dict, base_dir: str | None = None) -> dict:
         z1 = z0 if kind == "constant" else _passive(complex(r1, x1), "taper z_end")
         laws[flag] = ImpedanceLaw(kind, z0, z1)
     return laws
+
+
+# ----------------------------------------------------------------------------
+# Dielectric media of the snapshot's ``dielectrics`` rows (twod/geometry.py:149-160, 236-258,
+# 314-367, 392-405, 530-564, 683-706, 2349-2394)
+# ----------------------------------------------------------------------------
+MEDIUM_CSV_HEADER = ["frequency_hz", "eps_real", "eps_imag", "mu_real", "mu_imag"]
+MATERIAL_SINGULAR_TOL = EPS
+
+
+def _finite(z: complex, context: str) -> complex:
+    z = complex(z)
+    if not (math.isfinite(z.real) and math.isfinite(z.imag)):
+        raise ValueError(f"{context} contains non-finite value {z!r}.")
+    return z
+
+
+def passive_medium(eps: complex, mu: complex, context: str):
+    """Finite, non-singular eps and mu with Im <= 0 up to 64 ulp (exp(+j omega t))."""
+    eps = _finite(eps, f"{context} epsilon")
+    mu = _finite(mu, f"{context} mu")
+    if abs(eps) <= MATERIAL_SINGULAR_TOL:
+        raise ValueError(f"{context} has unsupported singular/near-ENZ epsilon {eps!r} (|epsilon| <= "
+                         f"{MATERIAL_SINGULAR_TOL:g}); it will not be replaced with free space.")
+    if abs(mu) <= MATERIAL_SINGULAR_TOL:
+        raise ValueError(f"{context} has unsupported singular/near-MNZ mu {mu!r} (|mu| <= "
+                         f"{MATERIAL_SINGULAR_TOL:g}); it will not be replaced with free space.")
+    for name, sym, v in (("epsilon", "epsilon", eps), ("mu", "mu", mu)):
+        if v.imag > 64.0 * np.finfo(float).eps * max(1.0, abs(v)):
+            raise ValueError(f"{context} {name} has gain-sign Im({sym})={v.imag:g}. For e^(+j*omega*t), passive "
+                             f"media require Im({sym}) <= 0; active/gain media are not supported.")
+    return eps, mu
+
+
+def causal_index(eps: complex, mu: complex) -> complex:
+    """Refractive index sqrt(eps mu) on the passive branch: Im n <= 0, and on the lossless branch
+    the sign that gives Re(mu / n) >= 0."""
+    n = complex(np.sqrt(complex(eps) * complex(mu)))
+    tol = 64.0 * np.finfo(float).eps * max(1.0, abs(n))
+    if n.imag > tol:
+        n = -n
+    elif abs(n.imag) <= tol and (complex(mu) / n).real < 0.0:
+        n = -n
+    return n
+
+
+@dataclass
+class Medium:
+    """A dielectric flag: inline (eps, mu) or a frequency table of both."""
+    eps: complex = 1.0 + 0.0j
+    mu: complex = 1.0 + 0.0j
+    freqs_ghz: np.ndarray | None = None
+    eps_values: np.ndarray | None = None
+    mu_values: np.ndarray | None = None
+
+    def sample(self, flag: int, frequency_ghz: float):
+        if self.freqs_ghz is None:
+            return passive_medium(self.eps, self.mu, f"Dielectric flag {flag}")
+        table = ImpedanceLaw("table", freqs_ghz=self.freqs_ghz, values=self.eps_values)
+        eps = table._sample(frequency_ghz)
+        mu = ImpedanceLaw("table", freqs_ghz=self.freqs_ghz, values=self.mu_values)._sample(frequency_ghz)
+        return passive_medium(eps, mu, f"Dielectric flag {flag} sampled at {frequency_ghz:g} GHz")
+
+
+def dielectric_media(snapshot: dict, base_dir: str | None = None) -> dict:
+    """flag -> Medium for every ``dielectrics`` row: ``[flag, eps_real, eps_imag, mu_real, mu_imag]``
+    or ``[flag, "file.csv"]`` (``frequency_hz,eps_real,eps_imag,mu_real,mu_imag``)."""
+    media = {}
+    folder = None
+    for row in snapshot.get("dielectrics", []) or []:
+        if not row:
+            continue
+        flag = _material_flag(row[0], "Dielectric material definition flag")
+        if flag in media:
+            raise ValueError(f"Duplicate dielectric material flag {flag}.")
+        if len(row) == 2:
+            if not str(row[1]).strip().lower().endswith(".csv"):
+                raise ValueError(f"Dielectric flag {flag}: inline material requires exactly 'flag eps_real eps_imag "
+                                 "mu_real mu_imag'; file-backed definitions require 'flag filename.csv'.")
+            folder = folder or material_base_dir(snapshot, base_dir)
+            path = _table_file(folder, flag, row[1])
+            rows = _read_csv_rows(path, MEDIUM_CSV_HEADER)
+            eps = np.array([complex(r[1], r[2]) for r in rows])
+            mu = np.array([complex(r[3], r[4]) for r in rows])
+            for i, (e, m) in enumerate(zip(eps, mu), start=1):
+                passive_medium(e, m, f"Dielectric CSV '{path}' data row {i}")
+            media[flag] = Medium(freqs_ghz=np.array([r[0] / 1.0e9 for r in rows]), eps_values=eps, mu_values=mu)
+            continue
+        if len(row) != 5 or any(str(t).strip() == "" for t in row):
+            raise ValueError(f"Dielectric flag {flag}: inline material requires exactly 'flag eps_real eps_imag "
+                             f"mu_real mu_imag' with no blank fields (got {len(row)} fields).")
+        er = _material_float(row[1], f"Dielectric flag {flag} epsilon real part")
+        ei = _material_float(row[2], f"Dielectric flag {flag} epsilon imaginary part")
+        mr = _material_float(row[3], f"Dielectric flag {flag} mu real part")
+        mi = _material_float(row[4], f"Dielectric flag {flag} mu imaginary part")
+        eps = _finite(complex(er, ei), f"Dielectric flag {flag} epsilon")
+        mu = _finite(complex(mr, mi), f"Dielectric flag {flag} mu")
+        eps, mu = passive_medium(eps, mu, f"Dielectric flag {flag}")
+        media[flag] = Medium(eps, mu)
+    return media
+
+
+def medium_of(media: dict, flag: int, frequency_ghz: float):
+    if flag <= 0:
+        return 1.0 + 0.0j, 1.0 + 0.0j
+    if flag not in media:
+        raise ValueError(f"Undefined dielectric flag {flag}.")
+    return media[flag].sample(flag, frequency_ghz)
+
+
+def material_wavelength(segments, media: dict, frequency_ghz: float) -> float:
+    """Shortest wavelength the mesh of ``segments`` must resolve at one frequency: air's, divided
+    by the largest |n| of the dielectric flags TYPE 3/4 (pos_mat) and TYPE 5 (both) segments
+    reference (twod/geometry.py:2394-2452)."""
+    flags = set()
+    for seg in segments:
+        props = list(seg.get("properties", []) or []) + [""] * 5
+        seg_type = _parse_int(props[0]) if str(props[0]).strip() else int(seg.get("seg_type", 0))
+        pos, neg = _parse_int(props[3]), _parse_int(props[4])
+        if seg_type in (3, 4) and pos > 0:
+            flags.add(pos)
+        elif seg_type == 5:
+            flags.update(f for f in (pos, neg) if f > 0)
+    index = 1.0
+    for flag in sorted(flags):
+        value = abs(causal_index(*medium_of(media, flag, frequency_ghz)))
+        if not math.isfinite(value) or value <= 0.0:
+            raise ValueError(f"Dielectric flag {flag} produced invalid refractive-index magnitude {value!r} at "
+                             f"{frequency_ghz:g} GHz.")
+        index = max(index, value)
+    return C0 / (float(frequency_ghz) * 1.0e9) / index
 
 
 # ----------------------------------------------------------------------------
@@ -598,7 +736,6 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     certification pair: its chains get ``refined_chain_counts`` of the base counts
     (twod/geometry.py:1788-1850)."""
     scale = unit_scale(units)
-    wavelength0 = C0 / (float(frequency_ghz) * 1e9)
     law_ghz = float(frequency_ghz if law_ghz is None else law_ghz)
     freqs = [float(frequency_ghz)] if mesh_frequencies is None else sorted({float(f) for f in mesh_frequencies})
     segments = list(snapshot.get("segments", []) or [])
@@ -612,13 +749,28 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
         raise ValueError("Internal 2-D certification refinement metadata does not match the geometry "
                          "segment count.")
     laws = impedance_laws(snapshot, material_base_dir)
+    media = dielectric_media(snapshot, material_base_dir)
+    wavelength0 = min(material_wavelength(segments, media, f) for f in freqs)
     welder, junctions = boundary_condition_junctions(segments, scale, laws, freqs)
     panels0, panels1, zs = [], [], []
+    kinds, dielectric_flags = set(), set()
     for s, seg in enumerate(segments):
         seg_type, n_prop, ibc = _parse_props(seg)
         name = str(seg.get("name", "segment"))
-        if seg_type != 2:
-            raise ValueError(f"Segment '{name}': only conductor (TYPE 2) contours are supported by the GPU path.")
+        if seg_type not in (2, 3):
+            raise ValueError(f"Segment '{name}': only conductor (TYPE 2) contours and dielectric (TYPE 3) "
+                             "interfaces with air are supported by the GPU path.")
+        kinds.add(seg_type)
+        # an explicit count is held to the media on the segment's own sides
+        local_wavelength = min(material_wavelength([seg], media, f) for f in freqs)
+        if seg_type == 3:
+            pos_mat = _parse_int((list(seg.get("properties", []) or []) + [0] * 5)[3])
+            if pos_mat <= 0:
+                raise ValueError(f"TYPE 3 panel '{name}' requires pos_mat > 0.")
+            if ibc > 0:
+                raise ValueError(f"Segment '{name}': an impedance sheet on a dielectric interface is not supported by "
+                                 "the GPU path.")
+            dielectric_flags.add(pos_mat)
         law = None
         if ibc > 0:
             if ibc not in laws:
@@ -652,10 +804,10 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
         seg_start = len(panels0)
         for (a, b, length), count in zip(ends, counts):
             if n_prop > 0:
-                minimum = max(1, int(math.ceil(length * MIN_EXPLICIT_PANELS_PER_WAVELENGTH / wavelength0)))
+                minimum = max(1, int(math.ceil(length * MIN_EXPLICIT_PANELS_PER_WAVELENGTH / local_wavelength)))
                 if count < minimum:
                     raise ValueError(f"Segment '{name}' explicit N={n_prop} under-resolves a {length:.6g} m "
-                                     f"primitive at wavelength {wavelength0:.6g} m: at least N={minimum} is "
+                                     f"primitive at wavelength {local_wavelength:.6g} m: at least N={minimum} is "
                                      "required.")
             if len(panels0) + count > max_panels:
                 raise ValueError("Discretization exceeds the configured panel limit.")
@@ -722,9 +874,16 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     prev[nxt] = np.arange(n, dtype=np.int32)
     starts = [int(e) for e in range(n) if prev[e] != e - 1]
     impedance = np.array(zs, complex)
+    medium = None
+    if 3 in kinds:
+        if kinds != {3} or len(dielectric_flags) != 1:
+            raise ValueError("Multi-region layouts (conductors beside or inside dielectrics, several dielectric "
+                             "media) are not supported by the GPU path: one homogeneous dielectric body in air is.")
+        flag = dielectric_flags.pop()
+        medium = (flag,) + tuple(medium_of(media, flag, law_ghz))
     mesh = ContourMesh(np.array(nodes)[first], p0, p1, nxt, prev, starts,
                        impedance if np.any(np.abs(impedance) > EPS) else None,
-                       None if np.array_equal(first, np.arange(n)) else first.copy())
+                       None if np.array_equal(first, np.arange(n)) else first.copy(), medium)
     _check_orientation(mesh)
     return mesh
 

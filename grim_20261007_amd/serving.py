@@ -64,6 +64,8 @@ def geometry_digest(mesh: ContourMesh) -> str:
         h.update(np.ascontiguousarray(a).tobytes())
     if mesh.impedance is not None:
         h.update(np.ascontiguousarray(mesh.impedance).tobytes())
+    if mesh.medium is not None:
+        h.update(np.array([complex(mesh.medium[1]), complex(mesh.medium[2])]).tobytes())
     return h.hexdigest()[:24]
 
 
@@ -221,7 +223,8 @@ class Server:
             if self.cache.put(key, item, nb) and done is not None:
                 done.set()
 
-        nbytes = request_bytes(mesh.n, len(req.angles_deg), len(req.pols), req.precision, cached is not None)
+        dofs = 2 * mesh.n if mesh.medium is not None else mesh.n
+        nbytes = request_bytes(dofs, len(req.angles_deg), len(req.pols), req.precision, cached is not None)
         try:
             self._admit(nbytes)
             try:
@@ -240,21 +243,25 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     solve and far field.  `store(item, nbytes)` receives the assembled + factored systems."""
     mesh = req.mesh
     if req.precision == "auto":
-        req.precision = choose_precision(mesh.n, len(req.angles_deg))
+        req.precision = choose_precision(2 * mesh.n if mesh.medium is not None else mesh.n, len(req.angles_deg))
     k = wavenumber(req.frequency_ghz)
     started = time.perf_counter()
     dev = torch.device(device)
     contour = cached["contour"] if cached else engine.DeviceContour(mesh, dev)
+    dofs = 2 * mesh.n if mesh.medium is not None else mesh.n
     if cached is None:
         if mesh.n not in ws:
             ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
-        z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
-        pair = z_tm._base                                         # batch [TE, TM]
-        if mesh.impedance is not None:
-            engine.impedance_systems(contour, k, pair, ws[mesh.n])
+        if mesh.medium is not None:
+            pair = engine.dielectric_systems(contour, k, ws[mesh.n])
+        else:
+            z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
+            pair = z_tm._base                                     # batch [TE, TM]
+            if mesh.impedance is not None:
+                engine.impedance_systems(contour, k, pair, ws[mesh.n])
         fac = engine.factor(pair, req.precision)
         if store is not None:
-            sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * mesh.n * mesh.n
+            sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * dofs * dofs
             store({"contour": contour, "factor": fac}, sysbytes)
     else:
         fac = cached["factor"]
@@ -265,20 +272,24 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
             t.record_stream(cur)
     angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, angles)
-    rhs = (torch.stack((-bdn, -bu)) if mesh.impedance is None     # [TE, TM]
-           else engine.impedance_rhs(contour, k, angles))
+    if mesh.medium is not None:
+        rhs = engine.dielectric_rhs(bu, bdn)
+    elif mesh.impedance is not None:
+        rhs = engine.impedance_rhs(contour, k, angles)
+    else:
+        rhs = torch.stack((-bdn, -bu))                            # [TE, TM]
     x, inf = engine.solve(fac, rhs, gate=(req.precision == "double"))
     if inf.get("factor") is not None and store is not None:
         # the mixed factorization fell back to FP64: later requests reuse the FP64 factors
         fac = inf.pop("factor")
-        store({"contour": contour, "factor": fac}, 2 * 32 * mesh.n * mesh.n)
+        store({"contour": contour, "factor": fac}, 2 * 32 * dofs * dofs)
     rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
     amp, sig, resid, info = {}, {}, {}, {}
     pending = []
     for i, pol in enumerate(("TE", "TM")):
         if pol not in req.pols:
             continue
-        a, s = engine.farfield(x[i], bu, bdn, k)
+        a, s = engine.farfield(engine.radiating_density(contour, k, x[i]), bu, bdn, k)
         pending.append((pol, a, s, rel[i]))
         # per result: refinement steps and error, the normwise backward error (the FP64 gate's
         # metric) and the precision the factor ended in

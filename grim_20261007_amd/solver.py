@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from . import engine
-from .mesh import C0, build_mesh, unit_scale
+from .mesh import C0, build_mesh, causal_index, unit_scale
 
 RCS_DB_FLOOR = 1e-12
 AMPLITUDE_CONVENTION = "A_physical_asymptotic = +j * B_stored"
@@ -34,6 +34,7 @@ MAX_PANELS_DEFAULT = 20_000
 MEMORY_LIMIT_FRACTION = 0.9        # twod/solver.py:1416
 FORMULATION = "2D combined regional potential S + gamma D (linear Galerkin)"
 DENSITY_FORMULATION = ("Multi-region indirect (exterior potential density; closed contours include a DLP term)")
+DIELECTRIC_DENSITY_FORMULATION = "Indirect dielectric (combined DLP + i*k0*SLP density)"
 SOLVER_BACKEND = "grim_20261007_amd (HIP, gfx950)"
 # twod/solver.py:953-1080
 DEFAULT_QUALITY_THRESHOLDS = {"residual_norm_max": 1.0e-6, "constraint_residual_norm_max": 1.0e-8,
@@ -210,16 +211,26 @@ def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, m
                                 law_ghz=float(f), mesh_frequencies=served)
 
 
-def _systems(contour, k, z_te, z_tm, bu, bdn, angles, workspace):
-    """[TE, TM] systems (2, N, N) and right-hand sides (2, A, N): the PEC combined potential
-    (TM Dirichlet rows -b_u, TE flux rows -b_dn), or, for contours with surface impedances,
-    the flux rows dn(u) + alpha u of both polarizations (engine.impedance_systems, which reads
-    the near blocks the assembly left in ``workspace``)."""
-    pair = z_tm._base if z_tm._base is not None else torch.stack((z_te, z_tm))   # [TE, TM]
+def system_dofs(mesh) -> int:
+    """Unknowns of the mesh's system: one per node, two (psi, phi) for a dielectric body."""
+    return 2 * mesh.n if mesh.medium is not None else mesh.n
+
+
+def _systems(contour, k, angles, workspace):
+    """[TE, TM] systems, right-hand sides (2, A, DOFs) and the plain loads b_u, b_dn (A, N):
+    the PEC combined potential (TM Dirichlet rows -b_u, TE flux rows -b_dn); for contours with
+    surface impedances the flux rows dn(u) + alpha u of both polarizations
+    (engine.impedance_systems, which reads the near blocks the assembly left in ``workspace``);
+    for a dielectric body the two-density system (engine.dielectric_systems)."""
+    bu, bdn = engine.loads(contour, k, angles)
+    if contour.mesh.medium is not None:
+        return engine.dielectric_systems(contour, k, workspace), engine.dielectric_rhs(bu, bdn), bu, bdn
+    z_tm, z_te = engine.assemble(contour, k, workspace)
+    pair = z_tm._base                                                      # [TE, TM]
     if contour.mesh.impedance is None:
-        return pair, torch.stack((-bdn, -bu))
-    return (engine.impedance_systems(contour, k, pair, workspace),
-            engine.impedance_rhs(contour, k, angles))
+        return pair, torch.stack((-bdn, -bu)), bu, bdn
+    return (engine.impedance_systems(contour, k, pair, workspace), engine.impedance_rhs(contour, k, angles),
+            bu, bdn)
 
 
 def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("TE", "TM"),
@@ -232,16 +243,15 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     dev = _device(device)
     k = wavenumber(frequency_ghz)
     order = [p for p in ("TE", "TM") if p in pols]
-    admit_device_memory(mesh.n, len(angles_deg), len(order), precision, dev, context, compute_condition_number)
+    admit_device_memory(system_dofs(mesh), len(angles_deg), len(order), precision, dev, context,
+                        compute_condition_number)
     contour = engine.DeviceContour(mesh, dev)
     ws = engine.assembly_workspace(mesh.n, dev)
-    z_tm, z_te = engine.assemble(contour, k, ws)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
-    bu, bdn = engine.loads(contour, k, angles)
+    pair, rhs_all, bu, bdn = _systems(contour, k, angles, ws)
+    del ws
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
-    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn, angles, ws)
-    del ws
     sel = [("TE", "TM").index(p) for p in order]
     systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
     rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
@@ -258,7 +268,7 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     out = {}
     for i, pol in enumerate(order):
         part = engine.system(final, i) if final.parts is not None else final
-        amp, sigma = engine.farfield(x[i], bu, bdn, k)
+        amp, sigma = engine.farfield(engine.radiating_density(contour, k, x[i]), bu, bdn, k)
         res = {"amplitude": amp.cpu().numpy(), "sigma": sigma.cpu().numpy(),
                "residual": rel[i].cpu().numpy(),
                "info": {kk: v for kk, v in info.items() if kk not in ("residual", "factor")},
@@ -268,7 +278,7 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
             res["condition"] = float(conds[i])
             res["condition_method"] = methods[i]
         if return_density:
-            res["density"] = x[i].cpu().numpy().T      # (N, A) nodal densities
+            res["density"] = x[i, :, :mesh.n].cpu().numpy().T      # (N, A) nodal (exterior) densities
         out[pol] = res
     return out
 
@@ -490,18 +500,16 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
     dev = _device(device)
     k = wavenumber(frequency_ghz)
     order = [p for p in ("TE", "TM") if p in pols]
-    admit_device_memory(mesh.n, len(incidence_deg) + len(observation_deg), len(order), precision, dev,
+    admit_device_memory(system_dofs(mesh), len(incidence_deg) + len(observation_deg), len(order), precision, dev,
                         "The bistatic 2-D solve", compute_condition_number)
     contour = engine.DeviceContour(mesh, dev)
     ws = engine.assembly_workspace(mesh.n, dev)
-    z_tm, z_te = engine.assemble(contour, k, ws)
     inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)
     obs = torch.tensor(observation_deg, dtype=torch.float64, device=dev)
-    bu, bdn = engine.loads(contour, k, inc)
+    pair, rhs_all, bu, bdn = _systems(contour, k, inc, ws)
+    del ws
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
-    pair, rhs_all = _systems(contour, k, z_te, z_tm, bu, bdn, inc, ws)
-    del ws
     sel = [("TE", "TM").index(p) for p in order]
     systems = pair if len(order) == 2 else pair[sel[0]].unsqueeze(0)
     rhs = rhs_all if len(order) == 2 else rhs_all[sel[0]].unsqueeze(0)
@@ -516,7 +524,7 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
     rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
     final = info.get("factor", fac)
     bu_o, bdn_o = engine.loads(contour, k, obs)
-    amp, sigma = engine.bistatic_farfield(x, bu_o, bdn_o, k)
+    amp, sigma = engine.bistatic_farfield(engine.radiating_density(contour, k, x), bu_o, bdn_o, k)
     out = {}
     for i, pol in enumerate(order):
         part = engine.system(final, i) if final.parts is not None else final
@@ -678,6 +686,7 @@ def compute_boundary_densities(geometry_snapshot: Dict[str, Any], frequency_ghz:
                           context="Boundary-density diagnostics")[pol]
     check_abort()
     nodal = res["density"][:, 0]
+    index = 1.0 if mesh.medium is None else max(1.0, abs(causal_index(mesh.medium[1], mesh.medium[2])))
     ids = mesh.elements
     density = 0.5 * (nodal[ids[:, 0]] + nodal[ids[:, 1]])      # linear basis at t = 1/2
     centers = 0.5 * (mesh.p0 + mesh.p1)
@@ -688,15 +697,15 @@ def compute_boundary_densities(geometry_snapshot: Dict[str, Any], frequency_ghz:
         "interpretation": ("Formulation-specific SLP/DLP representation density; do not interpret as "
                            "electric or magnetic surface current without a formulation- and "
                            "polarization-specific trace conversion."),
-        "formulation": DENSITY_FORMULATION,
+        "formulation": DENSITY_FORMULATION if mesh.medium is None else DIELECTRIC_DENSITY_FORMULATION,
         "frequency_ghz": frequency_ghz,
         "elevation_deg": float(elevation_deg),
         "polarization": pol,
         "coordinate_units": "meters",
         "length_units": "meters",
-        "mesh_wavelength_m": float(C0 / (mesh_ghz * 1e9)),
-        "mesh_max_refractive_index": 1.0,
-        "mesh_material_flags": [],
+        "mesh_wavelength_m": float(C0 / (mesh_ghz * 1e9) / index),
+        "mesh_max_refractive_index": float(index),
+        "mesh_material_flags": [] if mesh.medium is None else [int(mesh.medium[0])],
         "element_count": int(mesh.n),
         "node_count": int(mesh.n),
         "centers_x": centers[:, 0].tolist(),

@@ -83,6 +83,24 @@ int grim_assemble_near(int64_t n, const double* p0, const double* p1, const int3
                        double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
                        int64_t near_per_element, void* stream);
 
+/* S and K' (observation-normal derivative) + kp_mass M of one region's wavenumber, from the
+ * same traversal and near rules as grim_assemble (no D, W or jump terms): the interior
+ * operators of a dielectric body, twod/formulations/dielectric.py:64-69
+ * (_assemble_linear_operator_matrices with single- and double-layer destinations), kp_mass =
+ * 1/2 there.  Same arguments as grim_assemble, outputs s and kp (complex128 n x n, column-major;
+ * they may be blocks of a larger matrix through ld_s, ld_kp). */
+int grim_assemble_plain(int64_t n, const double* p0, const double* p1, const int32_t* next, const int32_t* prev,
+                        const int32_t* fix_rows, int64_t nfix, double k, int far_order, double* s, int64_t ld_s,
+                        double* kp, int64_t ld_kp, double kp_mass, void* workspace, int64_t workspace_bytes,
+                        int64_t near_per_element, void* stream);
+
+/* Scales the four n x n blocks of a complex128 2n x 2n column-major matrix in place:
+ * block (i, j) (rows i n .. i n + n - 1, columns j n ..) by scale[i + 2 j] (complex128 host
+ * array of 4).  Forms the two-density dielectric system from its assembled operators: eta Z_TM,
+ * -eta Z_TE (exterior, eta = j k0), -S1 and factor (K'1 + M/2) (interior), as
+ * twod/formulations/dielectric.py:17-81 scales them. */
+int grim_scale_blocks(int64_t n, double* z, int64_t ld, const double* scale, void* stream);
+
 /* Synchronises `stream` and returns the kernel status of the last
  * grim_assemble / grim_lu_factor that used `workspace` (0 ok, else a status
  * code as above). */

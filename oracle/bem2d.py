@@ -789,6 +789,53 @@ def robin_system(mesh: Mesh, k: float, element_impedance, pol: str, z_tm, z_te, 
     return flux, load
 
 
+def plain_operators(mesh: Mesh, k: float, rows_per_chunk: int = 16):
+    """S and K' (observation-normal derivative), node order, no mass terms, with the far and near
+    rules of ``assemble`` (twod/operators.py:2087-2607 with compute_double_layer and
+    obs_normal_deriv, as the interior traversal of twod/formulations/dielectric.py:64-66 asks)."""
+    E, N = len(mesh.length), len(mesh.nodes)
+    s_out = np.zeros((N, N), complex)
+    k_out = np.zeros((N, N), complex)
+    ids = mesh.elements
+    q = far_order(mesh, k)
+    for start in range(0, E, rows_per_chunk):
+        rows = np.arange(start, min(E, start + rows_per_chunk))
+        far = far_mask(mesh, rows)
+        sb, kb, _ = _far_rows(mesh, k, rows, far, q)
+        for a in range(2):
+            for b in range(2):
+                r, c = ids[rows, a][:, None], ids[None, :, b]
+                np.add.at(s_out, (r, c), sb[:, :, a, b])
+                np.add.at(k_out, (r, c), kb[:, :, a, b])
+    for e, f in np.argwhere(~far_mask(mesh)):
+        sb, kb = near_blocks(mesh, k, int(e), int(f))
+        for a in range(2):
+            for b in range(2):
+                s_out[ids[e, a], ids[f, b]] += sb[a, b]
+                k_out[ids[e, a], ids[f, b]] += kb[a, b]
+    return s_out, k_out
+
+
+def dielectric_system(mesh: Mesh, k0: float, z_tm0, z_te0, s1, kp1, factor, angles_deg):
+    """Two-density system of a homogeneous dielectric body in air (twod/formulations/dielectric.py:
+    17-91): exterior field (D + eta S) psi with eta = j k0, interior field S1 phi;
+        [ eta S0 + D0 + M/2         -S1              ] [psi]   [-b_u ]
+        [ -eta K'0 + W0 + eta M/2    factor (K'1 + M/2)] [phi] = [ b_dn]
+    factor = mu_air/mu (TM) or eps_air/eps (TE).  With gamma = 1/eta the exterior blocks are
+    eta Z_TM(k0) and -eta Z_TE(k0) of the PEC pair.  Returns (system, rhs); the far amplitude is
+    farfield(eta psi)."""
+    eta = 1j * k0
+    n = len(mesh.nodes)
+    m = weighted_mass(mesh, np.ones(len(mesh.length)))
+    z = np.zeros((2 * n, 2 * n), complex)
+    z[:n, :n] = eta * z_tm0
+    z[n:, :n] = -eta * z_te0
+    z[:n, n:] = -s1
+    z[n:, n:] = factor * (kp1 + 0.5 * m)
+    bu, bdn = incident_loads(mesh, k0, angles_deg)
+    return z, np.concatenate((-bu, bdn), axis=0)
+
+
 # ----------------------------------------------------------------------------
 # Condition estimate
 # ----------------------------------------------------------------------------

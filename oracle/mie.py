@@ -22,6 +22,15 @@ Pinned in tests/test_oracle_mie.py, without running the reference's code:
 * the reference-written golden amplitudes of the 64-chord ka = 1 and the
   400-chord ka = 10 circles (tests/golden), which agree with the series to
   the discretisation error (1.7e-3 and 4.3e-4 relative).
+
+``dielectric_backscatter_amplitude`` is the homogeneous dielectric cylinder's
+series (validation/cylinder.py:133-191): the interior field J_n(k1 r) matched to
+the exterior J_n + a_n H_n^(2) by continuity of u and of (1/zeta) du/dr, zeta =
+mu_r (TM) or eps_r (TE).  Pinned in tests/test_dielectric_cpu.py: eps = mu = 1
+gives a_n = 0 exactly (no scatterer); eps -> infinity in TM (zeta = mu fixed)
+tends to the PEC Dirichlet series; lossless media give |1 + 2 a_n| = 1; and the
+reference-written dielectric golden amplitudes agree with the series to the
+discretisation error.
 """
 import math
 
@@ -56,3 +65,23 @@ def pec_backscatter_amplitude(radius: float, k: float, pol: str) -> complex:
 
 def pec_sigma(radius: float, k: float, pol: str) -> float:
     return abs(pec_backscatter_amplitude(radius, k, pol)) ** 2 / (4.0 * k)
+
+
+def dielectric_coefficients(ka: float, eps_r: complex, mu_r: complex, pol: str):
+    """(n, a_n) of a homogeneous cylinder of relative (eps_r, mu_r) at k0 a = ka."""
+    index = np.sqrt(complex(eps_r) * complex(mu_r))
+    mx = index * ka
+    m = terms_for_ka(max(abs(ka), abs(mx)))
+    n = np.arange(-m, m + 1)
+    zeta = complex(mu_r) if pol.upper() == "TM" else complex(eps_r)
+    if pol.upper() not in ("TM", "TE"):
+        raise ValueError(pol)
+    p, q = index / zeta, 1.0           # k1 / zeta and k0 / 1, both divided by k0
+    num = -(p * sp.jvp(n, mx, 1) * sp.jv(n, ka) - q * sp.jv(n, mx) * sp.jvp(n, ka, 1))
+    den = p * sp.jvp(n, mx, 1) * sp.hankel2(n, ka) - q * sp.jv(n, mx) * sp.h2vp(n, ka, 1)
+    return n, num / den
+
+
+def dielectric_backscatter_amplitude(radius: float, k: float, eps_r: complex, mu_r: complex, pol: str) -> complex:
+    n, a = dielectric_coefficients(k * radius, eps_r, mu_r, pol)
+    return complex(-4.0j * np.sum(a * (-1.0) ** n))

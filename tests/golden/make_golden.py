@@ -16,7 +16,8 @@ inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
 Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
-                                               --ibc-zones | --condition]
+                                               --ibc-zones | --dielectric |
+                                               --condition]
 """
 from __future__ import annotations
 
@@ -507,6 +508,33 @@ def main_ibc_zones():
                   fn, [0.0, 30.0, 180.0, 290.0], material_base_dir=str(HERE))
 
 
+def _dielectric_snapshot(contours, media):
+    """contours: list of (points, N, dielectric flag); media: {flag: (eps, mu)} inline rows."""
+    segments = []
+    for i, (pts, n_prop, flag) in enumerate(contours):
+        seg = workloads._snapshot(pts, f"d{i}")["segments"][0]
+        seg["seg_type"] = 3
+        seg["properties"] = ["3", str(n_prop), "0", str(flag), "0"]
+        segments.append(seg)
+    rows = [[str(flag), repr(e.real), repr(e.imag), repr(m.real), repr(m.imag)] for flag, (e, m) in media.items()]
+    return {"segments": segments, "ibcs": [], "dielectrics": rows}
+
+
+def main_dielectric():
+    """Homogeneous lossless dielectric bodies in air: the reference's two-density
+    combined-potential system (twod/formulations/dielectric.py)."""
+    r = 0.1
+    run_zone_case("diel_circle48_er4", _dielectric_snapshot([(workloads.circle_points(r, 48), 1, 1)],
+                                                            {1: (4.0 + 0j, 1.0 + 0j)}),
+                  workloads.frequency_for_ka(1.0, r), [0.0, 90.0, 200.0], keep_matrix=True)
+    run_zone_case("diel_circle64_magnetic", _dielectric_snapshot([(workloads.circle_points(r, 64), 1, 2)],
+                                                                 {2: (2.5 + 0j, 1.6 + 0j)}),
+                  workloads.frequency_for_ka(1.5, r), [0.0, 45.0, 180.0, 300.0])
+    pts = workloads.naca0012_points(40)
+    run_zone_case("diel_naca80_er3", _dielectric_snapshot([(pts, 1, 1)], {1: (3.0 + 0j, 1.0 + 0j)}),
+                  workloads.frequency_for_segments_per_wavelength(pts, 10.0), [0.0, 30.0, 180.0])
+
+
 def run_condition_case(name, points, frequency_ghz, angles):
     """The reference's single-polarization entry (twod/solver.py:1984-2712) with
     compute_condition_number=True in both LU precisions: per channel the condition estimate and
@@ -580,6 +608,8 @@ if __name__ == "__main__":
         main_grim()
     elif "--bistatic" in sys.argv:
         main_bistatic()
+    elif "--dielectric" in sys.argv:
+        main_dielectric()
     elif "--ibc-zones" in sys.argv:
         main_ibc_zones()
     elif "--ibc" in sys.argv:
