@@ -12,11 +12,12 @@ extern "C" hipError_t grim_upload_rules(const double*, const double*, const doub
                                         const double*, const double*);
 extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
                                       const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
-                                      int64_t ldte, const int* fix_rows, int nfix, int plain, hipStream_t st);
+                                      int64_t ldte, const int* fix_rows, int nfix, int mode, hipStream_t st);
 extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1, const int* next, const int* prev,
                                        double k, int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
                                        double* blocks, int* kind, unsigned* status, double* ztm, int64_t ldtm,
-                                       double* zte, int64_t ldte, int plain, double plain_mass, hipStream_t st);
+                                       double* zte, int64_t ldte, int mode, double mass, const double* mass_coef,
+                                       hipStream_t st);
 extern "C" hipError_t grim_launch_loads(int n, int na, const double* p0, const double* p1, const int* prev, double k,
                                         const double* angles_deg, double* bu, double* bdn, int64_t ld,
                                         hipStream_t st);
@@ -34,6 +35,9 @@ extern "C" hipError_t grim_launch_robin_split(int q, int n, const double* p0, co
 extern "C" hipError_t grim_launch_robin_rhs(int n, int na, const double* p0, const double* p1, const int* prev,
                                             double k, const double* angles_deg, const double* alpha_e,
                                             const int* dirichlet, double* out, int64_t ld, hipStream_t st);
+extern "C" hipError_t grim_launch_route(int m, int n, const double* src, int64_t ld_src, const int* rows_src,
+                                       const int* cols_src, double* dst, int64_t ld_dst, const int* rows_dst,
+                                       const int* cols_dst, const double* w, hipStream_t st);
 extern "C" hipError_t grim_launch_scale_blocks(int64_t n, double* z, int64_t ld, const double* s, hipStream_t st);
 extern "C" int64_t grim_dense_aux_bytes(int n, int single);
 extern "C" hipError_t grim_dense_lu_factor(int batch, int n, void* a, int64_t lda, int64_t sa, int single, int* piv,
@@ -267,16 +271,17 @@ static bool assemble_args_ok(int64_t n, const double* p0, const double* p1, cons
 static int assemble_launch(int passes, int64_t n, const double* p0, const double* p1, const int32_t* next,
                            const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
                            double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, const AssemblyWork& w,
-                           int64_t cap, hipStream_t st, int plain = 0, double plain_mass = 0.0) {
+                           int64_t cap, hipStream_t st, int mode = 0, double mass = 0.0,
+                           const double* mass_coef = nullptr) {
     if (passes & 1) {
         zero_words<<<1, 64, 0, st>>>(w.status, 1);
         CHECK(hipGetLastError());
         CHECK(grim_launch_far(far_order, (int)n, p0, p1, next, prev, k, z_tm, ld_tm, z_te, ld_te, fix_rows,
-                              (int)nfix, plain, st));
+                              (int)nfix, mode, st));
     }
     if (passes & 2)
         CHECK(grim_launch_near((int)n, p0, p1, next, prev, k, w.counts, w.offsets, (int)cap, w.pair_e, w.pair_f,
-                               w.blocks, w.kind, w.status, z_tm, ld_tm, z_te, ld_te, plain, plain_mass, st));
+                               w.blocks, w.kind, w.status, z_tm, ld_tm, z_te, ld_te, mode, mass, mass_coef, st));
     return GRIM_OK;
 }
 
@@ -295,20 +300,31 @@ static int assemble_pass(int passes, int64_t n, const double* p0, const double* 
                            cap, (hipStream_t)stream);
 }
 
-extern "C" int grim_assemble_plain(int64_t n, const double* p0, const double* p1, const int32_t* next,
-                                   const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k,
-                                   int far_order, double* s, int64_t ld_s, double* kp, int64_t ld_kp,
-                                   double kp_mass, void* workspace, int64_t workspace_bytes,
-                                   int64_t near_per_element, void* stream) {
-    if (!workspace || !isfinite(kp_mass) ||
-        !assemble_args_ok(n, p0, p1, next, prev, fix_rows, nfix, k, far_order, s, ld_s, kp, ld_kp, near_per_element))
+extern "C" int grim_assemble_operators(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                       const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k,
+                                       int far_order, int which, double* a, int64_t ld_a, double* b, int64_t ld_b,
+                                       double mass, const double* mass_coef, void* workspace,
+                                       int64_t workspace_bytes, int64_t near_per_element, void* stream) {
+    if (!workspace || !isfinite(mass) || (which != 1 && which != 2) ||
+        !assemble_args_ok(n, p0, p1, next, prev, fix_rows, nfix, k, far_order, a, ld_a, b, ld_b, near_per_element))
         return GRIM_ERR_ARGS;
     const int64_t cap = n * near_per_element;
     if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
     AssemblyWork w = carve(workspace, n, cap);
     if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
-    return assemble_launch(3, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, s, ld_s, kp, ld_kp, w, cap,
-                           (hipStream_t)stream, 1, kp_mass);
+    return assemble_launch(3, n, p0, p1, next, prev, fix_rows, nfix, k, far_order, a, ld_a, b, ld_b, w, cap,
+                           (hipStream_t)stream, which, mass, mass_coef);
+}
+
+extern "C" int grim_route(int64_t m, int64_t n, const double* src, int64_t ld_src, const int32_t* rows_src,
+                          const int32_t* cols_src, double* dst, int64_t ld_dst, const int32_t* rows_dst,
+                          const int32_t* cols_dst, const double* weight, void* stream) {
+    if (m < 0 || n < 0 || m > (1 << 30) || n > 65535 || ld_src < 1 || ld_dst < 1) return GRIM_ERR_ARGS;
+    if (m == 0 || n == 0) return GRIM_OK;
+    if (!src || !rows_src || !cols_src || !dst || !rows_dst || !cols_dst || !weight) return GRIM_ERR_ARGS;
+    CHECK(grim_launch_route((int)m, (int)n, src, ld_src, rows_src, cols_src, dst, ld_dst, rows_dst, cols_dst, weight,
+                            (hipStream_t)stream));
+    return GRIM_OK;
 }
 
 extern "C" int grim_assemble(int64_t n, const double* p0, const double* p1, const int32_t* next,

@@ -79,10 +79,11 @@ __device__ __forceinline__ void zero_blk2(Blk2& b) {
 }
 
 // Far element-pair quadrature with the Q x Q Gauss rule (far.c:100-191 semantics,
-// kernels evaluated directly instead of through an interpolation table).  PLAIN gives the
-// operators themselves instead of the combined potential: tm = S, te = K', tmt = S^T and
-// tet = D^T (the K' of the reversed pair).
-template <int Q, bool PLAIN = false>
+// kernels evaluated directly instead of through an interpolation table).  MODE 0 gives the
+// combined potential; MODE 1 the operators S and K' (tm = S, te = K', tmt = S^T, tet = D^T,
+// the K' of the reversed pair); MODE 2 the operators D and W (tm = D, te = W, tmt = K'^T,
+// the D of the reversed pair, tet = W^T).
+template <int Q, int MODE = 0>
 __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k, Blk2& out) {
     cplx S[2][2], K[2][2], D[2][2];
 #pragma unroll
@@ -137,7 +138,7 @@ __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k,
             K[a][b] = cscale(K[a][b], scale);
             D[a][b] = cscale(D[a][b], scale);
         }
-    if (PLAIN) {
+    if (MODE == 1) {
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -161,6 +162,13 @@ __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k,
         for (int b = 0; b < 2; ++b) {
             const double tsign = (a == b) ? 1.0 : -1.0;
             const cplx w = {kk * S[a][b].re + tsign * q.re, kk * S[a][b].im + tsign * q.im};
+            if (MODE == 2) {
+                out.tm[a][b] = D[a][b];
+                out.te[a][b] = w;
+                out.tmt[b][a] = K[a][b];
+                out.tet[b][a] = w;
+                continue;
+            }
             out.tm[a][b] = {S[a][b].re - g * D[a][b].im, S[a][b].im + g * D[a][b].re};
             out.te[a][b] = {K[a][b].re + g * w.im, K[a][b].im - g * w.re};
             out.tmt[b][a] = {S[a][b].re - g * K[a][b].im, S[a][b].im + g * K[a][b].re};
@@ -168,10 +176,10 @@ __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k,
         }
 }
 
-template <int Q, bool PLAIN>
+template <int Q, int MODE>
 __device__ __forceinline__ void far_or_zero(const Elem& o, int e, int ne, const Elem& s, int f, int nf,
                                             double k, bool active, Blk2& out) {
-    if (active && is_far(o, e, ne, s, f, nf)) far_pair<Q, PLAIN>(o, s, k, out);
+    if (active && is_far(o, e, ne, s, f, nf)) far_pair<Q, MODE>(o, s, k, out);
     else zero_blk2(out);
 }
 
@@ -190,9 +198,9 @@ __device__ __forceinline__ void store_add(double* z, int64_t ld, int p, int q, c
 // same evaluations also give the transposed tile (rows in chunk, columns in
 // block I), written with the roles exchanged.  Every entry is written exactly
 // once, in a fixed summation order.  Entries whose observation-side neighbour
-// is not lane - 1 (lane 0, contour starts) are completed by far_fixup.  With PLAIN the two
-// outputs are S and K' (see far_pair).
-template <int Q, bool PLAIN>
+// is not lane - 1 (lane 0, contour starts) are completed by far_fixup.  MODE selects the two
+// outputs (see far_pair).
+template <int Q, int MODE>
 __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                  const int* __restrict__ next, const int* __restrict__ prev,
                                                  double k, int chunk, double* __restrict__ ztm, int64_t ldtm,
@@ -222,7 +230,7 @@ __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict
         cplx ltm0, ltm1, lte0, lte1, mtm0, mtm1, mte0, mte1;
         if (f == c0 || pf != f - 1) {
             Blk2 b;
-            far_or_zero<Q, PLAIN>(o, ee, ne, load_elem(p0, p1, pf), pf, next[pf], k, active, b);
+            far_or_zero<Q, MODE>(o, ee, ne, load_elem(p0, p1, pf), pf, next[pf], k, active, b);
             ltm0 = b.tm[0][1]; ltm1 = b.tm[1][1]; lte0 = b.te[0][1]; lte1 = b.te[1][1];
             mtm0 = b.tmt[1][0]; mtm1 = b.tmt[1][1]; mte0 = b.tet[1][0]; mte1 = b.tet[1][1];
         } else {
@@ -230,7 +238,7 @@ __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict
             mtm0 = pt_tm0; mtm1 = pt_tm1; mte0 = pt_te0; mte1 = pt_te1;
         }
         Blk2 b;
-        far_or_zero<Q, PLAIN>(o, ee, ne, load_elem(p0, p1, f), f, next[f], k, active, b);
+        far_or_zero<Q, MODE>(o, ee, ne, load_elem(p0, p1, f), f, next[f], k, active, b);
         // direct tile: column f, rows e (own, local 0) and next(e) (local 1 -> lane + 1)
         const cplx ctm0 = cadd(ltm0, b.tm[0][0]), ctm1 = cadd(ltm1, b.tm[1][0]);
         const cplx cte0 = cadd(lte0, b.te[0][0]), cte1 = cadd(lte1, b.te[1][0]);
@@ -261,7 +269,7 @@ __global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict
 //       Z[c, q] += blk(e', prev q).tm[1][1] + blk(e', q).tm[1][0]
 //   column c (entries written by a mirror tile,  block(q) >  block(c)):
 //       Z[q, c] += blk(e', prev q).tmt[1][1] + blk(e', q).tmt[0][1]
-template <int Q, bool PLAIN>
+template <int Q, int MODE>
 __global__ void __launch_bounds__(64) far_fixup(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                 const int* __restrict__ next, const int* __restrict__ prev,
                                                 double k, const int* __restrict__ rows, double* __restrict__ ztm,
@@ -280,7 +288,7 @@ __global__ void __launch_bounds__(64) far_fixup(int n, const double* __restrict_
     const int qc = q < n ? q : n - 1;
     const int col = lane == 0 ? prev[qb] : qc;
     Blk2 b;
-    far_or_zero<Q, PLAIN>(o, e, ne, load_elem(p0, p1, col), col, next[col], k, lane == 0 || q < n, b);
+    far_or_zero<Q, MODE>(o, e, ne, load_elem(p0, p1, col), col, next[col], k, lane == 0 || q < n, b);
     cplx ltm, lte, lmt, lme;
     ltm.re = __shfl_up(b.tm[1][1].re, 1, 64); ltm.im = __shfl_up(b.tm[1][1].im, 1, 64);
     lte.re = __shfl_up(b.te[1][1].re, 1, 64); lte.im = __shfl_up(b.te[1][1].im, 1, 64);
@@ -289,7 +297,7 @@ __global__ void __launch_bounds__(64) far_fixup(int n, const double* __restrict_
     const int pq = prev[qc];
     if (lane >= 2 && q < n && pq != q - 1) {
         Blk2 b1;
-        far_or_zero<Q, PLAIN>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
+        far_or_zero<Q, MODE>(o, e, ne, load_elem(p0, p1, pq), pq, next[pq], k, true, b1);
         ltm = b1.tm[1][1]; lte = b1.te[1][1]; lmt = b1.tmt[1][1]; lme = b1.tet[1][1];
     }
     if (lane == 0 || q >= n) return;
@@ -694,12 +702,14 @@ __device__ __forceinline__ void add_entry(double* z, int64_t ld, int p, int q, c
 }
 
 // Row-owner gather of near blocks and mass terms: thread p adds everything of node row p.
-// With plain != 0 the outputs are S and K' + plain_mass M instead (no D, W or jump terms).
+// mode 1: the outputs are S and K' + c M; mode 2: D + c M and W; c = mass_coef[e] per element
+// (or mass if mass_coef is null).  No other mass or jump terms in these modes.
 __global__ void __launch_bounds__(256) near_gather(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                    const int* __restrict__ next, const int* __restrict__ prev,
                                                    const int* __restrict__ offsets, const int* __restrict__ pair_f,
-                                                   const double* __restrict__ blocks, double k, int plain,
-                                                   double plain_mass, double* __restrict__ ztm, int64_t ldtm,
+                                                   const double* __restrict__ blocks, double k, int mode,
+                                                   double mass, const double* __restrict__ mass_coef,
+                                                   double* __restrict__ ztm, int64_t ldtm,
                                                    double* __restrict__ zte, int64_t ldte) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
@@ -723,11 +733,25 @@ __global__ void __launch_bounds__(256) near_gather(int n, const double* __restri
                     D[a][b] = rev < 0 ? cplx{0.0, 0.0} : cplx{R[8 + 2 * (2 * b + a)], R[8 + 2 * (2 * b + a) + 1]};
                 }
             Blk blk;
-            if (plain) {
+            if (mode == 1) {
                 for (int a = 0; a < 2; ++a)
                     for (int b = 0; b < 2; ++b) {
                         blk.tm[a][b] = S[a][b];
                         blk.te[a][b] = K[a][b];
+                    }
+            } else if (mode == 2) {
+                // D and the Maue W of combine(), without gamma
+                const double nn = o.nx * s.nx + o.ny * s.ny;
+                double denom = o.len * s.len;
+                if (denom < GRIM_EPS * GRIM_EPS) denom = GRIM_EPS * GRIM_EPS;
+                const cplx sum = cadd(cadd(S[0][0], S[0][1]), cadd(S[1][0], S[1][1]));
+                const cplx q = {sum.re / denom, sum.im / denom};
+                const double kk = -k * k * nn;
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b) {
+                        const double tsign = (a == b) ? 1.0 : -1.0;
+                        blk.tm[a][b] = D[a][b];
+                        blk.te[a][b] = {kk * S[a][b].re + tsign * q.re, kk * S[a][b].im + tsign * q.im};
                     }
             } else {
                 combine(S, K, D, o, s, k, blk);
@@ -740,10 +764,13 @@ __global__ void __launch_bounds__(256) near_gather(int n, const double* __restri
         }
         // mass of element e: L [[1/3, 1/6], [1/6, 1/3]]; TM + gamma/2 M, TE - M/2
         const int cols[2] = {e, next[e]};
+        const double c = mass_coef ? mass_coef[e] : mass;
         for (int b = 0; b < 2; ++b) {
             const double m = o.len * (loc == b ? 1.0 / 3.0 : 1.0 / 6.0);
-            if (plain) {
-                add_entry(zte, ldte, p, cols[b], cplx{plain_mass * m, 0.0});
+            if (mode == 1) {
+                add_entry(zte, ldte, p, cols[b], cplx{c * m, 0.0});
+            } else if (mode == 2) {
+                add_entry(ztm, ldtm, p, cols[b], cplx{c * m, 0.0});
             } else {
                 add_entry(ztm, ldtm, p, cols[b], cplx{0.0, 0.5 * m * g});
                 add_entry(zte, ldte, p, cols[b], cplx{-0.5 * m, 0.0});
@@ -852,7 +879,7 @@ extern "C" hipError_t grim_upload_rules(const double* gl8t, const double* gl8w, 
 }
 
 // Far pass launcher: one wavefront per (row block I, column chunk j) with j in block J >= I.
-template <bool PLAIN>
+template <int MODE>
 static hipError_t launch_far(int q, int n, const double* p0, const double* p1, const int* next, const int* prev,
                              double k, double* ztm, int64_t ldtm, double* zte, int64_t ldte, const int* fix_rows,
                              int nfix, hipStream_t st) {
@@ -867,36 +894,38 @@ static hipError_t launch_far(int q, int n, const double* p0, const double* p1, c
     long ntiles = 0;
     for (int I = 0; I < nb; ++I) ntiles += nc - I * cpb;
     if (q == 8)
-        far_kernel<8, PLAIN><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+        far_kernel<8, MODE><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
     else
-        far_kernel<16, PLAIN><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
+        far_kernel<16, MODE><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess || nfix == 0) return err;
     dim3 g2((n + 62) / 63, nfix);
-    if (q == 8) far_fixup<8, PLAIN><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
-    else far_fixup<16, PLAIN><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    if (q == 8) far_fixup<8, MODE><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
+    else far_fixup<16, MODE><<<g2, 64, 0, st>>>(n, p0, p1, next, prev, k, fix_rows, ztm, ldtm, zte, ldte);
     return hipGetLastError();
 }
 
-// plain = 0: the combined-potential pair (TM, TE); plain = 1: S and K' (far_pair).
+// mode 0: the combined-potential pair (TM, TE); 1: S and K'; 2: D and W (far_pair).
 extern "C" hipError_t grim_launch_far(int q, int n, const double* p0, const double* p1, const int* next,
                                       const int* prev, double k, double* ztm, int64_t ldtm, double* zte,
-                                      int64_t ldte, const int* fix_rows, int nfix, int plain, hipStream_t st) {
-    return plain ? launch_far<true>(q, n, p0, p1, next, prev, k, ztm, ldtm, zte, ldte, fix_rows, nfix, st)
-                 : launch_far<false>(q, n, p0, p1, next, prev, k, ztm, ldtm, zte, ldte, fix_rows, nfix, st);
+                                      int64_t ldte, const int* fix_rows, int nfix, int mode, hipStream_t st) {
+    if (mode == 1) return launch_far<1>(q, n, p0, p1, next, prev, k, ztm, ldtm, zte, ldte, fix_rows, nfix, st);
+    if (mode == 2) return launch_far<2>(q, n, p0, p1, next, prev, k, ztm, ldtm, zte, ldte, fix_rows, nfix, st);
+    return launch_far<0>(q, n, p0, p1, next, prev, k, ztm, ldtm, zte, ldte, fix_rows, nfix, st);
 }
 
 extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1, const int* next, const int* prev,
                                        double k, int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
                                        double* blocks, int* kind, unsigned* status, double* ztm, int64_t ldtm,
-                                       double* zte, int64_t ldte, int plain, double plain_mass, hipStream_t st) {
+                                       double* zte, int64_t ldte, int mode, double mass, const double* mass_coef,
+                                       hipStream_t st) {
     near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 0, counts, offsets, capacity, pair_e, pair_f, status);
     scan_counts<<<1, 1024, 0, st>>>(n, counts, offsets);
     near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 1, counts, offsets, capacity, pair_e, pair_f, status);
     near_blocks<<<capacity, 64, 0, st>>>(n, p0, p1, next, offsets, pair_e, pair_f, k, blocks, kind, status);
     near_adaptive<<<capacity, 64, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, k, blocks, kind, status);
-    near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, blocks, k, plain,
-                                                 plain_mass, ztm, ldtm, zte, ldte);
+    near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, blocks, k, mode, mass,
+                                                 mass_coef, ztm, ldtm, zte, ldte);
     return hipGetLastError();
 }
 
@@ -1150,6 +1179,35 @@ extern "C" hipError_t grim_launch_scale_blocks(int64_t n, double* z, int64_t ld,
     if (n == 0) return hipSuccess;
     dim3 grid((unsigned)((2 * n + 255) / 256), (unsigned)(2 * n));
     scale_blocks_kernel<<<grid, 256, 0, st>>>(n, z, ld, s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7]);
+    return hipGetLastError();
+}
+
+// dst[rows_dst[i], cols_dst[j]] += w[i] src[rows_src[i], cols_src[j]] for i < m, j < n (column-major
+// complex128; a negative destination index skips the entry).  Composes regional systems from
+// assembled operator blocks (twod/assembly/scatter.py:65-116, SystemScatter).
+__global__ void __launch_bounds__(256) route_kernel(int m, int n, const double* __restrict__ src, int64_t ld_src,
+                                                    const int* __restrict__ rows_src, const int* __restrict__ cols_src,
+                                                    double* __restrict__ dst, int64_t ld_dst,
+                                                    const int* __restrict__ rows_dst, const int* __restrict__ cols_dst,
+                                                    const double* __restrict__ w) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= m || j >= n) return;
+    const int rd = rows_dst[i], cd = cols_dst[j];
+    if (rd < 0 || cd < 0) return;
+    const double* a = src + 2 * ((int64_t)rows_src[i] + (int64_t)cols_src[j] * ld_src);
+    double* t = dst + 2 * ((int64_t)rd + (int64_t)cd * ld_dst);
+    const double wr = w[2 * i], wi = w[2 * i + 1], ar = a[0], ai = a[1];
+    t[0] += wr * ar - wi * ai;
+    t[1] += wr * ai + wi * ar;
+}
+
+extern "C" hipError_t grim_launch_route(int m, int n, const double* src, int64_t ld_src, const int* rows_src,
+                                       const int* cols_src, double* dst, int64_t ld_dst, const int* rows_dst,
+                                       const int* cols_dst, const double* w, hipStream_t st) {
+    if (m == 0 || n == 0) return hipSuccess;
+    dim3 grid((unsigned)((m + 255) / 256), (unsigned)n);
+    route_kernel<<<grid, 256, 0, st>>>(m, n, src, ld_src, rows_src, cols_src, dst, ld_dst, rows_dst, cols_dst, w);
     return hipGetLastError();
 }
 

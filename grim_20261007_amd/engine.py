@@ -248,7 +248,7 @@ def dielectric_systems(contour: DeviceContour, k0: float, workspace: torch.Tenso
         [ -eta Z_TE(k0)   factor (K'(k1) + M/2) ]
     with eta = j k0 (so that gamma = 1/eta in the PEC pair of k0), k1 = k0 n and factor =
     1/eps (TE) or 1/mu (TM).  The four blocks are assembled in place (grim_assemble at k0 into the
-    left block column, grim_assemble_plain at k1 into the right one), the system is copied for the
+    left block column, grim_assemble_operators at k1 into the right one), the system is copied for the
     second polarization, and grim_scale_blocks applies each polarization's block factors."""
     flag, eps, mu = contour.mesh.medium
     k1 = dielectric_wavenumber(contour.mesh, k0)
@@ -268,11 +268,10 @@ def dielectric_systems(contour: DeviceContour, k0: float, workspace: torch.Tenso
                                                      block(n, 0), ld, workspace.data_ptr(), workspace.numel(),
                                                      NEAR_PER_ELEMENT, st))
     native.check("grim_assemble (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
-    native.check("grim_assemble_plain", lib.grim_assemble_plain(*args, float(k1), contour.mesh.far_order(k1),
-                                                                 block(0, n), ld, block(n, n), ld, 0.5,
-                                                                 workspace.data_ptr(), workspace.numel(),
-                                                                 NEAR_PER_ELEMENT, st))
-    native.check("grim_assemble_plain (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
+    native.check("grim_assemble_operators", lib.grim_assemble_operators(
+        *args, float(k1), contour.mesh.far_order(k1), 1, block(0, n), ld, block(n, n), ld, 0.5, None,
+        workspace.data_ptr(), workspace.numel(), NEAR_PER_ELEMENT, st))
+    native.check("grim_assemble_operators (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
     pair[1].copy_(pair[0])
     eta = 1j * float(k0)
     one = 1.0 + 0.0j
@@ -298,6 +297,260 @@ def radiating_density(contour: DeviceContour, k0: float, x: torch.Tensor) -> tor
     if contour.mesh.medium is None:
         return x
     return (1j * float(k0) * x[..., :contour.n]).contiguous()
+
+
+# ----------------------------------------------------------------------------
+# Multi-region layouts (twod/formulations/regions.py, combined_regions.py)
+# ----------------------------------------------------------------------------
+@dataclass
+class RegionPlan:
+    """Interfaces and unknowns of a multi-region layout (twod/formulations/regions.py:66-119):
+    elements grouped by (minus, plus) region, interfaces in sorted key order, each interface's
+    nodes in the reference's node order, and one block of unknowns per interface side that
+    borders a region (minus first).  ``subs[rid]``: the region's elements (closed contours) and
+    the map from mesh node to the region's local node."""
+    ifaces: list
+    region_ifaces: dict
+    dofs: dict
+    n_dof: int
+    media: dict
+    subs: dict
+
+
+def region_plan(mesh: ContourMesh) -> RegionPlan:
+    ref = np.arange(mesh.n) if mesh.ref_node is None else np.asarray(mesh.ref_node)
+    groups = {}
+    for e, (rm, rp) in enumerate(mesh.regions):
+        groups.setdefault((int(rm), int(rp)), []).append(e)
+    ifaces, region_ifaces, dofs, n_dof = [], {}, {}, 0
+    for mi, ((rm, rp), eids) in enumerate(sorted(groups.items())):
+        nodes = np.array(sorted(eids, key=lambda v: ref[v]), np.int64)   # element e starts at node e
+        ifaces.append({"r_m": rm, "r_p": rp, "eids": np.array(eids, np.int64), "nodes": nodes})
+        for side, rid in (("minus", rm), ("plus", rp)):
+            if rid >= 0:
+                region_ifaces.setdefault(rid, []).append(mi)
+                dofs[mi, side] = (n_dof, len(nodes))
+                n_dof += len(nodes)
+    subs = {}
+    for rid, members in region_ifaces.items():
+        els = np.sort(np.concatenate([ifaces[mi]["eids"] for mi in members]))
+        local = np.full(mesh.n, -1, np.int64)
+        local[els] = np.arange(len(els))
+        subs[rid] = {"elements": els, "local": local}
+    return RegionPlan(ifaces, region_ifaces, dofs, n_dof, dict(mesh.region_media), subs)
+
+
+def _region_wavenumber(media, rid, k0):
+    eps, mu = media[rid]
+    index = causal_index(eps, mu)
+    tol = 64.0 * np.finfo(float).eps * max(1.0, abs(index))
+    if abs(index.imag) > tol or not index.real > 0.0:
+        raise ValueError(f"Region {rid} (eps={eps!r}, mu={mu!r}) has refractive index {index!r}: lossy and "
+                         "negative-index media need complex-argument Hankel kernels, which the GPU path does not "
+                         "evaluate. Use lossless media with positive indices or the reference's CPU path.")
+    return float(k0 * index.real)
+
+
+@dataclass
+class Route:
+    """dst[rows_dst, cols_dst] += weight * src[rows_src, cols_src] for one system (0 TE, 1 TM) or
+    right-hand side.  src: (region, operator) with operator 0 S, 1 K' + c M, 2 D - c M, 3 W of the
+    region's contours (local node numbers), or "bu" / "bdn" (mesh node numbers, all angles)."""
+    pol: int
+    rows_dst: np.ndarray
+    cols_dst: np.ndarray | None
+    src: tuple | str
+    rows_src: np.ndarray
+    cols_src: np.ndarray | None
+    weight: complex
+
+
+def region_mass_coefficients(plan: RegionPlan, rid: int) -> np.ndarray:
+    """Per element of region rid's contours: -1/2 where the region borders the element's interface
+    from its minus side, +1/2 from its plus side -- the mass of K' (regions.py:280-316: -M/2 + K' on
+    the minus side, M/2 + K' on the plus side); D takes the opposite, its trace jump
+    (combined_regions.py:139-141)."""
+    local = plan.subs[rid]["local"]
+    coef = np.empty(len(plan.subs[rid]["elements"]))
+    for mi in plan.region_ifaces[rid]:
+        ifc = plan.ifaces[mi]
+        coef[local[ifc["eids"]]] = -0.5 if ifc["r_m"] == rid else 0.5
+    return coef
+
+
+def region_routes(plan: RegionPlan, k_of) -> list:
+    """The routes that compose both polarizations' regional systems (regions.py:239-333 base rows,
+    combined_regions.py:43-62,116-146 D and W routes; ``k_of(rid)`` is the region's wavenumber)."""
+    routes = []
+
+    def cols(mj, rid):
+        src = plan.ifaces[mj]
+        start, count = plan.dofs[mj, "minus" if src["r_m"] == rid else "plus"]
+        return src["nodes"], np.arange(start, start + count)
+
+    def inv_beta(ifc, pol):
+        key = 1 if pol == "TM" else 0
+        return complex(plan.media[ifc["r_m"]][key]) / complex(plan.media[ifc["r_p"]][key])
+
+    def rows_of(mi, side):
+        start, count = plan.dofs[mi, side]
+        return np.arange(start, start + count)
+
+    for i, pol in enumerate(("TE", "TM")):
+        for mi, ifc in enumerate(plan.ifaces):
+            rm, rp, nodes = ifc["r_m"], ifc["r_p"], ifc["nodes"]
+            if rm < 0 or rp < 0:
+                rid = rp if rm < 0 else rm
+                local = plan.subs[rid]["local"]
+                rows = rows_of(mi, "plus" if rm < 0 else "minus")
+                for mj in plan.region_ifaces[rid]:
+                    src_nodes, cc = cols(mj, rid)
+                    routes.append(Route(i, rows, cc, (rid, 0 if pol == "TM" else 1), local[nodes], local[src_nodes],
+                                        1.0))
+            else:
+                rs, rt = rows_of(mi, "minus"), rows_of(mi, "plus")
+                for rid, sign, flux in ((rm, 1.0, 1.0), (rp, -1.0, inv_beta(ifc, pol))):
+                    local = plan.subs[rid]["local"]
+                    for mj in plan.region_ifaces[rid]:
+                        src_nodes, cc = cols(mj, rid)
+                        routes.append(Route(i, rs, cc, (rid, 1), local[nodes], local[src_nodes], sign * flux))
+                        routes.append(Route(i, rt, cc, (rid, 0), local[nodes], local[src_nodes], sign))
+        for rid, members in plan.region_ifaces.items():
+            local = plan.subs[rid]["local"]
+            for mi in members:
+                side = "minus" if plan.ifaces[mi]["r_m"] == rid else "plus"
+                gamma = (1.0 if side == "minus" else -1.0) / (1j * k_of(rid))
+                src_nodes, cc = cols(mi, rid)
+                for obs in members:
+                    tgt = plan.ifaces[obs]
+                    o_rows = local[tgt["nodes"]]
+                    if tgt["r_m"] < 0 or tgt["r_p"] < 0:
+                        rows = rows_of(obs, "plus" if tgt["r_m"] < 0 else "minus")
+                        if pol == "TM":
+                            routes.append(Route(i, rows, cc, (rid, 2), o_rows, local[src_nodes], gamma))
+                        else:
+                            routes.append(Route(i, rows, cc, (rid, 3), o_rows, local[src_nodes], -gamma))
+                    else:
+                        own = rid == tgt["r_m"]
+                        routes.append(Route(i, rows_of(obs, "plus"), cc, (rid, 2), o_rows, local[src_nodes],
+                                            gamma * (1.0 if own else -1.0)))
+                        routes.append(Route(i, rows_of(obs, "minus"), cc, (rid, 3), o_rows, local[src_nodes],
+                                            -gamma * (1.0 if own else -inv_beta(tgt, pol))))
+    return routes
+
+
+def region_load_routes(plan: RegionPlan) -> list:
+    """Routes of both polarizations' right-hand sides (regions.py:557-590): on interfaces that
+    border air, -b_dn (TE) or -b_u (TM) on conductor rows; transmission rows with air on their minus
+    side get -b_dn (flux) and -b_u (trace), with air on their plus side +(1/beta) b_dn and +b_u."""
+    routes = []
+    for i, pol in enumerate(("TE", "TM")):
+        key = 1 if pol == "TM" else 0
+        for mi, ifc in enumerate(plan.ifaces):
+            rm, rp, nodes = ifc["r_m"], ifc["r_p"], ifc["nodes"]
+            if rm < 0 or rp < 0:
+                if (rp if rm < 0 else rm) == 0:
+                    start, count = plan.dofs[mi, "plus" if rm < 0 else "minus"]
+                    routes.append(Route(i, np.arange(start, start + count), None, "bu" if pol == "TM" else "bdn",
+                                        nodes, None, -1.0))
+                continue
+            fl, tr = plan.dofs[mi, "minus"][0], plan.dofs[mi, "plus"][0]
+            rs, rt = np.arange(fl, fl + len(nodes)), np.arange(tr, tr + len(nodes))
+            if rm == 0:
+                routes.append(Route(i, rs, None, "bdn", nodes, None, -1.0))
+                routes.append(Route(i, rt, None, "bu", nodes, None, -1.0))
+            if rp == 0:
+                inv = complex(plan.media[rm][key]) / complex(plan.media[rp][key])
+                routes.append(Route(i, rs, None, "bdn", nodes, None, inv))
+                routes.append(Route(i, rt, None, "bu", nodes, None, 1.0))
+    return routes
+
+
+class _Router:
+    """grim_route launches on torch's current stream.  The index and weight tensors they read are
+    freed into torch's stream-ordered cache, so later work on that stream cannot overwrite them
+    before the launches have read them."""
+
+    def __init__(self, dev):
+        self.dev, self.st = dev, _stream(dev)
+
+    def __call__(self, dst, ld_dst, rows_dst, cols_dst, src, ld_src, rows_src, cols_src, weight):
+        m, n = len(rows_src), len(cols_src)
+        w = torch.as_tensor(np.full(m, complex(weight), np.complex128), device=self.dev)
+        ts = [torch.as_tensor(np.ascontiguousarray(v, np.int32), device=self.dev)
+              for v in (rows_src, cols_src, rows_dst, cols_dst)]
+        code = native.lib().grim_route(m, n, src.data_ptr(), ld_src, ts[0].data_ptr(), ts[1].data_ptr(),
+                                       dst.data_ptr(), ld_dst, ts[2].data_ptr(), ts[3].data_ptr(), w.data_ptr(),
+                                       self.st)
+        native.check("grim_route", code)
+
+
+def region_systems(contour: DeviceContour, k0: float, workspace: torch.Tensor, plan: RegionPlan) -> torch.Tensor:
+    """[TE, TM] systems (2, DOFs, DOFs) of a multi-region layout without surface impedances: per
+    region, S, K' + c M, D - c M and W of the region's contours at its wavenumber
+    (grim_assemble_operators, c from region_mass_coefficients), then the routes of region_routes
+    (grim_route)."""
+    mesh, dev = contour.mesh, contour.device
+    nd = plan.n_dof
+    pair = torch.zeros((2, nd, nd), dtype=torch.complex128, device=dev)
+    route = _Router(dev)
+    lib = native.lib()
+    ops, ks = {}, {}
+    for rid, sub in plan.subs.items():
+        k = ks[rid] = _region_wavenumber(plan.media, rid, k0)
+        els, local = sub["elements"], sub["local"]
+        nr = len(els)
+        rmesh = ContourMesh(mesh.p0[els], mesh.p0[els], mesh.p1[els], local[mesh.next[els]].astype(np.int32),
+                            local[mesh.prev[els]].astype(np.int32))
+        rc = DeviceContour(rmesh, dev)
+        coef = region_mass_coefficients(plan, rid)
+        c_k, c_d = torch.as_tensor(coef, device=dev), torch.as_tensor(-coef, device=dev)
+        mats = torch.empty((4, nr, nr), dtype=torch.complex128, device=dev)     # S, K', D, W
+        args = (nr, rc.p0.data_ptr(), rc.p1.data_ptr(), rc.next.data_ptr(), rc.prev.data_ptr(),
+                rc.fix_rows.data_ptr(), int(rc.fix_rows.numel()), float(k), rmesh.far_order(k))
+        for which, a, b, c in ((1, mats[0], mats[1], c_k), (2, mats[2], mats[3], c_d)):
+            # the workspace is sized for the whole mesh, so for any region's contours
+            code = lib.grim_assemble_operators(*args, which, a.data_ptr(), nr, b.data_ptr(), nr, 0.0, c.data_ptr(),
+                                               workspace.data_ptr(), workspace.numel(), NEAR_PER_ELEMENT, route.st)
+            native.check("grim_assemble_operators", code)
+            native.check("grim_assemble_operators (device)",
+                         lib.grim_workspace_status(workspace.data_ptr(), route.st))
+        ops[rid] = mats
+    for r in region_routes(plan, ks.__getitem__):
+        rid, op = r.src
+        route(pair[r.pol], nd, r.rows_dst, r.cols_dst, ops[rid][op], len(plan.subs[rid]["elements"]), r.rows_src,
+              r.cols_src, r.weight)
+    return pair
+
+
+def region_rhs(plan: RegionPlan, bu: torch.Tensor, bdn: torch.Tensor) -> torch.Tensor:
+    """Right-hand sides (2, A, DOFs) [TE, TM] of region_systems (routes of region_load_routes)."""
+    dev = bu.device
+    na, n = bu.shape
+    out = torch.zeros((2, na, plan.n_dof), dtype=torch.complex128, device=dev)
+    route = _Router(dev)
+    acols = np.arange(na)
+    loads = {"bu": bu, "bdn": bdn}
+    for r in region_load_routes(plan):
+        route(out[r.pol], plan.n_dof, r.rows_dst, acols, loads[r.src], n, r.rows_src, acols, r.weight)
+    return out
+
+
+def region_exterior_density(plan: RegionPlan, n: int, x: torch.Tensor) -> torch.Tensor:
+    """Nodal single-layer density (A, N) of the air-side unknowns of a solution x (A, DOFs), zero
+    on nodes that do not border air (twod/formulations/regions.py:593-609)."""
+    dev = x.device
+    na = x.shape[0]
+    out = torch.zeros((na, n), dtype=torch.complex128, device=dev)
+    route = _Router(dev)
+    acols = np.arange(na)
+    xc = x.contiguous()
+    for mi, ifc in enumerate(plan.ifaces):
+        for side, rid in (("minus", ifc["r_m"]), ("plus", ifc["r_p"])):
+            if rid == 0:
+                start, count = plan.dofs[mi, side]
+                route(out, n, ifc["nodes"], acols, xc, plan.n_dof, np.arange(start, start + count), acols, 1.0)
+    return out
 
 
 def bistatic_farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):

@@ -38,6 +38,8 @@ class ContourMesh:
     impedance: np.ndarray | None = None   # (N,) complex surface impedance per element (0: PEC), or None
     ref_node: np.ndarray | None = None    # (N,) the reference's node number of node p (None: p itself)
     medium: tuple | None = None           # (flag, eps, mu) of a dielectric body (TYPE 3), or None (conductor)
+    regions: np.ndarray | None = None     # (N, 2) (minus, plus) region per element of a multi-region layout
+    region_media: dict | None = None      # region -> (eps, mu) of a multi-region layout (0: air)
 
     @property
     def n(self) -> int:
@@ -752,25 +754,31 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     media = dielectric_media(snapshot, material_base_dir)
     wavelength0 = min(material_wavelength(segments, media, f) for f in freqs)
     welder, junctions = boundary_condition_junctions(segments, scale, laws, freqs)
-    panels0, panels1, zs = [], [], []
+    panels0, panels1, zs, keys_e = [], [], [], []
     kinds, dielectric_flags = set(), set()
     for s, seg in enumerate(segments):
         seg_type, n_prop, ibc = _parse_props(seg)
         name = str(seg.get("name", "segment"))
-        if seg_type not in (2, 3):
-            raise ValueError(f"Segment '{name}': only conductor (TYPE 2) contours and dielectric (TYPE 3) "
-                             "interfaces with air are supported by the GPU path.")
+        if seg_type not in (2, 3, 4, 5):
+            raise ValueError(f"Segment '{name}': TYPE {seg_type} boundaries (sheets) are not supported by the GPU "
+                             "path: conductors (TYPE 2), dielectric interfaces with air (TYPE 3), coated conductors "
+                             "(TYPE 4) and dielectric/dielectric interfaces (TYPE 5) are.")
         kinds.add(seg_type)
         # an explicit count is held to the media on the segment's own sides
         local_wavelength = min(material_wavelength([seg], media, f) for f in freqs)
+        props5 = list(seg.get("properties", []) or []) + [0] * 5
+        pos_mat, neg_mat = _parse_int(props5[3]), _parse_int(props5[4])
+        if seg_type in (3, 4) and pos_mat <= 0:
+            raise ValueError(f"TYPE {seg_type} panel '{name}' requires pos_mat > 0.")
+        if seg_type == 5 and (pos_mat <= 0 or neg_mat <= 0):
+            raise ValueError(f"TYPE 5 panel '{name}' requires pos_mat > 0 and neg_mat > 0.")
+        if seg_type in (3, 5) and ibc > 0:
+            raise ValueError(f"Segment '{name}': an impedance sheet on a dielectric interface is not supported by "
+                             "the GPU path.")
         if seg_type == 3:
-            pos_mat = _parse_int((list(seg.get("properties", []) or []) + [0] * 5)[3])
-            if pos_mat <= 0:
-                raise ValueError(f"TYPE 3 panel '{name}' requires pos_mat > 0.")
-            if ibc > 0:
-                raise ValueError(f"Segment '{name}': an impedance sheet on a dielectric interface is not supported by "
-                                 "the GPU path.")
             dielectric_flags.add(pos_mat)
+        # (minus region, plus region) of the segment's panels: 0 air, -1 conductor
+        region_key = {2: (0, -1), 3: (0, pos_mat), 4: (-1, pos_mat), 5: (neg_mat, pos_mat)}[seg_type]
         law = None
         if ibc > 0:
             if ibc not in laws:
@@ -780,9 +788,11 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
                 raise ValueError("Thin dielectric layers can be assigned only to a free TYPE 1 sheet.")
         wavelength = surface_wave_mesh_wavelength(wavelength0, law, ibc, freqs)
         pairs = list(seg.get("point_pairs", []) or [])
-        # TYPE 2: the user draws the normal into air; reverse the chain.
-        pairs = [{"x1": p.get("x2", 0.0), "y1": p.get("y2", 0.0), "x2": p.get("x1", 0.0), "y2": p.get("y1", 0.0)}
-                 for p in reversed(pairs)]
+        if seg_type in (2, 3):
+            # TYPE 2 and 3: the user draws the normal into air; reverse the chain so that the stored
+            # normal points to the plus side (TYPE 4 and 5 are drawn toward their plus side).
+            pairs = [{"x1": p.get("x2", 0.0), "y1": p.get("y2", 0.0), "x2": p.get("x1", 0.0), "y2": p.get("y1", 0.0)}
+                     for p in reversed(pairs)]
         ends = []
         for pair in pairs:
             a = np.array([float(pair["x1"]) * scale, float(pair["y1"]) * scale])
@@ -831,7 +841,8 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
         for length in seg_len:
             arc = (cum + 0.5 * length) / total if total > EPS else 0.5
             cum += length
-            zs.append(0j if law is None else law.impedance(ibc, law_ghz, 1.0 - arc))
+            zs.append(0j if law is None else law.impedance(ibc, law_ghz, 1.0 - arc if seg_type in (2, 3) else arc))
+            keys_e.append(region_key)
     if not panels0:
         raise ValueError("Geometry does not contain any valid discretized panels.")
     p0 = np.array(panels0)
@@ -874,32 +885,69 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     prev[nxt] = np.arange(n, dtype=np.int32)
     starts = [int(e) for e in range(n) if prev[e] != e - 1]
     impedance = np.array(zs, complex)
-    medium = None
-    if 3 in kinds:
-        if kinds != {3} or len(dielectric_flags) != 1:
-            raise ValueError("Multi-region layouts (conductors beside or inside dielectrics, several dielectric "
-                             "media) are not supported by the GPU path: one homogeneous dielectric body in air is.")
+    has_ibc = bool(np.any(np.abs(impedance) > EPS))
+    medium = regions = region_media = None
+    if kinds == {3} and len(dielectric_flags) == 1:
         flag = dielectric_flags.pop()
         medium = (flag,) + tuple(medium_of(media, flag, law_ghz))
+    elif kinds != {2}:
+        # several regions (twod/solver.py:1850-1865): coated conductors, conductors beside
+        # dielectrics, several media
+        if has_ibc:
+            raise ValueError("Surface impedances in multi-region layouts (coated impedance bodies, impedance "
+                             "contours beside dielectrics) are not supported by the GPU path.")
+        regions = np.array(keys_e, np.int64)
+        used = sorted({int(r) for r in regions.ravel() if r > 0})
+        if not np.any(regions == 0):
+            raise ValueError("The exterior region must be air: a layout needs TYPE 2 or TYPE 3 boundaries.")
+        region_media = {0: (1.0 + 0.0j, 1.0 + 0.0j)}
+        region_media.update({r: tuple(medium_of(media, r, law_ghz)) for r in used})
     mesh = ContourMesh(np.array(nodes)[first], p0, p1, nxt, prev, starts,
-                       impedance if np.any(np.abs(impedance) > EPS) else None,
-                       None if np.array_equal(first, np.arange(n)) else first.copy(), medium)
+                       impedance if has_ibc else None,
+                       None if np.array_equal(first, np.arange(n)) else first.copy(), medium, regions, region_media)
+    if regions is not None:
+        _check_region_contours(mesh)
     _check_orientation(mesh)
     return mesh
 
 
-def _check_orientation(mesh: ContourMesh) -> None:
-    """Each closed contour must run counterclockwise internally (normals into the conductor)."""
+def _contours(mesh: ContourMesh):
     seen = np.zeros(mesh.n, bool)
     for s in range(mesh.n):
         if seen[s]:
             continue
-        area, e = 0.0, s
+        cycle, e = [], s
         while not seen[e]:
             seen[e] = True
+            cycle.append(e)
+            e = int(mesh.next[e])
+        yield cycle
+
+
+def _check_region_contours(mesh: ContourMesh) -> None:
+    """In a multi-region layout every closed contour carries one boundary kind: junctions between
+    boundary kinds along a contour (the reference's junction conditions) are not built here."""
+    for cycle in _contours(mesh):
+        keys = {tuple(mesh.regions[e]) for e in cycle}
+        if len(keys) != 1:
+            raise ValueError("Junctions between boundary kinds along one contour are not supported by the GPU "
+                             "path in multi-region layouts: each closed contour must carry one boundary kind.")
+
+
+def _check_orientation(mesh: ContourMesh) -> None:
+    """Each closed contour's stored normals must point into the body it bounds: a conductor or
+    dielectric in air (TYPE 2, 3; drawn with the normal into air, reversed when stored) runs
+    counterclockwise as stored; the inner boundary of a coating or of a layered body (TYPE 4, 5;
+    drawn and stored with the normal out of the inner region) runs clockwise."""
+    for cycle in _contours(mesh):
+        area = 0.0
+        for e in cycle:
             a, b = mesh.p0[e], mesh.p1[e]
             area += a[0] * b[1] - b[0] * a[1]
-            e = int(mesh.next[e])
-        if area <= 0:
-            raise ValueError("Geometry orientation check failed: a TYPE 2 contour must be drawn with its "
+        inner = mesh.regions is not None and mesh.regions[cycle[0]][0] != 0
+        if inner and area >= 0:
+            raise ValueError("Geometry orientation check failed: a TYPE 4 or 5 contour must be drawn with its "
+                             "normal from the inner region into the pos_mat region (clockwise around the core).")
+        if not inner and area <= 0:
+            raise ValueError("Geometry orientation check failed: a TYPE 2 or 3 contour must be drawn with its "
                              "normal into the air (clockwise around the body).")

@@ -59,8 +59,9 @@ SIGNATURES = {
     "grim_copy_c128": (_i32, [_i64, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "grim_column_norms": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp]),
     "grim_robin_rows": (_i32, [_i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp]),
-    "grim_assemble_plain": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _i32, _vp, _i64, _vp, _i64, _dbl,
-                                    _vp, _i64, _i64, _vp]),
+    "grim_assemble_operators": (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _dbl, _i32, _i32, _vp, _i64, _vp, _i64,
+                                        _dbl, _vp, _vp, _i64, _i64, _vp]),
+    "grim_route": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "grim_scale_blocks": (_i32, [_i64, _vp, _i64, _vp, _vp]),
     "grim_robin_split": (_i32, [_i64, _vp, _vp, _vp, _vp, _dbl, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
                                  _vp, _i64, _vp, _i64, _vp]),

@@ -30,7 +30,7 @@ import torch
 
 from . import autotune, engine
 from .mesh import ContourMesh
-from .solver import wavenumber
+from .solver import system_dofs, wavenumber
 
 
 @dataclass
@@ -66,6 +66,9 @@ def geometry_digest(mesh: ContourMesh) -> str:
         h.update(np.ascontiguousarray(mesh.impedance).tobytes())
     if mesh.medium is not None:
         h.update(np.array([complex(mesh.medium[1]), complex(mesh.medium[2])]).tobytes())
+    if mesh.regions is not None:
+        h.update(np.ascontiguousarray(mesh.regions).tobytes())
+        h.update(np.array([complex(v) for r in sorted(mesh.region_media) for v in mesh.region_media[r]]).tobytes())
     return h.hexdigest()[:24]
 
 
@@ -223,8 +226,8 @@ class Server:
             if self.cache.put(key, item, nb) and done is not None:
                 done.set()
 
-        dofs = 2 * mesh.n if mesh.medium is not None else mesh.n
-        nbytes = request_bytes(dofs, len(req.angles_deg), len(req.pols), req.precision, cached is not None)
+        nbytes = request_bytes(system_dofs(mesh), len(req.angles_deg), len(req.pols), req.precision,
+                               cached is not None)
         try:
             self._admit(nbytes)
             try:
@@ -243,16 +246,19 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     solve and far field.  `store(item, nbytes)` receives the assembled + factored systems."""
     mesh = req.mesh
     if req.precision == "auto":
-        req.precision = choose_precision(2 * mesh.n if mesh.medium is not None else mesh.n, len(req.angles_deg))
+        req.precision = choose_precision(system_dofs(mesh), len(req.angles_deg))
     k = wavenumber(req.frequency_ghz)
     started = time.perf_counter()
     dev = torch.device(device)
     contour = cached["contour"] if cached else engine.DeviceContour(mesh, dev)
-    dofs = 2 * mesh.n if mesh.medium is not None else mesh.n
+    plan = cached.get("plan") if cached else (engine.region_plan(mesh) if mesh.regions is not None else None)
+    dofs = plan.n_dof if plan is not None else (2 * mesh.n if mesh.medium is not None else mesh.n)
     if cached is None:
         if mesh.n not in ws:
             ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
-        if mesh.medium is not None:
+        if plan is not None:
+            pair = engine.region_systems(contour, k, ws[mesh.n], plan)
+        elif mesh.medium is not None:
             pair = engine.dielectric_systems(contour, k, ws[mesh.n])
         else:
             z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
@@ -262,7 +268,7 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
         fac = engine.factor(pair, req.precision)
         if store is not None:
             sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * dofs * dofs
-            store({"contour": contour, "factor": fac}, sysbytes)
+            store({"contour": contour, "factor": fac, "plan": plan}, sysbytes)
     else:
         fac = cached["factor"]
         # the cached tensors are read on this worker's stream too
@@ -272,7 +278,9 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
             t.record_stream(cur)
     angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, angles)
-    if mesh.medium is not None:
+    if plan is not None:
+        rhs = engine.region_rhs(plan, bu, bdn)
+    elif mesh.medium is not None:
         rhs = engine.dielectric_rhs(bu, bdn)
     elif mesh.impedance is not None:
         rhs = engine.impedance_rhs(contour, k, angles)
@@ -282,14 +290,16 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     if inf.get("factor") is not None and store is not None:
         # the mixed factorization fell back to FP64: later requests reuse the FP64 factors
         fac = inf.pop("factor")
-        store({"contour": contour, "factor": fac}, 2 * 32 * dofs * dofs)
+        store({"contour": contour, "factor": fac, "plan": plan}, 2 * 32 * dofs * dofs)
     rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
     amp, sig, resid, info = {}, {}, {}, {}
     pending = []
     for i, pol in enumerate(("TE", "TM")):
         if pol not in req.pols:
             continue
-        a, s = engine.farfield(engine.radiating_density(contour, k, x[i]), bu, bdn, k)
+        rho = (engine.region_exterior_density(plan, mesh.n, x[i]) if plan is not None
+               else engine.radiating_density(contour, k, x[i]))
+        a, s = engine.farfield(rho, bu, bdn, k)
         pending.append((pol, a, s, rel[i]))
         # per result: refinement steps and error, the normwise backward error (the FP64 gate's
         # metric) and the precision the factor ended in

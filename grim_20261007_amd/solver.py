@@ -212,25 +212,36 @@ def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, m
 
 
 def system_dofs(mesh) -> int:
-    """Unknowns of the mesh's system: one per node, two (psi, phi) for a dielectric body."""
+    """Unknowns of the mesh's system: one per node, two (psi, phi) for a dielectric body, one per
+    node and bordering region for a multi-region layout."""
+    if mesh.regions is not None:
+        return engine.region_plan(mesh).n_dof
     return 2 * mesh.n if mesh.medium is not None else mesh.n
 
 
 def _systems(contour, k, angles, workspace):
-    """[TE, TM] systems, right-hand sides (2, A, DOFs) and the plain loads b_u, b_dn (A, N):
+    """[TE, TM] systems, right-hand sides (2, A, DOFs), the plain loads b_u, b_dn (A, N) and the map
+    from a solution (A, DOFs) to the single-layer density (A, N) that radiates its far field:
     the PEC combined potential (TM Dirichlet rows -b_u, TE flux rows -b_dn); for contours with
     surface impedances the flux rows dn(u) + alpha u of both polarizations
     (engine.impedance_systems, which reads the near blocks the assembly left in ``workspace``);
-    for a dielectric body the two-density system (engine.dielectric_systems)."""
+    for a dielectric body the two-density system (engine.dielectric_systems); for a multi-region
+    layout the regional system (engine.region_systems)."""
     bu, bdn = engine.loads(contour, k, angles)
-    if contour.mesh.medium is not None:
-        return engine.dielectric_systems(contour, k, workspace), engine.dielectric_rhs(bu, bdn), bu, bdn
+    mesh = contour.mesh
+    if mesh.regions is not None:
+        plan = engine.region_plan(mesh)
+        return (engine.region_systems(contour, k, workspace, plan), engine.region_rhs(plan, bu, bdn), bu, bdn,
+                lambda x: engine.region_exterior_density(plan, mesh.n, x))
+    if mesh.medium is not None:
+        return (engine.dielectric_systems(contour, k, workspace), engine.dielectric_rhs(bu, bdn), bu, bdn,
+                lambda x: engine.radiating_density(contour, k, x))
     z_tm, z_te = engine.assemble(contour, k, workspace)
     pair = z_tm._base                                                      # [TE, TM]
-    if contour.mesh.impedance is None:
-        return pair, torch.stack((-bdn, -bu)), bu, bdn
+    if mesh.impedance is None:
+        return pair, torch.stack((-bdn, -bu)), bu, bdn, lambda x: x
     return (engine.impedance_systems(contour, k, pair, workspace), engine.impedance_rhs(contour, k, angles),
-            bu, bdn)
+            bu, bdn, lambda x: x)
 
 
 def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("TE", "TM"),
@@ -248,7 +259,7 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     contour = engine.DeviceContour(mesh, dev)
     ws = engine.assembly_workspace(mesh.n, dev)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
-    pair, rhs_all, bu, bdn = _systems(contour, k, angles, ws)
+    pair, rhs_all, bu, bdn, radiate = _systems(contour, k, angles, ws)
     del ws
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
@@ -268,7 +279,8 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     out = {}
     for i, pol in enumerate(order):
         part = engine.system(final, i) if final.parts is not None else final
-        amp, sigma = engine.farfield(engine.radiating_density(contour, k, x[i]), bu, bdn, k)
+        rho = radiate(x[i])
+        amp, sigma = engine.farfield(rho, bu, bdn, k)
         res = {"amplitude": amp.cpu().numpy(), "sigma": sigma.cpu().numpy(),
                "residual": rel[i].cpu().numpy(),
                "info": {kk: v for kk, v in info.items() if kk not in ("residual", "factor")},
@@ -278,7 +290,8 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
             res["condition"] = float(conds[i])
             res["condition_method"] = methods[i]
         if return_density:
-            res["density"] = x[i, :, :mesh.n].cpu().numpy().T      # (N, A) nodal (exterior) densities
+            # (N, A) nodal densities: the exterior one of a multi-region layout, psi of a dielectric body
+            res["density"] = (rho if mesh.regions is not None else x[i, :, :mesh.n]).cpu().numpy().T
         out[pol] = res
     return out
 
@@ -506,7 +519,7 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
     ws = engine.assembly_workspace(mesh.n, dev)
     inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)
     obs = torch.tensor(observation_deg, dtype=torch.float64, device=dev)
-    pair, rhs_all, bu, bdn = _systems(contour, k, inc, ws)
+    pair, rhs_all, bu, bdn, radiate = _systems(contour, k, inc, ws)
     del ws
     if abort_event is not None and abort_event.is_set():
         raise InterruptedError("Solve canceled by user.")
@@ -524,7 +537,8 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
     rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
     final = info.get("factor", fac)
     bu_o, bdn_o = engine.loads(contour, k, obs)
-    amp, sigma = engine.bistatic_farfield(engine.radiating_density(contour, k, x), bu_o, bdn_o, k)
+    rho = torch.stack([radiate(x[i]) for i in range(x.shape[0])])
+    amp, sigma = engine.bistatic_farfield(rho, bu_o, bdn_o, k)
     out = {}
     for i, pol in enumerate(order):
         part = engine.system(final, i) if final.parts is not None else final
@@ -686,7 +700,14 @@ def compute_boundary_densities(geometry_snapshot: Dict[str, Any], frequency_ghz:
                           context="Boundary-density diagnostics")[pol]
     check_abort()
     nodal = res["density"][:, 0]
-    index = 1.0 if mesh.medium is None else max(1.0, abs(causal_index(mesh.medium[1], mesh.medium[2])))
+    if mesh.regions is not None:
+        flags = sorted(r for r in mesh.region_media if r > 0)
+        index = max([1.0] + [abs(causal_index(*mesh.region_media[r])) for r in flags])
+    elif mesh.medium is not None:
+        flags = [int(mesh.medium[0])]
+        index = max(1.0, abs(causal_index(mesh.medium[1], mesh.medium[2])))
+    else:
+        flags, index = [], 1.0
     ids = mesh.elements
     density = 0.5 * (nodal[ids[:, 0]] + nodal[ids[:, 1]])      # linear basis at t = 1/2
     centers = 0.5 * (mesh.p0 + mesh.p1)
@@ -705,7 +726,7 @@ def compute_boundary_densities(geometry_snapshot: Dict[str, Any], frequency_ghz:
         "length_units": "meters",
         "mesh_wavelength_m": float(C0 / (mesh_ghz * 1e9) / index),
         "mesh_max_refractive_index": float(index),
-        "mesh_material_flags": [] if mesh.medium is None else [int(mesh.medium[0])],
+        "mesh_material_flags": flags,
         "element_count": int(mesh.n),
         "node_count": int(mesh.n),
         "centers_x": centers[:, 0].tolist(),

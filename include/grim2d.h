@@ -83,16 +83,30 @@ int grim_assemble_near(int64_t n, const double* p0, const double* p1, const int3
                        double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
                        int64_t near_per_element, void* stream);
 
-/* S and K' (observation-normal derivative) + kp_mass M of one region's wavenumber, from the
- * same traversal and near rules as grim_assemble (no D, W or jump terms): the interior
- * operators of a dielectric body, twod/formulations/dielectric.py:64-69
- * (_assemble_linear_operator_matrices with single- and double-layer destinations), kp_mass =
- * 1/2 there.  Same arguments as grim_assemble, outputs s and kp (complex128 n x n, column-major;
- * they may be blocks of a larger matrix through ld_s, ld_kp). */
-int grim_assemble_plain(int64_t n, const double* p0, const double* p1, const int32_t* next, const int32_t* prev,
-                        const int32_t* fix_rows, int64_t nfix, double k, int far_order, double* s, int64_t ld_s,
-                        double* kp, int64_t ld_kp, double kp_mass, void* workspace, int64_t workspace_bytes,
-                        int64_t near_per_element, void* stream);
+/* The operators themselves, from the same traversal and near rules as grim_assemble, for
+ * formulations that weight them per region and interface: which = 1 gives a = S and
+ * b = K' (observation-normal derivative) + c M; which = 2 gives a = D + c M and b = W (Maue),
+ * with c = mass_coef[e] for each element e (device double[n]) or `mass` when mass_coef is NULL.
+ * Replaces twod/operators.py:2087-2607 (_assemble_linear_operator_matrices[_multi] with single-,
+ * double-layer and hypersingular outputs) as used by twod/formulations/dielectric.py:64-69 (S, K'
+ * + M/2 at the interior wavenumber) and twod/formulations/regions.py:239-333,
+ * combined_regions.py:116-146 (S, K', D, W of each region, the trace-jump mass in D).  Same
+ * arguments as grim_assemble otherwise; a and b are complex128 n x n, column-major, and may be
+ * blocks of larger matrices through ld_a, ld_b. */
+int grim_assemble_operators(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                            const int32_t* prev, const int32_t* fix_rows, int64_t nfix, double k, int far_order,
+                            int which, double* a, int64_t ld_a, double* b, int64_t ld_b, double mass,
+                            const double* mass_coef, void* workspace, int64_t workspace_bytes,
+                            int64_t near_per_element, void* stream);
+
+/* dst[rows_dst[i], cols_dst[j]] += weight[i] src[rows_src[i], cols_src[j]] for i < m, j < n
+ * (complex128, column-major; weight complex128 [m]; index arrays int32 device arrays; a
+ * negative destination index skips its row or column).  Composes a regional system, its loads
+ * and the exterior density from assembled operator blocks, as the reference's SystemScatter
+ * routes tiles (twod/assembly/scatter.py:65-116, formulations/regions.py:239-333, 557-609). */
+int grim_route(int64_t m, int64_t n, const double* src, int64_t ld_src, const int32_t* rows_src,
+               const int32_t* cols_src, double* dst, int64_t ld_dst, const int32_t* rows_dst,
+               const int32_t* cols_dst, const double* weight, void* stream);
 
 /* Scales the four n x n blocks of a complex128 2n x 2n column-major matrix in place:
  * block (i, j) (rows i n .. i n + n - 1, columns j n ..) by scale[i + 2 j] (complex128 host

@@ -836,6 +836,177 @@ def dielectric_system(mesh: Mesh, k0: float, z_tm0, z_te0, s1, kp1, factor, angl
     return z, np.concatenate((-bu, bdn), axis=0)
 
 
+def operator_set(mesh: Mesh, k: float, rows_per_chunk: int = 16):
+    """S, K', D and W (Maue) of one wavenumber over all element pairs, node order, no mass terms
+    (twod/operators.py:2087-2607; D(e, f) = K'(f, e)^T for near pairs)."""
+    E, N = len(mesh.length), len(mesh.nodes)
+    out = [np.zeros((N, N), complex) for _ in range(4)]
+    ids = mesh.elements
+    q = far_order(mesh, k)
+    nn = np.einsum("ed,fd->ef", mesh.normal, mesh.normal)
+    for start in range(0, E, rows_per_chunk):
+        rows = np.arange(start, min(E, start + rows_per_chunk))
+        far = far_mask(mesh, rows)
+        sb, kb, db = _far_rows(mesh, k, rows, far, q)
+        lp = mesh.length[rows][:, None] * mesh.length[None, :]
+        ssum = sb.sum(axis=(2, 3))
+        wb = (-(k * k) * nn[rows][:, :, None, None] * sb
+              + TANGENT[None, None] * (ssum / np.maximum(lp, EPS * EPS))[:, :, None, None])
+        for a in range(2):
+            for b in range(2):
+                r, c = ids[rows, a][:, None], ids[None, :, b]
+                for z, blk in zip(out, (sb, kb, db, wb)):
+                    np.add.at(z, (r, c), blk[:, :, a, b])
+    blocks = {}
+    near = np.argwhere(~far_mask(mesh))
+    for e, f in near:
+        for pair in ((int(e), int(f)), (int(f), int(e))):
+            if pair not in blocks:
+                blocks[pair] = near_blocks(mesh, k, *pair)
+    for e, f in near:
+        e, f = int(e), int(f)
+        sb, kb = blocks[(e, f)]
+        db = blocks[(f, e)][1].T
+        wb = maue(sb, k, mesh.normal[e], mesh.normal[f], mesh.length[e], mesh.length[f])
+        for a in range(2):
+            for b in range(2):
+                for z, blk in zip(out, (sb, kb, db, wb)):
+                    z[ids[e, a], ids[f, b]] += blk[a, b]
+    return tuple(out)
+
+
+def region_layout(mesh: Mesh, element_regions, media):
+    """Interfaces of a multi-region layout (twod/formulations/regions.py:66-119): elements grouped
+    by (minus, plus) region (-1 = conductor), interfaces in sorted key order with sorted node
+    lists, and one block of unknowns per interface side that borders a region (minus first).
+    ``media``: region -> (eps, mu); region 0 is air and carries the incident field."""
+    groups = {}
+    for e, key in enumerate(element_regions):
+        groups.setdefault(tuple(int(v) for v in key), []).append(e)
+    ifaces, region_ifaces, dofs, n_dof = [], {}, {}, 0
+    for mi, ((rm, rp), eids) in enumerate(sorted(groups.items())):
+        nodes = sorted({int(v) for e in eids for v in mesh.elements[e]})
+        ifaces.append(dict(r_m=rm, r_p=rp, eids=eids, nodes=nodes, n=len(nodes)))
+        for side, rid in (("minus", rm), ("plus", rp)):
+            if rid >= 0:
+                region_ifaces.setdefault(rid, []).append(mi)
+                dofs[mi, side] = (n_dof, len(nodes))
+                n_dof += len(nodes)
+    return dict(ifaces=ifaces, region_ifaces=region_ifaces, dofs=dofs, n_dof=n_dof, media=dict(media))
+
+
+def region_system(mesh: Mesh, layout, k0: float, pol: str, angles_deg):
+    """System, right-hand sides and exterior-density map of a multi-region layout without surface
+    impedances (twod/formulations/regions.py:239-333 base rows, combined_regions.py:21-83,116-146
+    D/W routes of closed interfaces, regions.py:557-609 loads and exterior density):
+    conductor rows carry +-M/2 + K' (TM: S) of their region plus K' (TM: S) from the region's
+    other interfaces; a transmission interface carries the flux row -M/2 + K'_m - (1/beta)(M/2 +
+    K'_p) and the trace row S_m - S_p with beta = mu_p/mu_m (TM) or eps_p/eps_m (TE); then every
+    closed interface adds gamma D (with the +-1/2 trace-jump mass) on trace rows and -gamma W on
+    flux rows of each observer in its region, gamma = (+-1)/(j k_region) by side."""
+    ifaces, dofs, media = layout["ifaces"], layout["dofs"], layout["media"]
+    key = 1 if pol == "TM" else 0                      # media[rid] = (eps, mu)
+    ks = {rid: k0 * complex(np.sqrt(complex(media[rid][0]) * complex(media[rid][1]))).real for rid in media}
+    ops = {rid: operator_set(mesh, ks[rid]) for rid in layout["region_ifaces"]}
+    mass = weighted_mass(mesh, np.ones(len(mesh.length)))
+    z = np.zeros((layout["n_dof"], layout["n_dof"]), complex)
+
+    def cols(mj, rid):
+        src = ifaces[mj]
+        start, count = dofs[mj, "minus" if src["r_m"] == rid else "plus"]
+        return src["nodes"], slice(start, start + count)
+
+    def inv_beta(ifc):
+        b = complex(media[ifc["r_p"]][key]) / complex(media[ifc["r_m"]][key])
+        return 1.0 / b
+
+    for mi, ifc in enumerate(ifaces):
+        rm, rp, nodes = ifc["r_m"], ifc["r_p"], ifc["nodes"]
+        m = mass[np.ix_(nodes, nodes)]
+        if rm < 0 or rp < 0:
+            rid = rp if rm < 0 else rm
+            rows = slice(*(lambda o, c: (o, o + c))(*dofs[mi, "plus" if rm < 0 else "minus"]))
+            s_k, kp_k = ops[rid][0], ops[rid][1]
+            for mj in layout["region_ifaces"][rid]:
+                src_nodes, cc = cols(mj, rid)
+                block = (s_k if pol == "TM" else kp_k)[np.ix_(nodes, src_nodes)].copy()
+                if mj == mi and pol == "TE":
+                    block += (0.5 if rm < 0 else -0.5) * m
+                z[rows, cc] += block
+        else:
+            ds, dt = dofs[mi, "minus"][0], dofs[mi, "plus"][0]
+            n = ifc["n"]
+            rs, rt = slice(ds, ds + n), slice(dt, dt + n)
+            inv = inv_beta(ifc)
+            for rid, sign, flux in ((rm, 1.0, 1.0), (rp, -1.0, inv)):
+                s_k, kp_k = ops[rid][0], ops[rid][1]
+                for mj in layout["region_ifaces"][rid]:
+                    src_nodes, cc = cols(mj, rid)
+                    kb = kp_k[np.ix_(nodes, src_nodes)].copy()
+                    if mj == mi:
+                        kb += (-0.5 if rid == rm else 0.5) * m
+                    z[rs, cc] += sign * flux * kb
+                    z[rt, cc] += sign * s_k[np.ix_(nodes, src_nodes)]
+    # D and W routes of closed interfaces
+    for rid, members in layout["region_ifaces"].items():
+        k = ks[rid]
+        d_k, w_k = ops[rid][2], ops[rid][3]
+        for mi in members:
+            src = ifaces[mi]
+            side = "minus" if src["r_m"] == rid else "plus"
+            gamma = (1.0 if side == "minus" else -1.0) / (1j * k)
+            src_nodes, cc = cols(mi, rid)
+            jump = (0.5 if side == "minus" else -0.5) * mass
+            for obs in members:
+                tgt = ifaces[obs]
+                o_nodes = tgt["nodes"]
+                dvals = d_k[np.ix_(o_nodes, src_nodes)] + jump[np.ix_(o_nodes, src_nodes)]
+                wvals = w_k[np.ix_(o_nodes, src_nodes)]
+                if tgt["r_m"] < 0 or tgt["r_p"] < 0:
+                    start, count = dofs[obs, "plus" if tgt["r_m"] < 0 else "minus"]
+                    rows = slice(start, start + count)
+                    if pol == "TM":
+                        z[rows, cc] += gamma * dvals          # every conductor node is a Dirichlet row
+                    else:
+                        z[rows, cc] += -gamma * wvals
+                else:
+                    fl, tr = dofs[obs, "minus"][0], dofs[obs, "plus"][0]
+                    n = tgt["n"]
+                    own = rid == tgt["r_m"]
+                    z[tr:tr + n, cc] += gamma * (1.0 if own else -1.0) * dvals
+                    z[fl:fl + n, cc] += -gamma * (1.0 if own else -inv_beta(tgt)) * wvals
+    # loads on the interfaces that border air, and the exterior density
+    bu, bdn = incident_loads(mesh, k0, angles_deg)
+    rhs = np.zeros((layout["n_dof"], bu.shape[1]), complex)
+    exterior = []
+    for mi, ifc in enumerate(ifaces):
+        rm, rp, nodes, n = ifc["r_m"], ifc["r_p"], ifc["nodes"], ifc["n"]
+        if rm < 0 or rp < 0:
+            if (rp if rm < 0 else rm) == 0:
+                o = dofs[mi, "plus" if rm < 0 else "minus"][0]
+                rhs[o:o + n] = -(bu if pol == "TM" else bdn)[nodes]
+                exterior.append((nodes, o))
+        else:
+            fl, tr = dofs[mi, "minus"][0], dofs[mi, "plus"][0]
+            if rm == 0:
+                rhs[fl:fl + n] -= bdn[nodes]
+                rhs[tr:tr + n] -= bu[nodes]
+                exterior.append((nodes, fl))
+            if rp == 0:
+                rhs[fl:fl + n] += inv_beta(ifc) * bdn[nodes]
+                rhs[tr:tr + n] += bu[nodes]
+                exterior.append((nodes, tr))
+    return z, rhs, exterior
+
+
+def exterior_density(mesh: Mesh, exterior, x):
+    """Nodal single-layer density of the air-side unknowns (regions.py:593-609), zero elsewhere."""
+    out = np.zeros((len(mesh.nodes), x.shape[1]), complex)
+    for nodes, offset in exterior:
+        out[nodes] += x[offset:offset + len(nodes)]
+    return out
+
+
 # ----------------------------------------------------------------------------
 # Condition estimate
 # ----------------------------------------------------------------------------

@@ -16,7 +16,7 @@ inputs (the geometry snapshot's vertices, frequency, angles) and the
 reference's outputs.  Nothing from the reference is stored except numbers.
 
 Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
-                                               --ibc-zones | --dielectric |
+                                               --ibc-zones | --dielectric | --regions |
                                                --condition]
 """
 from __future__ import annotations
@@ -535,6 +535,46 @@ def main_dielectric():
                   workloads.frequency_for_segments_per_wavelength(pts, 10.0), [0.0, 30.0, 180.0])
 
 
+def _region_snapshot(contours, media):
+    """contours: list of (points, N, TYPE, pos_mat, neg_mat) drawn as given (one segment each)."""
+    segments = []
+    for i, (pts, n_prop, seg_type, pos, neg) in enumerate(contours):
+        seg = workloads._snapshot(pts, f"r{i}")["segments"][0]
+        seg["seg_type"] = seg_type
+        seg["properties"] = [str(seg_type), str(n_prop), "0", str(pos), str(neg)]
+        segments.append(seg)
+    rows = [[str(flag), repr(e.real), repr(e.imag), repr(m.real), repr(m.imag)] for flag, (e, m) in media.items()]
+    return {"segments": segments, "ibcs": [], "dielectrics": rows}
+
+
+def main_regions():
+    """Multi-region layouts in air (twod/formulations/regions.py, combined_regions.py): a coated
+    PEC cylinder, a PEC cylinder beside a dielectric one, two dielectric media, and a two-layer
+    dielectric cylinder.  Every contour is drawn clockwise (TYPE 2/3 normals into air, TYPE 4/5
+    normals out of the inner region)."""
+    f = workloads.frequency_for_ka(1.0, 0.1)
+    run_zone_case("region_coated_pec",
+                  _region_snapshot([(workloads.circle_points(0.07, 24), 1, 4, 1, 0),
+                                    (workloads.circle_points(0.1, 40), 1, 3, 1, 0)], {1: (3.0 + 0j, 1.0 + 0j)}),
+                  f, [0.0, 90.0, 200.0], keep_matrix=True)
+    shift = np.array([0.3, 0.05])
+    run_zone_case("region_pec_beside_dielectric",
+                  _region_snapshot([(workloads.circle_points(0.1, 32), 1, 2, 0, 0),
+                                    (workloads.circle_points(0.08, 32) + shift, 1, 3, 1, 0)],
+                                   {1: (2.5 + 0j, 1.2 + 0j)}),
+                  f, [0.0, 45.0, 180.0, 300.0])
+    run_zone_case("region_two_media",
+                  _region_snapshot([(workloads.circle_points(0.1, 32), 1, 3, 1, 0),
+                                    (workloads.circle_points(0.08, 32) + shift, 1, 3, 2, 0)],
+                                   {1: (2.0 + 0j, 1.0 + 0j), 2: (4.0 + 0j, 1.5 + 0j)}),
+                  f, [0.0, 120.0, 250.0])
+    run_zone_case("region_two_layer",
+                  _region_snapshot([(workloads.circle_points(0.06, 24), 1, 5, 1, 2),
+                                    (workloads.circle_points(0.1, 40), 1, 3, 1, 0)],
+                                   {1: (2.0 + 0j, 1.0 + 0j), 2: (5.0 + 0j, 1.0 + 0j)}),
+                  f, [0.0, 60.0, 180.0])
+
+
 def run_condition_case(name, points, frequency_ghz, angles):
     """The reference's single-polarization entry (twod/solver.py:1984-2712) with
     compute_condition_number=True in both LU precisions: per channel the condition estimate and
@@ -608,6 +648,8 @@ if __name__ == "__main__":
         main_grim()
     elif "--bistatic" in sys.argv:
         main_bistatic()
+    elif "--regions" in sys.argv:
+        main_regions()
     elif "--dielectric" in sys.argv:
         main_dielectric()
     elif "--ibc-zones" in sys.argv:

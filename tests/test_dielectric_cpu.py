@@ -106,14 +106,14 @@ def test_dielectric_rows_and_refusals(tmp_path):
     # explicit counts are held to the medium's wavelength: 48 chords at eps 400 (n = 20) are too few
     with pytest.raises(ValueError, match="under-resolves"):
         build_mesh(_snapshot([["1", "400", "0", "1", "0"]]), "meters", 3 * f)
-    # several media, or conductors beside a dielectric, are multi-region layouts
+    # two media make a multi-region layout, not a dielectric body
     two = _snapshot([["1", "4", "0", "1", "0"], ["2", "2", "0", "1", "0"]])
     other = json.loads(json.dumps(two["segments"][0]))
     other["point_pairs"] = [{k: (v + 1.0) for k, v in p.items()} for p in other["point_pairs"]]
     other["properties"] = ["3", "1", "0", "2", "0"]
     two["segments"].append(other)
-    with pytest.raises(ValueError, match="Multi-region layouts"):
-        build_mesh(two, "meters", f)
+    mesh = build_mesh(two, "meters", f)
+    assert mesh.medium is None and mesh.region_media == {0: (1, 1), 1: (4, 1), 2: (2, 1)}
     # a CSV medium, interpolated at the solve frequency
     (tmp_path / "m.csv").write_text("frequency_hz,eps_real,eps_imag,mu_real,mu_imag\n1e8,4,0,1,0\n1e9,2,0,1.5,0\n")
     media = dielectric_media(_snapshot([["1", "m.csv"]]), str(tmp_path))

@@ -1,6 +1,6 @@
 """Homogeneous lossless dielectric bodies on the GPU against the reference's solves of the same
 snapshots (tests/golden/diel_*.npz): the two-density system (grim_assemble at k0 and
-grim_assemble_plain at k1 into the blocks of one matrix, grim_scale_blocks) entry by entry, the
+grim_assemble_operators at k1 into the blocks of one matrix, grim_scale_blocks) entry by entry, the
 exterior densities and amplitudes, the public entry in both precision forms, the serving loop,
 the bistatic diagonal, the boundary densities' labels, and the refusal of a lossy medium."""
 import json

@@ -1,0 +1,131 @@
+"""Multi-region layouts on the CPU (coated conductor, conductor beside a dielectric, two media, a
+two-layer dielectric; tests/golden/region_*.npz written by make_golden.py --regions): the product's
+meshes against the reference's, the oracle's restatement of the regional system
+(bem2d.region_system) against the reference's matrices, loads, densities and amplitudes, the
+product's route plan (engine.region_routes / region_load_routes, executed here with the oracle's
+operators) against the same matrices, and the layout refusals."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from grim_20261007_amd import engine, workloads
+from grim_20261007_amd.mesh import build_mesh
+from oracle import bem2d
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+CASES = ["region_coated_pec", "region_pec_beside_dielectric", "region_two_media", "region_two_layer"]
+
+
+def _case(name):
+    z = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)
+    return json.loads(str(z["snapshot_json"])), float(z["frequency_ghz"]), z["angles_deg"], z
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_region_meshes_match_reference(name):
+    snap, f, _, z = _case(name)
+    mesh = build_mesh(snap, "meters", f)
+    ref = np.arange(mesh.n) if mesh.ref_node is None else mesh.ref_node
+    assert np.array_equal(np.stack((ref, ref[mesh.next]), axis=1), z["elements"])
+    assert np.array_equal(mesh.nodes, z["nodes"][ref])
+    assert mesh.regions is not None and mesh.medium is None
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_region_system_matches_reference(name):
+    snap, f, angles, z = _case(name)
+    mesh = build_mesh(snap, "meters", f)
+    om = bem2d.mesh_from_nodes(z["nodes"], z["elements"])
+    # element e is the same panel in both numberings
+    layout = bem2d.region_layout(om, [tuple(r) for r in mesh.regions], mesh.region_media)
+    k0 = float(z["k0"])
+    for pol in ("TE", "TM"):
+        a, b, exterior = bem2d.region_system(om, layout, k0, pol, angles)
+        if f"Z_{pol}" in z:
+            assert np.max(np.abs(a - z[f"Z_{pol}"])) < 1e-14 * np.max(np.abs(z[f"Z_{pol}"]))
+        assert np.max(np.abs(b - z[f"rhs_{pol}"])) < 1e-15 * np.max(np.abs(b))
+        rho = bem2d.exterior_density(om, exterior, bem2d.solve(a, b))
+        assert np.max(np.abs(rho - z[f"density_{pol}"])) < 1e-12 * np.max(np.abs(rho))
+        amp = bem2d.farfield(om, k0, angles, rho)
+        assert np.max(np.abs(amp - z[f"amp_{pol}"]) / np.abs(z[f"amp_{pol}"])) < 1e-12
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_route_plan_restates_reference(name):
+    """The product's routes, executed with the oracle's S, K' + c M, D - c M, W of each region's
+    contours, give the reference's regional matrices and loads; the air-side unknowns give its
+    exterior densities."""
+    snap, f, angles, z = _case(name)
+    mesh = build_mesh(snap, "meters", f)
+    plan = engine.region_plan(mesh)
+    k0 = float(z["k0"])
+    ks = {rid: engine._region_wavenumber(plan.media, rid, k0) for rid in plan.subs}
+    ops = {}
+    for rid, sub in plan.subs.items():
+        els, local = sub["elements"], sub["local"]
+        sm = bem2d.mesh_from_nodes(mesh.p0[els], np.stack((np.arange(len(els)), local[mesh.next[els]]), axis=1))
+        s, kp, d, w = bem2d.operator_set(sm, ks[rid])
+        mc = bem2d.weighted_mass(sm, engine.region_mass_coefficients(plan, rid))
+        ops[rid] = (s, kp + mc, d - mc, w)
+    systems = np.zeros((2, plan.n_dof, plan.n_dof), complex)
+    for r in engine.region_routes(plan, ks.__getitem__):
+        rid, op = r.src
+        systems[r.pol][np.ix_(r.rows_dst, r.cols_dst)] += r.weight * ops[rid][op][np.ix_(r.rows_src, r.cols_src)]
+    full = bem2d.mesh_from_nodes(mesh.nodes, np.stack((np.arange(mesh.n), mesh.next), axis=1))
+    bu, bdn = bem2d.incident_loads(full, k0, angles)
+    loads = {"bu": bu, "bdn": bdn}
+    rhs = np.zeros((2, plan.n_dof, len(angles)), complex)
+    for r in engine.region_load_routes(plan):
+        rhs[r.pol][r.rows_dst] += r.weight * loads[r.src][r.rows_src]
+    ref = np.arange(mesh.n) if mesh.ref_node is None else mesh.ref_node
+    for i, pol in enumerate(("TE", "TM")):
+        if f"Z_{pol}" in z:
+            assert np.max(np.abs(systems[i] - z[f"Z_{pol}"])) < 1e-14 * np.max(np.abs(z[f"Z_{pol}"]))
+        assert np.max(np.abs(rhs[i] - z[f"rhs_{pol}"])) < 1e-15 * np.max(np.abs(rhs[i]))
+        x = np.linalg.solve(systems[i], rhs[i])
+        rho = np.zeros((mesh.n, len(angles)), complex)
+        for mi, ifc in enumerate(plan.ifaces):
+            for side, rid in (("minus", ifc["r_m"]), ("plus", ifc["r_p"])):
+                if rid == 0:
+                    o, c = plan.dofs[mi, side]
+                    rho[ifc["nodes"]] += x[o:o + c]
+        assert np.max(np.abs(rho - z[f"density_{pol}"][ref])) < 1e-12 * np.max(np.abs(rho))
+
+
+def _coated(inner_props, outer_props, media=(("1", "3", "0", "1", "0"),), inner=None):
+    pts_in = workloads.circle_points(0.07, 24) if inner is None else inner
+    snap = {"segments": [], "ibcs": [], "dielectrics": [list(m) for m in media]}
+    for pts, props in ((pts_in, inner_props), (workloads.circle_points(0.1, 40), outer_props)):
+        seg = workloads._snapshot(pts, "c")["segments"][0]
+        seg["properties"] = props
+        snap["segments"].append(seg)
+    return snap
+
+
+def test_region_refusals():
+    f = workloads.frequency_for_ka(1.0, 0.1)
+    assert build_mesh(_coated(["4", "1", "0", "1", "0"], ["3", "1", "0", "1", "0"]), "meters", f).regions is not None
+    with pytest.raises(ValueError, match="TYPE 4 panel 'c' requires pos_mat > 0"):
+        build_mesh(_coated(["4", "1", "0", "0", "0"], ["3", "1", "0", "1", "0"]), "meters", f)
+    with pytest.raises(ValueError, match="requires pos_mat > 0 and neg_mat > 0"):
+        build_mesh(_coated(["5", "1", "0", "1", "0"], ["3", "1", "0", "1", "0"]), "meters", f)
+    with pytest.raises(ValueError, match="orientation"):
+        build_mesh(_coated(["4", "1", "0", "1", "0"], ["3", "1", "0", "1", "0"],
+                           inner=workloads.circle_points(0.07, 24)[::-1]), "meters", f)
+    with pytest.raises(ValueError, match="Surface impedances in multi-region layouts"):
+        snap = _coated(["4", "1", "1", "1", "0"], ["3", "1", "0", "1", "0"])
+        snap["ibcs"] = [["1", "constant", "50", "0", "0", "0"]]
+        build_mesh(snap, "meters", f)
+    with pytest.raises(ValueError, match="sheets"):
+        build_mesh(_coated(["1", "1", "1", "0", "0"], ["3", "1", "0", "1", "0"]), "meters", f)
+    # one contour drawn as a TYPE 3 half and a TYPE 2 half: a junction between boundary kinds
+    snap = workloads._snapshot(workloads.circle_points(0.1, 40), "j")
+    seg = snap["segments"][0]
+    half = dict(seg, point_pairs=seg["point_pairs"][20:], properties=["2", "1", "0", "0", "0"])
+    seg["point_pairs"], seg["properties"] = seg["point_pairs"][:20], ["3", "1", "0", "1", "0"]
+    snap["segments"].append(half)
+    snap["dielectrics"] = [["1", "3", "0", "1", "0"]]
+    with pytest.raises(ValueError, match="Junctions between boundary kinds"):
+        build_mesh(snap, "meters", f)
