@@ -94,32 +94,45 @@ __device__ __forceinline__ void far_pair(const Elem& o, const Elem& s, double k,
         const double ti = Rule<Q>::t(qi), wi = Rule<Q>::w(qi);
         const double xo = fma(ti, o.sx, o.x0), yo = fma(ti, o.sy, o.y0);
         cplx ps0 = {0, 0}, ps1 = {0, 0}, pk0 = {0, 0}, pk1 = {0, 0}, pd0 = {0, 0}, pd1 = {0, 0};
-        for (int qj = 0; qj < Q; ++qj) {
-            const double tj = Rule<Q>::t(qj), wj = Rule<Q>::w(qj);
-            const double dx = xo - fma(tj, s.sx, s.x0);
-            const double dy = yo - fma(tj, s.sy, s.y0);
-            // r and 1/r from the hardware 1/sqrt estimate: two Newton steps for 1/r, one
-            // residual correction for r (r is the Hankel phase argument, so it is kept
-            // to the rounding of a correctly rounded sqrt)
-            const double r2 = fmax(fma(dx, dx, dy * dy), GRIM_EPS * GRIM_EPS);
-            double inv = __builtin_amdgcn_rsq(r2);
-            inv = inv * fma(-0.5 * r2, inv * inv, 1.5);
-            inv = inv * fma(-0.5 * r2, inv * inv, 1.5);
-            double r = r2 * inv;
-            r = fma(0.5 * inv, fma(-r, r, r2), r);
-            cplx g, h;
-            green_kernels(k, r, g, h);
-            const double po = (dx * o.nx + dy * o.ny) * inv;
-            const double pso = (dx * s.nx + dy * s.ny) * inv;
-            const double w0 = wj * (1.0 - tj), w1 = wj * tj;
-            ps0.re = fma(w0, g.re, ps0.re); ps0.im = fma(w0, g.im, ps0.im);
-            ps1.re = fma(w1, g.re, ps1.re); ps1.im = fma(w1, g.im, ps1.im);
-            const double kr_ = -h.re * po, ki_ = -h.im * po;
-            pk0.re = fma(w0, kr_, pk0.re); pk0.im = fma(w0, ki_, pk0.im);
-            pk1.re = fma(w1, kr_, pk1.re); pk1.im = fma(w1, ki_, pk1.im);
-            const double dr_ = h.re * pso, di_ = h.im * pso;
-            pd0.re = fma(w0, dr_, pd0.re); pd0.im = fma(w0, di_, pd0.im);
-            pd1.re = fma(w1, dr_, pd1.re); pd1.im = fma(w1, di_, pd1.im);
+        // two source points per step (Q is even): the pair's Hankel evaluations run as two
+        // independent chains; the sums take the points in order, as one point per step would
+        // unrolled for Q = 8 (fastest measured); kept rolled for Q = 16, which would spill
+        constexpr int QJ_UNROLL = Q == 8 ? Q / 2 : 1;
+#pragma unroll QJ_UNROLL
+        for (int qj = 0; qj < Q; qj += 2) {
+            double po[2], pso[2], r[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const double tj = Rule<Q>::t(qj + u);
+                const double dx = xo - fma(tj, s.sx, s.x0);
+                const double dy = yo - fma(tj, s.sy, s.y0);
+                // r and 1/r from the hardware 1/sqrt estimate: two Newton steps for 1/r, one
+                // residual correction for r (r is the Hankel phase argument, so it is kept
+                // to the rounding of a correctly rounded sqrt)
+                const double r2 = fmax(fma(dx, dx, dy * dy), GRIM_EPS * GRIM_EPS);
+                double iv = __builtin_amdgcn_rsq(r2);
+                iv = iv * fma(-0.5 * r2, iv * iv, 1.5);
+                iv = iv * fma(-0.5 * r2, iv * iv, 1.5);
+                double rr = r2 * iv;
+                r[u] = fma(0.5 * iv, fma(-rr, rr, r2), rr);
+                po[u] = (dx * o.nx + dy * o.ny) * iv;
+                pso[u] = (dx * s.nx + dy * s.ny) * iv;
+            }
+            cplx g[2], h[2];
+            green_kernels2(k, r[0], r[1], g[0], h[0], g[1], h[1]);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const double tj = Rule<Q>::t(qj + u), wj = Rule<Q>::w(qj + u);
+                const double w0 = wj * (1.0 - tj), w1 = wj * tj;
+                ps0.re = fma(w0, g[u].re, ps0.re); ps0.im = fma(w0, g[u].im, ps0.im);
+                ps1.re = fma(w1, g[u].re, ps1.re); ps1.im = fma(w1, g[u].im, ps1.im);
+                const double kr_ = -h[u].re * po[u], ki_ = -h[u].im * po[u];
+                pk0.re = fma(w0, kr_, pk0.re); pk0.im = fma(w0, ki_, pk0.im);
+                pk1.re = fma(w1, kr_, pk1.re); pk1.im = fma(w1, ki_, pk1.im);
+                const double dr_ = h[u].re * pso[u], di_ = h[u].im * pso[u];
+                pd0.re = fma(w0, dr_, pd0.re); pd0.im = fma(w0, di_, pd0.im);
+                pd1.re = fma(w1, dr_, pd1.re); pd1.im = fma(w1, di_, pd1.im);
+            }
         }
         const double wo0 = wi * (1.0 - ti), wo1 = wi * ti;
         S[0][0] = cadd(S[0][0], cscale(ps0, wo0)); S[0][1] = cadd(S[0][1], cscale(ps1, wo0));
@@ -201,7 +214,7 @@ __device__ __forceinline__ void store_add(double* z, int64_t ld, int p, int q, c
 // is not lane - 1 (lane 0, contour starts) are completed by far_fixup.  MODE selects the two
 // outputs (see far_pair).
 template <int Q, int MODE>
-__global__ void __launch_bounds__(64) far_kernel(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) far_kernel(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                  const int* __restrict__ next, const int* __restrict__ prev,
                                                  double k, int chunk, double* __restrict__ ztm, int64_t ldtm,
                                                  double* __restrict__ zte, int64_t ldte) {

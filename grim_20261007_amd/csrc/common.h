@@ -154,6 +154,120 @@ __device__ __noinline__ HK hankel_quarter_v(double x) {
     }
     return HK{g.re, g.im, h1.re, h1.im};
 }
+// Two arguments at once, for the far quadrature: the same operations as hankel_quarter_v on each
+// argument (so the same values, bit for bit), written as two independent chains that the
+// scheduler interleaves; the range forms are selected per argument instead of branched.
+struct HK2 { HK a, b; };
+
+__device__ __forceinline__ void hk_phase(double x, double& cps, double& cms) {
+    const double n = rint(x * 0.6366197723675814);
+    double t = fma(-n, 1.5707963267948966, x);
+    t = fma(-n, 6.123233995736766e-17, t);
+    const double t2 = t * t;
+    double ps = -7.647163731819816e-13;
+    ps = fma(ps, t2, 1.6059043836821613e-10);
+    ps = fma(ps, t2, -2.505210838544172e-08);
+    ps = fma(ps, t2, 2.7557319223985893e-06);
+    ps = fma(ps, t2, -1.984126984126984e-04);
+    ps = fma(ps, t2, 8.333333333333333e-03);
+    ps = fma(ps, t2, -1.6666666666666666e-01);
+    const double sn = fma(ps * t2, t, t);
+    double pc = 4.779477332387385e-14;
+    pc = fma(pc, t2, -1.1470745597729725e-11);
+    pc = fma(pc, t2, 2.08767569878681e-09);
+    pc = fma(pc, t2, -2.755731922398589e-07);
+    pc = fma(pc, t2, 2.48015873015873e-05);
+    pc = fma(pc, t2, -1.388888888888889e-03);
+    pc = fma(pc, t2, 4.1666666666666664e-02);
+    const double cs = fma(fma(pc, t2, -0.5), t2, 1.0);
+    const int q = (int)n & 3;
+    const double a = cs + sn, b = cs - sn;
+    cps = (q & 1) ? b : a;
+    cms = (q & 1) ? -a : b;
+    if (q & 2) { cps = -cps; cms = -cms; }
+}
+
+__device__ __forceinline__ HK hk_combine(double amp, double p0, double q0, double p1, double q1, double cps,
+                                         double cms) {
+    const double u0 = cps, v0 = cms;
+    const double h0re = amp * (p0 * u0 + q0 * v0), h0im = amp * (p0 * v0 - q0 * u0);
+    const double u1 = -cms, v1 = cps;
+    const double h1re = amp * (p1 * u1 + q1 * v1), h1im = amp * (p1 * v1 - q1 * u1);
+    return HK{-0.25 * h0im, 0.25 * h0re, -0.25 * h1im, 0.25 * h1re};
+}
+
+__device__ __forceinline__ HK hk_small(double x) {
+    const double y = 0.25 * x * x;
+    const double L = log(0.5 * x);
+    const double j0v = horner(GRIM_HK_A0, y);
+    const double j1v = 0.5 * x * horner(GRIM_HK_A1, y);
+    const double y0v = (2.0 / GRIM_PI) * L * j0v + horner(GRIM_HK_B0, y);
+    const double y1v = (2.0 / GRIM_PI) * L * j1v - 2.0 / (GRIM_PI * x) + 0.5 * x * horner(GRIM_HK_B1, y);
+    return HK{0.25 * y0v, 0.25 * j0v, 0.25 * y1v, 0.25 * j1v};
+}
+
+__device__ __forceinline__ HK hk_select(bool c, const HK& u, const HK& v) {
+    return HK{c ? u.gr : v.gr, c ? u.gi : v.gi, c ? u.hr : v.hr, c ? u.hi : v.hi};
+}
+
+__device__ __noinline__ HK2 hankel_quarter_v2(double xa, double xb) {
+    double rsa = __builtin_amdgcn_rsq(xa), rsb = __builtin_amdgcn_rsq(xb);
+    rsa = rsa * fma(-0.5 * xa, rsa * rsa, 1.5);
+    rsb = rsb * fma(-0.5 * xb, rsb * rsb, 1.5);
+    rsa = rsa * fma(-0.5 * xa, rsa * rsa, 1.5);
+    rsb = rsb * fma(-0.5 * xb, rsb * rsb, 1.5);
+    const double inva = rsa * rsa, invb = rsb * rsb;
+    const double ampa = (0.7978845608028654 * GRIM_SQRT_HALF) * rsa;
+    const double ampb = (0.7978845608028654 * GRIM_SQRT_HALF) * rsb;
+    // x >= 20 form for both
+    const double ra = GRIM_HANKEL_XM * inva, rb = GRIM_HANKEL_XM * invb;
+    const double wa = ra * ra, wb = rb * rb;
+    double p0a = horner(GRIM_HK_P0, wa), p0b = horner(GRIM_HK_P0, wb);
+    double q0a = ra * horner(GRIM_HK_Q0, wa), q0b = rb * horner(GRIM_HK_Q0, wb);
+    double p1a = horner(GRIM_HK_P1, wa), p1b = horner(GRIM_HK_P1, wb);
+    double q1a = ra * horner(GRIM_HK_Q1, wa), q1b = rb * horner(GRIM_HK_Q1, wb);
+    if (xa < GRIM_HANKEL_XM || xb < GRIM_HANKEL_XM) {
+        const double sa = GRIM_HANKEL_XS * inva, sb = GRIM_HANKEL_XS * invb;
+        const double ua = fma(sa * sa, GRIM_HANKEL_MID_SCALE, GRIM_HANKEL_MID_SHIFT);
+        const double ub = fma(sb * sb, GRIM_HANKEL_MID_SCALE, GRIM_HANKEL_MID_SHIFT);
+        const double mp0a = horner(GRIM_HK_P0M, ua), mp0b = horner(GRIM_HK_P0M, ub);
+        const double mq0a = sa * horner(GRIM_HK_Q0M, ua), mq0b = sb * horner(GRIM_HK_Q0M, ub);
+        const double mp1a = horner(GRIM_HK_P1M, ua), mp1b = horner(GRIM_HK_P1M, ub);
+        const double mq1a = sa * horner(GRIM_HK_Q1M, ua), mq1b = sb * horner(GRIM_HK_Q1M, ub);
+        const bool ma = xa < GRIM_HANKEL_XM, mb = xb < GRIM_HANKEL_XM;
+        p0a = ma ? mp0a : p0a; q0a = ma ? mq0a : q0a; p1a = ma ? mp1a : p1a; q1a = ma ? mq1a : q1a;
+        p0b = mb ? mp0b : p0b; q0b = mb ? mq0b : q0b; p1b = mb ? mp1b : p1b; q1b = mb ? mq1b : q1b;
+    }
+    double cpsa, cmsa, cpsb, cmsb;
+    hk_phase(xa, cpsa, cmsa);
+    hk_phase(xb, cpsb, cmsb);
+    if (xa >= 1.0e9 || xb >= 1.0e9) {
+        double s0, c0;
+        if (xa >= 1.0e9) { sincos(xa, &s0, &c0); cpsa = c0 + s0; cmsa = c0 - s0; }
+        if (xb >= 1.0e9) { sincos(xb, &s0, &c0); cpsb = c0 + s0; cmsb = c0 - s0; }
+    }
+    HK a = hk_combine(ampa, p0a, q0a, p1a, q1a, cpsa, cmsa);
+    HK b = hk_combine(ampb, p0b, q0b, p1b, q1b, cpsb, cmsb);
+    if (xa < GRIM_HANKEL_XS || xb < GRIM_HANKEL_XS) {
+        if (xa < GRIM_HANKEL_XS) a = hk_small(xa);
+        if (xb < GRIM_HANKEL_XS) b = hk_small(xb);
+    }
+    return HK2{a, b};
+}
+
+// green_kernels for two distances at once (see hankel_quarter_v2).
+__device__ __forceinline__ void green_kernels2(double k, double ra, double rb, cplx& ga, cplx& ha, cplx& gb,
+                                               cplx& hb) {
+    double xa = k * (ra > GRIM_EPS ? ra : GRIM_EPS), xb = k * (rb > GRIM_EPS ? rb : GRIM_EPS);
+    if (xa < 1e-12) xa = 1e-12;
+    if (xb < 1e-12) xb = 1e-12;
+    const HK2 v = hankel_quarter_v2(xa, xb);
+    ga = {v.a.gr, v.a.gi};
+    ha = cscale(cplx{v.a.hr, v.a.hi}, k);
+    gb = {v.b.gr, v.b.gi};
+    hb = cscale(cplx{v.b.hr, v.b.hi}, k);
+}
+
 __device__ __forceinline__ void hankel_quarter(double x, cplx& g, cplx& h1) {
     const HK v = hankel_quarter_v(x);
     g = {v.gr, v.gi};
