@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=128, help="requests in the timed serving region")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workers", type=int, default=4, help="requests in flight in the serving form")
+    ap.add_argument("--batch", type=int, default=4,
+                    help="requests of equal size a serving worker runs as one batched factor + solve")
     ap.add_argument("--n-side", type=int, default=1000)
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -273,12 +275,13 @@ def far_valu():
             "source": "profiles/far_pass_naca0012_n2000.md (rocprofv3 --pmc VALUBusy VALUUtilization)"}
 
 
-def serve(engine_mod, mesh, f, angles, precision, workers, nreq, warm):
-    """Closed-loop serving: nreq distinct requests (no cache reuse) with `workers` in flight.
-    Returns the results, the elapsed time, service times (ms) and the precision each request
-    ran in ('auto' resolves per request by the flop model)."""
+def serve(engine_mod, mesh, f, angles, precision, workers, nreq, warm, batch=1):
+    """Closed-loop serving: nreq distinct requests (no cache reuse) with `workers` in flight, each
+    worker running up to `batch` waiting requests as one batch.  Returns the results, the elapsed
+    time, service times (ms) and the precision each request ran in ('auto' resolves per request
+    by the flop model)."""
     from grim_20261007_amd import serving
-    srv = serving.Server(workers=workers)
+    srv = serving.Server(workers=workers, batch=batch)
     try:
         serving.run_stream(srv, [serving.Request(mesh, f, angles, precision=precision, key=f"w{precision}{i}")
                                  for i in range(warm)])
@@ -291,7 +294,7 @@ def serve(engine_mod, mesh, f, angles, precision, workers, nreq, warm):
     return res, elapsed, svc, chosen
 
 
-def stream_summary(workers: int, count: int = 32, seed: int = 11, rate_hz: float = 200.0):
+def stream_summary(workers: int, count: int = 32, seed: int = 11, rate_hz: float = 200.0, batch: int = 1):
     """The Poisson serving stream (star-shaped curves, N log-uniform in [200, 12000], 90 / 360 /
     1440 angles, 30 % reusing an earlier system): plain FP64 back to back against the serving
     form in the cost-model precision ('auto') and in forced mixed precision, on the same seeded
@@ -313,7 +316,7 @@ def stream_summary(workers: int, count: int = 32, seed: int = 11, rate_hz: float
            "fp64_plain": {"value": round(2 * len(stream) / t64, 3), "p99_ms": round(float(np.percentile(lat64, 99)), 3),
                           "form": "back to back, no arrivals, no reuse"}}
     for precision in ("auto", "mixed"):
-        srv = serving.Server(workers=workers)
+        srv = serving.Server(workers=workers, batch=batch)
         try:
             serving.run_stream(srv, [serving.Request(ms[0], freqs[0], angs[0], key="warm")])
             reqs = [serving.Request(m, f, a, precision=precision, key=it["key"])
@@ -574,10 +577,11 @@ def main():
     warm = max(args.warmup, 2 * args.workers)
     # the first server of a process runs about 12 % slower whatever its form (measured: 253 against
     # 288 solves/s for the same requests in the next server), so one untimed server goes first
-    serve(engine, mesh, f, angles, "auto", args.workers, warm, args.workers)
-    res_a, t_a, svc_a, chosen = serve(engine, mesh, f, angles, "auto", args.workers, args.steps, warm)
-    res_m, t_m, svc_m, _ = serve(engine, mesh, f, angles, "mixed", args.workers, args.steps, warm)
-    res_d, t_d, svc_d, _ = serve(engine, mesh, f, angles, "double", args.workers, args.steps, warm)
+    bt = args.batch
+    serve(engine, mesh, f, angles, "auto", args.workers, warm, args.workers, bt)
+    res_a, t_a, svc_a, chosen = serve(engine, mesh, f, angles, "auto", args.workers, args.steps, warm, bt)
+    res_m, t_m, svc_m, _ = serve(engine, mesh, f, angles, "mixed", args.workers, args.steps, warm, bt)
+    res_d, t_d, svc_d, _ = serve(engine, mesh, f, angles, "double", args.workers, args.steps, warm, bt)
     value = 2 * args.steps / t_a
     # every served mixed result within the tolerance of the FP64 path
     ref_sig = {pol: results["double"][0][pol][1].cpu().numpy() for pol in ("TE", "TM")}
@@ -617,7 +621,7 @@ def main():
         if stage:
             tf = lu_flops / (stage * 1e-3) / 1e12
             lu[key] = {"factor_ms": stage, "tflops": round(tf, 2), f"frac_of_{label}_peak": round(tf / peak, 4)}
-    stream = stream_summary(args.workers) if args.stream_count > 0 else None
+    stream = stream_summary(args.workers, batch=args.batch) if args.stream_count > 0 else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mesh, f, angles, args.cpu_seconds)
@@ -636,9 +640,13 @@ def main():
         "data": "synthetic NACA 0012 geometry",
         "config": {"workload": f"NACA 0012, chord 1 m, N={mesh.n} linear Galerkin unknowns, f={f:.4f} GHz "
                                f"(longest panel lambda/10), {len(angles)} monostatic angles, TE+TM co-polarized "
-                               f"requests; serving form with {args.workers} requests in flight (distinct requests, "
-                               "no cache reuse), each request in the precision the flop model picks for it",
+                               f"requests; serving form with {args.workers} workers in flight"
+                               + (f", each running up to {args.batch} waiting requests as one batched factor + "
+                                  "solve" if args.batch > 1 else "")
+                               + " (distinct requests, no cache reuse), each request in the precision the flop "
+                                 "model picks for it",
                    "n": mesh.n, "angles": len(angles), "frequency_ghz": f, "workers": args.workers,
+                   "batch": args.batch,
                    "precision_choice": chosen},
         "p50_ms": round(float(np.percentile(svc_a, 50)), 3), "p99_ms": round(float(np.percentile(svc_a, 99)), 3),
         "serving_vs_plain_fp64": round(value / plain64, 3),

@@ -63,15 +63,22 @@ def assembly_workspace(n: int, device) -> torch.Tensor:
     return torch.empty(int(nbytes), dtype=torch.uint8, device=device)
 
 
-def assemble(contour: DeviceContour, k: float, workspace: torch.Tensor | None = None, check: bool = True):
+def assemble(contour: DeviceContour, k: float, workspace: torch.Tensor | None = None, check: bool = True,
+             out: torch.Tensor | None = None):
     """TM and TE system matrices (complex128, column-major) from one fused traversal.
 
     Returns (z_tm, z_te), two views of one (2, N, N) tensor ``pair`` = [TE, TM]
     (also reachable as ``z_tm._base``) so that both can be factored and solved in
-    batched launches."""
+    batched launches.  ``out``, a contiguous (2, N, N) complex128 tensor (e.g. two systems of a
+    larger batch), receives the pair instead of a new allocation."""
     n = contour.n
     dev = contour.device
-    pair = torch.empty((2, n, n), dtype=torch.complex128, device=dev)
+    if out is None:
+        pair = torch.empty((2, n, n), dtype=torch.complex128, device=dev)
+    else:
+        if out.shape != (2, n, n) or out.dtype != torch.complex128 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (2, N, N) complex128 tensor")
+        pair = out
     z_te, z_tm = pair[0], pair[1]
     ws = workspace if workspace is not None else assembly_workspace(n, dev)
     st = _stream(dev)
@@ -650,6 +657,19 @@ def system(f: Factor, i: int) -> Factor:
                   None if f.z is None else f.z[i:i + 1], f.n, 1, f.fallback_reason)
 
 
+def systems(f: Factor, i0: int, i1: int) -> Factor:
+    """Factor of systems i0 .. i1-1 of a batch (views, no copies; they keep the whole batch's
+    storage alive)."""
+    if f.parts is not None:
+        parts = f.parts[i0:i1]
+        reason = next((p.fallback_reason for p in parts if p.fallback_reason), "")
+        return Factor(f.precision, None, None, None, None if f.z is None else f.z[i0:i1], f.n, i1 - i0, reason,
+                      parts=parts)
+    return Factor(f.precision, f.lu[i0:i1], f.piv[i0:i1], f.aux[i0:i1], None if f.z is None else f.z[i0:i1],
+                  f.n, i1 - i0, f.fallback_reason,
+                  norm_inf=None if f.norm_inf is None else f.norm_inf[i0:i1])
+
+
 def _solve_parts(parts, bb: torch.Tensor, gate: bool):
     """Solve each system with its own factor and merge the results into one batch."""
     xs, infos = [], []
@@ -668,6 +688,15 @@ def _solve_parts(parts, bb: torch.Tensor, gate: bool):
     errs = [inf["refinement_error"] for inf in infos if "refinement_error" in inf]
     if errs:
         info["refinement_error"] = max(errs)
+    # per-system statements, one entry per system of the batch
+    for name, fallback in (("backward_errors", "backward_error"), ("refinement_errors", "refinement_error"),
+                           ("steps", "refinement_steps")):
+        vals = [inf[name][0] if name in inf else inf.get(fallback) for inf in infos]
+        if any(v is not None for v in vals):
+            info[name] = vals
+    bes = [v for v in info.get("backward_errors", []) if v is not None]
+    if bes:
+        info["backward_error"] = max(bes)
     if any(a is not b for a, b in zip(new_parts, parts)):
         head = parts[0]
         info["factor"] = Factor("mixed", None, None, None, torch.cat([p.z for p in new_parts]), head.n,
@@ -748,15 +777,28 @@ def solve(f: Factor, b: torch.Tensor, gate: bool = True):
     # that residual: a per-result statement of its accuracy next to the refinement error
     if f.norm_inf is None:
         f.norm_inf = norm_inf(f.z)
-    info["backward_error"] = backward_error(f.norm_inf, x, bb, info["residual"])
+    errs = backward_errors(f.norm_inf, x, bb, info["residual"])
+    info["backward_error"] = float(errs.max())
+    info["backward_errors"] = [float(e) for e in errs]
+    # the refinement's own measure per system: max over columns of max|r| / max|b| (max|b| = 0
+    # counted as 1, as the refinement does)
+    bmax = torch.amax(torch.abs(bb), dim=2)
+    rmax = torch.amax(torch.abs(info["residual"]), dim=2)
+    info["refinement_errors"] = [float(e) for e in
+                                 torch.amax(rmax / torch.where(bmax > 0, bmax, torch.ones_like(bmax)), dim=1).cpu()]
     return (x if b.dim() == 3 else x[0]), info
 
 
-def backward_error(a_inf: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor) -> float:
-    """max over columns of max|r| / (||Z||_inf max|x| + max|b|), batched (B, A, N)."""
+def backward_errors(a_inf: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor) -> np.ndarray:
+    """Per system of a batch (B, A, N): max over columns of max|r| / (||Z||_inf max|x| + max|b|)."""
     ri = torch.amax(torch.abs(r), dim=2)
     den = a_inf[:, None] * torch.amax(torch.abs(x), dim=2) + torch.amax(torch.abs(b), dim=2)
-    return float(torch.max(ri / den))
+    return torch.amax(ri / den, dim=1).cpu().numpy()
+
+
+def backward_error(a_inf: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor) -> float:
+    """max over systems and columns of max|r| / (||Z||_inf max|x| + max|b|), batched (B, A, N)."""
+    return float(np.max(backward_errors(a_inf, x, b, r)))
 
 
 def ctypes_int():
@@ -770,30 +812,39 @@ def ctypes_double():
 
 
 def _backward_error_gate(f: Factor, x: torch.Tensor, b: torch.Tensor):
-    """Normwise backward error <= 1e-12 with at most two corrections (linalg/dense.py:257-280)."""
+    """Normwise backward error <= 1e-12 with at most two corrections (linalg/dense.py:257-280).
+
+    Each system of a batch is gated on its own, as the reference gates each matrix: a system
+    within the limit takes no correction, a correction is kept only if it lowers that system's
+    error, and its corrections run as batch-1 calls.  So a batch of several requests' systems
+    returns, system by system, what separate calls return."""
     z = f.z
     if f.norm_inf is None:
         f.norm_inf = norm_inf(z)
     a_inf = f.norm_inf                                                  # (B,)
-    steps = 0
-
-    def metrics(xx):
-        r = residual(z, xx, b)
-        return r, backward_error(a_inf, xx, b, r)
-
-    r, err = metrics(x)
-    for _ in range(2):
-        if err <= BACKWARD_ERROR_MAX:
-            break
-        cand = x + lu_solve_inplace(f, r.clone())
-        r2, err2 = metrics(cand)
-        if err2 >= err:
-            break
-        x, r, err = cand, r2, err2
-        steps += 1
-    if not math.isfinite(err) or err > BACKWARD_ERROR_MAX:
-        raise RuntimeError(f"normwise backward error {err} exceeds the release limit {BACKWARD_ERROR_MAX}.")
-    return x, {"refinement_steps": steps, "backward_error": err, "residual": r}
+    r = residual(z, x, b)
+    errs = backward_errors(a_inf, x, b, r)
+    nb = errs.shape[0]
+    steps = np.zeros(nb, dtype=np.int64)
+    for i in range(nb):
+        for _ in range(2):
+            if errs[i] <= BACKWARD_ERROR_MAX:
+                break
+            fi = system(f, i)
+            cand = x[i:i + 1] + lu_solve_inplace(fi, r[i:i + 1].clone())
+            r2 = residual(z[i:i + 1], cand, b[i:i + 1])
+            err2 = float(backward_errors(a_inf[i:i + 1], cand, b[i:i + 1], r2)[0])
+            if err2 >= errs[i]:
+                break
+            x[i:i + 1] = cand
+            r[i:i + 1] = r2
+            errs[i] = err2
+            steps[i] += 1
+    bad = [float(e) for e in errs if not (math.isfinite(float(e)) and e <= BACKWARD_ERROR_MAX)]
+    if bad:
+        raise RuntimeError(f"normwise backward error {bad[0]} exceeds the release limit {BACKWARD_ERROR_MAX}.")
+    return x, {"refinement_steps": int(steps.max()), "backward_error": float(errs.max()),
+               "backward_errors": [float(e) for e in errs], "steps": [int(v) for v in steps], "residual": r}
 
 
 def norm_inf(z: torch.Tensor) -> torch.Tensor:

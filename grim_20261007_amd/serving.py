@@ -17,10 +17,9 @@ before it is allocated.
 from __future__ import annotations
 
 import hashlib
-import queue
 import threading
 import time
-from collections import OrderedDict
+from collections import OrderedDict, deque
 from concurrent.futures import Future
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
@@ -128,10 +127,32 @@ class SystemCache:
             return True
 
 
+def batch_signature(req: Request) -> Optional[tuple]:
+    """Requests with equal signatures can share factor and solve launches: plain PEC contours
+    (one unknown per node) of the same size, angle count and precision form.  None: the request
+    runs on its own (impedance, dielectric and multi-region systems)."""
+    mesh = req.mesh
+    if mesh.impedance is not None or mesh.medium is not None or mesh.regions is not None:
+        return None
+    precision = req.precision
+    if precision == "auto":
+        precision = choose_precision(mesh.n, len(req.angles_deg))
+    return (mesh.n, len(req.angles_deg), precision)
+
+
 class Server:
+    """``workers`` threads, each with its own HIP stream; ``batch`` > 1 lets a worker take up to
+    that many waiting requests of equal signature (``batch_signature``) and run their factor and
+    solve as one batched launch sequence (``execute_batch``), which spreads the latency-bound
+    panel chain of the LU over all of them."""
+
     def __init__(self, device=None, workers: int = 4, hbm_budget_bytes: Optional[int] = None,
-                 cache_bytes: Optional[int] = None, tuning=None):
+                 cache_bytes: Optional[int] = None, tuning=None, batch: int = 4):
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        if batch < 1:
+            raise ValueError("batch must be at least 1")
+        self.batch = int(batch)
+        self.batches: List[int] = []          # sizes of the request groups run, in order
         # launch-shape table from the autotuner (a dict or a JSON path); shapes never change results
         self.tuning = autotune.resolve(tuning)
         if self.tuning is not None:
@@ -146,7 +167,10 @@ class Server:
         self.inflight: Dict[tuple, threading.Event] = {}
         self.inflight_lock = threading.Lock()
         self.waited = 0
-        self.queue: "queue.Queue" = queue.Queue()
+        # waiting requests, in arrival order; a worker takes the head and, with batch > 1, the
+        # waiting requests of the same signature behind it
+        self.pending: "deque" = deque()
+        self.pending_cv = threading.Condition()
         self.results: List[Result] = []
         self.workers = [threading.Thread(target=self._worker, args=(i,), daemon=True) for i in range(workers)]
         self.stop = False
@@ -172,14 +196,57 @@ class Server:
         fut: Future = Future()
         if req.arrival == 0.0:
             req.arrival = time.perf_counter()
-        self.queue.put((req, fut))
+        with self.pending_cv:
+            self.pending.append((req, fut))
+            self.pending_cv.notify()
         return fut
 
+    def submit_many(self, reqs: List[Request]) -> List[Future]:
+        """Submit several requests at once (they are queued together, so a batching worker
+        sees all of them)."""
+        futs = []
+        now = time.perf_counter()
+        with self.pending_cv:
+            for req in reqs:
+                fut: Future = Future()
+                if req.arrival == 0.0:
+                    req.arrival = now
+                self.pending.append((req, fut))
+                futs.append(fut)
+            self.pending_cv.notify_all()
+        return futs
+
     def close(self):
-        for _ in self.workers:
-            self.queue.put(None)
+        with self.pending_cv:
+            self.stop = True
+            self.pending_cv.notify_all()
         for t in self.workers:
             t.join()
+
+    def _take(self):
+        """Next group of waiting requests (None once closed and drained): the head, plus with
+        batch > 1 the waiting requests of its signature, up to ``batch`` and half the byte
+        budget, in arrival order."""
+        with self.pending_cv:
+            while not self.pending and not self.stop:
+                self.pending_cv.wait()
+            if not self.pending:
+                return None
+            head = self.pending.popleft()
+            group = [head]
+            sig = batch_signature(head[0]) if self.batch > 1 else None
+            if sig is not None:
+                unit = request_bytes(sig[0], sig[1], 2, sig[2], False)
+                keep = deque()
+                while self.pending:
+                    item = self.pending.popleft()
+                    if (len(group) < self.batch and batch_signature(item[0]) == sig
+                            and (len(group) + 1) * unit <= self.budget // 2):
+                        group.append(item)
+                    else:
+                        keep.append(item)
+                self.pending = keep
+            return group
 
     # -- execution -----------------------------------------------------------
     def _worker(self, index: int):
@@ -187,14 +254,71 @@ class Server:
         stream = torch.cuda.Stream(device=self.device)
         ws = {}
         while True:
-            item = self.queue.get()
-            if item is None:
+            group = self._take()
+            if group is None:
                 return
-            req, fut = item
-            try:
-                with torch.cuda.stream(stream):
+            with torch.cuda.stream(stream):
+                if len(group) > 1:
+                    self._run_group(group, ws)
+                    continue
+                req, fut = group[0]
+                try:
+                    self.batches.append(1)
                     res = self._run(req, ws)
+                    fut.set_result(res)
+                except BaseException as exc:   # surfaced through the future
+                    fut.set_exception(exc)
+
+    def _run_group(self, group, ws: dict):
+        """Requests of one signature.  Those whose system is neither cached nor being built
+        run as one batch (each one the single-flight owner of its key); the rest run on their
+        own afterwards, as does every request of a batch that failed (redone request by
+        request, so one bad system fails only its own request)."""
+        owners, alone = [], []
+        with self.inflight_lock:
+            for req, fut in group:
+                if req.precision == "auto":
+                    req.precision = choose_precision(req.mesh.n, len(req.angles_deg))
+                key = (req.key or geometry_digest(req.mesh), float(wavenumber(req.frequency_ghz)), req.precision)
+                if self.cache.get(key) is None and key not in self.inflight:
+                    self.inflight[key] = threading.Event()
+                    owners.append((req, fut, key))
+                else:
+                    alone.append((req, fut))
+        results = None
+        if len(owners) > 1:
+            reqs = [o[0] for o in owners]
+            nbytes = sum(request_bytes(system_dofs(r.mesh), len(r.angles_deg), len(r.pols), r.precision, False)
+                         for r in reqs)
+
+            def store(i, item, nb):
+                self.cache.put(owners[i][2], item, nb)
+
+            try:
+                self._admit(nbytes)
+                try:
+                    results = execute_batch(reqs, self.device, ws, store)
+                finally:
+                    self._release(nbytes)
+            except Exception:
+                results = None
+        # the batch's factors are in the cache (waiters find them there) or it failed (the
+        # requests are redone below, each its own owner again)
+        with self.inflight_lock:
+            for _, _, key in owners:
+                ev = self.inflight.pop(key, None)
+                if ev is not None:
+                    ev.set()
+        if results is not None:
+            self.batches.append(len(owners))
+            for (req, fut, key), res in zip(owners, results):
                 fut.set_result(res)
+        else:
+            alone = [(r, f) for r, f, _ in owners] + alone
+        for req, fut in alone:
+            try:
+                self.batches.append(1)
+                fut.set_result(self._run(req, ws))
             except BaseException as exc:   # surfaced through the future
                 fut.set_exception(exc)
 
@@ -301,10 +425,7 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
                else engine.radiating_density(contour, k, x[i]))
         a, s = engine.farfield(rho, bu, bdn, k)
         pending.append((pol, a, s, rel[i]))
-        # per result: refinement steps and error, the normwise backward error (the FP64 gate's
-        # metric) and the precision the factor ended in
-        info[pol] = {kk: v for kk, v in inf.items() if kk not in ("residual", "factor")}
-        info[pol]["precision_used"] = engine.system(fac, i).precision if fac.parts is not None else fac.precision
+        info[pol] = system_info(inf, fac, i)
     torch.cuda.current_stream().synchronize()
     for pol, a, s, r in pending:
         amp[pol] = a.cpu().numpy()
@@ -315,11 +436,100 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     return Result(req.rid, amp, sig, resid, cached is not None, finished - arrival, started, finished, info)
 
 
+def system_info(inf: dict, fac, i: int) -> dict:
+    """Accuracy statement of system i of a solve: refinement steps and error, the normwise
+    backward error (the FP64 gate's metric) and the precision its factor ended in."""
+    out = {kk: v for kk, v in inf.items()
+           if kk not in ("residual", "factor", "backward_errors", "refinement_errors", "steps", "precisions")}
+    if "backward_errors" in inf:
+        out["backward_error"] = inf["backward_errors"][i]
+    if "refinement_errors" in inf:
+        out["refinement_error"] = inf["refinement_errors"][i]
+    if "steps" in inf:
+        out["refinement_steps"] = inf["steps"][i]
+    out["precision_used"] = engine.system(fac, i).precision if fac.parts is not None else fac.precision
+    return out
+
+
+def execute_batch(reqs: List[Request], device, ws: dict, store=None) -> List[Result]:
+    """Requests of one batch signature (plain PEC contours with equal N, angle count and
+    precision), none of them cached: each is assembled into its two systems [TE, TM] of one
+    (2B, N, N) batch, the batch is factored and solved in single batched launch sequences, and
+    each request's far field comes from its own systems.  Every kernel works on each system of
+    a batch with the same operations whatever the batch, and the FP64 gate and the mixed
+    refinement decide per system, so each request gets, bit for bit, what ``execute`` gives it
+    alone.  ``store(i, item, nbytes)`` receives request i's assembled + factored systems (views
+    into the batch's storage)."""
+    dev = torch.device(device)
+    head = reqs[0]
+    n, na, precision = head.mesh.n, len(head.angles_deg), head.precision
+    for r in reqs:
+        if batch_signature(r) != (n, na, precision) or r.precision != precision:
+            raise ValueError("execute_batch needs requests of one signature")
+    nb = len(reqs)
+    started = time.perf_counter()
+    if n not in ws:
+        ws[n] = engine.assembly_workspace(n, dev)
+    systems = torch.empty((2 * nb, n, n), dtype=torch.complex128, device=dev)
+    rhs = torch.empty((2 * nb, na, n), dtype=torch.complex128, device=dev)
+    contours, ks, loads = [], [], []
+    for i, req in enumerate(reqs):
+        k = wavenumber(req.frequency_ghz)
+        contour = engine.DeviceContour(req.mesh, dev)
+        engine.assemble(contour, k, ws[n], out=systems[2 * i:2 * i + 2])
+        angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
+        bu, bdn = engine.loads(contour, k, angles)
+        torch.neg(bdn, out=rhs[2 * i])
+        torch.neg(bu, out=rhs[2 * i + 1])
+        contours.append(contour)
+        ks.append(k)
+        loads.append((bu, bdn))
+    fac = engine.factor(systems, precision)
+    sysbytes = 2 * (16 + (8 if precision == "mixed" else 16)) * n * n
+    if store is not None:
+        for i in range(nb):
+            store(i, {"contour": contours[i], "factor": engine.systems(fac, 2 * i, 2 * i + 2), "plan": None},
+                  sysbytes)
+    x, inf = engine.solve(fac, rhs, gate=(precision == "double"))
+    if inf.get("factor") is not None:
+        # some systems fell back to the FP64 LU: later requests reuse those factors
+        fac = inf.pop("factor")
+        if store is not None:
+            for i in range(nb):
+                store(i, {"contour": contours[i], "factor": engine.systems(fac, 2 * i, 2 * i + 2), "plan": None},
+                      2 * 32 * n * n)
+    rel = engine.relative_residual(fac.z, x, rhs, inf.get("residual"))
+    pending = []
+    for i, req in enumerate(reqs):
+        bu, bdn = loads[i]
+        for j, pol in enumerate(("TE", "TM")):
+            if pol not in req.pols:
+                continue
+            a, sg = engine.farfield(x[2 * i + j], bu, bdn, ks[i])
+            pending.append((i, pol, a, sg, rel[2 * i + j], system_info(inf, fac, 2 * i + j)))
+    torch.cuda.current_stream().synchronize()
+    finished = time.perf_counter()
+    out = [Result(r.rid, {}, {}, {}, False, finished - (r.arrival or started), started, finished, {})
+           for r in reqs]
+    for i, pol, a, sg, r, inf_i in pending:
+        out[i].amplitude[pol] = a.cpu().numpy()
+        out[i].sigma[pol] = sg.cpu().numpy()
+        out[i].residual[pol] = float(r.max())
+        out[i].info[pol] = inf_i
+    return out
+
+
 def run_stream(server: Server, requests: List[Request], rate_hz: Optional[float] = None,
                rng: Optional[np.random.Generator] = None) -> Tuple[List[Result], float]:
     """Submit requests with Poisson arrivals at rate_hz (None: all at once); wait for all."""
     rng = rng or np.random.default_rng(0)
     t0 = time.perf_counter()
+    if not rate_hz:
+        for req in requests:
+            req.arrival = t0
+        futs = server.submit_many(requests)
+        results = [f.result() for f in futs]
+        return results, time.perf_counter() - t0
     futs = []
     t_next = t0
     for req in requests:
