@@ -572,9 +572,9 @@ def main():
     # the server's default, the cost-model precision per request ('auto'); forced mixed and
     # forced FP64 serving run on the same requests beside it.
     torch.cuda.synchronize()
-    # each form warms its own server with two rounds of requests (first-use allocations of every
-    # worker's stream pool fall outside the timed region)
-    warm = max(args.warmup, 2 * args.workers)
+    # each form warms its own server with two full rounds of batches (first-use allocations of
+    # every worker's stream pool, a whole batch's systems included, fall outside the timed region)
+    warm = max(args.warmup, 2 * args.workers * args.batch)
     # the first server of a process runs about 12 % slower whatever its form (measured: 253 against
     # 288 solves/s for the same requests in the next server), so one untimed server goes first
     bt = args.batch
