@@ -18,6 +18,18 @@ extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1
                                        double* blocks, int* kind, unsigned* status, double* ztm, int64_t ldtm,
                                        double* zte, int64_t ldte, int mode, double mass, const double* mass_coef,
                                        hipStream_t st);
+extern "C" hipError_t grim_launch_near_list(int n, const double* p0, const double* p1, const int* next, int* counts,
+                                            int* offsets, int capacity, int* pair_e, int* pair_f, unsigned* status,
+                                            hipStream_t st);
+extern "C" hipError_t grim_launch_poly(int degree, int n, int q, const double* p0, const double* p1, const int* next,
+                                       const int* prev, double k, const int* offsets, int capacity,
+                                       const int* pair_e, const int* pair_f, double* bs, double* bk, double* bte,
+                                       unsigned* status, double* ztm, int64_t ldtm, double* zte, int64_t ldte,
+                                       hipStream_t st);
+extern "C" hipError_t grim_launch_poly_loads(int degree, int n, int na, const double* p0, const double* p1,
+                                             const int* prev, double k, const double* angles_deg, int order,
+                                             const double* qt, const double* qw, double* bu, double* bdn, int64_t ld,
+                                             hipStream_t st);
 extern "C" hipError_t grim_launch_loads(int n, int na, const double* p0, const double* p1, const int* prev, double k,
                                         const double* angles_deg, double* bu, double* bdn, int64_t ld,
                                         hipStream_t st);
@@ -377,9 +389,81 @@ extern "C" int grim_workspace_status(void* workspace, void* stream) {
     CHECK(hipMemcpyAsync(&h, workspace, sizeof(unsigned), hipMemcpyDeviceToHost, st));
     CHECK(hipStreamSynchronize(st));
     if (h & GRIM_DEV_NEAR_CAPACITY) return GRIM_ERR_CAPACITY;
-    if (h & (GRIM_DEV_ADAPTIVE_DEPTH | GRIM_DEV_SELF_SERIES)) return GRIM_ERR_NOT_CONVERGED;
+    if (h & GRIM_DEV_POLY_INTERSECT) return GRIM_ERR_GEOMETRY;
+    if (h & (GRIM_DEV_ADAPTIVE_DEPTH | GRIM_DEV_SELF_SERIES | GRIM_DEV_POLY_NEAR)) return GRIM_ERR_NOT_CONVERGED;
     if (h & GRIM_DEV_BARRIER_TIMEOUT) return GRIM_ERR_TIMEOUT;
     if (h & GRIM_DEV_ZERO_PIVOT) return GRIM_ERR_SINGULAR;
+    return GRIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Degree-2 and degree-3 Galerkin bases (poly.hip)
+// ---------------------------------------------------------------------------
+struct PolyWork {
+    unsigned* status;
+    int *counts, *offsets, *pair_e, *pair_f;
+    double *bs, *bk, *bte;
+    int64_t bytes;
+};
+
+static PolyWork carve_poly(void* base, int64_t n, int64_t cap, int degree) {
+    PolyWork w;
+    char* p = (char*)base;
+    int64_t off = 0;
+    const int64_t table = align_up(16 * (int64_t)(degree + 1) * (degree + 1) * n * n);
+    w.status = (unsigned*)(p + off); off += kAlign;
+    w.counts = (int*)(p + off); off += align_up(4 * n);
+    w.offsets = (int*)(p + off); off += align_up(4 * (n + 1));
+    w.pair_e = (int*)(p + off); off += align_up(4 * cap);
+    w.pair_f = (int*)(p + off); off += align_up(4 * cap);
+    w.bs = (double*)(p + off); off += table;
+    w.bk = (double*)(p + off); off += table;
+    w.bte = (double*)(p + off); off += table;
+    w.bytes = off;
+    return w;
+}
+
+static bool poly_size_ok(int64_t n, int degree) {
+    return (degree == 2 || degree == 3) && n >= 2 && n * degree <= 65535;
+}
+
+extern "C" int64_t grim_assemble_poly_workspace_bytes(int64_t n, int degree, int64_t near_per_element) {
+    if (!poly_size_ok(n, degree) || near_per_element <= 0) return -1;
+    return carve_poly(nullptr, n, n * near_per_element, degree).bytes;
+}
+
+extern "C" int grim_assemble_poly(int64_t n, int degree, const double* p0, const double* p1, const int32_t* next,
+                                  const int32_t* prev, double k, int far_order, double* z_tm, int64_t ld_tm,
+                                  double* z_te, int64_t ld_te, void* workspace, int64_t workspace_bytes,
+                                  int64_t near_per_element, void* stream) {
+    if (!poly_size_ok(n, degree) || !p0 || !p1 || !next || !prev || !z_tm || !z_te || !workspace ||
+        ld_tm < n * degree || ld_te < n * degree || !(k > 0) || !isfinite(k) || near_per_element <= 0 ||
+        (far_order != 9 && far_order != 16))
+        return GRIM_ERR_ARGS;
+    const int64_t cap = n * near_per_element;
+    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
+    const PolyWork w = carve_poly(workspace, n, cap, degree);
+    if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
+    hipStream_t st = (hipStream_t)stream;
+    zero_words<<<1, 64, 0, st>>>(w.status, 1);
+    CHECK(hipGetLastError());
+    CHECK(grim_launch_near_list((int)n, p0, p1, next, w.counts, w.offsets, (int)cap, w.pair_e, w.pair_f, w.status,
+                                st));
+    CHECK(grim_launch_poly(degree, (int)n, far_order, p0, p1, next, prev, k, w.offsets, (int)cap, w.pair_e,
+                           w.pair_f, w.bs, w.bk, w.bte, w.status, z_tm, ld_tm, z_te, ld_te, st));
+    return GRIM_OK;
+}
+
+extern "C" int grim_loads_poly(int64_t n, int degree, int64_t na, const double* p0, const double* p1,
+                               const int32_t* prev, double k, const double* angles_deg, int64_t order,
+                               const double* rule_t, const double* rule_w, double* bu, double* bdn, int64_t ld,
+                               void* stream) {
+    if (!poly_size_ok(n, degree) || na < 0 || na > 65535 || !p0 || !p1 || !prev || !angles_deg || (!bu && !bdn) ||
+        ld < n * degree || !(k > 0) || !isfinite(k) || order < 1 || order > 256 || !rule_t || !rule_w)
+        return GRIM_ERR_ARGS;
+    if (na == 0) return GRIM_OK;
+    CHECK(grim_launch_poly_loads(degree, (int)n, (int)na, p0, p1, prev, k, angles_deg, (int)order, rule_t, rule_w,
+                                 bu, bdn, ld, (hipStream_t)stream));
     return GRIM_OK;
 }
 

@@ -942,6 +942,16 @@ extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1
     return hipGetLastError();
 }
 
+// The near-pair CSR lists alone (count, scan, list), for the polynomial near pass.
+extern "C" hipError_t grim_launch_near_list(int n, const double* p0, const double* p1, const int* next, int* counts,
+                                            int* offsets, int capacity, int* pair_e, int* pair_f, unsigned* status,
+                                            hipStream_t st) {
+    near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 0, counts, offsets, capacity, pair_e, pair_f, status);
+    scan_counts<<<1, 1024, 0, st>>>(n, counts, offsets);
+    near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 1, counts, offsets, capacity, pair_e, pair_f, status);
+    return hipGetLastError();
+}
+
 // Impedance (Robin) rows of closed contours, in place over a column-major pair
 // of rows x cols arrays: x <- x + ax[r] y on every row, y <- y on Dirichlet rows
 // and x + ay[r] y elsewhere (both from the original x, y).  With x, y the TE and

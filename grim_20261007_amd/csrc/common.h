@@ -20,6 +20,7 @@ enum {
     GRIM_ERR_CAPACITY = 7,
     GRIM_ERR_SINGULAR = 8,
     GRIM_ERR_TIMEOUT = 9,
+    GRIM_ERR_GEOMETRY = 10,
 };
 
 // Device status word bits (written by kernels, read by the host once per call).
@@ -29,6 +30,8 @@ enum {
     GRIM_DEV_SELF_SERIES = 4,
     GRIM_DEV_ZERO_PIVOT = 8,
     GRIM_DEV_BARRIER_TIMEOUT = 16,
+    GRIM_DEV_POLY_NEAR = 32,        // polynomial near block did not converge (refine the mesh)
+    GRIM_DEV_POLY_INTERSECT = 64,   // polynomial quadrature met coincident integration points
 };
 
 // Launch-shape knobs (grim_set_launch_shape): far-pass source columns per

@@ -58,7 +58,10 @@ class DeviceContour:
         return self.n * self.n * (16 if precision == "double" else 8)
 
 
-def assembly_workspace(n: int, device) -> torch.Tensor:
+def assembly_workspace(n: int, device, degree: int = 1) -> torch.Tensor:
+    """Assembly workspace of an n-element contour with elements of the given degree."""
+    if degree > 1:
+        return poly_workspace(n, degree, device)
     nbytes = native.lib().grim_assemble_workspace_bytes(n, NEAR_PER_ELEMENT)
     return torch.empty(int(nbytes), dtype=torch.uint8, device=device)
 
@@ -71,6 +74,8 @@ def assemble(contour: DeviceContour, k: float, workspace: torch.Tensor | None = 
     (also reachable as ``z_tm._base``) so that both can be factored and solved in
     batched launches.  ``out``, a contiguous (2, N, N) complex128 tensor (e.g. two systems of a
     larger batch), receives the pair instead of a new allocation."""
+    if contour.mesh.degree > 1:
+        return assemble_poly(contour, k, workspace, out)
     n = contour.n
     dev = contour.device
     if out is None:
@@ -92,8 +97,90 @@ def assemble(contour: DeviceContour, k: float, workspace: torch.Tensor | None = 
     return z_tm, z_te
 
 
+# ----------------------------------------------------------------------------
+# Degree-2 and degree-3 bases (twod/basis.py, twod/polynomial_quadrature.py)
+# ----------------------------------------------------------------------------
+POLY_NEAR_MESSAGE = "Polynomial near quadrature did not converge; refine the mesh."   # polynomial_quadrature.py:143
+POLY_INTERSECT_MESSAGE = "Polynomial quadrature encountered intersecting integration points."   # :104
+
+
+def poly_workspace(n: int, degree: int, device) -> torch.Tensor:
+    nbytes = int(native.lib().grim_assemble_poly_workspace_bytes(n, degree, NEAR_PER_ELEMENT))
+    if nbytes < 0:
+        raise ValueError(f"no polynomial assembly for n={n}, degree={degree}")
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def assemble_poly(contour: DeviceContour, k: float, workspace: torch.Tensor | None = None,
+                  out: torch.Tensor | None = None):
+    """TM and TE systems of a contour with degree-2 or degree-3 elements (mesh.degree), N d x N d
+    complex128 column-major; returns (z_tm, z_te) views of one (2, Nd, Nd) [TE, TM] tensor (also
+    ``z_tm._base``), written into ``out`` when given.  Raises ValueError where the reference's
+    polynomial near quadrature raises (non-convergence, coincident points)."""
+    mesh = contour.mesh
+    d = mesh.degree
+    if d not in (2, 3):
+        raise ValueError("assemble_poly needs a mesh of degree 2 or 3")
+    n, m = contour.n, mesh.dofs
+    dev = contour.device
+    if out is None:
+        pair = torch.empty((2, m, m), dtype=torch.complex128, device=dev)
+    else:
+        if out.shape != (2, m, m) or out.dtype != torch.complex128 or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (2, Nd, Nd) complex128 tensor")
+        pair = out
+    z_te, z_tm = pair[0], pair[1]
+    ws = workspace if workspace is not None else poly_workspace(n, d, dev)
+    st = _stream(dev)
+    native.check("grim_assemble_poly", native.lib().grim_assemble_poly(
+        n, d, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.next.data_ptr(), contour.prev.data_ptr(),
+        float(k), mesh.far_order(k), z_tm.data_ptr(), m, z_te.data_ptr(), m, ws.data_ptr(), ws.numel(),
+        NEAR_PER_ELEMENT, st))
+    code = native.lib().grim_workspace_status(ws.data_ptr(), st)
+    if code == 6:
+        raise ValueError(POLY_NEAR_MESSAGE)
+    if code == 10:
+        raise ValueError(POLY_INTERSECT_MESSAGE)
+    native.check("grim_assemble_poly (device)", code)
+    return z_tm, z_te
+
+
+_RULES: dict = {}
+
+
+def _gauss_rule(order: int, device) -> tuple:
+    key = (order, str(device))
+    if key not in _RULES:
+        x, w = np.polynomial.legendre.leggauss(order)
+        _RULES[key] = (torch.tensor(0.5 * (x + 1.0), dtype=torch.float64, device=device),
+                       torch.tensor(0.5 * w, dtype=torch.float64, device=device))
+    return _RULES[key]
+
+
+def loads_poly(contour: DeviceContour, k: float, angles_deg: torch.Tensor, want_u=True, want_dn=True):
+    """Plane-wave loads of a degree-2 / degree-3 contour, (A, N d) complex128 (column-major
+    N d x A): Gauss moments of the nodal basis (twod/basis.py:89-104, assembly/kernels.py:236-259)."""
+    mesh = contour.mesh
+    d = mesh.degree
+    na = int(angles_deg.numel())
+    dev = contour.device
+    m = mesh.dofs
+    order = mesh.load_order(k)
+    qt, qw = _gauss_rule(order, dev)
+    bu = torch.empty((na, m), dtype=torch.complex128, device=dev) if want_u else None
+    bdn = torch.empty((na, m), dtype=torch.complex128, device=dev) if want_dn else None
+    native.check("grim_loads_poly", native.lib().grim_loads_poly(
+        contour.n, d, na, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.prev.data_ptr(), float(k),
+        angles_deg.data_ptr(), order, qt.data_ptr(), qw.data_ptr(), native.ptr(bu), native.ptr(bdn), m,
+        _stream(dev)))
+    return bu, bdn
+
+
 def loads(contour: DeviceContour, k: float, angles_deg: torch.Tensor, want_u=True, want_dn=True):
-    """Nodal plane-wave loads (A, N) complex128 tensors (column-major N x A)."""
+    """Nodal plane-wave loads (A, N) complex128 tensors (column-major N x A); (A, N d) for a contour
+    of degree d > 1 (loads_poly)."""
+    if contour.mesh.degree > 1:
+        return loads_poly(contour, k, angles_deg, want_u, want_dn)
     na = int(angles_deg.numel())
     dev = contour.device
     bu = torch.empty((na, contour.n), dtype=torch.complex128, device=dev) if want_u else None

@@ -40,10 +40,36 @@ class ContourMesh:
     medium: tuple | None = None           # (flag, eps, mu) of a dielectric body (TYPE 3), or None (conductor)
     regions: np.ndarray | None = None     # (N, 2) (minus, plus) region per element of a multi-region layout
     region_media: dict | None = None      # region -> (eps, mu) of a multi-region layout (0: air)
+    degree: int = 1                       # Galerkin basis degree (1, 2 or 3; twod/basis.py)
 
     @property
     def n(self) -> int:
         return len(self.p0)
+
+    @property
+    def dofs(self) -> int:
+        """Basis nodes of a conductor contour: the element ends plus degree - 1 interior nodes per
+        element (twod/basis.py:47-63)."""
+        return self.n * self.degree
+
+    def basis_elements(self) -> np.ndarray:
+        """(N, degree + 1) node ids per element, local order: start, end, then the interior nodes
+        at 1/d .. (d-1)/d, numbered after every end node, element by element (basis.py:47-63)."""
+        d = self.degree
+        ends = self.elements
+        if d == 1:
+            return ends
+        inner = self.n + (d - 1) * np.arange(self.n)[:, None] + np.arange(d - 1)[None, :]
+        return np.concatenate((ends, inner), axis=1)
+
+    def basis_nodes(self) -> np.ndarray:
+        """(dofs, 2) coordinates of the basis nodes (ends first, then interior, as basis_elements)."""
+        d = self.degree
+        if d == 1:
+            return self.p0.copy()
+        t = np.arange(1, d) / d
+        inner = self.p0[:, None, :] + t[None, :, None] * (self.p1 - self.p0)[:, None, :]
+        return np.concatenate((self.p0, inner.reshape(-1, 2)), axis=0)
 
     @property
     def elements(self) -> np.ndarray:
@@ -66,12 +92,23 @@ class ContourMesh:
         return rows[mask].astype(np.int32)
 
     def far_order(self, k: float) -> int:
-        """W-calibrated far rule of the paired traversal (twod/operators.py:1509-1527)."""
+        """W-calibrated far rule of the paired traversal (twod/operators.py:1502-1527): 8 points for
+        degree 1, 9 for degrees 2 and 3, inside the calibrated range; 16 outside it.  (The graded
+        rule of a tile never exceeds the configured 8 points, so the W floor decides.)"""
         lengths = self.lengths
         lmax, lmin = float(lengths.max()), float(lengths.min())
         if 1e-4 <= abs(k) * lmax <= 3.0 and lmin >= 1e-3 * lmax:
-            return 8
+            return 8 if self.degree == 1 else 9
         return 16
+
+    def load_order(self, k: float) -> int:
+        """Gauss order of the polynomial plane-wave moments: max(12, degree + 3 + ceil(|k| L_max))
+        (twod/basis.py:89-97; taken over the whole contour, which only adds points to elements
+        whose own bound is lower)."""
+        order = max(12, self.degree + 3 + int(math.ceil(abs(k) * float(self.lengths.max()))))
+        if order > 256:
+            raise ValueError("Polynomial excitation needs a finer wavelength mesh.")
+        return order
 
 
 def unit_scale(units: str) -> float:

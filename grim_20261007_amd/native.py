@@ -66,13 +66,18 @@ SIGNATURES = {
     "grim_robin_split": (_i32, [_i64, _vp, _vp, _vp, _vp, _dbl, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _i64,
                                  _vp, _i64, _vp, _i64, _vp]),
     "grim_robin_rhs": (_i32, [_i64, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "grim_assemble_poly_workspace_bytes": (_i64, [_i64, _i32, _i64]),
+    "grim_assemble_poly": (_i32, [_i64, _i32, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _i64, _vp, _i64, _vp, _i64, _i64,
+                                  _vp]),
+    "grim_loads_poly": (_i32, [_i64, _i32, _i64, _vp, _vp, _vp, _dbl, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp]),
     "grim_set_launch_shape": (_i32, [_i32, _i64, _i32]),
     "grim_clear_launch_shapes": (_i32, []),
     "grim_launch_shape": (_i32, [_i32, _i64]),
 }
 
 STATUS = {0: "ok", 1: "size or index out of range", 3: "invalid arguments", 5: "HIP runtime failure",
-          6: "not converged", 7: "near-pair capacity exceeded", 8: "zero pivot"}
+          6: "not converged", 7: "near-pair capacity exceeded", 8: "zero pivot",
+          10: "coincident integration points"}
 
 
 class GrimError(RuntimeError):

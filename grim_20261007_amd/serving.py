@@ -132,7 +132,8 @@ def batch_signature(req: Request) -> Optional[tuple]:
     (one unknown per node) of the same size, angle count and precision form.  None: the request
     runs on its own (impedance, dielectric and multi-region systems)."""
     mesh = req.mesh
-    if mesh.impedance is not None or mesh.medium is not None or mesh.regions is not None:
+    if (mesh.impedance is not None or mesh.medium is not None or mesh.regions is not None
+            or mesh.degree != 1):
         return None
     precision = req.precision
     if precision == "auto":
@@ -376,19 +377,20 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     dev = torch.device(device)
     contour = cached["contour"] if cached else engine.DeviceContour(mesh, dev)
     plan = cached.get("plan") if cached else (engine.region_plan(mesh) if mesh.regions is not None else None)
-    dofs = plan.n_dof if plan is not None else (2 * mesh.n if mesh.medium is not None else mesh.n)
+    dofs = plan.n_dof if plan is not None else system_dofs(mesh)
     if cached is None:
-        if mesh.n not in ws:
-            ws[mesh.n] = engine.assembly_workspace(mesh.n, dev)
+        wkey = (mesh.n, mesh.degree)
+        if wkey not in ws:
+            ws[wkey] = engine.assembly_workspace(mesh.n, dev, mesh.degree)
         if plan is not None:
-            pair = engine.region_systems(contour, k, ws[mesh.n], plan)
+            pair = engine.region_systems(contour, k, ws[wkey], plan)
         elif mesh.medium is not None:
-            pair = engine.dielectric_systems(contour, k, ws[mesh.n])
+            pair = engine.dielectric_systems(contour, k, ws[wkey])
         else:
-            z_tm, z_te = engine.assemble(contour, k, ws[mesh.n])
+            z_tm, z_te = engine.assemble(contour, k, ws[wkey])
             pair = z_tm._base                                     # batch [TE, TM]
             if mesh.impedance is not None:
-                engine.impedance_systems(contour, k, pair, ws[mesh.n])
+                engine.impedance_systems(contour, k, pair, ws[wkey])
         fac = engine.factor(pair, req.precision)
         if store is not None:
             sysbytes = 2 * (16 + (8 if req.precision == "mixed" else 16)) * dofs * dofs
@@ -468,15 +470,15 @@ def execute_batch(reqs: List[Request], device, ws: dict, store=None) -> List[Res
             raise ValueError("execute_batch needs requests of one signature")
     nb = len(reqs)
     started = time.perf_counter()
-    if n not in ws:
-        ws[n] = engine.assembly_workspace(n, dev)
+    if (n, 1) not in ws:
+        ws[(n, 1)] = engine.assembly_workspace(n, dev)
     systems = torch.empty((2 * nb, n, n), dtype=torch.complex128, device=dev)
     rhs = torch.empty((2 * nb, na, n), dtype=torch.complex128, device=dev)
     contours, ks, loads = [], [], []
     for i, req in enumerate(reqs):
         k = wavenumber(req.frequency_ghz)
         contour = engine.DeviceContour(req.mesh, dev)
-        engine.assemble(contour, k, ws[n], out=systems[2 * i:2 * i + 2])
+        engine.assemble(contour, k, ws[(n, 1)], out=systems[2 * i:2 * i + 2])
         angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
         bu, bdn = engine.loads(contour, k, angles)
         torch.neg(bdn, out=rhs[2 * i])

@@ -197,7 +197,20 @@ def admit_device_memory(n: int, nangles: int, npols: int, precision: str, device
     return need / gib
 
 
-def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, material_base_dir):
+def basis_order(execution_options) -> int:
+    """The Galerkin degree an execution-options record asks for (execution/options.py:72-83, 425):
+    1 (default), 2 or 3."""
+    if execution_options is None:
+        return 1
+    if not isinstance(execution_options, dict):
+        raise ValueError("Execution settings must contain supported fields only.")
+    order = execution_options.get("basis_order", 1)
+    if type(order) is not int or order not in (1, 2, 3):
+        raise ValueError("Boundary polynomial degree must be 1, 2 or 3.")
+    return order
+
+
+def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, material_base_dir, degree=1):
     """(mesh frequency, mesh) that frequency ``f`` solves on.  Without a reference frequency the mesh
     is built at f for f alone; with one, every frequency shares the mesh of the shortest wavelength
     over the sweep and the reference (twod/solver.py:2195-2215, geometry.py:2454-2471), whose junction
@@ -207,8 +220,13 @@ def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, m
         mesh_ghz = max(served)
     else:
         served, mesh_ghz = [float(f)], float(f)
-    return mesh_ghz, build_mesh(snapshot, units, mesh_ghz, max_panels, material_base_dir=material_base_dir,
-                                law_ghz=float(f), mesh_frequencies=served)
+    mesh = build_mesh(snapshot, units, mesh_ghz, max_panels, material_base_dir=material_base_dir,
+                      law_ghz=float(f), mesh_frequencies=served)
+    if degree > 1:
+        if mesh.impedance is not None or mesh.medium is not None or mesh.regions is not None:
+            raise ValueError("Polynomial bases of degree 2 and 3 are built for PEC contours only.")
+        mesh.degree = int(degree)
+    return mesh_ghz, mesh
 
 
 def system_dofs(mesh) -> int:
@@ -216,7 +234,7 @@ def system_dofs(mesh) -> int:
     node and bordering region for a multi-region layout."""
     if mesh.regions is not None:
         return engine.region_plan(mesh).n_dof
-    return 2 * mesh.n if mesh.medium is not None else mesh.n
+    return 2 * mesh.n if mesh.medium is not None else mesh.dofs
 
 
 def _systems(contour, k, angles, workspace):
@@ -257,7 +275,7 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     admit_device_memory(system_dofs(mesh), len(angles_deg), len(order), precision, dev, context,
                         compute_condition_number)
     contour = engine.DeviceContour(mesh, dev)
-    ws = engine.assembly_workspace(mesh.n, dev)
+    ws = engine.assembly_workspace(mesh.n, dev, mesh.degree)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
     pair, rhs_all, bu, bdn, radiate = _systems(contour, k, angles, ws)
     del ws
@@ -291,7 +309,7 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
             res["condition_method"] = methods[i]
         if return_density:
             # (N, A) nodal densities: the exterior one of a multi-region layout, psi of a dielectric body
-            res["density"] = (rho if mesh.regions is not None else x[i, :, :mesh.n]).cpu().numpy().T
+            res["density"] = (rho if mesh.regions is not None else x[i, :, :mesh.dofs]).cpu().numpy().T
         out[pol] = res
     return out
 
@@ -341,9 +359,11 @@ class _Record:
         self.columns = 0
         self.batches = 0
         self.factorizations = 0
+        self.degrees: List[int] = []
 
     def add_mesh(self, mesh, mesh_ghz):
         self.panel_counts.append(mesh.n)
+        self.degrees.append(int(getattr(mesh, "degree", 1)))
         self.lengths += [float(np.min(mesh.lengths)), float(np.max(mesh.lengths))]
         self.mesh_ghz.append(float(mesh_ghz))
 
@@ -375,6 +395,9 @@ class _Record:
             "mesh_wavelength_m": wl[0] if len(set(round(v, 15) for v in wl)) == 1 else None,
             "mesh_wavelength_min_m": min(wl), "mesh_wavelength_max_m": max(wl),
             "mesh_max_refractive_index": 1.0, "mesh_material_flags": [],
+            "polynomial_degree": self.degrees[0] if self.degrees else 1,
+            "polynomial_degree_min": min(self.degrees) if self.degrees else 1,
+            "polynomial_degree_max": max(self.degrees) if self.degrees else 1,
             "formulation": FORMULATION, "solver_method": "dense_lu", "solver_method_requested": str(solver_method),
             "residual_norm_max": rmax, "residual_norm_mean": rmean, "residual_nonfinite_count": nonfinite,
             "constraint_residual_norm_max": 0.0, "constraint_residual_norm_mean": 0.0,
@@ -415,6 +438,7 @@ def solve_monostatic_rcs_2d_single_polarization(
         raise ValueError("Only the physical RCS normalization is supported.")
     pol = _normalize_pol(polarization)
     unit_scale(geometry_units)
+    degree = basis_order(execution_options)
     evaluate_quality_gate({}, quality_thresholds)            # refuse unknown thresholds up front
     samples = []
     record = _Record(geometry_snapshot)
@@ -424,7 +448,7 @@ def solve_monostatic_rcs_2d_single_polarization(
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve canceled by user.")
         mesh_ghz, mesh = _frequency_mesh(geometry_snapshot, geometry_units, f, freqs, mesh_reference_ghz, max_panels,
-                                         material_base_dir)
+                                         material_base_dir, degree)
         record.add_mesh(mesh, mesh_ghz)
         res = solve_frequency(mesh, f, angs, (pol,), precision, device, abort_event=abort_event,
                               compute_condition_number=compute_condition_number,
@@ -462,6 +486,7 @@ def solve_monostatic_rcs_2d(
     if str(rcs_normalization_mode).strip().lower() != "physical":
         raise ValueError("Only the physical RCS normalization is supported.")
     unit_scale(geometry_units)
+    degree = basis_order(execution_options)
     evaluate_quality_gate({}, quality_thresholds)
     co = {"VV": [], "HH": []}
     flat = []
@@ -472,7 +497,7 @@ def solve_monostatic_rcs_2d(
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve canceled by user.")
         mesh_ghz, mesh = _frequency_mesh(geometry_snapshot, geometry_units, f, freqs, mesh_reference_ghz, max_panels,
-                                         material_base_dir)
+                                         material_base_dir, degree)
         record.add_mesh(mesh, mesh_ghz)
         res = solve_frequency(mesh, f, angs, ("TE", "TM"), precision, device, abort_event=abort_event,
                               compute_condition_number=compute_condition_number)

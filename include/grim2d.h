@@ -6,7 +6,8 @@
  *   - int status return: 0 ok, 1 size/index out of range, 3 invalid arguments,
  *     5 HIP runtime failure, 6 adaptive near rule not converged / refinement
  *     stalled, 7 near-pair capacity exceeded, 8 zero pivot, 9 a device-wide
- *     barrier of the tall-panel LU did not complete (bounded wait expired);
+ *     barrier of the tall-panel LU did not complete (bounded wait expired),
+ *     10 coincident integration points in a polynomial near block;
  *   - sizes are int64_t; complex arrays are interleaved double[2] (complex128)
  *     or float[2] (complex64), column-major, leading dimension in elements;
  *   - caller-owned outputs and workspace (all pointers are DEVICE pointers
@@ -119,6 +120,34 @@ int grim_scale_blocks(int64_t n, double* z, int64_t ld, const double* scale, voi
  * grim_assemble / grim_lu_factor that used `workspace` (0 ok, else a status
  * code as above). */
 int grim_workspace_status(void* workspace, void* stream);
+
+/* Degree-2 and degree-3 Galerkin bases (execution option basis_order 2 / 3).  The contour's n
+ * elements keep their end nodes 0 .. n-1 (element e from node e to next[e]); each element adds
+ * degree - 1 interior nodes n + (degree - 1) e + m at (m + 1) / degree along it, so a system has
+ * n * degree unknowns (twod/basis.py:47-63).  grim_assemble_poly writes the TM and TE systems
+ * (complex128 n*degree x n*degree, column-major): far pairs with the Q x Q Gauss rule, Q = far_order
+ * (9 inside the W-calibrated range, else 16; twod/operators.py:1502-1527, 2335-2518 with width
+ * degree + 1), every other pair with the adaptive polynomial near rule (orders 20 / 36, then 72 / 144
+ * for a self pair or at depth 24, otherwise bisection; t^4 corner rule on self and touching pairs,
+ * exact log moments on self blocks; twod/polynomial_quadrature.py:23-156), the Maue W of each
+ * block and the element mass.  Replaces the polynomial branch of _assemble_multi and
+ * _near_pair_blocks (twod/operators.py:2053-2067).  The workspace holds the near-pair lists and
+ * three tables of n x n element blocks (48 (degree + 1)^2 n^2 bytes).  grim_workspace_status
+ * afterwards returns 6 where the reference raises "Polynomial near quadrature did not converge"
+ * and 10 where it raises "Polynomial quadrature encountered intersecting integration points". */
+int64_t grim_assemble_poly_workspace_bytes(int64_t n, int degree, int64_t near_per_element);
+int grim_assemble_poly(int64_t n, int degree, const double* p0, const double* p1, const int32_t* next,
+                       const int32_t* prev, double k, int far_order, double* z_tm, int64_t ld_tm, double* z_te,
+                       int64_t ld_te, void* workspace, int64_t workspace_bytes, int64_t near_per_element,
+                       void* stream);
+
+/* Plane-wave loads of a degree-2 / degree-3 contour: Gauss moments of the nodal basis with the
+ * `order`-point rule rule_t / rule_w (device arrays, nodes and weights on [0, 1]), the reference's
+ * order being max(12, degree + 3 + ceil(|k| L_max)) (twod/basis.py:89-104, used by
+ * twod/assembly/kernels.py:236-259).  Outputs complex128 (n * degree) x na as grim_loads. */
+int grim_loads_poly(int64_t n, int degree, int64_t na, const double* p0, const double* p1, const int32_t* prev,
+                    double k, const double* angles_deg, int64_t order, const double* rule_t, const double* rule_w,
+                    double* bu, double* bdn, int64_t ld, void* stream);
 
 /* Plane-wave loads of all nodes for na incidence angles (degrees, device
  * array): bu = int phi_i exp(j k d.r) ds, bdn = int phi_i (j k d.n) exp(j k d.r) ds,

@@ -17,7 +17,7 @@ reference's outputs.  Nothing from the reference is stored except numbers.
 
 Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
                                                --ibc-zones | --dielectric | --regions |
-                                               --condition]
+                                               --condition | --poly]
 """
 from __future__ import annotations
 
@@ -48,7 +48,7 @@ EXECUTION = {"assembly_threads": 1, "blas_threads": 8, "factorization": "dense",
 
 
 def run_case(name, points, frequency_ghz, angles, keep_matrix=True, matrix_rows=None,
-             density_angles=None, keep_rhs=True):
+             density_angles=None, keep_rhs=True, execution=None):
     import ghost_backend.twod.solver as rcs
 
     captured = []
@@ -74,7 +74,7 @@ def run_case(name, points, frequency_ghz, angles, keep_matrix=True, matrix_rows=
         result = rcs.solve_monostatic_rcs_2d(
             snapshot, frequencies_ghz=[frequency_ghz], elevations_deg=list(map(float, angles)),
             geometry_units="meters", strict_quality_gate=False, max_panels=100000,
-            solver_method="direct", execution_options=EXECUTION)
+            solver_method="direct", execution_options=execution or EXECUTION)
         seconds = time.perf_counter() - start
     finally:
         rcs._solve_fields = original
@@ -112,6 +112,7 @@ def run_case(name, points, frequency_ghz, angles, keep_matrix=True, matrix_rows=
                                    "amplitude_version", "rcs_log_unit", "rcs_linear_quantity",
                                    "polarizations", "polarization_mapping") if k in result}
     out["result_meta_json"] = np.array(json.dumps(meta, default=str))
+    out["polynomial_degree"] = int(result["metadata"].get("polynomial_degree", 1))
     np.savez_compressed(HERE / f"{name}.npz", **out)
     print(f"{name}: N={len(te['nodes'])} f={frequency_ghz:.6g} GHz angles={len(angles)} "
           f"reference {seconds:.2f} s", flush=True)
@@ -637,8 +638,27 @@ def main_condition():
     run_condition_case("cond_naca80", pts, workloads.frequency_for_segments_per_wavelength(pts, 10.0), [0.0, 30.0])
 
 
+def main_poly():
+    """Degree-2 and degree-3 Galerkin bases (execution option basis_order, twod/basis.py)."""
+    for degree in (2, 3):
+        execution = dict(EXECUTION, basis_order=degree)
+        r = 0.1
+        run_case(f"poly{degree}_circle32_ka1", workloads.circle_points(r, 32), workloads.frequency_for_ka(1.0, r),
+                 [0.0, 45.0, 90.0, 200.0], execution=execution)
+        pts = workloads.naca0012_points(24)
+        run_case(f"poly{degree}_naca48", pts, workloads.frequency_for_segments_per_wavelength(pts, 5.0),
+                 [0.0, 30.0, 90.0, 180.0, 300.0], execution=execution)
+        rng = np.random.default_rng(20261204 + degree)
+        a, phi = workloads.star_shape(rng)
+        pts = workloads.star_points(40, 0.5, a, phi)
+        run_case(f"poly{degree}_star40", pts, workloads.frequency_for_segments_per_wavelength(pts, 4.0),
+                 [0.0, 111.0, 250.0], execution=execution)
+
+
 if __name__ == "__main__":
-    if "--condition" in sys.argv:
+    if "--poly" in sys.argv:
+        main_poly()
+    elif "--condition" in sys.argv:
         main_condition()
     elif "--certified" in sys.argv:
         main_certified()

@@ -6,6 +6,8 @@ import numpy as np
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SMALL = ["circle64_ka1", "naca80", "star96", "star128_aspect4"]
 ALL = SMALL + ["c1_circle400_ka10"]
+# degree-2 and degree-3 Galerkin bases (execution option basis_order)
+POLY = [f"poly{d}_{case}" for d in (2, 3) for case in ("circle32_ka1", "naca48", "star40")]
 
 
 def load(name):

@@ -1,0 +1,111 @@
+"""Degree-2 and degree-3 Galerkin bases on the GPU against the reference's own solves
+(tests/golden/poly*.npz, written by make_golden.py --poly): both system matrices, the loads,
+the nodal densities and the public entry's sample amplitudes."""
+import json
+
+import numpy as np
+import pytest
+
+from tests import golden_cases
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def _mesh(case):
+    from grim_20261007_amd.mesh import build_mesh
+    mesh = build_mesh(golden_cases.snapshot(case), "meters", float(case["frequency_ghz"]), 100000)
+    mesh.degree = int(case["polynomial_degree"])
+    return mesh
+
+
+@pytest.mark.parametrize("name", golden_cases.POLY)
+def test_poly_systems_and_loads_match_reference(name):
+    from grim_20261007_amd import engine
+    dev = _dev()
+    case = golden_cases.load(name)
+    mesh = _mesh(case)
+    k = float(case["k0"])
+    print(f"{name}: degree {mesh.degree}, {mesh.n} elements, {mesh.dofs} unknowns", flush=True)
+    contour = engine.DeviceContour(mesh, dev)
+    z_tm, z_te = engine.assemble(contour, k, engine.assembly_workspace(mesh.n, dev, mesh.degree))
+    for pol, z in (("TM", z_tm), ("TE", z_te)):
+        got = z.cpu().numpy().T
+        ref = case[f"Z_{pol}"]
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        print(f"{name} {pol}: matrix max rel err {err:.2e}", flush=True)
+        assert err < 1e-11
+    angles = torch.tensor(case["angles_deg"], dtype=torch.float64, device=dev)
+    bu, bdn = engine.loads(contour, k, angles)
+    for pol, b in (("TM", -bu), ("TE", -bdn)):
+        got = b.cpu().numpy().T
+        ref = case[f"rhs_{pol}"]
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        print(f"{name} {pol}: loads max rel err {err:.2e}", flush=True)
+        assert err < 1e-12
+
+
+@pytest.mark.parametrize("name", golden_cases.POLY)
+def test_poly_public_entry_matches_reference(name):
+    from grim_20261007_amd.solver import solve_frequency, solve_monostatic_rcs_2d
+    _dev()
+    case = golden_cases.load(name)
+    degree = int(case["polynomial_degree"])
+    f = float(case["frequency_ghz"])
+    angles = case["angles_deg"].tolist()
+    res = solve_monostatic_rcs_2d(golden_cases.snapshot(case), [f], angles, geometry_units="meters",
+                                  solver_method="direct", max_panels=100000,
+                                  execution_options={"basis_order": degree})
+    assert res["metadata"]["polynomial_degree"] == degree
+    meta = json.loads(str(case["result_meta_json"]))
+    for key, value in meta.items():
+        assert res[key] == value, key
+    for export, internal in (("VV", "TE"), ("HH", "TM")):
+        rows = res["co_solved_samples"][export]
+        amp = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+        ref = case[f"sample_amp_{internal}"]
+        err = np.max(np.abs(amp - ref) / np.abs(ref))
+        db = np.max(np.abs(np.array([r["rcs_db"] for r in rows]) - case[f"sample_db_{internal}"]))
+        print(f"{name} {export}: amplitude {err:.2e}, dB {db:.2e}", flush=True)
+        assert err < 1e-10 and db < 1e-8
+    # nodal densities of both polarizations (the reference's return_density)
+    out = solve_frequency(_mesh(case), f, angles, ("TE", "TM"), return_density=True)
+    for pol in ("TE", "TM"):
+        got = out[pol]["density"]
+        ref = case[f"density_{pol}"]
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        print(f"{name} {pol}: density max rel err {err:.2e}", flush=True)
+        assert err < 1e-10
+
+
+def test_poly_mixed_precision_matches_double():
+    from grim_20261007_amd.solver import solve_frequency
+    _dev()
+    case = golden_cases.load("poly3_naca48")
+    f = float(case["frequency_ghz"])
+    angles = np.linspace(0.0, 350.0, 36).tolist()
+    a = solve_frequency(_mesh(case), f, angles, ("TE", "TM"), precision="double")
+    b = solve_frequency(_mesh(case), f, angles, ("TE", "TM"), precision="mixed")
+    for pol in ("TE", "TM"):
+        err = np.max(np.abs(b[pol]["amplitude"] - a[pol]["amplitude"]) / np.abs(a[pol]["amplitude"]))
+        db = np.max(np.abs(10 * np.log10(b[pol]["sigma"]) - 10 * np.log10(a[pol]["sigma"])))
+        print(f"poly3 mixed vs double {pol}: amplitude {err:.1e}, dB {db:.1e}", flush=True)
+        assert err < 1e-9 and db < 0.05
+        assert np.max(b[pol]["residual"]) < 1e-10
+
+
+def test_poly_rejects_other_layouts():
+    from grim_20261007_amd.solver import solve_monostatic_rcs_2d
+    case = golden_cases.load("poly2_circle32_ka1")
+    snap = golden_cases.snapshot(case)
+    snap["segments"][0]["properties"][0] = "3"      # a dielectric body
+    with pytest.raises(ValueError):
+        solve_monostatic_rcs_2d(snap, [float(case["frequency_ghz"])], [0.0], geometry_units="meters",
+                                execution_options={"basis_order": 2})
