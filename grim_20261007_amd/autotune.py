@@ -4,7 +4,8 @@ Three launch shapes of the request path depend on the problem size and the devic
 
 * far-pass source columns per wavefront (knob 0, keyed by n): few columns give
   more wavefronts (better fill of the 1024 SIMDs at small n), more columns
-  re-evaluate fewer chunk-boundary pairs;
+  give fewer chunk-boundary columns (shared between two wavefronts by atomic
+  adds onto cleared entries) and longer register-carried walks;
 * columns per wavefront of the factorization's 64-row triangular solves (knob 1,
   keyed by the columns of one call times its batch): more columns give more
   independent chains per wavefront but fewer wavefronts;
@@ -34,7 +35,7 @@ from .mesh import build_mesh
 from .solver import wavenumber
 
 FAR_COLUMNS, TRSM_COLUMNS, SOLVE_COLUMNS = 0, 1, 2
-# table measured by tune() on an MI355X (far pass n = 250-8000, factors of n = 2000, solves of
+# table measured by tune() on an MI355X (far pass n = 500-12000, factors of n = 2000, solves of
 # 90-1440 angles)
 MI355X_TABLE = Path(__file__).resolve().parent / "tuning_mi355x.json"
 KNOB_NAMES = {FAR_COLUMNS: "far_columns", TRSM_COLUMNS: "trsm_columns", SOLVE_COLUMNS: "solve_columns"}

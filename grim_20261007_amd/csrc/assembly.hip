@@ -205,14 +205,53 @@ __device__ __forceinline__ void store_add(double* z, int64_t ld, int p, int q, c
 // Far pass over unordered element pairs.  A wavefront owns 64 consecutive
 // observation elements (lanes, block I) and walks a chunk of `chunk` source
 // columns (inside column block J >= I) in order.  Node entry Z[p, q] gathers
-//   blk(p, prev q)[0][1] + blk(p, q)[0][0] + blk(prev p, prev q)[1][1] + blk(prev p, q)[1][0]:
-// the source-side neighbour is carried in registers along the walk, the
-// observation-side neighbour comes from lane - 1 by a shuffle.  For J > I the
-// same evaluations also give the transposed tile (rows in chunk, columns in
-// block I), written with the roles exchanged.  Every entry is written exactly
-// once, in a fixed summation order.  Entries whose observation-side neighbour
-// is not lane - 1 (lane 0, contour starts) are completed by far_fixup.  MODE selects the two
-// outputs (see far_pair).
+//   (blk(p, prev q)[0][1] + blk(prev p, prev q)[1][1])      source-side part P1
+// + (blk(p, q)[0][0] + blk(prev p, q)[1][0])                  own part P2
+// (always in this association, so no entry's bits depend on the chunk): the
+// observation-side neighbour comes from lane - 1 by a shuffle, and P1 of the
+// next column is formed from the current column's blocks and carried along the
+// walk.  For J > I the same evaluations also give the transposed tile (rows in
+// the chunk, columns in block I), written with the roles exchanged.
+// A chunk whose first column continues the previous chunk of its row block
+// does not evaluate that column's source-side neighbour again: the previous
+// chunk adds P1 after its last column and this chunk adds P2, both by atomic
+// adds onto entries far_zero cleared.  Two addends onto zero give the same sum
+// in either order, so every entry is still written deterministically.  Only the
+// first chunk of a row block and columns that start a contour evaluate their
+// neighbour.  Entries whose observation-side neighbour is not lane - 1 (lane 0,
+// contour starts) are completed by far_fixup.  MODE selects the two outputs
+// (see far_pair).
+__device__ __forceinline__ cplx shfl_up1(cplx v) {
+    return {__shfl_up(v.re, 1, 64), __shfl_up(v.im, 1, 64)};
+}
+
+__device__ __forceinline__ void atomic_add_entry(double* z, int64_t ld, int p, int q, cplx v) {
+    double* t = z + 2 * ((int64_t)p + (int64_t)q * ld);
+    atomicAdd(t, v.re);
+    atomicAdd(t + 1, v.im);
+}
+
+// Column c (a multiple of `chunk`, continuing a contour) receives two atomic partial sums in
+// the row blocks whose chunk boundary it is: rows below 64 ceil(c / 64) of column c (direct
+// tiles) and columns below 64 floor(c / 64) of row c (mirror tiles).  One workgroup per c.
+__global__ void __launch_bounds__(256) far_zero(int n, int chunk, const int* __restrict__ prev,
+                                                double* __restrict__ ztm, int64_t ldtm, double* __restrict__ zte,
+                                                int64_t ldte) {
+    const int c = (blockIdx.x + 1) * chunk;
+    if (c >= n || prev[c] != c - 1) return;
+    const int rows = min(n, 64 * ((c + 63) / 64)), cols = 64 * (c / 64);
+    for (int r = threadIdx.x; r < rows; r += 256) {
+        double* a = ztm + 2 * ((int64_t)r + (int64_t)c * ldtm);
+        double* b = zte + 2 * ((int64_t)r + (int64_t)c * ldte);
+        a[0] = 0.0; a[1] = 0.0; b[0] = 0.0; b[1] = 0.0;
+    }
+    for (int r = threadIdx.x; r < cols; r += 256) {
+        double* a = ztm + 2 * ((int64_t)c + (int64_t)r * ldtm);
+        double* b = zte + 2 * ((int64_t)c + (int64_t)r * ldte);
+        a[0] = 0.0; a[1] = 0.0; b[0] = 0.0; b[1] = 0.0;
+    }
+}
+
 template <int Q, int MODE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) far_kernel(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                  const int* __restrict__ next, const int* __restrict__ prev,
@@ -236,42 +275,69 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) fa
     const int c0 = tile.y * chunk;
     const int c1 = min(n, c0 + chunk);
     const bool mirror = (c0 / 64) > tile.x;
-    cplx pd_tm0 = {0, 0}, pd_tm1 = {0, 0}, pd_te0 = {0, 0}, pd_te1 = {0, 0};   // direct: column next(f)
-    cplx pt_tm0 = {0, 0}, pt_tm1 = {0, 0}, pt_te0 = {0, 0}, pt_te1 = {0, 0};   // mirror: row next(f)
+    const bool first_chunk = c0 == tile.x * 64;
+    // P1 of the current column: direct TM / TE (column f, rows e) and mirror TM / TE (row f, columns e)
+    cplx qtm = {0, 0}, qte = {0, 0}, rtm = {0, 0}, rte = {0, 0};
+    bool shared_first = false;   // this chunk's first column gets its P1 from the previous chunk
     for (int f = c0; f < c1; ++f) {
         const int pf = prev[f];
-        cplx ltm0, ltm1, lte0, lte1, mtm0, mtm1, mte0, mte1;
         if (f == c0 || pf != f - 1) {
-            Blk2 b;
-            far_or_zero<Q, MODE>(o, ee, ne, load_elem(p0, p1, pf), pf, next[pf], k, active, b);
-            ltm0 = b.tm[0][1]; ltm1 = b.tm[1][1]; lte0 = b.te[0][1]; lte1 = b.te[1][1];
-            mtm0 = b.tmt[1][0]; mtm1 = b.tmt[1][1]; mte0 = b.tet[1][0]; mte1 = b.tet[1][1];
-        } else {
-            ltm0 = pd_tm0; ltm1 = pd_tm1; lte0 = pd_te0; lte1 = pd_te1;
-            mtm0 = pt_tm0; mtm1 = pt_tm1; mte0 = pt_te0; mte1 = pt_te1;
+            if (f == c0 && !first_chunk && pf == f - 1) {
+                shared_first = true;
+                qtm = qte = rtm = rte = {0.0, 0.0};
+            } else {
+                Blk2 b;
+                far_or_zero<Q, MODE>(o, ee, ne, load_elem(p0, p1, pf), pf, next[pf], k, active, b);
+                const cplx utm = shfl_up1(b.tm[1][1]), ute = shfl_up1(b.te[1][1]);
+                const cplx vtm = shfl_up1(b.tmt[1][1]), vte = shfl_up1(b.tet[1][1]);
+                qtm = rowok ? cadd(b.tm[0][1], utm) : b.tm[0][1];
+                qte = rowok ? cadd(b.te[0][1], ute) : b.te[0][1];
+                rtm = rowok ? cadd(b.tmt[1][0], vtm) : b.tmt[1][0];
+                rte = rowok ? cadd(b.tet[1][0], vte) : b.tet[1][0];
+            }
         }
         Blk2 b;
         far_or_zero<Q, MODE>(o, ee, ne, load_elem(p0, p1, f), f, next[f], k, active, b);
-        // direct tile: column f, rows e (own, local 0) and next(e) (local 1 -> lane + 1)
-        const cplx ctm0 = cadd(ltm0, b.tm[0][0]), ctm1 = cadd(ltm1, b.tm[1][0]);
-        const cplx cte0 = cadd(lte0, b.te[0][0]), cte1 = cadd(lte1, b.te[1][0]);
-        pd_tm0 = b.tm[0][1]; pd_tm1 = b.tm[1][1]; pd_te0 = b.te[0][1]; pd_te1 = b.te[1][1];
-        // mirror tile: row f, columns e (local 0) and next(e) (local 1 -> lane + 1)
-        const cplx rtm0 = cadd(mtm0, b.tmt[0][0]), rtm1 = cadd(mtm1, b.tmt[0][1]);
-        const cplx rte0 = cadd(mte0, b.tet[0][0]), rte1 = cadd(mte1, b.tet[0][1]);
-        pt_tm0 = b.tmt[1][0]; pt_tm1 = b.tmt[1][1]; pt_te0 = b.tet[1][0]; pt_te1 = b.tet[1][1];
-        cplx utm, ute, vtm, vte;
-        utm.re = __shfl_up(ctm1.re, 1, 64); utm.im = __shfl_up(ctm1.im, 1, 64);
-        ute.re = __shfl_up(cte1.re, 1, 64); ute.im = __shfl_up(cte1.im, 1, 64);
-        vtm.re = __shfl_up(rtm1.re, 1, 64); vtm.im = __shfl_up(rtm1.im, 1, 64);
-        vte.re = __shfl_up(rte1.re, 1, 64); vte.im = __shfl_up(rte1.im, 1, 64);
+        // own parts P2: direct column f from blk(e, f)[0][0] + blk(e - 1, f)[1][0], mirror row f from
+        // blk(e, f).tmt[0][0] + blk(e - 1, f).tmt[0][1]
+        const cplx u0 = shfl_up1(b.tm[1][0]), u1 = shfl_up1(b.te[1][0]);
+        const cplx u2 = shfl_up1(b.tmt[0][1]), u3 = shfl_up1(b.tet[0][1]);
+        const cplx otm = rowok ? cadd(b.tm[0][0], u0) : b.tm[0][0];
+        const cplx ote = rowok ? cadd(b.te[0][0], u1) : b.te[0][0];
+        const cplx mtm = rowok ? cadd(b.tmt[0][0], u2) : b.tmt[0][0];
+        const cplx mte = rowok ? cadd(b.tet[0][0], u3) : b.tet[0][0];
         if (active) {
-            store_add(ztm, ldtm, e, f, rowok ? cadd(ctm0, utm) : ctm0, false);
-            store_add(zte, ldte, e, f, rowok ? cadd(cte0, ute) : cte0, false);
-            if (mirror) {
-                store_add(ztm, ldtm, f, e, rowok ? cadd(rtm0, vtm) : rtm0, false);
-                store_add(zte, ldte, f, e, rowok ? cadd(rte0, vte) : rte0, false);
+            if (f == c0 && shared_first) {
+                atomic_add_entry(ztm, ldtm, e, f, otm);
+                atomic_add_entry(zte, ldte, e, f, ote);
+                if (mirror) {
+                    atomic_add_entry(ztm, ldtm, f, e, mtm);
+                    atomic_add_entry(zte, ldte, f, e, mte);
+                }
+            } else {
+                store_add(ztm, ldtm, e, f, cadd(qtm, otm), false);
+                store_add(zte, ldte, e, f, cadd(qte, ote), false);
+                if (mirror) {
+                    store_add(ztm, ldtm, f, e, cadd(rtm, mtm), false);
+                    store_add(zte, ldte, f, e, cadd(rte, mte), false);
+                }
             }
+        }
+        // P1 of column f + 1 from this column's blocks
+        const cplx w0 = shfl_up1(b.tm[1][1]), w1 = shfl_up1(b.te[1][1]);
+        const cplx w2 = shfl_up1(b.tmt[1][1]), w3 = shfl_up1(b.tet[1][1]);
+        qtm = rowok ? cadd(b.tm[0][1], w0) : b.tm[0][1];
+        qte = rowok ? cadd(b.te[0][1], w1) : b.te[0][1];
+        rtm = rowok ? cadd(b.tmt[1][0], w2) : b.tmt[1][0];
+        rte = rowok ? cadd(b.tet[1][0], w3) : b.tet[1][0];
+    }
+    // the next chunk of this row block takes its first column's P1 from here
+    if (active && c1 < n && prev[c1] == c1 - 1) {
+        atomic_add_entry(ztm, ldtm, e, c1, qtm);
+        atomic_add_entry(zte, ldte, e, c1, qte);
+        if ((c1 / 64) > tile.x) {
+            atomic_add_entry(ztm, ldtm, c1, e, rtm);
+            atomic_add_entry(zte, ldte, c1, e, rte);
         }
     }
 }
@@ -897,15 +963,18 @@ static hipError_t launch_far(int q, int n, const double* p0, const double* p1, c
                              double k, double* ztm, int64_t ldtm, double* zte, int64_t ldte, const int* fix_rows,
                              int nfix, hipStream_t st) {
     // columns per wavefront: enough wavefronts (about 8-16 K) to fill 1024 SIMDs
-    // with their tails overlapping; each chunk re-evaluates one column pair.
+    // with their tails overlapping; chunk boundaries share their column by atomics
+    // onto entries far_zero cleared, so a chunk evaluates only its own columns.
     int chunk = grim_shape_value(GRIM_SHAPE_FAR_COLUMNS, n);
-    if (chunk == 0) {
-        chunk = 4;
-        while (chunk < 32 && (double)n * n / (128.0 * chunk) > 16384.0) chunk *= 2;
-    }
+    if (chunk == 0) chunk = n < 1414 ? 1 : n < 5656 ? 2 : 8;   // measured on MI355X, n = 500-12000
     const int nc = (n + chunk - 1) / chunk, cpb = 64 / chunk, nb = (n + 63) / 64;
     long ntiles = 0;
     for (int I = 0; I < nb; ++I) ntiles += nc - I * cpb;
+    if (nc > 1) {
+        far_zero<<<(unsigned)(nc - 1), 256, 0, st>>>(n, chunk, prev, ztm, ldtm, zte, ldte);
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) return err;
+    }
     if (q == 8)
         far_kernel<8, MODE><<<(unsigned)ntiles, 64, 0, st>>>(n, p0, p1, next, prev, k, chunk, ztm, ldtm, zte, ldte);
     else
