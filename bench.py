@@ -668,7 +668,7 @@ def main():
                              "p99_ms": round(float(np.percentile(lat32, 99)), 3), "stage_ms": stage_ms},
         "accuracy": {"max_rcs_db_error_vs_fp64": max(db_err, serve_db), "max_amplitude_rel_error": amp_err,
                      "max_refined_residual": max(resid, serve_res), "refinement_steps": steps_mixed},
-        "roofline": {"kernel": "far pass (far_kernel + far_fixup, grim_assemble_far)", "bound": "valu",
+        "roofline": {"kernel": "far pass (far_zero_all + far_kernel + far_fixup, grim_assemble_far)", "bound": "valu",
                      "achieved": round(achieved, 3), "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64, 4), "traffic": far_traffic(),
                      "model": f"{pairs} unordered far pairs x {q}x{q} points x {FLOPS_PER_EVAL} FP64 flops "
