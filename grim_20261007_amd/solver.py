@@ -197,17 +197,84 @@ def admit_device_memory(n: int, nangles: int, npols: int, precision: str, device
     return need / gib
 
 
-def basis_order(execution_options) -> int:
-    """The Galerkin degree an execution-options record asks for (execution/options.py:72-83, 425):
-    1 (default), 2 or 3."""
-    if execution_options is None:
-        return 1
-    if not isinstance(execution_options, dict):
+# The reference's execution-settings record (execution/options.py:16-34): every field with its
+# default.  The dense GPU path honours basis_order and mesh_strategy 'global'; the CPU execution
+# controls (threads, tiles, storage, RAM budget, batch size) have no meaning on it and are only
+# validated.
+EXECUTION_DEFAULTS = {
+    "version": 1, "factorization": "dense", "mesh_strategy": "global", "basis_order": 1,
+    "compressed_storage_mib": 2048, "ram_budget_gib": None, "temporary_directory": "",
+    "assembly_threads": 1, "blas_threads": 1, "rhs_compression": "auto", "angle_batch_size": 256,
+    "assembly_tile": 0, "far_quadrature_order": 0, "far_grading": True, "dense_residual_storage": "auto",
+    "compressed_far_method": "full", "frequency_preconditioner": "off",
+}
+
+
+def execution_record(execution_options, scattering: str = "monostatic") -> dict:
+    """The complete, validated settings record of one solve, with the reference's messages
+    (execution/options.py:72-118, validate_for_run :422-428).  Factorization choices other than
+    dense run the dense GPU LU (the hierarchical and compressed backends approximate it); mesh
+    strategies other than 'global' (local and adaptive hp meshing) are not built here."""
+    import os
+    value = {} if execution_options is None else execution_options
+    if not isinstance(value, dict) or set(value) - set(EXECUTION_DEFAULTS):
         raise ValueError("Execution settings must contain supported fields only.")
-    order = execution_options.get("basis_order", 1)
-    if type(order) is not int or order not in (1, 2, 3):
+    out = dict(EXECUTION_DEFAULTS)
+    out.update(value)
+    if type(out["version"]) is not int or out["version"] != 1:
+        raise ValueError("Unsupported execution settings version.")
+    if out["factorization"] not in ("dense", "hierarchical", "auto", "compressed", "adaptive"):
+        raise ValueError("Choose dense, hierarchical, auto, compressed, or adaptive factorization.")
+    if type(out["basis_order"]) is not int or out["basis_order"] not in (1, 2, 3):
         raise ValueError("Boundary polynomial degree must be 1, 2 or 3.")
-    return order
+    if out["mesh_strategy"] not in ("global", "local", "adaptive"):
+        raise ValueError("Mesh strategy must be global, local or adaptive.")
+    if out["rhs_compression"] not in ("off", "auto", "on"):
+        raise ValueError("RHS compression must be off, auto, or on.")
+    if out["dense_residual_storage"] not in ("auto", "memory", "disk"):
+        raise ValueError("Dense residual storage must be auto, memory, or disk.")
+    if out["compressed_far_method"] not in ("full", "verified_cur"):
+        raise ValueError("Compressed far method must be full or verified_cur (experimental).")
+    if out["frequency_preconditioner"] not in ("off", "reuse"):
+        raise ValueError("Frequency preconditioner must be off or reuse (experimental).")
+    storage = out["compressed_storage_mib"]
+    if type(storage) is not int or not (storage == 0 or 16 <= storage <= 1048576):
+        raise ValueError("compressed_storage_mib must be 0 for automatic or an integer from 16 to 1048576.")
+    for key, lower, upper in (("angle_batch_size", 1, 256), ("assembly_tile", 0, 65536),
+                              ("far_quadrature_order", 0, 64)):
+        number = out[key]
+        if type(number) is not int or not lower <= number <= upper:
+            raise ValueError(f"{key} must be an integer from {lower} to {upper}.")
+    threads = out["assembly_threads"]
+    if threads != "auto" and (type(threads) is not int or not 1 <= threads <= 1024):
+        raise ValueError("Assembly threads must be auto or an integer from 1 to 1024.")
+    threads = out["blas_threads"]
+    if threads != "auto" and (type(threads) is not int or not 1 <= threads <= 1024):
+        raise ValueError("BLAS threads must be auto or an integer from 1 to 1024.")
+    ram = out["ram_budget_gib"]
+    if ram is not None and (type(ram) not in (int, float) or not math.isfinite(ram) or ram <= 0):
+        raise ValueError("RAM budget must be positive GiB or null for available memory.")
+    if type(out["far_grading"]) is not bool:
+        raise ValueError("Far grading must be true or false.")
+    directory = out["temporary_directory"]
+    if not isinstance(directory, str) or any(c in directory for c in "\r\n\x00"):
+        raise ValueError("Temporary directory must be a path string.")
+    if directory and not os.path.isabs(directory):
+        raise ValueError("Temporary directory must be absolute or empty for the system temporary directory.")
+    if (out["mesh_strategy"] == "adaptive" or out["basis_order"] > 1) and scattering != "monostatic":
+        raise ValueError("Adaptive polynomial meshing supports 2D monostatic runs only.")
+    if out["mesh_strategy"] != "global":
+        raise ValueError(f"Mesh strategy '{out['mesh_strategy']}' (local / adaptive hp meshing) is not built "
+                         "on the MI355X path; use 'global'.")
+    if out["far_quadrature_order"] not in (0, 8):
+        raise ValueError("A far quadrature order other than the W-calibrated rule is not built on the MI355X "
+                         "path; use 0.")
+    return out
+
+
+def basis_order(execution_options) -> int:
+    """The Galerkin degree an execution-settings record asks for: 1 (default), 2 or 3."""
+    return int(execution_record(execution_options)["basis_order"])
 
 
 def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, material_base_dir, degree=1):

@@ -78,12 +78,30 @@ def test_gauss_rules():
 
 
 def test_options_record():
-    from grim_20261007_amd.solver import basis_order
+    """The execution-settings record: the reference's fields, defaults and messages
+    (execution/options.py:16-118, 422-428)."""
+    from grim_20261007_amd.solver import basis_order, execution_record
     assert basis_order(None) == 1
     assert basis_order({"basis_order": 3}) == 3
     for bad in ({"basis_order": 4}, {"basis_order": 2.0}):
         with pytest.raises(ValueError, match="Boundary polynomial degree must be 1, 2 or 3."):
             basis_order(bad)
+    # the record the golden runs pass to the reference is accepted as it is
+    golden = {"assembly_threads": 1, "blas_threads": 8, "factorization": "dense", "mesh_strategy": "global",
+              "basis_order": 2, "rhs_compression": "off"}
+    rec = execution_record(golden)
+    assert rec["basis_order"] == 2 and rec["angle_batch_size"] == 256 and rec["far_grading"] is True
+    cases = [({"foo": 1}, "supported fields only"), ({"version": 2}, "Unsupported execution settings version"),
+             ({"factorization": "lu"}, "Choose dense, hierarchical"), ({"mesh_strategy": "x"}, "global, local or adaptive"),
+             ({"angle_batch_size": 0}, "angle_batch_size must be an integer from 1 to 256"),
+             ({"assembly_threads": 0}, "Assembly threads must be auto"), ({"far_grading": 1}, "true or false"),
+             ({"ram_budget_gib": -1.0}, "RAM budget"), ({"temporary_directory": "rel/dir"}, "absolute or empty"),
+             ({"mesh_strategy": "adaptive"}, "not built")]
+    for bad, msg in cases:
+        with pytest.raises(ValueError, match=msg):
+            execution_record(bad)
+    with pytest.raises(ValueError, match="2D monostatic runs only"):
+        execution_record({"basis_order": 2}, scattering="bistatic")
 
 
 def test_load_order():
