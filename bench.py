@@ -621,6 +621,23 @@ def main():
         if stage:
             tf = lu_flops / (stage * 1e-3) / 1e12
             lu[key] = {"factor_ms": stage, "tflops": round(tf, 2), f"frac_of_{label}_peak": round(tf / peak, 4)}
+    # the same factorization as the serving form runs it: the systems of `batch` requests in one
+    # call (random systems of the same size; the pivot chain and the GEMM shapes are what count)
+    if args.batch > 1:
+        nsys = 2 * args.batch
+        g = torch.Generator(device="cpu").manual_seed(5)
+        zb = torch.randn((nsys, mesh.n, mesh.n), dtype=torch.complex128, generator=g).to(dev)
+        lt = Timer(torch)
+        for prec, peak, label in (("double", PEAK_FP64, "fp64"), ("mixed", PEAK_FP32, "fp32")):
+            for i in range(3):
+                (lt.run if i else (lambda name, fn: fn()))(prec, lambda: engine.factor(zb, prec, check=False))
+            ms = lt.mean_ms(prec)
+            tf = nsys * (8.0 / 3.0) * mesh.n ** 3 / (ms * 1e-3) / 1e12
+            lu[("complex128" if prec == "double" else "complex64") + f"_batch{nsys}"] = {
+                "factor_ms": round(ms, 3), "systems": nsys, "tflops": round(tf, 2),
+                f"frac_of_{label}_peak": round(tf / peak, 4)}
+        del zb
+        torch.cuda.empty_cache()
     stream = stream_summary(args.workers, batch=args.batch) if args.stream_count > 0 else None
     cpu = None
     if not args.no_cpu_baseline:

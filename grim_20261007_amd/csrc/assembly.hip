@@ -698,80 +698,95 @@ __global__ void __launch_bounds__(64) near_blocks(int n, const double* __restric
 }
 
 // Recursive four-way bisection with rtol 1e-9 and depth 12 (operators.py:1172-1292),
-// run depth first with an explicit stack; one wavefront per adaptive pair.
+// run depth first with an explicit stack.  One workgroup of four wavefronts per adaptive pair,
+// a grid-stride loop over the listed pairs: the four child boxes of a node are integrated at
+// once, one per wavefront (each with the wave-parallel 16 x 16 rule), so a pair's recursion
+// takes one box time per node instead of four.  Every box and every sum is formed in the same
+// order as by a single wavefront.
 #define GRIM_ADAPT_STACK 48
-__global__ void __launch_bounds__(64) near_adaptive(int n, const double* __restrict__ p0, const double* __restrict__ p1,
-                                                    const int* __restrict__ offsets, const int* __restrict__ pair_e,
-                                                    const int* __restrict__ pair_f, double k,
-                                                    double* __restrict__ blocks, const int* __restrict__ kind,
-                                                    unsigned* __restrict__ status) {
-    const int idx = blockIdx.x;
-    if (idx >= offsets[n] || kind[idx] != 3) return;
+__global__ void __launch_bounds__(256) near_adaptive(int n, const double* __restrict__ p0, const double* __restrict__ p1,
+                                                     const int* __restrict__ offsets, const int* __restrict__ pair_e,
+                                                     const int* __restrict__ pair_f, double k,
+                                                     double* __restrict__ blocks, const int* __restrict__ kind,
+                                                     unsigned* __restrict__ status) {
     __shared__ double st_iv[GRIM_ADAPT_STACK][4];
     __shared__ double st_blk[GRIM_ADAPT_STACK][16];
     __shared__ int st_depth[GRIM_ADAPT_STACK];
-    const int lane = threadIdx.x;
-    const int e = pair_e[idx], f = pair_f[idx];
-    const Elem o = load_elem(p0, p1, e), s = load_elem(p0, p1, f);
-    const double floor_ = 2.220446049250313e-16 * fmax(1.0, o.len * s.len);
-    SK root;
-    box16(o, s, k, 0.0, 1.0, 0.0, 1.0, root);
-    int top = 0;
-    if (lane == 0) {
-        st_iv[0][0] = 0.0; st_iv[0][1] = 1.0; st_iv[0][2] = 0.0; st_iv[0][3] = 1.0;
-        st_depth[0] = 0;
-    }
-    if (lane < 16) st_blk[0][lane] = root.v[lane];
-    top = 1;
-    __syncthreads();
-    double total[16];
-    for (int i = 0; i < 16; ++i) total[i] = 0.0;
-    bool failed = false;
-    while (top > 0) {
-        --top;
-        const double oa = st_iv[top][0], ob = st_iv[top][1], sa = st_iv[top][2], sb = st_iv[top][3];
-        const int depth = st_depth[top];
-        double coarse[16];
-        for (int i = 0; i < 16; ++i) coarse[i] = st_blk[top][i];
-        __syncthreads();
-        const double om = 0.5 * (oa + ob), sm = 0.5 * (sa + sb);
-        const double iv[4][4] = {{oa, om, sa, sm}, {oa, om, sm, sb}, {om, ob, sa, sm}, {om, ob, sm, sb}};
-        SK kid[4];
-        for (int c = 0; c < 4; ++c) box16(o, s, k, iv[c][0], iv[c][1], iv[c][2], iv[c][3], kid[c]);
-        double err = 0.0;
-        for (int part = 0; part < 2; ++part) {
-            double diff2 = 0.0, denom = 0.0;
-            for (int c = 0; c < 4; ++c) {
-                double n2 = 0.0;
-                for (int i = 0; i < 8; ++i) n2 += kid[c].v[8 * part + i] * kid[c].v[8 * part + i];
-                denom += sqrt(n2);
+    __shared__ double kid_blk[4][16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int listed = offsets[n];
+    for (int idx = blockIdx.x; idx < listed; idx += gridDim.x) {
+        if (kind[idx] != 3) continue;                      // uniform over the workgroup
+        const int e = pair_e[idx], f = pair_f[idx];
+        const Elem o = load_elem(p0, p1, e), s = load_elem(p0, p1, f);
+        const double floor_ = 2.220446049250313e-16 * fmax(1.0, o.len * s.len);
+        if (wave == 0) {
+            SK root;
+            box16(o, s, k, 0.0, 1.0, 0.0, 1.0, root);
+            if (lane == 0) {
+                st_iv[0][0] = 0.0; st_iv[0][1] = 1.0; st_iv[0][2] = 0.0; st_iv[0][3] = 1.0;
+                st_depth[0] = 0;
             }
-            for (int i = 0; i < 8; ++i) {
-                const double fine = kid[0].v[8 * part + i] + kid[1].v[8 * part + i] + kid[2].v[8 * part + i] +
-                                    kid[3].v[8 * part + i];
-                const double d = fine - coarse[8 * part + i];
-                diff2 += d * d;
-            }
-            err = fmax(err, sqrt(diff2) / fmax(denom, floor_));
+            if (lane < 16) st_blk[0][lane] = root.v[lane];
         }
-        if (err <= 1e-9 || depth >= 12 || top + 4 > GRIM_ADAPT_STACK) {
-            if (err > 1e-9) failed = true;
-            for (int c = 0; c < 4; ++c)
-                for (int i = 0; i < 16; ++i) total[i] += kid[c].v[i];
-        } else {
-            for (int c = 3; c >= 0; --c) {
-                if (lane == 0) {
-                    for (int j = 0; j < 4; ++j) st_iv[top][j] = iv[c][j];
-                    st_depth[top] = depth + 1;
+        int top = 1;
+        __syncthreads();
+        double total[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) total[i] = 0.0;
+        bool failed = false;
+        while (top > 0) {
+            --top;
+            const double oa = st_iv[top][0], ob = st_iv[top][1], sa = st_iv[top][2], sb = st_iv[top][3];
+            const int depth = st_depth[top];
+            double coarse[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) coarse[i] = st_blk[top][i];
+            const double om = 0.5 * (oa + ob), sm = 0.5 * (sa + sb);
+            const double iv[4][4] = {{oa, om, sa, sm}, {oa, om, sm, sb}, {om, ob, sa, sm}, {om, ob, sm, sb}};
+            {
+                SK kid;
+                box16(o, s, k, iv[wave][0], iv[wave][1], iv[wave][2], iv[wave][3], kid);
+                if (lane < 16) kid_blk[wave][lane] = kid.v[lane];
+            }
+            __syncthreads();
+            double err = 0.0;
+            for (int part = 0; part < 2; ++part) {
+                double diff2 = 0.0, denom = 0.0;
+                for (int c = 0; c < 4; ++c) {
+                    double n2 = 0.0;
+                    for (int i = 0; i < 8; ++i) n2 += kid_blk[c][8 * part + i] * kid_blk[c][8 * part + i];
+                    denom += sqrt(n2);
                 }
-                if (lane < 16) st_blk[top][lane] = kid[c].v[lane];
-                ++top;
+                for (int i = 0; i < 8; ++i) {
+                    const double fine = kid_blk[0][8 * part + i] + kid_blk[1][8 * part + i] +
+                                        kid_blk[2][8 * part + i] + kid_blk[3][8 * part + i];
+                    const double d = fine - coarse[8 * part + i];
+                    diff2 += d * d;
+                }
+                err = fmax(err, sqrt(diff2) / fmax(denom, floor_));
             }
+            if (err <= 1e-9 || depth >= 12 || top + 4 > GRIM_ADAPT_STACK) {
+                if (err > 1e-9) failed = true;
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) total[i] += kid_blk[c][i];
+            } else {
+                for (int c = 3; c >= 0; --c) {
+                    if (tid == 0) {
+                        for (int j = 0; j < 4; ++j) st_iv[top][j] = iv[c][j];
+                        st_depth[top] = depth + 1;
+                    }
+                    if (tid < 16) st_blk[top][tid] = kid_blk[c][tid];
+                    ++top;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        if (failed && tid == 0) atomicOr(status, (unsigned)GRIM_DEV_ADAPTIVE_DEPTH);
+        if (tid < 16) blocks[16 * (int64_t)idx + tid] = total[tid];
+        __syncthreads();                                   // the stack is reused by the next pair
     }
-    if (failed && lane == 0) atomicOr(status, (unsigned)GRIM_DEV_ADAPTIVE_DEPTH);
-    if (lane < 16) blocks[16 * (int64_t)idx + lane] = total[lane];
 }
 
 __device__ __forceinline__ int find_pair(const int* __restrict__ offsets, const int* __restrict__ pair_f, int row, int col) {
@@ -1018,7 +1033,8 @@ extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1
     scan_counts<<<1, 1024, 0, st>>>(n, counts, offsets);
     near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 1, counts, offsets, capacity, pair_e, pair_f, status);
     near_blocks<<<capacity, 64, 0, st>>>(n, p0, p1, next, offsets, pair_e, pair_f, k, blocks, kind, status);
-    near_adaptive<<<capacity, 64, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, k, blocks, kind, status);
+    near_adaptive<<<capacity < 1024 ? capacity : 1024, 256, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, k, blocks,
+                                                                     kind, status);
     near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, blocks, k, mode, mass,
                                                  mass_coef, ztm, ldtm, zte, ldte);
     return hipGetLastError();
