@@ -360,6 +360,7 @@ __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__
     __shared__ T cand_v[2][16];
     __shared__ int cand_i[2][16];
     __shared__ T prow_re[2][W], prow_im[2][W], crow_re[2][W], crow_im[2][W];
+    __shared__ T pinv[2][2];   // 1 / pivot, formed once by the pivot row's thread
     __shared__ int lpiv[W];
     __shared__ int zero_pivot;
     __shared__ int list_dst[2 * W], list_src[2 * W];
@@ -418,6 +419,10 @@ __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__
                 if (row == p) {
 #pragma unroll
                     for (int cc = 0; cc < W; ++cc) { prow_re[buf][cc] = vr[r][cc]; prow_im[buf][cc] = vi[r][cc]; }
+                    const T dr = vr[r][c], di = vi[r][c];
+                    const T den = dr * dr + di * di;
+                    pinv[buf][0] = den > 0 ? dr / den : 0;
+                    pinv[buf][1] = den > 0 ? -di / den : 0;
                 }
                 if (row == c) {
 #pragma unroll
@@ -425,9 +430,7 @@ __global__ void __launch_bounds__(1024) panel_leaf(int m, int w, T* __restrict__
                 }
             }
             __syncthreads();
-            const T dr = prow_re[buf][c], di = prow_im[buf][c];
-            const T den = dr * dr + di * di;
-            const T ir = den > 0 ? dr / den : 0, ii = den > 0 ? -di / den : 0;
+            const T ir = pinv[buf][0], ii = pinv[buf][1];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int row = tid + 1024 * r;
