@@ -22,14 +22,21 @@ REFINE_MAX_STEPS = 10        # linalg/refined_lu.py:49
 BACKWARD_ERROR_MAX = 1e-12   # twod/constants.py:24
 
 
-def request_device_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool = False) -> int:
+def request_device_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool = False,
+                         degree: int = 1) -> int:
     """Device bytes one co-polarized request allocates: the two assembled systems and their
     factors (unless cached), the loads, right-hand sides, solutions and residuals, refinement
-    and assembly workspace."""
+    and assembly workspace.  n counts unknowns; with degree-2 / degree-3 elements (n / degree
+    elements) the assembly workspace holds three tables of element-pair blocks,
+    48 (degree + 1)^2 (n / degree)^2 bytes."""
     z = 16 * n * n
     lu = (8 if precision == "mixed" else 16) * n * n
     vec = 16 * n * nangles
-    work = 0 if cached else NEAR_PER_ELEMENT * n * 160 + 64 * n
+    if degree > 1:
+        ne = n // degree
+        work = 0 if cached else 48 * (degree + 1) ** 2 * ne * ne + NEAR_PER_ELEMENT * ne * 16 + 64 * ne
+    else:
+        work = 0 if cached else NEAR_PER_ELEMENT * n * 160 + 64 * n
     refine = (24 * n * nangles) if precision == "mixed" else 0
     per_pol = vec * 3 + refine                       # rhs, solution, residual
     systems = 0 if cached else 2 * (z + lu)          # both polarizations assembled together

@@ -86,9 +86,9 @@ def choose_precision(n: int, nangles: int, corrections: Optional[int] = None) ->
     return "mixed" if mixed < plain else "double"
 
 
-def request_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool) -> int:
+def request_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool, degree: int = 1) -> int:
     """Device bytes a request allocates (systems, factors, loads, RHS, solutions, workspace)."""
-    return engine.request_device_bytes(n, nangles, npols, precision, cached)
+    return engine.request_device_bytes(n, nangles, npols, precision, cached, degree)
 
 
 class SystemCache:
@@ -153,7 +153,7 @@ class Server:
         if batch < 1:
             raise ValueError("batch must be at least 1")
         self.batch = int(batch)
-        self.batches: List[int] = []          # sizes of the request groups run, in order
+        self.batches: "deque" = deque(maxlen=4096)   # sizes of the latest request groups run, in order
         # launch-shape table from the autotuner (a dict or a JSON path); shapes never change results
         self.tuning = autotune.resolve(tuning)
         if self.tuning is not None:
@@ -352,7 +352,7 @@ class Server:
                 done.set()
 
         nbytes = request_bytes(system_dofs(mesh), len(req.angles_deg), len(req.pols), req.precision,
-                               cached is not None)
+                               cached is not None, mesh.degree)
         try:
             self._admit(nbytes)
             try:
