@@ -291,7 +291,7 @@ def solve_monostatic_rcs_2d_certified(
         quality_thresholds: Optional[Dict[str, Any]] = None,
         mesh_convergence_policy: Optional[Dict[str, Any]] = None, max_panels: int = MAX_PANELS_DEFAULT,
         mesh_reference_ghz: Optional[float] = None, rcs_normalization_mode: str = "physical",
-        abort_event=None, solver_method: str = "auto", *, precision: str = "double",
+        abort_event=None, solver_method: str = "auto", *, execution_options=None, precision: str = "double",
         device=None) -> Dict[str, Any]:
     """Certified co-polarized monostatic solve (twod/solver.py:3979-4005, 3685-3878).
 
@@ -308,7 +308,8 @@ def solve_monostatic_rcs_2d_certified(
         parts = [solve_monostatic_rcs_2d_certified(
             geometry_snapshot, [f], elevations_deg, geometry_units, material_base_dir, None, quality_thresholds,
             mesh_convergence_policy, max_panels, mesh_reference_ghz, rcs_normalization_mode, abort_event,
-            solver_method, precision=precision, device=device) for f in freqs]
+            solver_method, execution_options=execution_options, precision=precision, device=device)
+            for f in freqs]
         merged = {k: v for k, v in parts[0].items() if k not in ("samples", "co_solved_samples", "metadata")}
         merged["samples"] = [row for part in parts for row in part["samples"]]
         merged["co_solved_samples"] = {pol: [row for part in parts for row in part["co_solved_samples"][pol]]
@@ -323,7 +324,8 @@ def solve_monostatic_rcs_2d_certified(
                   geometry_units=geometry_units, material_base_dir=material_base_dir,
                   quality_thresholds=quality_thresholds, strict_quality_gate=True, compute_condition_number=True,
                   max_panels=max_panels, mesh_reference_ghz=mesh_reference_ghz, rcs_normalization_mode=rcs_normalization_mode,
-                  abort_event=abort_event, solver_method=solver_method, precision=precision)
+                  abort_event=abort_event, solver_method=solver_method, precision=precision,
+                  execution_options=execution_options)
     fine_snapshot = certification_fine_snapshot(geometry_snapshot, policy["fine_factor"])
     base, fine = _run_pair(solve_monostatic_rcs_2d, geometry_snapshot, fine_snapshot, kwargs, device)
     if progress_callback is not None:
@@ -345,6 +347,8 @@ def solve_monostatic_rcs_2d_certified(
                      "published_mesh": "fine", "geometry_model": "piecewise_linear_input",
                      "geometry_approximation_certified": False, "refinement_kind": "mesh",
                      "base_panel_count": base_panels, "fine_panel_count": fine_panels,
+                     "base_polynomial_degree": int(base["metadata"].get("polynomial_degree", 1)),
+                     "fine_polynomial_degree": int(fine["metadata"].get("polynomial_degree", 1)),
                      "panel_refinement_ratio": float(fine_panels) / float(base_panels) if base_panels else
                      float("nan")})
         if not gate["passed"]:

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import cmath
 import math
+import time
 from typing import Any, Callable, Dict, List, Optional
 
 import numpy as np
@@ -272,6 +273,14 @@ def execution_record(execution_options, scattering: str = "monostatic") -> dict:
     return out
 
 
+def _execution_metadata(metadata: dict, settings: dict, started: float) -> None:
+    """The effective settings record and the wall time of the call, as the reference's public
+    entries record them (execution/options.py:651-666)."""
+    metadata["execution_options"] = dict(settings, basis_order=metadata.get("polynomial_degree",
+                                                                            settings["basis_order"]))
+    metadata["execution_wall_seconds"] = time.perf_counter() - started
+
+
 def basis_order(execution_options) -> int:
     """The Galerkin degree an execution-settings record asks for: 1 (default), 2 or 3."""
     return int(execution_record(execution_options)["basis_order"])
@@ -505,7 +514,9 @@ def solve_monostatic_rcs_2d_single_polarization(
         raise ValueError("Only the physical RCS normalization is supported.")
     pol = _normalize_pol(polarization)
     unit_scale(geometry_units)
-    degree = basis_order(execution_options)
+    started = time.perf_counter()
+    settings = execution_record(execution_options)
+    degree = settings["basis_order"]
     evaluate_quality_gate({}, quality_thresholds)            # refuse unknown thresholds up front
     samples = []
     record = _Record(geometry_snapshot)
@@ -533,6 +544,7 @@ def solve_monostatic_rcs_2d_single_polarization(
     metadata.update({"polarization_internal": pol, "polarization_user": pol, "polarization_aliases": [pol],
                      "polarization_export": pol, "polarization_export_alias": alias,
                      "rcs_normalization_mode": "physical", "cfie_alpha": cfie_alpha})
+    _execution_metadata(metadata, settings, started)
     _apply_quality_gate(metadata, quality_thresholds, strict_quality_gate)
     return {"solver": "2d_bie_mom_rcs", "scattering_mode": "monostatic",
             "amplitude_convention": AMPLITUDE_CONVENTION, "amplitude_version": 2,
@@ -553,7 +565,9 @@ def solve_monostatic_rcs_2d(
     if str(rcs_normalization_mode).strip().lower() != "physical":
         raise ValueError("Only the physical RCS normalization is supported.")
     unit_scale(geometry_units)
-    degree = basis_order(execution_options)
+    started = time.perf_counter()
+    settings = execution_record(execution_options)
+    degree = settings["basis_order"]
     evaluate_quality_gate({}, quality_thresholds)
     co = {"VV": [], "HH": []}
     flat = []
@@ -589,6 +603,7 @@ def solve_monostatic_rcs_2d(
     metadata = record.metadata(compute_condition_number, precision, solver_method)
     metadata.update({"polarizations": ["VV", "HH"], "co_solve_shared_discretization": True,
                      "rcs_normalization_mode": "physical"})
+    _execution_metadata(metadata, settings, started)
     _apply_quality_gate(metadata, quality_thresholds, strict_quality_gate)
     return {"solver": "2d_bie_mom_rcs", "scattering_mode": "monostatic",
             "amplitude_convention": AMPLITUDE_CONVENTION, "amplitude_version": 2,

@@ -131,3 +131,27 @@ def test_poly_request_through_server():
     for res in (first, again):
         for pol in ("TE", "TM"):
             assert np.array_equal(res.amplitude[pol], direct[pol]["amplitude"])
+
+
+def test_poly_certified_pair_and_execution_metadata():
+    """The certified entry carries execution_options to both solves: a degree-2 base/fine pair
+    publishes the fine result, with both degrees in the gate and the effective settings record."""
+    from grim_20261007_amd.certify import certification_fine_snapshot, solve_monostatic_rcs_2d_certified
+    from grim_20261007_amd.solver import solve_monostatic_rcs_2d
+    _dev()
+    case = golden_cases.load("poly2_circle32_ka1")
+    f = float(case["frequency_ghz"])
+    angles = [0.0, 90.0, 180.0]
+    opts = {"basis_order": 2}
+    cert = solve_monostatic_rcs_2d_certified(golden_cases.snapshot(case), [f], angles, geometry_units="meters",
+                                             max_panels=100000, execution_options=opts)
+    meta = cert["metadata"]
+    assert meta["polynomial_degree"] == 2 and meta["execution_options"]["basis_order"] == 2
+    for export in ("VV", "HH"):
+        gate = meta["mesh_convergence"][export]
+        assert gate["passed"] and gate["base_polynomial_degree"] == 2 and gate["fine_polynomial_degree"] == 2
+    fine = solve_monostatic_rcs_2d(certification_fine_snapshot(golden_cases.snapshot(case), 1.5), [f], angles,
+                                   geometry_units="meters", max_panels=100000, compute_condition_number=True,
+                                   execution_options=opts)
+    for a, b in zip(cert["samples"], fine["samples"]):
+        assert a["rcs_amp_real"] == b["rcs_amp_real"] and a["rcs_amp_imag"] == b["rcs_amp_imag"]
