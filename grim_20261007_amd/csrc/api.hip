@@ -41,9 +41,10 @@ extern "C" hipError_t grim_launch_robin_rows(int64_t rows, int64_t cols, double*
                                              hipStream_t st);
 extern "C" hipError_t grim_launch_robin_split(int q, int n, const double* p0, const double* p1, const int* next,
                                               const int* prev, double k, int nblocks, const int* walk,
-                                              const int* offsets, const int* pair_f, const double* blocks,
-                                              const double* c_te, const double* c_tm, const int* flags, double* zte,
-                                              int64_t ldte, double* ztm, int64_t ldtm, hipStream_t st);
+                                              const int* offsets, const int* pair_f, int capacity,
+                                              const double* blocks, const double* c_te, const double* c_tm,
+                                              const int* flags, double* zte, int64_t ldte, double* ztm, int64_t ldtm,
+                                              hipStream_t st);
 extern "C" hipError_t grim_launch_robin_rhs(int n, int na, const double* p0, const double* p1, const int* prev,
                                             double k, const double* angles_deg, const double* alpha_e,
                                             const int* dirichlet, double* out, int64_t ld, hipStream_t st);
@@ -51,6 +52,22 @@ extern "C" hipError_t grim_launch_route(int m, int n, const double* src, int64_t
                                        const int* cols_src, double* dst, int64_t ld_dst, const int* rows_dst,
                                        const int* cols_dst, const double* w, hipStream_t st);
 extern "C" hipError_t grim_launch_scale_blocks(int64_t n, double* z, int64_t ld, const double* s, hipStream_t st);
+extern "C" hipError_t grim_launch_near_pairs(int n, const double* p0, const double* p1, const int* next, double k,
+                                             int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
+                                             double* blocks, int* kind, unsigned* status, hipStream_t st);
+extern "C" hipError_t grim_launch_sheet_far(int q, int ne, const double* p0, const double* p1, const int* nid,
+                                            double k, double* bs, hipStream_t st);
+extern "C" hipError_t grim_launch_sheet_near_store(int ne, const int* offsets, int capacity, const int* pair_e,
+                                                   const int* pair_f, const double* blocks, double* bs,
+                                                   hipStream_t st);
+extern "C" hipError_t grim_launch_sheet_gather(int hyper, int nn, int ne, const double* p0, const double* p1,
+                                               const int* own_ptr, const int* own_el, const int* own_loc,
+                                               const double* bs, double k, const double* coef, const int* pinned,
+                                               double* z, int64_t ld, hipStream_t st);
+extern "C" hipError_t grim_launch_sheet_loads(int nn, int na, const double* p0, const double* p1, const int* own_ptr,
+                                              const int* own_el, const int* own_loc, double k,
+                                              const double* angles_deg, double* bu, double* bdn, int64_t ld,
+                                              hipStream_t st);
 extern "C" int64_t grim_dense_aux_bytes(int n, int single);
 extern "C" hipError_t grim_dense_lu_factor(int batch, int n, void* a, int64_t lda, int64_t sa, int single, int* piv,
                                            int64_t spiv, void* aux, int64_t saux, unsigned* status, hipStream_t st);
@@ -485,6 +502,88 @@ extern "C" int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx
     return GRIM_OK;
 }
 
+extern "C" int grim_farfield_potential(int64_t n, int64_t na, const double* x, int64_t ldx, const double* b,
+                                       int64_t ldb, double k, double* amp, double* sigma, void* stream) {
+    if (n < 1 || na < 0 || !x || !b || !amp || ldx < n || ldb < n || !(k > 0) || !isfinite(k)) return GRIM_ERR_ARGS;
+    if (na == 0) return GRIM_OK;
+    CHECK(grim_launch_farfield((int)n, (int)na, x, ldx, b, nullptr, ldb, k, amp, sigma, (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sheet (TYPE 1) and sheet + PEC systems (sheet.hip)
+// ---------------------------------------------------------------------------
+struct SheetWork {
+    AssemblyWork near;   // status word, near-pair lists and linear near blocks
+    double* bs;          // element-pair S table, ne x ne blocks of 2 x 2 complex
+    int64_t bytes;
+};
+
+static SheetWork carve_sheet(void* base, int64_t ne, int64_t cap) {
+    SheetWork w;
+    w.near = carve(base, ne, cap);
+    w.bs = (double*)((char*)base + w.near.bytes);
+    w.bytes = w.near.bytes + align_up(64 * ne * ne);
+    return w;
+}
+
+static bool sheet_size_ok(int64_t ne, int64_t nn) {
+    return ne >= 1 && ne <= 65535 && nn >= 2 && nn <= 65535;
+}
+
+extern "C" int64_t grim_sheet_workspace_bytes(int64_t ne, int64_t near_per_element) {
+    if (ne < 1 || ne > 65535 || near_per_element <= 0) return -1;
+    return carve_sheet(nullptr, ne, ne * near_per_element).bytes;
+}
+
+extern "C" int grim_assemble_sheet(int64_t ne, int64_t nn, const double* p0, const double* p1,
+                                   const int32_t* next_ids, const int32_t* own_ptr, const int32_t* own_el,
+                                   const int32_t* own_loc, double k, int far_order_tm, int far_order_te,
+                                   const double* coef_tm, const double* coef_te, const int32_t* pinned,
+                                   double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te, void* workspace,
+                                   int64_t workspace_bytes, int64_t near_per_element, void* stream) {
+    if (!sheet_size_ok(ne, nn) || !p0 || !p1 || !next_ids || !own_ptr || !own_el || !own_loc || !workspace ||
+        (!z_tm && !z_te) || (z_tm && (!coef_tm || ld_tm < nn)) || (z_te && (!coef_te || ld_te < nn)) ||
+        !(k > 0) || !isfinite(k) || near_per_element <= 0 ||
+        (z_tm && far_order_tm != 8 && far_order_tm != 16) || (z_te && far_order_te != 8 && far_order_te != 16))
+        return GRIM_ERR_ARGS;
+    const int64_t cap = ne * near_per_element;
+    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
+    const SheetWork w = carve_sheet(workspace, ne, cap);
+    if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
+    hipStream_t st = (hipStream_t)stream;
+    zero_words<<<1, 64, 0, st>>>(w.near.status, 1);
+    CHECK(hipGetLastError());
+    CHECK(grim_launch_near_pairs((int)ne, p0, p1, next_ids, k, w.near.counts, w.near.offsets, (int)cap,
+                                 w.near.pair_e, w.near.pair_f, w.near.blocks, w.near.kind, w.near.status, st));
+    const int q0 = z_tm ? far_order_tm : far_order_te;
+    CHECK(grim_launch_sheet_far(q0, (int)ne, p0, p1, next_ids, k, w.bs, st));
+    CHECK(grim_launch_sheet_near_store((int)ne, w.near.offsets, (int)cap, w.near.pair_e, w.near.pair_f,
+                                       w.near.blocks, w.bs, st));
+    if (z_tm)
+        CHECK(grim_launch_sheet_gather(0, (int)nn, (int)ne, p0, p1, own_ptr, own_el, own_loc, w.bs, k, coef_tm,
+                                       nullptr, z_tm, ld_tm, st));
+    if (z_te) {
+        // the far pass rewrites only far blocks: the near blocks stored above stay
+        if (far_order_te != q0) CHECK(grim_launch_sheet_far(far_order_te, (int)ne, p0, p1, next_ids, k, w.bs, st));
+        CHECK(grim_launch_sheet_gather(1, (int)nn, (int)ne, p0, p1, own_ptr, own_el, own_loc, w.bs, k, coef_te,
+                                       pinned, z_te, ld_te, st));
+    }
+    return GRIM_OK;
+}
+
+extern "C" int grim_loads_sheet(int64_t ne, int64_t nn, int64_t na, const double* p0, const double* p1,
+                                const int32_t* own_ptr, const int32_t* own_el, const int32_t* own_loc, double k,
+                                const double* angles_deg, double* bu, double* bdn, int64_t ld, void* stream) {
+    if (!sheet_size_ok(ne, nn) || na < 0 || na > 65535 || !p0 || !p1 || !own_ptr || !own_el || !own_loc ||
+        !angles_deg || (!bu && !bdn) || ld < nn || !(k > 0) || !isfinite(k))
+        return GRIM_ERR_ARGS;
+    if (na == 0) return GRIM_OK;
+    CHECK(grim_launch_sheet_loads((int)nn, (int)na, p0, p1, own_ptr, own_el, own_loc, k, angles_deg, bu, bdn, ld,
+                                  (hipStream_t)stream));
+    return GRIM_OK;
+}
+
 static bool batch_ok(int64_t batch) { return batch >= 1 && batch <= 65535; }
 
 extern "C" int64_t grim_lu_aux_bytes(int64_t n, int precision) {
@@ -794,7 +893,8 @@ extern "C" int grim_robin_split(int64_t n, const double* p0, const double* p1, c
     const AssemblyWork w = carve(const_cast<void*>(workspace), n, cap);
     if (w.bytes > workspace_bytes) return GRIM_ERR_RANGE;
     CHECK(grim_launch_robin_split(far_order, (int)n, p0, p1, next, prev, k, (int)nblocks, walk, w.offsets, w.pair_f,
-                                  w.blocks, c_te, c_tm, flags, z_te, ld_te, z_tm, ld_tm, (hipStream_t)stream));
+                                  (int)cap, w.blocks, c_te, c_tm, flags, z_te, ld_te, z_tm, ld_tm,
+                                  (hipStream_t)stream));
     return GRIM_OK;
 }
 

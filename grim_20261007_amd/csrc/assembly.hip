@@ -659,10 +659,10 @@ __device__ void touching16(const Elem& o, const Elem& s, double k, bool obs_star
 __global__ void __launch_bounds__(64) near_blocks(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                   const int* __restrict__ next, const int* __restrict__ offsets,
                                                   const int* __restrict__ pair_e, const int* __restrict__ pair_f,
-                                                  double k, double* __restrict__ blocks, int* __restrict__ kind,
-                                                  unsigned* __restrict__ status) {
+                                                  int capacity, double k, double* __restrict__ blocks,
+                                                  int* __restrict__ kind, unsigned* __restrict__ status) {
     const int idx = blockIdx.x;
-    if (idx >= offsets[n]) return;
+    if (idx >= offsets[n] || offsets[n] > capacity) return;   // overflow: the lists were not written
     const int e = pair_e[idx], f = pair_f[idx];
     const Elem o = load_elem(p0, p1, e), s = load_elem(p0, p1, f);
     SK r;
@@ -706,7 +706,7 @@ __global__ void __launch_bounds__(64) near_blocks(int n, const double* __restric
 #define GRIM_ADAPT_STACK 48
 __global__ void __launch_bounds__(256) near_adaptive(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                      const int* __restrict__ offsets, const int* __restrict__ pair_e,
-                                                     const int* __restrict__ pair_f, double k,
+                                                     const int* __restrict__ pair_f, int capacity, double k,
                                                      double* __restrict__ blocks, const int* __restrict__ kind,
                                                      unsigned* __restrict__ status) {
     __shared__ double st_iv[GRIM_ADAPT_STACK][4];
@@ -715,6 +715,7 @@ __global__ void __launch_bounds__(256) near_adaptive(int n, const double* __rest
     __shared__ double kid_blk[4][16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int listed = offsets[n];
+    if (listed > capacity) return;                          // overflow: the lists were not written
     for (int idx = blockIdx.x; idx < listed; idx += gridDim.x) {
         if (kind[idx] != 3) continue;                      // uniform over the workgroup
         const int e = pair_e[idx], f = pair_f[idx];
@@ -812,12 +813,12 @@ __device__ __forceinline__ void add_entry(double* z, int64_t ld, int p, int q, c
 __global__ void __launch_bounds__(256) near_gather(int n, const double* __restrict__ p0, const double* __restrict__ p1,
                                                    const int* __restrict__ next, const int* __restrict__ prev,
                                                    const int* __restrict__ offsets, const int* __restrict__ pair_f,
-                                                   const double* __restrict__ blocks, double k, int mode,
+                                                   int capacity, const double* __restrict__ blocks, double k, int mode,
                                                    double mass, const double* __restrict__ mass_coef,
                                                    double* __restrict__ ztm, int64_t ldtm,
                                                    double* __restrict__ zte, int64_t ldte) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= n || offsets[n] > capacity) return;           // overflow: the lists were not written
     const double g = -1.0 / k;
     const int elems[2] = {p, prev[p]};
     for (int loc = 0; loc < 2; ++loc) {
@@ -887,33 +888,6 @@ __global__ void __launch_bounds__(256) near_gather(int n, const double* __restri
 // ---------------------------------------------------------------------------
 // Plane-wave loads and far field
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void sinc_d(double u, double& f, double& fp) {
-    if (fabs(u) < 0.01) {
-        const double v2 = u * u;
-        f = 1 + v2 * (-1.0 / 6 + v2 * (1.0 / 120 + v2 * (-1.0 / 5040 + v2 / 362880)));
-        fp = u * (-1.0 / 6 + v2 * (1.0 / 60 + v2 * (-1.0 / 1680 + v2 * (1.0 / 90720 - v2 / 7983360))));
-    } else {
-        double s, c;
-        sincos(u, &s, &c);
-        f = s / u;
-        fp = (u * c - s) / (2 * u * u);
-    }
-}
-
-// Moments of element e for direction (dx, dy): i0, i1 and j k d.n (kernels.py:20-39, 260-285).
-__device__ __forceinline__ void elem_moments(const Elem& E, double k, double dx, double dy, cplx& i0, cplx& i1,
-                                             cplx& dn) {
-    const double z = k * (E.sx * dx + E.sy * dy);
-    double f, fp;
-    sinc_d(0.5 * z, f, fp);
-    double s, c;
-    sincos(k * (E.cx * dx + E.cy * dy), &s, &c);
-    const cplx ph = {E.len * c, E.len * s};
-    i0 = cmul(ph, cplx{0.5 * f, fp});
-    i1 = cmul(ph, cplx{0.5 * f, -fp});
-    dn = {0.0, k * (E.nx * dx + E.ny * dy)};
-}
-
 // bu[p, a] = i0(p) + i1(prev p); bdn likewise with j k d.n.  angles in degrees.
 __global__ void __launch_bounds__(256) loads_kernel(int n, int na, const double* __restrict__ p0,
                                                     const double* __restrict__ p1, const int* __restrict__ prev,
@@ -936,7 +910,8 @@ __global__ void __launch_bounds__(256) loads_kernel(int n, int na, const double*
     if (bdn) { bdn[at] = v.re; bdn[at + 1] = v.im; }
 }
 
-// B(a) = sum_p x[p,a] bu[p,a] + gamma sum_p x[p,a] bdn[p,a]; sigma = |B|^2 / (4k).
+// B(a) = sum_p x[p,a] bu[p,a] + gamma sum_p x[p,a] bdn[p,a]; sigma = |B|^2 / (4k).  With bdn null,
+// B(a) = sum_p x[p,a] bu[p,a]: the far field of one potential whose plane-wave moments bu holds.
 __global__ void __launch_bounds__(256) farfield_kernel(int n, const double* __restrict__ x, int64_t ldx,
                                                        const double* __restrict__ bu, const double* __restrict__ bdn,
                                                        int64_t ldb, double k, double* __restrict__ amp,
@@ -948,8 +923,10 @@ __global__ void __launch_bounds__(256) farfield_kernel(int n, const double* __re
         const double xr = x[ix], xi = x[ix + 1];
         s0r += xr * bu[ib] - xi * bu[ib + 1];
         s0i += xr * bu[ib + 1] + xi * bu[ib];
-        s1r += xr * bdn[ib] - xi * bdn[ib + 1];
-        s1i += xr * bdn[ib + 1] + xi * bdn[ib];
+        if (bdn) {
+            s1r += xr * bdn[ib] - xi * bdn[ib + 1];
+            s1i += xr * bdn[ib + 1] + xi * bdn[ib];
+        }
     }
     __shared__ double red[4][4];
     s0r = wave_sum(s0r); s0i = wave_sum(s0i); s1r = wave_sum(s1r); s1i = wave_sum(s1i);
@@ -1032,11 +1009,28 @@ extern "C" hipError_t grim_launch_near(int n, const double* p0, const double* p1
     near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 0, counts, offsets, capacity, pair_e, pair_f, status);
     scan_counts<<<1, 1024, 0, st>>>(n, counts, offsets);
     near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 1, counts, offsets, capacity, pair_e, pair_f, status);
-    near_blocks<<<capacity, 64, 0, st>>>(n, p0, p1, next, offsets, pair_e, pair_f, k, blocks, kind, status);
-    near_adaptive<<<capacity < 1024 ? capacity : 1024, 256, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, k, blocks,
-                                                                     kind, status);
-    near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, blocks, k, mode, mass,
+    near_blocks<<<capacity, 64, 0, st>>>(n, p0, p1, next, offsets, pair_e, pair_f, capacity, k, blocks, kind,
+                                         status);
+    near_adaptive<<<capacity < 1024 ? capacity : 1024, 256, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, capacity, k,
+                                                                     blocks, kind, status);
+    near_gather<<<(n + 255) / 256, 256, 0, st>>>(n, p0, p1, next, prev, offsets, pair_f, capacity, blocks, k, mode,
+                                                 mass,
                                                  mass_coef, ztm, ldtm, zte, ldte);
+    return hipGetLastError();
+}
+
+// The near-pair lists and the linear S / K' blocks of every listed pair (no gather), for callers
+// that form node entries from element-pair blocks themselves (sheet systems).
+extern "C" hipError_t grim_launch_near_pairs(int n, const double* p0, const double* p1, const int* next, double k,
+                                             int* counts, int* offsets, int capacity, int* pair_e, int* pair_f,
+                                             double* blocks, int* kind, unsigned* status, hipStream_t st) {
+    near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 0, counts, offsets, capacity, pair_e, pair_f, status);
+    scan_counts<<<1, 1024, 0, st>>>(n, counts, offsets);
+    near_list<<<n, 64, 0, st>>>(n, p0, p1, next, 1, counts, offsets, capacity, pair_e, pair_f, status);
+    near_blocks<<<capacity, 64, 0, st>>>(n, p0, p1, next, offsets, pair_e, pair_f, capacity, k, blocks, kind,
+                                         status);
+    near_adaptive<<<capacity < 1024 ? capacity : 1024, 256, 0, st>>>(n, p0, p1, offsets, pair_e, pair_f, capacity, k,
+                                                                     blocks, kind, status);
     return hipGetLastError();
 }
 
@@ -1157,11 +1151,13 @@ __global__ void __launch_bounds__(64) robin_split(int n, const double* __restric
                                                   const int* __restrict__ next, const int* __restrict__ prev,
                                                   double k, const int* __restrict__ walk, int chunk,
                                                   const int* __restrict__ offsets, const int* __restrict__ pair_f,
-                                                  const double* __restrict__ blocks, const double* __restrict__ c_te,
+                                                  int capacity, const double* __restrict__ blocks,
+                                                  const double* __restrict__ c_te,
                                                   const double* __restrict__ c_tm, const int* __restrict__ flags,
                                                   double* __restrict__ zte, int64_t ldte, double* __restrict__ ztm,
                                                   int64_t ldtm) {
     const int lane = threadIdx.x;
+    if (offsets[n] > capacity) return;                      // overflow: the lists were not written
     const int w = walk[64 * (int64_t)blockIdx.x + lane];
     const bool active = w >= 0 && w < n;
     const int e = active ? w : 0;
@@ -1257,18 +1253,19 @@ __global__ void __launch_bounds__(256) robin_rhs_kernel(int n, int na, const dou
 
 extern "C" hipError_t grim_launch_robin_split(int q, int n, const double* p0, const double* p1, const int* next,
                                               const int* prev, double k, int nblocks, const int* walk,
-                                              const int* offsets, const int* pair_f, const double* blocks,
-                                              const double* c_te, const double* c_tm, const int* flags, double* zte,
-                                              int64_t ldte, double* ztm, int64_t ldtm, hipStream_t st) {
+                                              const int* offsets, const int* pair_f, int capacity,
+                                              const double* blocks, const double* c_te, const double* c_tm,
+                                              const int* flags, double* zte, int64_t ldte, double* ztm, int64_t ldtm,
+                                              hipStream_t st) {
     if (nblocks == 0) return hipSuccess;
     const int chunk = 32;
     dim3 grid((unsigned)nblocks, (unsigned)((n + chunk - 1) / chunk));
     if (q == 16)
-        robin_split<16><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, walk, chunk, offsets, pair_f, blocks, c_te,
-                                             c_tm, flags, zte, ldte, ztm, ldtm);
+        robin_split<16><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, walk, chunk, offsets, pair_f, capacity, blocks,
+                                             c_te, c_tm, flags, zte, ldte, ztm, ldtm);
     else
-        robin_split<8><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, walk, chunk, offsets, pair_f, blocks, c_te,
-                                            c_tm, flags, zte, ldte, ztm, ldtm);
+        robin_split<8><<<grid, 64, 0, st>>>(n, p0, p1, next, prev, k, walk, chunk, offsets, pair_f, capacity, blocks,
+                                            c_te, c_tm, flags, zte, ldte, ztm, ldtm);
     return hipGetLastError();
 }
 

@@ -322,6 +322,34 @@ __device__ __forceinline__ bool is_far(const Elem& a, int ea, int na, const Elem
     return dist / scale >= 3.0;
 }
 
+// Plane-wave moments of straight elements
+__device__ __forceinline__ void sinc_d(double u, double& f, double& fp) {
+    if (fabs(u) < 0.01) {
+        const double v2 = u * u;
+        f = 1 + v2 * (-1.0 / 6 + v2 * (1.0 / 120 + v2 * (-1.0 / 5040 + v2 / 362880)));
+        fp = u * (-1.0 / 6 + v2 * (1.0 / 60 + v2 * (-1.0 / 1680 + v2 * (1.0 / 90720 - v2 / 7983360))));
+    } else {
+        double s, c;
+        sincos(u, &s, &c);
+        f = s / u;
+        fp = (u * c - s) / (2 * u * u);
+    }
+}
+
+// Moments of element e for direction (dx, dy): i0, i1 and j k d.n (kernels.py:20-39, 260-285).
+__device__ __forceinline__ void elem_moments(const Elem& E, double k, double dx, double dy, cplx& i0, cplx& i1,
+                                             cplx& dn) {
+    const double z = k * (E.sx * dx + E.sy * dy);
+    double f, fp;
+    sinc_d(0.5 * z, f, fp);
+    double s, c;
+    sincos(k * (E.cx * dx + E.cy * dy), &s, &c);
+    const cplx ph = {E.len * c, E.len * s};
+    i0 = cmul(ph, cplx{0.5 * f, fp});
+    i1 = cmul(ph, cplx{0.5 * f, -fp});
+    dn = {0.0, k * (E.nx * dx + E.ny * dy)};
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

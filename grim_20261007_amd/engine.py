@@ -23,15 +23,21 @@ BACKWARD_ERROR_MAX = 1e-12   # twod/constants.py:24
 
 
 def request_device_bytes(n: int, nangles: int, npols: int, precision: str, cached: bool = False,
-                         degree: int = 1) -> int:
+                         degree: int = 1, sheet_elements: int = 0) -> int:
     """Device bytes one co-polarized request allocates: the two assembled systems and their
     factors (unless cached), the loads, right-hand sides, solutions and residuals, refinement
     and assembly workspace.  n counts unknowns; with degree-2 / degree-3 elements (n / degree
     elements) the assembly workspace holds three tables of element-pair blocks,
-    48 (degree + 1)^2 (n / degree)^2 bytes."""
+    48 (degree + 1)^2 (n / degree)^2 bytes.  A sheet mesh (sheet_elements > 0 panels) solves one
+    polarization at a time over a table of 64 ne^2 bytes of element-pair S blocks."""
     z = 16 * n * n
     lu = (8 if precision == "mixed" else 16) * n * n
     vec = 16 * n * nangles
+    if sheet_elements > 0:
+        ne = sheet_elements
+        work = 64 * ne * ne + NEAR_PER_ELEMENT * ne * 160 + 64 * ne
+        refine = (24 * n * nangles) if precision == "mixed" else 0
+        return z + lu + 2 * vec + vec * 3 + refine + work
     if degree > 1:
         ne = n // degree
         work = 0 if cached else 48 * (degree + 1) ** 2 * ne * ne + NEAR_PER_ELEMENT * ne * 16 + 64 * ne
@@ -211,6 +217,102 @@ def farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
 
 
 ETA0 = 376.730313668      # free-space wave impedance (twod/constants.py)
+
+
+# ----------------------------------------------------------------------------
+# Sheets (TYPE 1 open contours), alone or beside PEC contours (twod/formulations/sheet.py)
+# ----------------------------------------------------------------------------
+class DeviceSheet:
+    """One polarization's sheet mesh resident on one GPU: element geometry, the near test's end ids,
+    the node-to-element map and the pinned strip-end rows."""
+
+    def __init__(self, mesh, device: torch.device | str = "cuda"):
+        self.mesh = mesh
+        self.device = torch.device(device)
+        native.ensure_device(self.device.index if self.device.index is not None else torch.cuda.current_device())
+        self.n, self.ne = mesh.n, mesh.ne
+        dev = self.device
+        self.p0 = torch.as_tensor(np.ascontiguousarray(mesh.p0, np.float64), device=dev)
+        self.p1 = torch.as_tensor(np.ascontiguousarray(mesh.p1, np.float64), device=dev)
+        self.next_ids = torch.as_tensor(mesh.next_ids(), device=dev)
+        own_ptr, own_el, own_loc = mesh.owners()
+        self.own_ptr = torch.as_tensor(own_ptr, device=dev)
+        self.own_el = torch.as_tensor(own_el, device=dev)
+        self.own_loc = torch.as_tensor(own_loc, device=dev)
+        pinned = np.zeros(mesh.n, np.int32)
+        pinned[mesh.endpoints] = 1
+        self.pinned = torch.as_tensor(pinned, device=dev)
+        self.endpoints = torch.as_tensor(mesh.endpoints, device=dev)
+
+
+def sheet_workspace(ne: int, device) -> torch.Tensor:
+    nbytes = int(native.lib().grim_sheet_workspace_bytes(ne, NEAR_PER_ELEMENT))
+    if nbytes < 0:
+        raise ValueError("Sheet meshes are limited to 65535 panels.")
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def sheet_coefficients(mesh, k: float, pol: str) -> np.ndarray:
+    """Element mass coefficients: Zs / (j k eta0) (TM), j k Zs / eta0 (TE); 0 on PEC elements
+    (twod/formulations/sheet.py:12-22)."""
+    z = np.asarray(mesh.impedance, complex)
+    return z / (1j * k * ETA0) if pol == "TM" else (1j * k / ETA0) * z
+
+
+def sheet_system(sheet: DeviceSheet, k: float, pol: str, workspace: torch.Tensor | None = None,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """The (N, N) complex128 system (column-major) of one polarization: S - c M (TM) or W - c M with
+    the strip-end rows pinned (TE)."""
+    n, ne, dev = sheet.n, sheet.ne, sheet.device
+    z = torch.empty((n, n), dtype=torch.complex128, device=dev) if out is None else out
+    ws = workspace if workspace is not None else sheet_workspace(ne, dev)
+    coef = torch.as_tensor(sheet_coefficients(sheet.mesh, k, pol), device=dev)
+    te = pol == "TE"
+    st = _stream(dev)
+    code = native.lib().grim_assemble_sheet(
+        ne, n, sheet.p0.data_ptr(), sheet.p1.data_ptr(), sheet.next_ids.data_ptr(), sheet.own_ptr.data_ptr(),
+        sheet.own_el.data_ptr(), sheet.own_loc.data_ptr(), float(k), sheet.mesh.far_order(k, False),
+        sheet.mesh.far_order(k, True), None if te else coef.data_ptr(), coef.data_ptr() if te else None,
+        sheet.pinned.data_ptr() if te else None, None if te else z.data_ptr(), n, z.data_ptr() if te else None, n,
+        ws.data_ptr(), ws.numel(), NEAR_PER_ELEMENT, st)
+    native.check("grim_assemble_sheet", code)
+    native.check("grim_assemble_sheet (device)", native.lib().grim_workspace_status(ws.data_ptr(), st))
+    return z
+
+
+def sheet_loads(sheet: DeviceSheet, k: float, angles_deg: torch.Tensor):
+    """Plane-wave loads (A, N) of the sheet mesh's nodes: b_u and b_dn."""
+    na = int(angles_deg.numel())
+    bu = torch.empty((na, sheet.n), dtype=torch.complex128, device=sheet.device)
+    bdn = torch.empty((na, sheet.n), dtype=torch.complex128, device=sheet.device)
+    code = native.lib().grim_loads_sheet(sheet.ne, sheet.n, na, sheet.p0.data_ptr(), sheet.p1.data_ptr(),
+                                         sheet.own_ptr.data_ptr(), sheet.own_el.data_ptr(),
+                                         sheet.own_loc.data_ptr(), float(k), angles_deg.data_ptr(), bu.data_ptr(),
+                                         bdn.data_ptr(), sheet.n, _stream(sheet.device))
+    native.check("grim_loads_sheet", code)
+    return bu, bdn
+
+
+def sheet_rhs(sheet: DeviceSheet, pol: str, bu: torch.Tensor, bdn: torch.Tensor) -> torch.Tensor:
+    """-b_u (TM); b_dn with the pinned strip-end rows zero (TE) (twod/formulations/sheet.py:33-39)."""
+    if pol == "TM":
+        return -bu
+    rhs = bdn.clone()
+    if sheet.endpoints.numel():
+        rhs[:, sheet.endpoints] = 0
+    return rhs
+
+
+def farfield_potential(x: torch.Tensor, b: torch.Tensor, k: float):
+    """Amplitude B (A,) = sum_p x[a, p] b[a, p] of one potential (b = b_u: single layer, b_dn: double
+    layer) and sigma_2D (A,)."""
+    na, n = x.shape
+    amp = torch.empty(na, dtype=torch.complex128, device=x.device)
+    sigma = torch.empty(na, dtype=torch.float64, device=x.device)
+    code = native.lib().grim_farfield_potential(n, na, x.data_ptr(), n, b.data_ptr(), n, float(k), amp.data_ptr(),
+                                                sigma.data_ptr(), _stream(x.device))
+    native.check("grim_farfield_potential", code)
+    return amp, sigma
 
 
 @dataclass
@@ -657,13 +759,15 @@ def region_exterior_density(plan: RegionPlan, n: int, x: torch.Tensor) -> torch.
 def bistatic_farfield(x: torch.Tensor, bu: torch.Tensor, bdn: torch.Tensor, k: float):
     """Bistatic amplitudes B[b, i, s] = sum_p x[b, i, p] (bu[s, p] + gamma bdn[s, p]), gamma = 1/(j k),
     for densities x (B, A_inc, N) and observation-direction loads (A_obs, N); with s = i this is
-    the matched monostatic projection.  One complex GEMM per batch on the library's FP64 MFMA
-    kernel (the observation operator is shared: stride 0).  Returns (amplitude, sigma_2D), both
-    (B, A_inc, A_obs)."""
+    the matched monostatic projection.  With bdn None, B[b, i, s] = sum_p x[b, i, p] bu[s, p]: the
+    far field of the one potential whose loads bu holds.  One complex GEMM per batch on the
+    library's FP64 MFMA kernel (the observation operator is shared: stride 0).  Returns
+    (amplitude, sigma_2D), both (B, A_inc, A_obs)."""
     xb = _as_batch(x).contiguous()
     nb, ninc, n = xb.shape
     nobs = bu.shape[0]
-    proj_t = (bu + bdn * complex(0.0, -1.0 / float(k))).transpose(0, 1).contiguous()   # (N, A_obs)
+    proj = bu if bdn is None else bu + bdn * complex(0.0, -1.0 / float(k))
+    proj_t = proj.transpose(0, 1).contiguous()   # (N, A_obs)
     c = torch.zeros((nb, ninc, nobs), dtype=torch.complex128, device=x.device)
     code = native.lib().grim_gemm_minus(nb, nobs, ninc, n, proj_t.data_ptr(), nobs, 0, xb.data_ptr(), n, ninc * n,
                                         c.data_ptr(), nobs, ninc * nobs, 64, _stream(x.device))
@@ -884,10 +988,15 @@ def solve(f: Factor, b: torch.Tensor, gate: bool = True):
 
 
 def backward_errors(a_inf: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor) -> np.ndarray:
-    """Per system of a batch (B, A, N): max over columns of max|r| / (||Z||_inf max|x| + max|b|)."""
+    """Per system of a batch (B, A, N): max over columns of max|r| / (||Z||_inf max|x| + max|b|); a
+    column with a zero denominator (b = 0, x = 0) counts 0, or infinity if its residual is not zero
+    (linalg/dense.py:257-263)."""
     ri = torch.amax(torch.abs(r), dim=2)
     den = a_inf[:, None] * torch.amax(torch.abs(x), dim=2) + torch.amax(torch.abs(b), dim=2)
-    return torch.amax(ri / den, dim=1).cpu().numpy()
+    pos = den > 0
+    ratio = torch.where(pos, ri / torch.where(pos, den, torch.ones_like(den)),
+                        torch.where(ri > 0, torch.full_like(ri, math.inf), torch.zeros_like(ri)))
+    return torch.amax(ratio, dim=1).cpu().numpy()
 
 
 def backward_error(a_inf: torch.Tensor, x: torch.Tensor, b: torch.Tensor, r: torch.Tensor) -> float:

@@ -792,14 +792,17 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     wavelength0 = min(material_wavelength(segments, media, f) for f in freqs)
     welder, junctions = boundary_condition_junctions(segments, scale, laws, freqs)
     panels0, panels1, zs, keys_e = [], [], [], []
+    types_e, flags_e, names_e = [], [], []
     kinds, dielectric_flags = set(), set()
     for s, seg in enumerate(segments):
         seg_type, n_prop, ibc = _parse_props(seg)
         name = str(seg.get("name", "segment"))
-        if seg_type not in (2, 3, 4, 5):
-            raise ValueError(f"Segment '{name}': TYPE {seg_type} boundaries (sheets) are not supported by the GPU "
-                             "path: conductors (TYPE 2), dielectric interfaces with air (TYPE 3), coated conductors "
-                             "(TYPE 4) and dielectric/dielectric interfaces (TYPE 5) are.")
+        if seg_type not in (1, 2, 3, 4, 5):
+            raise ValueError(f"Segment '{name}': TYPE {seg_type} boundaries are not supported by the GPU path: "
+                             "sheets (TYPE 1), conductors (TYPE 2), dielectric interfaces with air (TYPE 3), "
+                             "coated conductors (TYPE 4) and dielectric/dielectric interfaces (TYPE 5) are.")
+        if seg_type == 1 and ibc <= 0:
+            raise ValueError(f"TYPE 1 panel '{name}' requires IBC > 0 in coupled dielectric mode.")
         kinds.add(seg_type)
         # an explicit count is held to the media on the segment's own sides
         local_wavelength = min(material_wavelength([seg], media, f) for f in freqs)
@@ -815,15 +818,19 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
         if seg_type == 3:
             dielectric_flags.add(pos_mat)
         # (minus region, plus region) of the segment's panels: 0 air, -1 conductor
-        region_key = {2: (0, -1), 3: (0, pos_mat), 4: (-1, pos_mat), 5: (neg_mat, pos_mat)}[seg_type]
+        region_key = {1: (0, 0), 2: (0, -1), 3: (0, pos_mat), 4: (-1, pos_mat), 5: (neg_mat, pos_mat)}[seg_type]
         law = None
         if ibc > 0:
             if ibc not in laws:
                 raise ValueError(f"Undefined IBC flag {ibc}.")
             law = laws[ibc]
             if law.kind == "thin_dielectric":
-                raise ValueError("Thin dielectric layers can be assigned only to a free TYPE 1 sheet.")
-        wavelength = surface_wave_mesh_wavelength(wavelength0, law, ibc, freqs)
+                if seg_type != 1:
+                    raise ValueError("Thin dielectric layers can be assigned only to a free TYPE 1 sheet.")
+                raise ValueError(f"Segment '{name}': transmitting thin dielectric layers on TYPE 1 sheets are not "
+                                 "supported by the GPU path; resistive and reactive sheets (impedance laws) are.")
+        # bound surface waves size opaque conductor segments only (twod/geometry.py:765, 1540-1545)
+        wavelength = wavelength0 if seg_type == 1 else surface_wave_mesh_wavelength(wavelength0, law, ibc, freqs)
         pairs = list(seg.get("point_pairs", []) or [])
         if seg_type in (2, 3):
             # TYPE 2 and 3: the user draws the normal into air; reverse the chain so that the stored
@@ -880,10 +887,16 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
             cum += length
             zs.append(0j if law is None else law.impedance(ibc, law_ghz, 1.0 - arc if seg_type in (2, 3) else arc))
             keys_e.append(region_key)
+            types_e.append(seg_type)
+            flags_e.append(ibc)
+            names_e.append(name.strip() or "__type1_sheet__")
     if not panels0:
         raise ValueError("Geometry does not contain any valid discretized panels.")
     p0 = np.array(panels0)
     p1 = np.array(panels1)
+    if 1 in kinds:
+        return SheetLayout(p0, p1, np.array(zs, complex), np.array(types_e, np.int64), np.array(flags_e, np.int64),
+                           names_e)
     # node welding: first appearance, snap key or within the tolerance
     keys: dict = {}
     nodes: list = []
@@ -988,3 +1001,254 @@ def _check_orientation(mesh: ContourMesh) -> None:
         if not inner and area <= 0:
             raise ValueError("Geometry orientation check failed: a TYPE 2 or 3 contour must be drawn with its "
                              "normal into the air (clockwise around the body).")
+
+
+# ----------------------------------------------------------------------------
+# Sheets (TYPE 1 open contours), alone or beside PEC contours
+# ----------------------------------------------------------------------------
+SHEET_MIXED_MESSAGE = (
+    "Mixed TYPE 1 sheet + (IBC-coated body / dielectric body / layered coating) geometries are not currently "
+    "supported. Supported options: all-sheet (any number of TYPE 1 sheets, each with its own Z_s), or sheet + "
+    "pure-PEC TYPE 2 body (mixed sheet+PEC is handled by _solve_mixed_sheet_pec).  For a tapered resistive "
+    "treatment on a coated body, use TYPE 2 with a tapered IBC row -- that's the physically correct model for "
+    "'resistance transitioning from air to a conducting body'.")
+SHEET_ATTACHED_MESSAGE = ("TE sheet attached to a PEC body requires a coupled junction condition; this geometry is "
+                          "not supported. Model the fin as a closed finite-thickness PEC contour.")
+SHEET_BRANCH_MESSAGE = "TE sheet branching junctions require an unsupported junction condition."
+
+
+class SheetMesh:
+    """Linear mesh of one polarization of a sheet layout: element e runs from node elements[e, 0] to
+    node elements[e, 1], nodes numbered by first appearance and shared only between panels of one
+    interface signature (twod/geometry.py:2189-2290), so a node may have one element (a strip end),
+    two, or more.  ``endpoints`` are the TE strip ends whose rows are pinned."""
+
+    degree = 1
+
+    def __init__(self, nodes, p0, p1, elements, impedance, sheet, endpoints):
+        self.nodes = np.asarray(nodes, float)
+        self.p0 = np.ascontiguousarray(p0, float)
+        self.p1 = np.ascontiguousarray(p1, float)
+        self.elements = np.asarray(elements, np.int64)
+        self.impedance = np.asarray(impedance, complex)
+        self.sheet = np.asarray(sheet, bool)
+        self.endpoints = np.asarray(endpoints, np.int64)
+
+    @property
+    def n(self) -> int:
+        return int(len(self.nodes))
+
+    @property
+    def ne(self) -> int:
+        return int(len(self.p0))
+
+    @property
+    def lengths(self) -> np.ndarray:
+        return np.linalg.norm(self.p1 - self.p0, axis=1)
+
+    @property
+    def normals(self) -> np.ndarray:
+        t = (self.p1 - self.p0) / self.lengths[:, None]
+        return np.stack((-t[:, 1], t[:, 0]), axis=1)
+
+    def far_order(self, k: float, hypersingular: bool = True) -> int:
+        """Far rule: the W-calibrated 8 / 16 points of the hypersingular traversal (twod/operators.py:
+        1502-1527), 8 for the single layer alone (its graded rule never exceeds the configured 8)."""
+        if not hypersingular:
+            return 8
+        lengths = self.lengths
+        lmax, lmin = float(lengths.max()), float(lengths.min())
+        return 8 if 1e-4 <= abs(k) * lmax <= 3.0 and lmin >= 1e-3 * lmax else 16
+
+    def owners(self):
+        """(own_ptr, own_el, own_loc): the elements of each node in CSR form, ascending element order."""
+        pairs = sorted((int(self.elements[e, c]), e, c) for e in range(self.ne) for c in (0, 1))
+        ptr = np.zeros(self.n + 1, np.int32)
+        for nid, _, _ in pairs:
+            ptr[nid + 1] += 1
+        return (np.cumsum(ptr).astype(np.int32), np.array([e for _, e, _ in pairs], np.int32),
+                np.array([c for _, _, c in pairs], np.int32))
+
+    def next_ids(self) -> np.ndarray:
+        """Id of each element's end in the near test's convention: the first element starting at that
+        node, else an id >= ne of the node alone."""
+        starts = {}
+        for e in range(self.ne):
+            starts.setdefault(int(self.elements[e, 0]), e)
+        out = np.empty(self.ne, np.int32)
+        for e in range(self.ne):
+            end = int(self.elements[e, 1])
+            out[e] = starts.get(end, self.ne + end)
+        return out
+
+
+def _closed_conductor_loops(p0, p1, body) -> set:
+    """Panels of closed one-sided conductor loops (their impedance flags do not split nodes,
+    twod/geometry.py:2143-2186)."""
+    welder = _Welder()
+    ends = {i: (welder.key(p0[i]), welder.key(p1[i])) for i in np.flatnonzero(body)}
+    incidence: dict = {}
+    for i, keys in ends.items():
+        for key in keys:
+            incidence.setdefault(key, []).append(i)
+    closed, remaining = set(), set(ends)
+    while remaining:
+        component, stack = set(), [remaining.pop()]
+        while stack:
+            i = stack.pop()
+            if i in component:
+                continue
+            component.add(i)
+            for key in ends[i]:
+                stack.extend(j for j in incidence[key] if j not in component)
+        remaining -= component
+        if all(len(incidence[key]) == 2 for i in component for key in ends[i]):
+            closed |= component
+    return closed
+
+
+def _orient_sheets(p0, p1, sheet):
+    """TE: walk each non-branching sheet chain from a free end (its lowest-index panel's first free
+    end, else that panel's start) and flip panels to follow the walk (twod/geometry.py:2292-2343)."""
+    p0, p1 = p0.copy(), p1.copy()
+    welder = _Welder()
+    ends, incidence = {}, {}
+    for i in np.flatnonzero(sheet):
+        ends[i] = (welder.key(p0[i]), welder.key(p1[i]))
+        for key in ends[i]:
+            incidence.setdefault(key, []).append(int(i))
+    if any(len(v) > 2 for v in incidence.values()):
+        raise ValueError(SHEET_BRANCH_MESSAGE)
+    remaining = {i for v in incidence.values() for i in v}
+    flipped = np.zeros(len(p0), bool)
+    while remaining:
+        first = min(remaining)
+        component, stack = set(), [first]
+        while stack:
+            i = stack.pop()
+            if i in component:
+                continue
+            component.add(i)
+            for key in ends[i]:
+                stack.extend(j for j in incidence[key] if j not in component)
+        start = ends[first][0]
+        found = False
+        for i in sorted(component):
+            for key in ends[i]:
+                if len(incidence[key]) == 1:
+                    start, found = key, True
+                    break
+            if found:
+                break
+        while True:
+            choices = [i for i in incidence[start] if i in remaining]
+            if not choices:
+                break
+            i = min(choices)
+            head, tail = ends[i]
+            if head != start:
+                p0[i], p1[i] = p1[i].copy(), p0[i].copy()
+                flipped[i] = True
+                head, tail = tail, head
+            remaining.discard(i)
+            start = tail
+    return p0, p1, flipped
+
+
+def _sheet_endpoints(nodes, p0, p1, elements, sheet, keys) -> np.ndarray:
+    """Strip ends: nodes that one sheet panel end lands on (counted by welded vertex).  Raises for a
+    sheet that touches a body, which needs a coupled junction condition
+    (twod/solver.py:1683-1736)."""
+    count, sheet_nodes, body_keys = {}, set(), set()
+    for e in range(len(p0)):
+        if not sheet[e]:
+            body_keys.update(keys[int(n)] for n in elements[e])
+            continue
+        for n in elements[e]:
+            sheet_nodes.add(int(n))
+            count[keys[int(n)]] = count.get(keys[int(n)], 0) + 1
+    attached = bool(body_keys.intersection(count))
+    if body_keys and not attached:
+        starts, dirs = p0[~sheet], p1[~sheet] - p0[~sheet]
+        len2 = np.sum(dirs * dirs, axis=1)
+        for n in sheet_nodes:
+            if count[keys[n]] != 1:
+                continue
+            rel = nodes[n] - starts
+            t = np.clip(np.sum(rel * dirs, axis=1) / len2, 0.0, 1.0)
+            if np.any(np.linalg.norm(rel - t[:, None] * dirs, axis=1) <= 1e-9):
+                attached = True
+                break
+    if attached:
+        raise ValueError(SHEET_ATTACHED_MESSAGE)
+    return np.array(sorted(n for n in sheet_nodes if count[keys[n]] == 1), np.int64)
+
+
+class SheetLayout:
+    """Panels of a geometry with TYPE 1 sheets: all sheets, or sheets beside pure-PEC TYPE 2 contours
+    (twod/solver.py:1557-1633, 1739-1777).  The two polarizations mesh it differently: TE orients every
+    sheet chain and lets sheets share nodes whatever their impedance flag or name; TM shares a node
+    only between panels of one flag and one sheet name (twod/geometry.py:2189-2290).  ``mesh(pol)``
+    builds that polarization's SheetMesh."""
+
+    degree = 1
+    medium = None
+    regions = None
+
+    def __init__(self, p0, p1, impedance, types, flags, names):
+        self.p0, self.p1 = np.asarray(p0, float), np.asarray(p1, float)
+        self.impedance = np.asarray(impedance, complex)
+        self.types, self.flags, self.names = np.asarray(types), np.asarray(flags), list(names)
+        self.sheet = self.types == 1
+        body = ~self.sheet
+        if np.any(body & ((self.types != 2) | (np.abs(self.impedance) > EPS))):
+            raise ValueError(SHEET_MIXED_MESSAGE)
+        for e in np.flatnonzero(self.sheet):
+            if abs(self.impedance[e]) <= EPS:
+                raise ValueError(f"TYPE 1 panel '{self.names[e]}' has zero impedance; provide non-zero IBC for "
+                                 "sheet mode.")
+        self.closed = _closed_conductor_loops(self.p0, self.p1, body)
+        self._meshes = {}
+
+    @property
+    def n(self) -> int:
+        """Panel count."""
+        return int(len(self.p0))
+
+    @property
+    def lengths(self) -> np.ndarray:
+        return np.linalg.norm(self.p1 - self.p0, axis=1)
+
+    @property
+    def all_sheet(self) -> bool:
+        return bool(np.all(self.sheet))
+
+    def mesh(self, pol: str) -> SheetMesh:
+        pol = pol.upper()
+        if pol not in self._meshes:
+            self._meshes[pol] = self._build(pol)
+        return self._meshes[pol]
+
+    def _build(self, pol: str) -> SheetMesh:
+        te = pol == "TE"
+        p0, p1 = (_orient_sheets(self.p0, self.p1, self.sheet)[:2] if te else (self.p0.copy(), self.p1.copy()))
+        welder = _Welder()
+        index, nodes, keys = {}, [], []
+        elements = np.zeros((len(p0), 2), np.int64)
+        for e in range(len(p0)):
+            if self.sheet[e]:
+                sig = (1, 0, "") if te else (1, int(self.flags[e]), self.names[e])
+            else:
+                sig = (2, 0 if e in self.closed else int(self.flags[e]), "")
+            for c, xy in enumerate((p0[e], p1[e])):
+                key = welder.key(xy)
+                found = index.get((key, sig))
+                if found is None:
+                    found = len(nodes)
+                    index[(key, sig)] = found
+                    nodes.append(np.array(xy, float))
+                    keys.append(key)
+                elements[e, c] = found
+        nodes = np.array(nodes)
+        endpoints = (_sheet_endpoints(nodes, p0, p1, elements, self.sheet, keys) if te else np.zeros(0, np.int64))
+        return SheetMesh(nodes, p0, p1, elements, self.impedance, self.sheet, endpoints)

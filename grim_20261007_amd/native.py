@@ -33,6 +33,11 @@ SIGNATURES = {
     "grim_workspace_status": (_i32, [_vp, _vp]),
     "grim_loads": (_i32, [_i64, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _i64, _vp]),
     "grim_farfield": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp, _i64, _dbl, _vp, _vp, _vp]),
+    "grim_farfield_potential": (_i32, [_i64, _i64, _vp, _i64, _vp, _i64, _dbl, _vp, _vp, _vp]),
+    "grim_sheet_workspace_bytes": (_i64, [_i64, _i64]),
+    "grim_assemble_sheet": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _i32, _i32, _vp, _vp, _vp,
+                                   _vp, _i64, _vp, _i64, _vp, _i64, _i64, _vp]),
+    "grim_loads_sheet": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _i64, _vp]),
     "grim_lu_aux_bytes": (_i64, [_i64, _i32]),
     "grim_lu_factor": (_i32, [_i64, _i64, _vp, _i64, _i64, _i32, _vp, _i64, _vp, _i64, _vp, _vp]),
     "grim_lu_solve_workspace_bytes": (_i64, [_i64, _i64, _i64, _i32]),

@@ -28,8 +28,8 @@ import numpy as np
 import torch
 
 from . import autotune, engine
-from .mesh import ContourMesh
-from .solver import system_dofs, wavenumber
+from .mesh import ContourMesh, SheetLayout
+from .solver import _solve_sheet_frequency, system_dofs, wavenumber
 
 
 @dataclass
@@ -323,8 +323,20 @@ class Server:
             except BaseException as exc:   # surfaced through the future
                 fut.set_exception(exc)
 
+    def _run_sheet(self, req: Request) -> Result:
+        mesh = req.mesh
+        tm = mesh.mesh("TM")
+        nbytes = engine.request_device_bytes(tm.n, len(req.angles_deg), 1, "double", sheet_elements=tm.ne)
+        self._admit(nbytes)
+        try:
+            return execute_sheet(req, self.device)
+        finally:
+            self._release(nbytes)
+
     def _run(self, req: Request, ws: dict) -> Result:
         mesh = req.mesh
+        if isinstance(mesh, SheetLayout):
+            return self._run_sheet(req)
         if req.precision == "auto":
             req.precision = choose_precision(mesh.n, len(req.angles_deg))
         k = wavenumber(req.frequency_ghz)
@@ -364,6 +376,21 @@ class Server:
                 with self.inflight_lock:
                     self.inflight.pop(key, None)
                 done.set()
+
+
+def execute_sheet(req: Request, device) -> Result:
+    """A sheet layout's request: each polarization on its own mesh, factored and solved alone (no
+    factor cache)."""
+    if req.precision == "auto":
+        req.precision = choose_precision(system_dofs(req.mesh), len(req.angles_deg))
+    started = time.perf_counter()
+    res = _solve_sheet_frequency(req.mesh, req.frequency_ghz, list(map(float, req.angles_deg)), tuple(req.pols),
+                                 req.precision, device, False, None, False, "The served sheet solve")
+    finished = time.perf_counter()
+    info = {pol: {"precision": r["precision_used"], "formulation": r["formulation"]} for pol, r in res.items()}
+    return Result(req.rid, {p: r["amplitude"] for p, r in res.items()}, {p: r["sigma"] for p, r in res.items()},
+                  {p: float(np.max(r["residual"])) for p, r in res.items()}, False,
+                  finished - (req.arrival or started), started, finished, info)
 
 
 def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store=None) -> Result:

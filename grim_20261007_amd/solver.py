@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from . import engine
-from .mesh import C0, build_mesh, causal_index, unit_scale
+from .mesh import C0, SheetLayout, build_mesh, causal_index, unit_scale
 
 RCS_DB_FLOOR = 1e-12
 AMPLITUDE_CONVENTION = "A_physical_asymptotic = +j * B_stored"
@@ -177,12 +177,12 @@ def _summarize_residuals(values):
 
 
 def admit_device_memory(n: int, nangles: int, npols: int, precision: str, device, context: str,
-                        condition: bool = False, degree: int = 1) -> float:
+                        condition: bool = False, degree: int = 1, sheet_elements: int = 0) -> float:
     """Refuse a solve whose device footprint exceeds 0.9 of the GPU memory it can use now (free
     memory plus what PyTorch's allocator holds unused), before anything is assembled, as the
     reference refuses host memory (twod/solver.py:1419-1554, 2342-2400).  Returns the estimate
     in GiB."""
-    need = engine.request_device_bytes(n, nangles, npols, precision, degree=degree)
+    need = engine.request_device_bytes(n, nangles, npols, precision, degree=degree, sheet_elements=sheet_elements)
     if condition:   # equilibration scales, two probe columns and a 256 x 256 staging block per system
         need += 2 * (3 * 8 * n + 16 * 2 * 2 * n + 16 * 256 * 256)
     free, _total = torch.cuda.mem_get_info(device)
@@ -298,6 +298,8 @@ def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, m
         served, mesh_ghz = [float(f)], float(f)
     mesh = build_mesh(snapshot, units, mesh_ghz, max_panels, material_base_dir=material_base_dir,
                       law_ghz=float(f), mesh_frequencies=served)
+    if degree > 1 and isinstance(mesh, SheetLayout):
+        raise ValueError("Polynomial bases of degree 2 and 3 are built for PEC contours only.")
     if degree > 1:
         if mesh.impedance is not None or mesh.medium is not None or mesh.regions is not None:
             raise ValueError("Polynomial bases of degree 2 and 3 are built for PEC contours only.")
@@ -307,7 +309,10 @@ def _frequency_mesh(snapshot, units, f, freqs, mesh_reference_ghz, max_panels, m
 
 def system_dofs(mesh) -> int:
     """Unknowns of the mesh's system: one per node, two (psi, phi) for a dielectric body, one per
-    node and bordering region for a multi-region layout."""
+    node and bordering region for a multi-region layout; for a sheet layout the nodes of its TM mesh
+    (the TE mesh never has more)."""
+    if isinstance(mesh, SheetLayout):
+        return mesh.mesh("TM").n
     if mesh.regions is not None:
         return engine.region_plan(mesh).n_dof
     return 2 * mesh.n if mesh.medium is not None else mesh.dofs
@@ -338,6 +343,94 @@ def _systems(contour, k, angles, workspace):
             bu, bdn, lambda x: x)
 
 
+SHEET_DENSITY_MESSAGE = (   # twod/solver.py:1636-1657
+    "TYPE 1 free-floating sheets are not supported by compute_boundary_densities (the boundary-density export "
+    "path).  They ARE supported for RCS: use solve_monostatic_rcs_2d or solve_bistatic_rcs_2d, which route sheet "
+    "geometries to the dedicated sheet BIEs (_solve_tm_sheet / _solve_te_sheet / _solve_mixed_sheet_pec).")
+
+SHEET_FORMULATIONS = {
+    (True, "TM"): "2D sheet BIE (TM: single-layer representation)",
+    (True, "TE"): "2D sheet BIE (TE: double-layer / hypersingular representation)",
+    (False, "TM"): "2D mixed sheet+PEC BIE (TM: unified SLP representation)",
+    (False, "TE"): "2D mixed sheet+PEC BIE (TE: unified DLP / hypersingular representation)",
+}   # twod/solver.py:2409-2439
+
+
+def _sheet_solve(layout: SheetLayout, pol: str, k: float, angles: torch.Tensor, precision: str, dev,
+                 compute_condition_number: bool, abort_event, context: str, extra_angles: int = 0):
+    """One polarization of a sheet layout on its own mesh: the system S - c M (TM) or W - c M with
+    pinned strip ends (TE), loads -b_u / b_dn, factor and solve (twod/solver.py:1746-1787)."""
+    mesh = layout.mesh(pol)
+    admit_device_memory(mesh.n, int(angles.numel()) + extra_angles, 1, precision, dev, context,
+                        compute_condition_number, sheet_elements=mesh.ne)
+    sheet = engine.DeviceSheet(mesh, dev)
+    z = engine.sheet_system(sheet, k, pol)
+    bu, bdn = engine.sheet_loads(sheet, k, angles)
+    rhs = engine.sheet_rhs(sheet, pol, bu, bdn).unsqueeze(0)
+    if abort_event is not None and abort_event.is_set():
+        raise InterruptedError("Solve canceled by user.")
+    fac = engine.factor(z.unsqueeze(0), precision)
+    del z
+    conds = methods = None
+    if compute_condition_number:
+        from . import condition
+        conds, methods, replacement = condition.condition_estimates(fac)
+        if replacement is not None:
+            fac = replacement
+    x, info = engine.solve(fac, rhs)
+    rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
+    final = info.get("factor", fac)
+    part = engine.system(final, 0) if final.parts is not None else final
+    res = {"residual": rel[0].cpu().numpy(), "info": {kk: v for kk, v in info.items() if kk not in ("residual", "factor")},
+           "k": k, "n": mesh.n, "precision_used": part.precision, "fallback_reason": part.fallback_reason,
+           "formulation": SHEET_FORMULATIONS[(layout.all_sheet, pol)]}
+    if conds is not None:
+        res["condition"] = float(conds[0])
+        res["condition_method"] = methods[0]
+    return sheet, x[0], bu, bdn, res
+
+
+def _solve_sheet_frequency(layout: SheetLayout, frequency_ghz: float, angles_deg: List[float], pols,
+                           precision: str, device, return_density: bool, abort_event,
+                           compute_condition_number: bool, context: str) -> Dict[str, Dict[str, Any]]:
+    """Sheets alone or beside PEC contours (twod/solver.py:2409-2460): per polarization its own mesh
+    and the single-layer (TM) or double-layer (TE) matched far field of the density
+    (twod/operators.py:2925-3023)."""
+    dev = _device(device)
+    k = wavenumber(frequency_ghz)
+    angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
+    out = {}
+    for pol in [p for p in ("TE", "TM") if p in pols]:
+        _sheet, x, bu, bdn, res = _sheet_solve(layout, pol, k, angles, precision, dev, compute_condition_number,
+                                               abort_event, context)
+        amp, sigma = engine.farfield_potential(x, bu if pol == "TM" else bdn, k)
+        res.update({"amplitude": amp.cpu().numpy(), "sigma": sigma.cpu().numpy()})
+        if return_density:
+            res["density"] = x.cpu().numpy().T
+        out[pol] = res
+    return out
+
+
+def _solve_sheet_bistatic_frequency(layout: SheetLayout, frequency_ghz: float, incidence_deg: List[float],
+                                    observation_deg: List[float], pols, precision: str, device, abort_event,
+                                    compute_condition_number: bool) -> Dict[str, Dict[str, Any]]:
+    """Bistatic sheets: every incidence solved, each density projected on every observation
+    direction with its polarization's potential."""
+    dev = _device(device)
+    k = wavenumber(frequency_ghz)
+    inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)
+    obs = torch.tensor(observation_deg, dtype=torch.float64, device=dev)
+    out = {}
+    for pol in [p for p in ("TE", "TM") if p in pols]:
+        sheet, x, _bu, _bdn, res = _sheet_solve(layout, pol, k, inc, precision, dev, compute_condition_number,
+                                                abort_event, "The bistatic 2-D solve", len(observation_deg))
+        bu_o, bdn_o = engine.sheet_loads(sheet, k, obs)
+        amp, sigma = engine.bistatic_farfield(x, bu_o if pol == "TM" else bdn_o, None, k)
+        res.update({"amplitude": amp[0].cpu().numpy(), "sigma": sigma[0].cpu().numpy()})
+        out[pol] = res
+    return out
+
+
 def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("TE", "TM"),
                     precision: str = "double", device=None, return_density: bool = False,
                     abort_event=None, compute_condition_number: bool = False,
@@ -345,6 +438,9 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     """Assemble both systems once, then factor, solve and project each requested polarization.
     Per polarization: amplitude, sigma, relative residual per angle, refinement info, the
     condition estimate when requested, and the precision the factor ended in."""
+    if isinstance(mesh, SheetLayout):
+        return _solve_sheet_frequency(mesh, frequency_ghz, angles_deg, pols, precision, device, return_density,
+                                      abort_event, compute_condition_number, context)
     dev = _device(device)
     k = wavenumber(frequency_ghz)
     order = [p for p in ("TE", "TM") if p in pols]
@@ -436,6 +532,7 @@ class _Record:
         self.batches = 0
         self.factorizations = 0
         self.degrees: List[int] = []
+        self.formulation = FORMULATION
 
     def add_mesh(self, mesh, mesh_ghz):
         self.panel_counts.append(mesh.n)
@@ -452,6 +549,7 @@ class _Record:
         if res["fallback_reason"] and res["fallback_reason"] not in self.reasons:
             self.reasons.append(res["fallback_reason"])
         self.precisions.append(res["precision_used"])
+        self.formulation = res.get("formulation", self.formulation)
         self.columns += nangles
         self.batches += 1
 
@@ -474,7 +572,7 @@ class _Record:
             "polynomial_degree": self.degrees[0] if self.degrees else 1,
             "polynomial_degree_min": min(self.degrees) if self.degrees else 1,
             "polynomial_degree_max": max(self.degrees) if self.degrees else 1,
-            "formulation": FORMULATION, "solver_method": "dense_lu", "solver_method_requested": str(solver_method),
+            "formulation": self.formulation, "solver_method": "dense_lu", "solver_method_requested": str(solver_method),
             "residual_norm_max": rmax, "residual_norm_mean": rmean, "residual_nonfinite_count": nonfinite,
             "constraint_residual_norm_max": 0.0, "constraint_residual_norm_mean": 0.0,
             "constraint_residual_applicable": False,
@@ -617,6 +715,9 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
                              abort_event=None, compute_condition_number: bool = False) -> Dict[str, Dict[str, Any]]:
     """Both systems from one assembly; solve for every incidence, then project every solution
     on every observation direction (twod/solver.py:3476-3517 with observation angles)."""
+    if isinstance(mesh, SheetLayout):
+        return _solve_sheet_bistatic_frequency(mesh, frequency_ghz, incidence_deg, observation_deg, pols, precision,
+                                               device, abort_event, compute_condition_number)
     dev = _device(device)
     k = wavenumber(frequency_ghz)
     order = [p for p in ("TE", "TM") if p in pols]
@@ -801,6 +902,8 @@ def compute_boundary_densities(geometry_snapshot: Dict[str, Any], frequency_ghz:
         raise ValueError("frequency_ghz must be a positive finite value.")
     mesh_ghz, mesh = _frequency_mesh(geometry_snapshot, geometry_units, frequency_ghz, [frequency_ghz],
                                      mesh_reference_ghz, max_panels, material_base_dir)
+    if isinstance(mesh, SheetLayout):
+        raise ValueError(SHEET_DENSITY_MESSAGE)
     check_abort()
     res = solve_frequency(mesh, frequency_ghz, [float(elevation_deg)], (pol,), precision, device,
                           return_density=True, abort_event=abort_event,

@@ -165,6 +165,41 @@ int grim_loads(int64_t n, int64_t na, const double* p0, const double* p1, const 
 int grim_farfield(int64_t n, int64_t na, const double* x, int64_t ldx, const double* bu, const double* bdn,
                   int64_t ldb, double k, double* amp, double* sigma, void* stream);
 
+/* Far field of one potential: B[a] = sum_p x[p,a] b[p,a] with b the plane-wave moments of that
+ * potential (bu for a single layer, bdn for a double layer), sigma[a] = |B[a]|^2 / (4k).  Replaces
+ * twod/operators.py:2925-3023 (_farfield_linear_density_many, matched projection, potential 'SLP' or
+ * 'DLP') as the sheet solves use it (twod/solver.py:1780-1787).  amp: complex128 [na]; sigma may be
+ * NULL. */
+int grim_farfield_potential(int64_t n, int64_t na, const double* x, int64_t ldx, const double* b, int64_t ldb,
+                            double k, double* amp, double* sigma, void* stream);
+
+/* Sheet (TYPE 1 open contour) systems, alone or beside PEC contours, linear Galerkin bases on a
+ * general node map (twod/formulations/sheet.py:6-30, twod/solver.py:1746-1787):
+ *   z_tm = S - diag-block mass with element coefficient coef_tm (Zs / (j k eta0) on sheets, 0 on PEC)
+ *   z_te = W - mass with coef_te (j k Zs / eta0), rows with pinned[p] != 0 replaced by the identity
+ *          row (the strip ends, twod/solver.py:1683-1736)
+ * where S is the single layer (far rule far_order_tm, 8 or 16) and W the Maue hypersingular form of
+ * the element-pair S blocks (twod/operators.py:2799-2858; far rule far_order_te, 8 or 16).  Elements
+ * are given by p0, p1 (ne x 2 float64); next_ids[e] is the element starting at element e's end node,
+ * or an id >= ne of that node alone when none does (the ids of the near test); nodes own their
+ * elements through own_ptr (nn + 1) / own_el / own_loc (local 0 = start, 1 = end); coef_* are
+ * complex128 [ne]; pinned int32 [nn] or NULL.  Either output may be NULL (that polarization is not
+ * formed).  Status as grim_assemble (grim_workspace_status on the workspace afterwards).
+ * ne, nn <= 65535. */
+int64_t grim_sheet_workspace_bytes(int64_t ne, int64_t near_per_element);
+int grim_assemble_sheet(int64_t ne, int64_t nn, const double* p0, const double* p1, const int32_t* next_ids,
+                        const int32_t* own_ptr, const int32_t* own_el, const int32_t* own_loc, double k,
+                        int far_order_tm, int far_order_te, const double* coef_tm, const double* coef_te,
+                        const int32_t* pinned, double* z_tm, int64_t ld_tm, double* z_te, int64_t ld_te,
+                        void* workspace, int64_t workspace_bytes, int64_t near_per_element, void* stream);
+
+/* Plane-wave loads of a sheet mesh's nodes (as grim_loads, over the node map of
+ * grim_assemble_sheet): the TM load is -bu, the TE load bdn with pinned rows zero
+ * (twod/formulations/sheet.py:33-39).  complex128 nn x na, leading dimension ld. */
+int grim_loads_sheet(int64_t ne, int64_t nn, int64_t na, const double* p0, const double* p1, const int32_t* own_ptr,
+                     const int32_t* own_el, const int32_t* own_loc, double k, const double* angles_deg, double* bu,
+                     double* bdn, int64_t ld, void* stream);
+
 /* Batched dense LU.  All dense calls take `batch` equally strided matrices
  * (strides in complex elements for matrices/vectors, in elements for int32
  * arrays, in bytes for aux), e.g. the TE and TM systems of one request.

@@ -53,6 +53,12 @@ What each function follows in the reference (paths under ``tools/GHOST/ghost_bac
                           regions.py:557-590); ``robin_alpha`` follows
                           regions.py:66-119 (TM nodes of PEC elements: alpha 0).
 
+* ``sheet_system``        twod/formulations/sheet.py:6-39 (TM: S - Zs/(j k eta0) M,
+                          -b_u; TE: W - j k Zs/eta0 M, b_dn, strip-end rows pinned),
+                          S with the configured 8-point far rule and W from the
+                          W-only traversal (twod/operators.py:2799-2858, far rule
+                          as ``far_order``), mass twod/assembly/mass.py:6-24.
+
 Pinning (see tests/test_oracle_*.py): every function above is checked against
 the reference's own outputs (tests/golden/*.npz, written by
 tests/golden/make_golden.py), against closed forms (Mie series, exact
@@ -1065,3 +1071,60 @@ def condition_estimate(z, precision: str = "double"):
     finally:
         np.random.set_state(saved)
     return est if math.isfinite(est) else float("inf")
+
+
+def sheet_operators(mesh: Mesh, k: float, q: int, rows_per_chunk: int = 16):
+    """Single layer S and its Maue hypersingular form W (node order) of a mesh of open or closed
+    chains, far pairs by the q-point rule, near pairs by ``near_blocks``."""
+    E, N = len(mesh.length), len(mesh.nodes)
+    s_out = np.zeros((N, N), complex)
+    w_out = np.zeros((N, N), complex)
+    ids = mesh.elements
+    nn = np.einsum("ed,fd->ef", mesh.normal, mesh.normal)
+    for start in range(0, E, rows_per_chunk):
+        rows = np.arange(start, min(E, start + rows_per_chunk))
+        far = far_mask(mesh, rows)
+        sb = _far_rows(mesh, k, rows, far, q)[0]
+        lp = np.maximum(mesh.length[rows][:, None] * mesh.length[None, :], EPS * EPS)
+        wb = (-(k * k) * nn[rows][:, :, None, None] * sb
+              + TANGENT[None, None] * (sb.sum(axis=(2, 3)) / lp)[:, :, None, None])
+        for a in range(2):
+            for b in range(2):
+                r, c = ids[rows, a][:, None], ids[None, :, b]
+                np.add.at(s_out, (r, c), sb[:, :, a, b])
+                np.add.at(w_out, (r, c), wb[:, :, a, b])
+    for e, f in np.argwhere(~far_mask(mesh)):
+        e, f = int(e), int(f)
+        sb = near_blocks(mesh, k, e, f)[0]
+        wb = maue(sb, k, mesh.normal[e], mesh.normal[f], mesh.length[e], mesh.length[f])
+        for a in range(2):
+            for b in range(2):
+                s_out[ids[e, a], ids[f, b]] += sb[a, b]
+                w_out[ids[e, a], ids[f, b]] += wb[a, b]
+    return s_out, w_out
+
+
+def sheet_system(mesh: Mesh, k: float, element_impedance, pol: str, angles_deg, endpoints=()):
+    """System and right-hand sides of sheets (alone or beside PEC contours, Zs = 0 there) in one
+    polarization: TM  S - (Zs / (j k eta0)) M, -b_u;  TE  W - (j k Zs / eta0) M, b_dn, with the
+    strip-end rows replaced by identity rows and zero loads."""
+    z = np.asarray(element_impedance, complex)
+    bu, bdn = incident_loads(mesh, k, angles_deg)
+    if pol == "TM":
+        s = sheet_operators(mesh, k, 8)[0]
+        return s - weighted_mass(mesh, z / (1j * k * ETA0)), -bu
+    w = sheet_operators(mesh, k, far_order(mesh, k))[1]
+    matrix = w - weighted_mass(mesh, (1j * k / ETA0) * z)
+    rhs = bdn.copy()
+    ends = np.asarray(endpoints, np.int64)
+    if ends.size:
+        matrix[ends] = 0
+        matrix[ends, ends] = 1
+        rhs[ends] = 0
+    return matrix, rhs
+
+
+def sheet_farfield(mesh: Mesh, k: float, angles_deg, x, pol: str):
+    """Single-layer (TM) or double-layer (TE) matched far field of the nodal density x (N x A)."""
+    bu, bdn = incident_loads(mesh, k, angles_deg)
+    return np.einsum("na,na->a", x, bu if pol == "TM" else bdn)

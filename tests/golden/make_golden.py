@@ -17,7 +17,7 @@ reference's outputs.  Nothing from the reference is stored except numbers.
 
 Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
                                                --ibc-zones | --dielectric | --regions |
-                                               --condition | --poly]
+                                               --condition | --poly | --sheets]
 """
 from __future__ import annotations
 
@@ -655,8 +655,106 @@ def main_poly():
                  [0.0, 111.0, 250.0], execution=execution)
 
 
+def run_sheet_case(name, snapshot, frequency_ghz, angles):
+    """The reference's co-polarized solve of a layout with TYPE 1 sheets (sheets alone or beside PEC
+    contours): per polarization its own mesh (TE orients sheet chains and shares their nodes, TM
+    splits nodes by impedance flag and sheet name), system matrix, right-hand sides, nodal densities
+    and amplitudes from the _solve_fields hook, plus the public samples and the formulation."""
+    import ghost_backend.twod.solver as rcs
+
+    captured = []
+    original = rcs._solve_fields
+
+    def recording(mesh, matrix, k0, angs, rhs_builder, diagnostics, label, **kwargs):
+        entry = {"k0": float(k0), "label": str(label), "matrix": np.array(matrix, dtype=np.complex128, copy=True)}
+        entry["rhs"] = np.array(rhs_builder(np.asarray(angs, float)), dtype=np.complex128)
+        entry["nodes"] = np.array([n.xy for n in mesh.nodes], dtype=float)
+        entry["elements"] = np.array([e.node_ids for e in mesh.elements], dtype=np.int64)
+        kwargs = dict(kwargs)
+        kwargs["return_density"] = True
+        out = original(mesh, matrix, k0, angs, rhs_builder, diagnostics, label, **kwargs)
+        entry["density"] = np.array(out[3], dtype=np.complex128)
+        entry["amplitude"] = np.array(out[1], dtype=np.complex128)
+        captured.append(entry)
+        return out
+
+    rcs._solve_fields = recording
+    try:
+        result = rcs.solve_monostatic_rcs_2d(
+            snapshot, frequencies_ghz=[frequency_ghz], elevations_deg=list(map(float, angles)),
+            geometry_units="meters", strict_quality_gate=False, max_panels=100000, solver_method="direct",
+            execution_options=EXECUTION)
+    finally:
+        rcs._solve_fields = original
+    if len(captured) != 2:
+        raise RuntimeError(f"{name}: expected two captured systems, got {len(captured)}")
+    out = {"snapshot_json": np.array(json.dumps(snapshot)), "frequency_ghz": float(frequency_ghz),
+           "angles_deg": np.asarray(angles, float), "k0": captured[0]["k0"]}
+    for pol, entry in zip(("TE", "TM"), captured):
+        out[f"label_{pol}"] = np.array(entry["label"])
+        out[f"nodes_{pol}"] = entry["nodes"]
+        out[f"elements_{pol}"] = entry["elements"]
+        out[f"Z_{pol}"] = entry["matrix"]
+        out[f"rhs_{pol}"] = entry["rhs"]
+        out[f"density_{pol}"] = entry["density"]
+        out[f"amp_{pol}"] = entry["amplitude"]
+    for export, pol in (("VV", "TE"), ("HH", "TM")):
+        rows = result["co_solved_samples"][export]
+        out[f"sample_amp_{pol}"] = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+        out[f"sample_db_{pol}"] = np.array([r["rcs_db"] for r in rows])
+    out["formulation"] = np.array(str(result["metadata"].get("formulation")))
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+    print(f"{name}: {result['metadata'].get('formulation')}; nodes TE {len(out['nodes_TE'])} "
+          f"TM {len(out['nodes_TM'])}, panels {len(out['elements_TE'])}", flush=True)
+
+
+def _chain(name, vertices, count, seg_type=1, ibc=0, closed=False):
+    """One segment of ``count`` equal-share panels along a polyline (closed: drawn so that the stored
+    normal points into air), one panel per point pair, N = 1 per primitive."""
+    v = np.asarray(vertices, float)
+    if closed:
+        if workloads.signed_area(v) > 0:
+            v = v[::-1]
+        v = np.vstack((v, v[0]))
+    lengths = np.linalg.norm(np.diff(v, axis=0), axis=1)
+    counts = np.maximum(1, np.floor(count * lengths / lengths.sum()).astype(int))
+    while counts.sum() < count:
+        counts[np.argmax(lengths / counts)] += 1
+    pairs = []
+    for a, b, m in zip(v[:-1], v[1:], counts):
+        pts = np.linspace(a, b, int(m) + 1)
+        pairs += [{"x1": float(p[0]), "y1": float(p[1]), "x2": float(q[0]), "y2": float(q[1])}
+                  for p, q in zip(pts[:-1], pts[1:])]
+    return {"name": name, "seg_type": seg_type, "properties": [str(seg_type), "1", str(ibc), "0", "0"],
+            "point_pairs": pairs}
+
+
+def main_sheets():
+    """TYPE 1 sheets: the reference's bent-sheet fixture (tests/general_fixtures.py:48-53), the flat
+    strip of its transparency test (tests/test_2d_capability_acceptance.py:303-322), two named sheets
+    meeting at a point (one drawn toward the junction), and a sheet beside a PEC body."""
+    run_sheet_case("sheet_bent192", {
+        "segments": [_chain("bent sheet", [[-.065, -.015], [0, 0], [.05, .04]], 192, ibc=1)],
+        "ibcs": [["1", "constant", "75", "0", "0", "0"]], "dielectrics": []}, 3.0, [0.0, 30.0, 90.0, 150.0, 200.0])
+    run_sheet_case("sheet_strip48", {
+        "segments": [_chain("sheet", [(-0.1, 0.0), (0.1, 0.0)], 48, ibc=1)],
+        "ibcs": [["1", "constant", "75", "0", "0", "0"]], "dielectrics": []}, 0.8, [-35.0, 0.0, 40.0])
+    run_sheet_case("sheet_two_named", {
+        "segments": [_chain("left", [(-0.1, 0.0), (0.0, 0.0)], 30, ibc=1),
+                     _chain("right", [(0.08, 0.03), (0.0, 0.0)], 26, ibc=2)],
+        "ibcs": [["1", "constant", "100", "0", "0", "0"], ["2", "constant", "50", "-30", "0", "0"]],
+        "dielectrics": []}, 2.0, [0.0, 45.0, 100.0, 270.0])
+    run_sheet_case("sheet_beside_pec", {
+        "segments": [_chain("PEC", [[-.06, -.025], [-.06, .025], [.0, .025], [.0, -.025]], 64, seg_type=2,
+                            closed=True),
+                     _chain("card", [(0.02, -0.03), (0.05, 0.0), (0.09, 0.01)], 40, ibc=1)],
+        "ibcs": [["1", "constant", "30", "20", "0", "0"]], "dielectrics": []}, 2.5, [0.0, 60.0, 180.0, 300.0])
+
+
 if __name__ == "__main__":
-    if "--poly" in sys.argv:
+    if "--sheets" in sys.argv:
+        main_sheets()
+    elif "--poly" in sys.argv:
         main_poly()
     elif "--condition" in sys.argv:
         main_condition()

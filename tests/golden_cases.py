@@ -17,3 +17,6 @@ def load(name):
 def snapshot(case):
     from grim_20261007_amd.workloads import _snapshot
     return _snapshot(case["points"], "contour")
+
+# TYPE 1 sheets alone and beside a PEC body (make_golden.py --sheets)
+SHEETS = ["sheet_bent192", "sheet_strip48", "sheet_two_named", "sheet_beside_pec"]

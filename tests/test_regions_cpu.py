@@ -118,8 +118,10 @@ def test_region_refusals():
         snap = _coated(["4", "1", "1", "1", "0"], ["3", "1", "0", "1", "0"])
         snap["ibcs"] = [["1", "constant", "50", "0", "0", "0"]]
         build_mesh(snap, "meters", f)
-    with pytest.raises(ValueError, match="sheets"):
-        build_mesh(_coated(["1", "1", "1", "0", "0"], ["3", "1", "0", "1", "0"]), "meters", f)
+    with pytest.raises(ValueError, match="Mixed TYPE 1 sheet"):
+        snap = _coated(["1", "1", "1", "0", "0"], ["3", "1", "0", "1", "0"])
+        snap["ibcs"] = [["1", "constant", "50", "0", "0", "0"]]
+        build_mesh(snap, "meters", f)
     # one contour drawn as a TYPE 3 half and a TYPE 2 half: a junction between boundary kinds
     snap = workloads._snapshot(workloads.circle_points(0.1, 40), "j")
     seg = snap["segments"][0]
