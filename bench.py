@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--n-side", type=int, default=1000)
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sheet", action="store_true", help="skip the sheet (TYPE 1) request timing")
     ap.add_argument("--tuning", default="mi355x",
                     help="launch-shape table: 'mi355x' (shipped, measured by the autotuner), a JSON path, "
                          "'tune' (measure now) or 'none' (built-in rules)")
@@ -116,6 +117,62 @@ class Timer:
         self.torch.cuda.synchronize()
         v = [a.elapsed_time(b) for a, b in self.acc.get(name, [])]
         return float(np.mean(v)) if v else float("nan")
+
+
+def sheet_workload(n: int, nangles: int):
+    """A 75 ohm resistive strip bent into a 120-degree circular arc of radius 0.5 m (TYPE 1 open
+    contour), n panels of lambda/10."""
+    from grim_20261007_amd.mesh import C0, build_mesh
+    t = np.linspace(np.pi / 6, 5 * np.pi / 6, n + 1)
+    pts = 0.5 * np.stack((np.cos(t), np.sin(t)), axis=1)
+    chord = float(np.linalg.norm(pts[1] - pts[0]))
+    f = C0 / (10.0 * chord) / 1e9
+    seg = {"name": "arc", "seg_type": 1, "properties": ["1", "1", "1", "0", "0"],
+           "point_pairs": [{"x1": float(a[0]), "y1": float(a[1]), "x2": float(b[0]), "y2": float(b[1])}
+                           for a, b in zip(pts[:-1], pts[1:])]}
+    snap = {"segments": [seg], "ibcs": [["1", "constant", "75", "0", "0", "0"]], "dielectrics": []}
+    layout = build_mesh(snap, "meters", f)
+    return layout, f, np.arange(nangles) * (360.0 / nangles)
+
+
+def sheet_summary(n: int = 2000, nangles: int = 720, reps: int = 3):
+    """One sheet request (TE + TM, each polarization on its own mesh, both systems factored and solved
+    as one batch of two) timed stage by stage with HIP events."""
+    import torch
+    from grim_20261007_amd import engine
+    from grim_20261007_amd.solver import wavenumber
+    dev = torch.device("cuda", 0)
+    layout, f, angles = sheet_workload(n, nangles)
+    k = wavenumber(f)
+    angles_t = torch.tensor(angles, dtype=torch.float64, device=dev)
+    meshes = [layout.mesh(p) for p in ("TE", "TM")]
+    assert np.array_equal(meshes[0].elements, meshes[1].elements)
+    sheets = [engine.DeviceSheet(m, dev) for m in meshes]
+    ws = engine.sheet_workspace(meshes[0].ne, dev)
+    nn = meshes[0].n
+    timer = Timer(torch)
+    for rep in range(reps + 1):
+        tm = timer.run if rep else (lambda name, fn: fn())
+        z = torch.empty((2, nn, nn), dtype=torch.complex128, device=dev)
+        rhs, proj = [], []
+        # the arc's TE and TM meshes coincide: one traversal forms both systems
+        tm("assemble", lambda: engine.sheet_systems(sheets[0], k, ws, te=z[0], tm=z[1]))
+        for i, pol in enumerate(("TE", "TM")):
+            bu, bdn = tm("loads", lambda: engine.sheet_loads(sheets[i], k, angles_t))
+            rhs.append(engine.sheet_rhs(sheets[i], pol, bu, bdn))
+            proj.append(bu if pol == "TM" else bdn)
+        rhs = torch.stack(rhs)
+        fac = tm("factor_double", lambda: engine.factor(z, "double", check=False))
+        x, info = tm("solve_double", lambda: engine.solve(fac, rhs))
+        for i in range(2):
+            tm("farfield", lambda: engine.farfield_potential(x[i], proj[i], k))
+    stage = {name: round(timer.mean_ms(name) * (2 if name in ("loads", "farfield") else 1), 3) for name in timer.acc}
+    total = sum(stage.values())
+    return {"workload": f"75 ohm resistive strip, 120-degree arc of radius 0.5 m, {meshes[0].ne} panels "
+                        f"({nn} nodes, strip ends pinned in TE), f={f:.4f} GHz (panels lambda/10), {len(angles)} "
+                        "monostatic angles, TE + TM", "request_ms": round(total, 3),
+            "solves_per_s": round(2e3 / total, 3), "stage_ms": stage,
+            "source": "HIP events; stage loads / farfield are both polarizations"}
 
 
 def request(engine, contour, k, angles_t, precision, timer=None, keep=False):
@@ -639,6 +696,7 @@ def main():
         del zb
         torch.cuda.empty_cache()
     stream = stream_summary(args.workers, batch=args.batch) if args.stream_count > 0 else None
+    sheet = sheet_summary() if not args.no_sheet else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mesh, f, angles, args.cpu_seconds)
@@ -699,6 +757,7 @@ def main():
         "lu_n2000": lu,
         "cpu_baseline": cpu,
         "stream": stream,
+        "sheet_n2000": sheet,
         "launch_shapes": shapes,
     }
     print(json.dumps(line))

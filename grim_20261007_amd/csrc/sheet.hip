@@ -24,8 +24,10 @@
 #include "common.h"
 
 // ---------------------------------------------------------------------------
-// Far pairs: one thread per ordered element pair (observation e = lane of the block row, source
-// f = blockIdx.y); pairs that are not far are left to the near pass.
+// Far pairs: one thread per unordered element pair e < f (e = lane of the block row, f =
+// blockIdx.y) writes block (e, f) and its transpose, block (f, e).  Pairs that are not far are
+// left to the near pass, which stores both orders (the test is symmetric, so the two passes agree
+// on every pair).
 // ---------------------------------------------------------------------------
 template <int Q>
 __global__ void __launch_bounds__(64) sheet_far(int ne, const double* __restrict__ p0, const double* __restrict__ p1,
@@ -38,7 +40,7 @@ __global__ void __launch_bounds__(64) sheet_far(int ne, const double* __restrict
     }
     __syncthreads();
     const int e = blockIdx.x * 64 + lane, f = blockIdx.y;
-    if (e >= ne || e == f) return;
+    if (e >= ne || e >= f) return;
     const Elem o = load_elem(p0, p1, e), s = load_elem(p0, p1, f);
     if (!is_far(o, e, nid[e], s, f, nid[f])) return;   // the near pass lists exactly the others
     cplx S[2][2];
@@ -74,12 +76,15 @@ __global__ void __launch_bounds__(64) sheet_far(int ne, const double* __restrict
     }
     const double scale = o.len * s.len;
     double* out = bs + 8 * ((int64_t)f * ne + e);
+    double* mirror = bs + 8 * ((int64_t)e * ne + f);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             out[2 * (2 * a + b)] = S[a][b].re * scale;
             out[2 * (2 * a + b) + 1] = S[a][b].im * scale;
+            mirror[2 * (2 * b + a)] = S[a][b].re * scale;
+            mirror[2 * (2 * b + a) + 1] = S[a][b].im * scale;
         }
 }
 
@@ -122,6 +127,7 @@ __global__ void __launch_bounds__(256) sheet_gather(int nn, int ne, const double
             for (int c = own_ptr[q]; c < own_ptr[q + 1]; ++c) {
                 const int f = own_el[c], b = own_loc[c];
                 const double* B = bs + 8 * ((int64_t)f * ne + e);
+                const int ab = 2 * a + b;
                 double vr, vi;
                 if (HYPER) {
                     const Elem s = load_elem(p0, p1, f);
@@ -130,11 +136,11 @@ __global__ void __launch_bounds__(256) sheet_gather(int nn, int ne, const double
                     const double sr = (B[0] + B[2]) + (B[4] + B[6]), si = (B[1] + B[3]) + (B[5] + B[7]);
                     const double tsign = a == b ? 1.0 : -1.0;
                     const double nn_ = kk * (o.nx * s.nx + o.ny * s.ny);
-                    vr = nn_ * B[2 * (2 * a + b)] + tsign * (sr / denom);
-                    vi = nn_ * B[2 * (2 * a + b) + 1] + tsign * (si / denom);
+                    vr = nn_ * B[2 * ab] + tsign * (sr / denom);
+                    vi = nn_ * B[2 * ab + 1] + tsign * (si / denom);
                 } else {
-                    vr = B[2 * (2 * a + b)];
-                    vi = B[2 * (2 * a + b) + 1];
+                    vr = B[2 * ab];
+                    vi = B[2 * ab + 1];
                 }
                 if (e == f) {
                     const double m = o.len * (a == b ? 1.0 / 3.0 : 1.0 / 6.0);

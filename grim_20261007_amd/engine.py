@@ -28,8 +28,9 @@ def request_device_bytes(n: int, nangles: int, npols: int, precision: str, cache
     factors (unless cached), the loads, right-hand sides, solutions and residuals, refinement
     and assembly workspace.  n counts unknowns; with degree-2 / degree-3 elements (n / degree
     elements) the assembly workspace holds three tables of element-pair blocks,
-    48 (degree + 1)^2 (n / degree)^2 bytes.  A sheet mesh (sheet_elements > 0 panels) solves one
-    polarization at a time over a table of 64 ne^2 bytes of element-pair S blocks."""
+    48 (degree + 1)^2 (n / degree)^2 bytes.  A sheet mesh (sheet_elements > 0 panels) assembles its
+    npols systems one after the other over one table of 64 ne^2 bytes of element-pair S blocks, each
+    with its own loads."""
     z = 16 * n * n
     lu = (8 if precision == "mixed" else 16) * n * n
     vec = 16 * n * nangles
@@ -37,7 +38,7 @@ def request_device_bytes(n: int, nangles: int, npols: int, precision: str, cache
         ne = sheet_elements
         work = 64 * ne * ne + NEAR_PER_ELEMENT * ne * 160 + 64 * ne
         refine = (24 * n * nangles) if precision == "mixed" else 0
-        return z + lu + 2 * vec + vec * 3 + refine + work
+        return npols * (z + lu + 2 * vec + vec * 3 + refine) + work
     if degree > 1:
         ne = n // degree
         work = 0 if cached else 48 * (degree + 1) ** 2 * ne * ne + NEAR_PER_ELEMENT * ne * 16 + 64 * ne
@@ -263,21 +264,30 @@ def sheet_system(sheet: DeviceSheet, k: float, pol: str, workspace: torch.Tensor
                  out: torch.Tensor | None = None) -> torch.Tensor:
     """The (N, N) complex128 system (column-major) of one polarization: S - c M (TM) or W - c M with
     the strip-end rows pinned (TE)."""
+    n = sheet.n
+    z = torch.empty((n, n), dtype=torch.complex128, device=sheet.device) if out is None else out
+    sheet_systems(sheet, k, workspace, **{pol.lower(): z})
+    return z
+
+
+def sheet_systems(sheet: DeviceSheet, k: float, workspace: torch.Tensor | None = None,
+                  te: torch.Tensor | None = None, tm: torch.Tensor | None = None) -> None:
+    """Both polarizations' systems from one traversal when their meshes coincide (a sheet whose TE
+    chains need no flip and whose TM nodes are not split): one near pass and one table of S blocks
+    feed both gathers.  Either output may be None; the pinned rows apply to TE."""
     n, ne, dev = sheet.n, sheet.ne, sheet.device
-    z = torch.empty((n, n), dtype=torch.complex128, device=dev) if out is None else out
     ws = workspace if workspace is not None else sheet_workspace(ne, dev)
-    coef = torch.as_tensor(sheet_coefficients(sheet.mesh, k, pol), device=dev)
-    te = pol == "TE"
+    c_te = torch.as_tensor(sheet_coefficients(sheet.mesh, k, "TE"), device=dev) if te is not None else None
+    c_tm = torch.as_tensor(sheet_coefficients(sheet.mesh, k, "TM"), device=dev) if tm is not None else None
     st = _stream(dev)
     code = native.lib().grim_assemble_sheet(
         ne, n, sheet.p0.data_ptr(), sheet.p1.data_ptr(), sheet.next_ids.data_ptr(), sheet.own_ptr.data_ptr(),
         sheet.own_el.data_ptr(), sheet.own_loc.data_ptr(), float(k), sheet.mesh.far_order(k, False),
-        sheet.mesh.far_order(k, True), None if te else coef.data_ptr(), coef.data_ptr() if te else None,
-        sheet.pinned.data_ptr() if te else None, None if te else z.data_ptr(), n, z.data_ptr() if te else None, n,
+        sheet.mesh.far_order(k, True), native.ptr(c_tm), native.ptr(c_te),
+        sheet.pinned.data_ptr() if te is not None else None, native.ptr(tm), n, native.ptr(te), n,
         ws.data_ptr(), ws.numel(), NEAR_PER_ELEMENT, st)
     native.check("grim_assemble_sheet", code)
     native.check("grim_assemble_sheet (device)", native.lib().grim_workspace_status(ws.data_ptr(), st))
-    return z
 
 
 def sheet_loads(sheet: DeviceSheet, k: float, angles_deg: torch.Tensor):

@@ -356,38 +356,65 @@ SHEET_FORMULATIONS = {
 }   # twod/solver.py:2409-2439
 
 
-def _sheet_solve(layout: SheetLayout, pol: str, k: float, angles: torch.Tensor, precision: str, dev,
+def _sheet_solve(layout: SheetLayout, pols, k: float, angles: torch.Tensor, precision: str, dev,
                  compute_condition_number: bool, abort_event, context: str, extra_angles: int = 0):
-    """One polarization of a sheet layout on its own mesh: the system S - c M (TM) or W - c M with
-    pinned strip ends (TE), loads -b_u / b_dn, factor and solve (twod/solver.py:1746-1787)."""
-    mesh = layout.mesh(pol)
-    admit_device_memory(mesh.n, int(angles.numel()) + extra_angles, 1, precision, dev, context,
-                        compute_condition_number, sheet_elements=mesh.ne)
-    sheet = engine.DeviceSheet(mesh, dev)
-    z = engine.sheet_system(sheet, k, pol)
-    bu, bdn = engine.sheet_loads(sheet, k, angles)
-    rhs = engine.sheet_rhs(sheet, pol, bu, bdn).unsqueeze(0)
-    if abort_event is not None and abort_event.is_set():
-        raise InterruptedError("Solve canceled by user.")
-    fac = engine.factor(z.unsqueeze(0), precision)
-    del z
-    conds = methods = None
-    if compute_condition_number:
-        from . import condition
-        conds, methods, replacement = condition.condition_estimates(fac)
-        if replacement is not None:
-            fac = replacement
-    x, info = engine.solve(fac, rhs)
-    rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
-    final = info.get("factor", fac)
-    part = engine.system(final, 0) if final.parts is not None else final
-    res = {"residual": rel[0].cpu().numpy(), "info": {kk: v for kk, v in info.items() if kk not in ("residual", "factor")},
-           "k": k, "n": mesh.n, "precision_used": part.precision, "fallback_reason": part.fallback_reason,
-           "formulation": SHEET_FORMULATIONS[(layout.all_sheet, pol)]}
-    if conds is not None:
-        res["condition"] = float(conds[0])
-        res["condition_method"] = methods[0]
-    return sheet, x[0], bu, bdn, res
+    """The requested polarizations of a sheet layout, each on its own mesh: the system S - c M (TM) or
+    W - c M with pinned strip ends (TE), loads -b_u / b_dn (twod/solver.py:1746-1787).  When both
+    meshes have the same node count the two systems are factored and solved as one batch of two.
+    Yields (pol, sheet, density (A, N), b_u, b_dn, result fields)."""
+    order = [p for p in ("TE", "TM") if p in pols]
+    meshes = {p: layout.mesh(p) for p in order}
+    if len(order) == 2 and meshes["TE"].n == meshes["TM"].n:
+        groups = [tuple(order)]
+    else:
+        groups = [(p,) for p in order]
+    for group in groups:
+        n = meshes[group[0]].n
+        ne = max(meshes[p].ne for p in group)
+        admit_device_memory(n, int(angles.numel()) + extra_angles, len(group), precision, dev, context,
+                            compute_condition_number, sheet_elements=ne)
+        z = torch.empty((len(group), n, n), dtype=torch.complex128, device=dev)
+        ws = engine.sheet_workspace(ne, dev)
+        sheets, loads, rhs = [], [], []
+        te, tm = meshes.get("TE"), meshes.get("TM")
+        shared = (len(group) == 2 and np.array_equal(te.p0, tm.p0) and np.array_equal(te.p1, tm.p1)
+                  and np.array_equal(te.elements, tm.elements))
+        if shared:                              # one traversal for both (pinned rows: TE only)
+            sheet = engine.DeviceSheet(te, dev)
+            engine.sheet_systems(sheet, k, ws, te=z[0], tm=z[1])
+        for i, pol in enumerate(group):
+            sheet = engine.DeviceSheet(meshes[pol], dev)
+            if not shared:
+                engine.sheet_system(sheet, k, pol, ws, out=z[i])
+            bu, bdn = engine.sheet_loads(sheet, k, angles)
+            sheets.append(sheet)
+            loads.append((bu, bdn))
+            rhs.append(engine.sheet_rhs(sheet, pol, bu, bdn))
+        del ws
+        rhs = torch.stack(rhs)
+        if abort_event is not None and abort_event.is_set():
+            raise InterruptedError("Solve canceled by user.")
+        fac = engine.factor(z, precision)
+        del z
+        conds = methods = None
+        if compute_condition_number:
+            from . import condition
+            conds, methods, replacement = condition.condition_estimates(fac)
+            if replacement is not None:
+                fac = replacement
+        x, info = engine.solve(fac, rhs)
+        rel = engine.relative_residual(fac.z, x, rhs, info.get("residual"))
+        final = info.get("factor", fac)
+        for i, pol in enumerate(group):
+            part = engine.system(final, i) if final.parts is not None else final
+            res = {"residual": rel[i].cpu().numpy(),
+                   "info": {kk: v for kk, v in info.items() if kk not in ("residual", "factor")},
+                   "k": k, "n": n, "precision_used": part.precision, "fallback_reason": part.fallback_reason,
+                   "formulation": SHEET_FORMULATIONS[(layout.all_sheet, pol)]}
+            if conds is not None:
+                res["condition"] = float(conds[i])
+                res["condition_method"] = methods[i]
+            yield pol, sheets[i], x[i], loads[i][0], loads[i][1], res
 
 
 def _solve_sheet_frequency(layout: SheetLayout, frequency_ghz: float, angles_deg: List[float], pols,
@@ -400,9 +427,8 @@ def _solve_sheet_frequency(layout: SheetLayout, frequency_ghz: float, angles_deg
     k = wavenumber(frequency_ghz)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
     out = {}
-    for pol in [p for p in ("TE", "TM") if p in pols]:
-        _sheet, x, bu, bdn, res = _sheet_solve(layout, pol, k, angles, precision, dev, compute_condition_number,
-                                               abort_event, context)
+    for pol, _sheet, x, bu, bdn, res in _sheet_solve(layout, pols, k, angles, precision, dev,
+                                                     compute_condition_number, abort_event, context):
         amp, sigma = engine.farfield_potential(x, bu if pol == "TM" else bdn, k)
         res.update({"amplitude": amp.cpu().numpy(), "sigma": sigma.cpu().numpy()})
         if return_density:
@@ -421,9 +447,8 @@ def _solve_sheet_bistatic_frequency(layout: SheetLayout, frequency_ghz: float, i
     inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)
     obs = torch.tensor(observation_deg, dtype=torch.float64, device=dev)
     out = {}
-    for pol in [p for p in ("TE", "TM") if p in pols]:
-        sheet, x, _bu, _bdn, res = _sheet_solve(layout, pol, k, inc, precision, dev, compute_condition_number,
-                                                abort_event, "The bistatic 2-D solve", len(observation_deg))
+    for pol, sheet, x, _bu, _bdn, res in _sheet_solve(layout, pols, k, inc, precision, dev, compute_condition_number,
+                                                      abort_event, "The bistatic 2-D solve", len(observation_deg)):
         bu_o, bdn_o = engine.sheet_loads(sheet, k, obs)
         amp, sigma = engine.bistatic_farfield(x, bu_o if pol == "TM" else bdn_o, None, k)
         res.update({"amplitude": amp[0].cpu().numpy(), "sigma": sigma[0].cpu().numpy()})

@@ -749,6 +749,15 @@ def main_sheets():
                             closed=True),
                      _chain("card", [(0.02, -0.03), (0.05, 0.0), (0.09, 0.01)], 40, ibc=1)],
         "ibcs": [["1", "constant", "30", "20", "0", "0"]], "dielectrics": []}, 2.5, [0.0, 60.0, 180.0, 300.0])
+    # one primitive, automatic panel count (air wavelength), a linear taper along the drawing direction
+    taper = _chain("taper", [(-0.15, 0.02), (0.15, -0.03)], 1, ibc=1)
+    taper["properties"][1] = "0"
+    run_sheet_case("sheet_taper_auto", {"segments": [taper], "ibcs": [["1", "linear", "20", "0", "600", "-100"]],
+                                        "dielectrics": []}, 2.0, [0.0, 20.0, 90.0, 210.0])
+    # a closed resistive ring: no strip end, one TE chain with no free end
+    run_sheet_case("sheet_ring", {
+        "segments": [_chain("ring", workloads.circle_points(0.05, 40), 40, ibc=1, closed=True)],
+        "ibcs": [["1", "constant", "120", "-40", "0", "0"]], "dielectrics": []}, 1.5, [0.0, 90.0, 225.0])
 
 
 if __name__ == "__main__":

@@ -19,4 +19,4 @@ def snapshot(case):
     return _snapshot(case["points"], "contour")
 
 # TYPE 1 sheets alone and beside a PEC body (make_golden.py --sheets)
-SHEETS = ["sheet_bent192", "sheet_strip48", "sheet_two_named", "sheet_beside_pec"]
+SHEETS = ["sheet_bent192", "sheet_strip48", "sheet_two_named", "sheet_beside_pec", "sheet_taper_auto", "sheet_ring"]
