@@ -16,7 +16,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libgrim2d.so"
-SOURCES = ("assembly.hip", "poly.hip", "sheet.hip", "dense.hip", "api.hip")
+SOURCES = ("assembly.hip", "poly.hip", "sheet.hip", "lossy.hip", "dense.hip", "api.hip")
 ARCH = os.environ.get("GRIM_OFFLOAD_ARCH", "gfx950")
 
 

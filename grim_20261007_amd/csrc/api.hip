@@ -68,6 +68,13 @@ extern "C" hipError_t grim_launch_sheet_loads(int nn, int na, const double* p0, 
                                               const int* own_el, const int* own_loc, double k,
                                               const double* angles_deg, double* bu, double* bdn, int64_t ld,
                                               hipStream_t st);
+extern "C" hipError_t grim_launch_lossy(int q, int n, const double* p0, const double* p1, const int* next,
+                                        const int* prev, double kr, double ki, const int* offsets, int capacity,
+                                        const int* pair_e, const int* pair_f, double* blocks, int* kind,
+                                        unsigned* status, double* ts, double* tk, double mass_k, double mass_d,
+                                        const double* coef_k, const double* coef_d, double* s_out, double* k_out,
+                                        double* d_out, double* w_out, int64_t ld, hipStream_t st);
+extern "C" hipError_t grim_launch_hankel2_probe(int m, const double* z, double* out, hipStream_t st);
 extern "C" int64_t grim_dense_aux_bytes(int n, int single);
 extern "C" hipError_t grim_dense_lu_factor(int batch, int n, void* a, int64_t lda, int64_t sa, int single, int* piv,
                                            int64_t spiv, void* aux, int64_t saux, unsigned* status, hipStream_t st);
@@ -581,6 +588,62 @@ extern "C" int grim_loads_sheet(int64_t ne, int64_t nn, int64_t na, const double
     if (na == 0) return GRIM_OK;
     CHECK(grim_launch_sheet_loads((int)nn, (int)na, p0, p1, own_ptr, own_el, own_loc, k, angles_deg, bu, bdn, ld,
                                   (hipStream_t)stream));
+    return GRIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lossy media: operators at a complex wavenumber (lossy.hip)
+// ---------------------------------------------------------------------------
+struct LossyWork {
+    AssemblyWork near;   // status word, near-pair lists and near blocks
+    double *ts, *tk;     // element-pair S and K' tables
+    int64_t bytes;
+};
+
+static LossyWork carve_lossy(void* base, int64_t n, int64_t cap) {
+    LossyWork w;
+    w.near = carve(base, n, cap);
+    const int64_t table = align_up(64 * n * n);
+    w.ts = (double*)((char*)base + w.near.bytes);
+    w.tk = (double*)((char*)base + w.near.bytes + table);
+    w.bytes = w.near.bytes + 2 * table;
+    return w;
+}
+
+extern "C" int64_t grim_assemble_operators_lossy_workspace_bytes(int64_t n, int64_t near_per_element) {
+    if (n < 2 || n > 65535 || near_per_element <= 0) return -1;
+    return carve_lossy(nullptr, n, n * near_per_element).bytes;
+}
+
+extern "C" int grim_assemble_operators_lossy(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                             const int32_t* prev, double k_re, double k_im, int far_order,
+                                             double* s, double* kp, double* d, double* w, int64_t ld,
+                                             double mass_k, double mass_d, const double* coef_k,
+                                             const double* coef_d, void* workspace, int64_t workspace_bytes,
+                                             int64_t near_per_element, void* stream) {
+    if (n < 2 || n > 65535 || !p0 || !p1 || !next || !prev || !workspace || (!s && !kp && !d && !w) || ld < n ||
+        !(k_re > 0) || !isfinite(k_re) || !isfinite(k_im) || k_im > 0 || near_per_element <= 0 ||
+        !isfinite(mass_k) || !isfinite(mass_d) || (far_order != 8 && far_order != 16))
+        return GRIM_ERR_ARGS;
+    const int64_t cap = n * near_per_element;
+    if (cap > (int64_t)1 << 31) return GRIM_ERR_RANGE;
+    const LossyWork wk = carve_lossy(workspace, n, cap);
+    if (wk.bytes > workspace_bytes) return GRIM_ERR_RANGE;
+    hipStream_t st = (hipStream_t)stream;
+    zero_words<<<1, 64, 0, st>>>(wk.near.status, 1);
+    CHECK(hipGetLastError());
+    CHECK(grim_launch_near_list((int)n, p0, p1, next, wk.near.counts, wk.near.offsets, (int)cap, wk.near.pair_e,
+                                wk.near.pair_f, wk.near.status, st));
+    CHECK(grim_launch_lossy(far_order, (int)n, p0, p1, next, prev, k_re, k_im, wk.near.offsets, (int)cap,
+                            wk.near.pair_e, wk.near.pair_f, wk.near.blocks, wk.near.kind, wk.near.status, wk.ts,
+                            wk.tk, mass_k, mass_d, coef_k, coef_d, s, kp, d, w, ld, st));
+    return GRIM_OK;
+}
+
+extern "C" int grim_hankel2_probe(int64_t m, const double* z, double* out, void* stream) {
+    if (m < 0 || m > (int64_t)1 << 30 || (m > 0 && (!z || !out))) return GRIM_ERR_ARGS;
+    if (m == 0) return GRIM_OK;
+    CHECK(grim_launch_hankel2_probe((int)m, z, out, (hipStream_t)stream));
     return GRIM_OK;
 }
 

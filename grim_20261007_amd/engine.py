@@ -440,18 +440,68 @@ def impedance_rhs(contour: DeviceContour, k: float, angles: torch.Tensor) -> tor
     return out
 
 
-def dielectric_wavenumber(mesh: ContourMesh, k0: float) -> float:
-    """Interior wavenumber k0 n of a homogeneous dielectric body.  Only lossless positive-index media
-    are evaluated here: their kernels are the real-argument Hankel functions of the rest of the
-    assembly; a lossy or negative-index medium needs complex-argument Hankel kernels."""
-    flag, eps, mu = mesh.medium
-    index = causal_index(eps, mu)
+def medium_wavenumber(index: complex, k0: float, what: str):
+    """k0 n for a medium of causal index n: a float for a lossless medium (the real-argument
+    kernels), a complex number k0 n with Im <= 0 for a lossy one (grim_assemble_operators_lossy).
+    Negative-index media (Re n <= 0) are refused."""
     tol = 64.0 * np.finfo(float).eps * max(1.0, abs(index))
-    if abs(index.imag) > tol or not index.real > 0.0:
-        raise ValueError(f"Dielectric flag {flag} (eps={eps!r}, mu={mu!r}) has refractive index {index!r}: lossy and "
-                         "negative-index media need complex-argument Hankel kernels, which the GPU path does not "
-                         "evaluate. Use a lossless medium with a positive index or the reference's CPU path.")
-    return float(k0 * index.real)
+    if not index.real > 0.0:
+        raise ValueError(f"{what} has refractive index {index!r}: negative-index media are not evaluated by the GPU "
+                         "path. Use a medium with a positive index or the reference's CPU path.")
+    if abs(index.imag) <= tol:
+        return float(k0 * index.real)
+    return complex(k0 * index)
+
+
+def is_lossy(k) -> bool:
+    return isinstance(k, complex)
+
+
+def lossy_workspace_bytes(mesh) -> int:
+    """Device bytes of the complex-wavenumber operator pass a mesh needs beyond the real assembly:
+    its two tables of element-pair blocks (128 n^2 bytes) and near lists for the largest lossy contour
+    set (a lossy dielectric body, or the contours of a lossy region); 0 without lossy media."""
+    if getattr(mesh, "medium", None) is not None:
+        n = mesh.n if abs(causal_index(mesh.medium[1], mesh.medium[2]).imag) > 0 else 0
+    elif getattr(mesh, "regions", None) is not None:
+        n = 0
+        for rid, (eps, mu) in mesh.region_media.items():
+            if rid > 0 and abs(causal_index(eps, mu).imag) > 0:
+                n = max(n, int(np.count_nonzero(np.any(np.asarray(mesh.regions) == rid, axis=1))))
+    else:
+        n = 0
+    return 0 if n == 0 else 128 * n * n + NEAR_PER_ELEMENT * n * 160 + 64 * n
+
+
+def lossy_workspace(n: int, device) -> torch.Tensor:
+    nbytes = int(native.lib().grim_assemble_operators_lossy_workspace_bytes(n, NEAR_PER_ELEMENT))
+    if nbytes < 0:
+        raise ValueError("Lossy media are evaluated for contours of at most 65535 panels.")
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def assemble_lossy(contour: DeviceContour, k: complex, s=None, kp=None, d=None, w=None, ld: int | None = None,
+                   mass_k: float = 0.0, mass_d: float = 0.0, coef_k=None, coef_d=None,
+                   workspace: torch.Tensor | None = None) -> None:
+    """S, K' + c_k M, D + c_d M and W of a contour set at complex k (pointers into column-major
+    outputs with leading dimension ld; any may be None) from one traversal."""
+    n, dev = contour.n, contour.device
+    ws = workspace if workspace is not None else lossy_workspace(n, dev)
+    st = _stream(dev)
+    lib = native.lib()
+    code = lib.grim_assemble_operators_lossy(
+        n, contour.p0.data_ptr(), contour.p1.data_ptr(), contour.next.data_ptr(), contour.prev.data_ptr(),
+        float(k.real), float(k.imag), contour.mesh.far_order(k), s, kp, d, w, n if ld is None else ld,
+        float(mass_k), float(mass_d), native.ptr(coef_k), native.ptr(coef_d), ws.data_ptr(), ws.numel(),
+        NEAR_PER_ELEMENT, st)
+    native.check("grim_assemble_operators_lossy", code)
+    native.check("grim_assemble_operators_lossy (device)", lib.grim_workspace_status(ws.data_ptr(), st))
+
+
+def dielectric_wavenumber(mesh: ContourMesh, k0: float):
+    """Interior wavenumber k0 n of a homogeneous dielectric body (complex for a lossy medium)."""
+    flag, eps, mu = mesh.medium
+    return medium_wavenumber(causal_index(eps, mu), k0, f"Dielectric flag {flag} (eps={eps!r}, mu={mu!r})")
 
 
 def dielectric_systems(contour: DeviceContour, k0: float, workspace: torch.Tensor) -> torch.Tensor:
@@ -481,10 +531,13 @@ def dielectric_systems(contour: DeviceContour, k0: float, workspace: torch.Tenso
                                                      block(n, 0), ld, workspace.data_ptr(), workspace.numel(),
                                                      NEAR_PER_ELEMENT, st))
     native.check("grim_assemble (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
-    native.check("grim_assemble_operators", lib.grim_assemble_operators(
-        *args, float(k1), contour.mesh.far_order(k1), 1, block(0, n), ld, block(n, n), ld, 0.5, None,
-        workspace.data_ptr(), workspace.numel(), NEAR_PER_ELEMENT, st))
-    native.check("grim_assemble_operators (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
+    if is_lossy(k1):
+        assemble_lossy(contour, k1, s=block(0, n), kp=block(n, n), ld=ld, mass_k=0.5)
+    else:
+        native.check("grim_assemble_operators", lib.grim_assemble_operators(
+            *args, float(k1), contour.mesh.far_order(k1), 1, block(0, n), ld, block(n, n), ld, 0.5, None,
+            workspace.data_ptr(), workspace.numel(), NEAR_PER_ELEMENT, st))
+        native.check("grim_assemble_operators (device)", lib.grim_workspace_status(workspace.data_ptr(), st))
     pair[1].copy_(pair[0])
     eta = 1j * float(k0)
     one = 1.0 + 0.0j
@@ -555,13 +608,7 @@ def region_plan(mesh: ContourMesh) -> RegionPlan:
 
 def _region_wavenumber(media, rid, k0):
     eps, mu = media[rid]
-    index = causal_index(eps, mu)
-    tol = 64.0 * np.finfo(float).eps * max(1.0, abs(index))
-    if abs(index.imag) > tol or not index.real > 0.0:
-        raise ValueError(f"Region {rid} (eps={eps!r}, mu={mu!r}) has refractive index {index!r}: lossy and "
-                         "negative-index media need complex-argument Hankel kernels, which the GPU path does not "
-                         "evaluate. Use lossless media with positive indices or the reference's CPU path.")
-    return float(k0 * index.real)
+    return medium_wavenumber(causal_index(eps, mu), k0, f"Region {rid} (eps={eps!r}, mu={mu!r})")
 
 
 @dataclass
@@ -719,6 +766,10 @@ def region_systems(contour: DeviceContour, k0: float, workspace: torch.Tensor, p
         coef = region_mass_coefficients(plan, rid)
         c_k, c_d = torch.as_tensor(coef, device=dev), torch.as_tensor(-coef, device=dev)
         mats = torch.empty((4, nr, nr), dtype=torch.complex128, device=dev)     # S, K', D, W
+        if is_lossy(k):         # on the current stream, as the routes
+            assemble_lossy(rc, k, *(m.data_ptr() for m in mats), ld=nr, coef_k=c_k, coef_d=c_d)
+            ops[rid] = mats
+            continue
         args = (nr, rc.p0.data_ptr(), rc.p1.data_ptr(), rc.next.data_ptr(), rc.prev.data_ptr(),
                 rc.fix_rows.data_ptr(), int(rc.fix_rows.numel()), float(k), rmesh.far_order(k))
         for which, a, b, c in ((1, mats[0], mats[1], c_k), (2, mats[2], mats[3], c_d)):

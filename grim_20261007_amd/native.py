@@ -34,6 +34,10 @@ SIGNATURES = {
     "grim_loads": (_i32, [_i64, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _i64, _vp]),
     "grim_farfield": (_i32, [_i64, _i64, _vp, _i64, _vp, _vp, _i64, _dbl, _vp, _vp, _vp]),
     "grim_farfield_potential": (_i32, [_i64, _i64, _vp, _i64, _vp, _i64, _dbl, _vp, _vp, _vp]),
+    "grim_assemble_operators_lossy_workspace_bytes": (_i64, [_i64, _i64]),
+    "grim_assemble_operators_lossy": (_i32, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp, _i64,
+                                             _dbl, _dbl, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "grim_hankel2_probe": (_i32, [_i64, _vp, _vp, _vp]),
     "grim_sheet_workspace_bytes": (_i64, [_i64, _i64]),
     "grim_assemble_sheet": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _i32, _i32, _vp, _vp, _vp,
                                    _vp, _i64, _vp, _i64, _vp, _i64, _i64, _vp]),

@@ -365,6 +365,8 @@ class Server:
 
         nbytes = request_bytes(system_dofs(mesh), len(req.angles_deg), len(req.pols), req.precision,
                                cached is not None, mesh.degree)
+        if cached is None:
+            nbytes += engine.lossy_workspace_bytes(mesh)
         try:
             self._admit(nbytes)
             try:

@@ -177,12 +177,13 @@ def _summarize_residuals(values):
 
 
 def admit_device_memory(n: int, nangles: int, npols: int, precision: str, device, context: str,
-                        condition: bool = False, degree: int = 1, sheet_elements: int = 0) -> float:
+                        condition: bool = False, degree: int = 1, sheet_elements: int = 0, extra: int = 0) -> float:
     """Refuse a solve whose device footprint exceeds 0.9 of the GPU memory it can use now (free
     memory plus what PyTorch's allocator holds unused), before anything is assembled, as the
     reference refuses host memory (twod/solver.py:1419-1554, 2342-2400).  Returns the estimate
     in GiB."""
-    need = engine.request_device_bytes(n, nangles, npols, precision, degree=degree, sheet_elements=sheet_elements)
+    need = engine.request_device_bytes(n, nangles, npols, precision, degree=degree,
+                                       sheet_elements=sheet_elements) + int(extra)
     if condition:   # equilibration scales, two probe columns and a 256 x 256 staging block per system
         need += 2 * (3 * 8 * n + 16 * 2 * 2 * n + 16 * 256 * 256)
     free, _total = torch.cuda.mem_get_info(device)
@@ -470,7 +471,7 @@ def solve_frequency(mesh, frequency_ghz: float, angles_deg: List[float], pols=("
     k = wavenumber(frequency_ghz)
     order = [p for p in ("TE", "TM") if p in pols]
     admit_device_memory(system_dofs(mesh), len(angles_deg), len(order), precision, dev, context,
-                        compute_condition_number, mesh.degree)
+                        compute_condition_number, mesh.degree, extra=engine.lossy_workspace_bytes(mesh))
     contour = engine.DeviceContour(mesh, dev)
     ws = engine.assembly_workspace(mesh.n, dev, mesh.degree)
     angles = torch.tensor(angles_deg, dtype=torch.float64, device=dev)
@@ -747,7 +748,7 @@ def solve_bistatic_frequency(mesh, frequency_ghz: float, incidence_deg: List[flo
     k = wavenumber(frequency_ghz)
     order = [p for p in ("TE", "TM") if p in pols]
     admit_device_memory(system_dofs(mesh), len(incidence_deg) + len(observation_deg), len(order), precision, dev,
-                        "The bistatic 2-D solve", compute_condition_number)
+                        "The bistatic 2-D solve", compute_condition_number, extra=engine.lossy_workspace_bytes(mesh))
     contour = engine.DeviceContour(mesh, dev)
     ws = engine.assembly_workspace(mesh.n, dev)
     inc = torch.tensor(incidence_deg, dtype=torch.float64, device=dev)

@@ -200,6 +200,25 @@ int grim_loads_sheet(int64_t ne, int64_t nn, int64_t na, const double* p0, const
                      const int32_t* own_el, const int32_t* own_loc, double k, const double* angles_deg, double* bu,
                      double* bdn, int64_t ld, void* stream);
 
+/* Operators of closed contours at a complex wavenumber k = k_re + j k_im (k_re > 0, k_im <= 0: a
+ * lossy medium under e^{+j omega t}), in the layout and with the mass terms of
+ * grim_assemble_operators, all four from one traversal: s = S, kp = K' + c_k M, d = D + c_d M,
+ * w = the Maue W (any may be NULL), c_k / c_d from coef_k / coef_d per element (float64 [n]) or the
+ * scalars mass_k / mass_d.  Replaces twod/operators.py:2087-2607 at complex k with the complex
+ * Hankel functions of twod/special.py:220-264 (SciPy's hankel2 in the reference).  n <= 65535.
+ * Status as grim_assemble (grim_workspace_status on the workspace afterwards). */
+int64_t grim_assemble_operators_lossy_workspace_bytes(int64_t n, int64_t near_per_element);
+int grim_assemble_operators_lossy(int64_t n, const double* p0, const double* p1, const int32_t* next,
+                                  const int32_t* prev, double k_re, double k_im, int far_order, double* s, double* kp,
+                                  double* d, double* w, int64_t ld, double mass_k, double mass_d,
+                                  const double* coef_k, const double* coef_d, void* workspace,
+                                  int64_t workspace_bytes, int64_t near_per_element, void* stream);
+
+/* (j/4) H0^(2)(z) and (j/4) H1^(2)(z) at m complex points z (device arrays: z complex128 [m], out
+ * float64 [4 m] = g re, g im, h1 re, h1 im), the device routine the lossy operators use; Re z > 0,
+ * Im z <= 0.  Replaces twod/special.py:220-264 (scipy.special.hankel2). */
+int grim_hankel2_probe(int64_t m, const double* z, double* out, void* stream);
+
 /* Batched dense LU.  All dense calls take `batch` equally strided matrices
  * (strides in complex elements for matrices/vectors, in elements for int32
  * arrays, in bytes for aux), e.g. the TE and TM systems of one request.

@@ -17,7 +17,7 @@ reference's outputs.  Nothing from the reference is stored except numbers.
 
 Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
                                                --ibc-zones | --dielectric | --regions |
-                                               --condition | --poly | --sheets]
+                                               --condition | --poly | --sheets | --lossy]
 """
 from __future__ import annotations
 
@@ -576,6 +576,35 @@ def main_regions():
                   f, [0.0, 60.0, 180.0])
 
 
+def main_lossy():
+    """Lossy media (complex wavenumbers, the reference's complex Hankel evaluation through SciPy): a
+    lossy dielectric cylinder, a lossy magnetic one, a PEC cylinder under a lossy coating, a
+    two-layer cylinder of lossy media, and a thin coating of the medium of the reference's own
+    airfoil.geo (flag 1: eps = 101.19 - 11.88j, mu = 3.45 - 3.47j) on a PEC cylinder."""
+    r = 0.1
+    run_zone_case("lossy_diel_circle48", _dielectric_snapshot([(workloads.circle_points(r, 48), 1, 1)],
+                                                              {1: (4.0 - 1.0j, 1.0 + 0j)}),
+                  workloads.frequency_for_ka(1.0, r), [0.0, 90.0, 200.0], keep_matrix=True)
+    run_zone_case("lossy_diel_magnetic64", _dielectric_snapshot([(workloads.circle_points(r, 64), 1, 2)],
+                                                                {2: (2.5 - 0.3j, 1.6 - 0.5j)}),
+                  workloads.frequency_for_ka(1.5, r), [0.0, 45.0, 180.0, 300.0])
+    f = workloads.frequency_for_ka(1.0, 0.1)
+    run_zone_case("lossy_region_coated_pec",
+                  _region_snapshot([(workloads.circle_points(0.07, 24), 1, 4, 1, 0),
+                                    (workloads.circle_points(0.1, 40), 1, 3, 1, 0)], {1: (3.0 - 0.6j, 1.2 - 0.2j)}),
+                  f, [0.0, 90.0, 200.0], keep_matrix=True)
+    run_zone_case("lossy_region_two_layer",
+                  _region_snapshot([(workloads.circle_points(0.06, 24), 1, 5, 1, 2),
+                                    (workloads.circle_points(0.1, 40), 1, 3, 1, 0)],
+                                   {1: (2.0 - 0.2j, 1.0 + 0j), 2: (5.0 - 1.5j, 1.0 - 0.1j)}),
+                  f, [0.0, 60.0, 180.0])
+    run_zone_case("lossy_airfoil_coating",
+                  _region_snapshot([(workloads.circle_points(0.045, 88), 1, 4, 1, 0),
+                                    (workloads.circle_points(0.05, 96), 1, 3, 1, 0)],
+                                   {1: (101.19 - 11.88j, 3.45 - 3.47j)}),
+                  0.3, [0.0, 90.0, 180.0])
+
+
 def run_condition_case(name, points, frequency_ghz, angles):
     """The reference's single-polarization entry (twod/solver.py:1984-2712) with
     compute_condition_number=True in both LU precisions: per channel the condition estimate and
@@ -761,7 +790,9 @@ def main_sheets():
 
 
 if __name__ == "__main__":
-    if "--sheets" in sys.argv:
+    if "--lossy" in sys.argv:
+        main_lossy()
+    elif "--sheets" in sys.argv:
         main_sheets()
     elif "--poly" in sys.argv:
         main_poly()

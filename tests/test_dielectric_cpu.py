@@ -121,14 +121,19 @@ def test_dielectric_rows_and_refusals(tmp_path):
     assert abs(eps - 3.0) < 1e-15 and abs(mu - 1.25) < 1e-15
 
 
-def test_causal_index_and_lossy_refusal():
+def test_causal_index_and_medium_wavenumbers():
+    """Lossless media give a real interior wavenumber (the real-argument kernels), lossy ones the
+    complex k0 n with Im <= 0 (the complex-argument operators), negative-index ones are refused."""
     assert causal_index(4.0, 1.0) == 2.0
     n = causal_index(4.0 - 0.4j, 1.0)
     assert n.imag < 0 and n.real > 0
     assert causal_index(-4.0, -1.0) == -2.0        # double negative: Re(mu/n) >= 0 takes Re n < 0
     f = workloads.frequency_for_ka(1.0, 0.1)
     mesh = build_mesh(_snapshot([["1", "4", "-0.4", "1", "0"]]), "meters", f)
-    with pytest.raises(ValueError, match="complex-argument Hankel"):
-        engine.dielectric_wavenumber(mesh, 20.0)
+    k1 = engine.dielectric_wavenumber(mesh, 20.0)
+    assert engine.is_lossy(k1) and abs(k1 - 20.0 * n) < 1e-13 and k1.imag < 0
     mesh = build_mesh(_snapshot([["1", "4", "0", "1", "0"]]), "meters", f)
-    assert engine.dielectric_wavenumber(mesh, 20.0) == 40.0
+    assert engine.dielectric_wavenumber(mesh, 20.0) == 40.0 and not engine.is_lossy(40.0)
+    mesh = build_mesh(_snapshot([["1", "-4", "0", "-1", "0"]]), "meters", f)
+    with pytest.raises(ValueError, match="negative-index media"):
+        engine.dielectric_wavenumber(mesh, 20.0)

@@ -2,7 +2,8 @@
 snapshots (tests/golden/diel_*.npz): the two-density system (grim_assemble at k0 and
 grim_assemble_operators at k1 into the blocks of one matrix, grim_scale_blocks) entry by entry, the
 exterior densities and amplitudes, the public entry in both precision forms, the serving loop,
-the bistatic diagonal, the boundary densities' labels, and the refusal of a lossy medium."""
+the bistatic diagonal, the boundary densities' labels, and the refusal of a negative-index medium
+(lossy media: test_gpu_lossy.py)."""
 import json
 from pathlib import Path
 
@@ -124,10 +125,10 @@ def test_dielectric_serving_bistatic_and_densities():
     assert np.max(np.abs(got - want)) < 1e-10 * np.max(np.abs(want))
 
 
-def test_lossy_dielectric_is_refused():
+def test_negative_index_dielectric_is_refused():
     _need_gpu()
     from grim_20261007_amd.solver import solve_monostatic_rcs_2d
     snap, f, angles, z = _case("diel_circle48_er4")
-    snap["dielectrics"] = [["1", "4", "-0.4", "1", "0"]]
-    with pytest.raises(ValueError, match="complex-argument Hankel"):
+    snap["dielectrics"] = [["1", "-4", "0", "-1", "0"]]
+    with pytest.raises(ValueError, match="negative-index media"):
         solve_monostatic_rcs_2d(snap, [f], angles, geometry_units="meters", max_panels=100000)
