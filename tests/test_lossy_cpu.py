@@ -37,3 +37,17 @@ def test_laguerre_header_matches_generator():
     buf = io.StringIO()
     gen["main"](buf)
     assert buf.getvalue() == open(os.path.join(CSRC, "lossy_rules.h")).read()
+
+
+def test_asymptotic_coefficients():
+    """a_k(nu) = prod (4 nu^2 - (2j - 1)^2) / (8 j) from exact rational arithmetic."""
+    from fractions import Fraction
+    text = open(os.path.join(CSRC, "lossy_rules.h")).read()
+    for nu in (0, 1):
+        line = [l for l in text.splitlines() if f"GRIM_HASYM{nu}[" in l][0]
+        got = [float(v) for v in line.split("{")[1].split("}")[0].split(",")]
+        a = Fraction(1)
+        for k in range(20):
+            if k:
+                a *= Fraction(4 * nu * nu - (2 * k - 1) ** 2, 8 * k)
+            assert abs(got[k] - float(a)) <= 1e-15 * abs(float(a)), (nu, k)
