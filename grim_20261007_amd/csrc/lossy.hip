@@ -484,6 +484,19 @@ __global__ void __launch_bounds__(64) lossy_far(int n, const double* __restrict_
     const Elem o = load_elem(p0, p1, e), s = load_elem(p0, p1, f);
     if (!is_far(o, e, next[e], s, f, next[f])) return;   // the near pass lists exactly the others
     const cplx k = {kr, ki};
+    const int64_t ef = 8 * ((int64_t)f * n + e), fe = 8 * ((int64_t)e * n + f);
+    // Negligible kernels (twod/operators.py:1355-1363, 2373-2379): with lower = centre distance less
+    // half of each length, -Im(k) lower >= 40 bounds |G| below e^{-40} of the near scale; the
+    // reference neither evaluates nor scatters such pairs, so their blocks are zero.
+    if (ki < 0.0) {
+        const double cdx = o.cx - s.cx, cdy = o.cy - s.cy;
+        const double lower = sqrt(cdx * cdx + cdy * cdy) - 0.5 * (o.len + s.len);
+        if (-ki * lower >= 40.0) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) ts[ef + i] = ts[fe + i] = tk[ef + i] = tk[fe + i] = 0.0;
+            return;
+        }
+    }
     cplx S[2][2], Ke[2][2], Kf[2][2];   // Kf[b][a]: observation basis b of f, source basis a of e
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -516,7 +529,6 @@ __global__ void __launch_bounds__(64) lossy_far(int n, const double* __restrict_
         }
     }
     const double scale = o.len * s.len;
-    const int64_t ef = 8 * ((int64_t)f * n + e), fe = 8 * ((int64_t)e * n + f);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
