@@ -53,6 +53,10 @@ What each function follows in the reference (paths under ``tools/GHOST/ghost_bac
                           regions.py:557-590); ``robin_alpha`` follows
                           regions.py:66-119 (TM nodes of PEC elements: alpha 0).
 
+* complex wavenumbers     (lossy media) ``green_kernels`` through SciPy's hankel2 as the
+                          reference does (twod/special.py:220-264), the self series at
+                          complex z (twod/operators.py:418-501), ``region_system`` and the
+                          dielectric operators at k0 n (principal root).
 * ``sheet_system``        twod/formulations/sheet.py:6-39 (TM: S - Zs/(j k eta0) M,
                           -b_u; TE: W - j k Zs/eta0 M, b_dn, strip-end rows pinned),
                           S with the configured 8-point far rule and W from the
@@ -67,6 +71,7 @@ or against brute-force quadrature.
 """
 from __future__ import annotations
 
+import cmath
 import math
 from dataclasses import dataclass
 
@@ -188,9 +193,16 @@ def gauss(order: int):
     return _GAUSS[order]
 
 
-def green_kernels(k: float, r: np.ndarray):
-    """(j/4) H0^(2)(k r) and (j/4) k H1^(2)(k r) for real k > 0."""
+def green_kernels(k, r: np.ndarray):
+    """(j/4) H0^(2)(k r) and (j/4) k H1^(2)(k r): real k > 0 through the real Bessel functions; a
+    complex k (lossy medium) through SciPy's hankel2, the reference's own evaluation
+    (twod/special.py:220-264, twod/operators.py:935-945: |k r| <= 1e-12 replaced by 1e-12)."""
     r = np.maximum(np.asarray(r, float), EPS)
+    if isinstance(k, complex):
+        from scipy.special import hankel2
+        kr = np.asarray(k * r, dtype=np.complex128)
+        kr = np.where(np.abs(kr) <= 1e-12, 1e-12 + 0j, kr)
+        return 0.25j * hankel2(0, kr), 0.25j * k * hankel2(1, kr)
     kr = np.maximum(k * r, 1e-12)
     g = 0.25 * y0(kr) + 0.25j * j0(kr)
     h = 0.25 * k * y1(kr) + 0.25j * k * j1(kr)
@@ -218,7 +230,7 @@ def self_single_layer(length: float, k: float) -> np.ndarray:
     c_diag = (2.0 / 3.0, -1.0, 0.0, 1.0 / 3.0)
     c_off = (1.0 / 3.0, 0.0, 0.0, -1.0 / 3.0)
     two_pi = 2.0 / math.pi
-    log_term = math.log(z / 2.0) + EULER_GAMMA
+    log_term = (cmath.log(z / 2.0) if isinstance(z, complex) else math.log(z / 2.0)) + EULER_GAMMA
     zq = (z / 2.0) ** 2
     bd = bo = 0.0 + 0.0j
     alpha, harmonic, m = 1.0, 0.0, 0
@@ -901,6 +913,12 @@ def region_layout(mesh: Mesh, element_regions, media):
     return dict(ifaces=ifaces, region_ifaces=region_ifaces, dofs=dofs, n_dof=n_dof, media=dict(media))
 
 
+def _medium_k(k0: float, medium):
+    """k0 n (principal root): real for a lossless medium, complex (Im <= 0) for a lossy one."""
+    n = complex(np.sqrt(complex(medium[0]) * complex(medium[1])))
+    return k0 * n.real if n.imag == 0.0 else k0 * n
+
+
 def region_system(mesh: Mesh, layout, k0: float, pol: str, angles_deg):
     """System, right-hand sides and exterior-density map of a multi-region layout without surface
     impedances (twod/formulations/regions.py:239-333 base rows, combined_regions.py:21-83,116-146
@@ -912,7 +930,7 @@ def region_system(mesh: Mesh, layout, k0: float, pol: str, angles_deg):
     flux rows of each observer in its region, gamma = (+-1)/(j k_region) by side."""
     ifaces, dofs, media = layout["ifaces"], layout["dofs"], layout["media"]
     key = 1 if pol == "TM" else 0                      # media[rid] = (eps, mu)
-    ks = {rid: k0 * complex(np.sqrt(complex(media[rid][0]) * complex(media[rid][1]))).real for rid in media}
+    ks = {rid: _medium_k(k0, media[rid]) for rid in media}
     ops = {rid: operator_set(mesh, ks[rid]) for rid in layout["region_ifaces"]}
     mass = weighted_mass(mesh, np.ones(len(mesh.length)))
     z = np.zeros((layout["n_dof"], layout["n_dof"]), complex)
