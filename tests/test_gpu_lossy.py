@@ -132,3 +132,35 @@ def test_lossy_dielectric_cylinder_converges_to_mie():
         assert errs[pol][2] < errs[pol][1] < errs[pol][0]
         assert errs[pol][1] / errs[pol][2] > 3.0            # second order: about 4 per halving
         assert errs[pol][2] < 5e-3
+
+
+def test_lossy_served_and_bistatic():
+    """A lossy coated cylinder served (alone, then from the factor cache with the same angles) gives
+    the direct solve's amplitudes bit for bit; the bistatic grid's diagonal is the monostatic
+    amplitude."""
+    _need_gpu()
+    from grim_20261007_amd.mesh import build_mesh
+    from grim_20261007_amd.serving import Request, Server
+    from grim_20261007_amd.solver import solve_bistatic_rcs_2d, solve_frequency
+    snap, f, angles, z = _case("lossy_region_coated_pec")
+    mesh = build_mesh(snap, "meters", f)
+    direct = solve_frequency(mesh, f, angles, ("TE", "TM"), "double")
+    srv = Server(workers=2, batch=4)
+    try:
+        first = srv.submit(Request(build_mesh(snap, "meters", f), f, np.asarray(angles), precision="double",
+                                   key="lossy")).result()
+        again = srv.submit(Request(build_mesh(snap, "meters", f), f, np.asarray(angles), precision="double",
+                                   key="lossy")).result()
+    finally:
+        srv.close()
+    assert again.cached and not first.cached
+    for res in (first, again):
+        for pol in ("TE", "TM"):
+            assert np.array_equal(res.amplitude[pol], direct[pol]["amplitude"])
+    bi = solve_bistatic_rcs_2d(snap, [f], angles, angles, geometry_units="meters", max_panels=100000)
+    for export, pol in (("VV", "TE"), ("HH", "TM")):
+        rows = [r for r in bi["samples"] if r["polarization"] == export and r["theta_inc_deg"] == r["theta_scat_deg"]]
+        diag = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+        err = np.max(np.abs(diag - z[f"sample_amp_{pol}"])) / np.max(np.abs(z[f"sample_amp_{pol}"]))
+        print(f"bistatic diagonal {pol}: {err:.2e}", flush=True)
+        assert err < 1e-10
