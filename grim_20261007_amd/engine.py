@@ -591,7 +591,8 @@ def region_plan(mesh: ContourMesh) -> RegionPlan:
     ifaces, region_ifaces, dofs, n_dof = [], {}, {}, 0
     for mi, ((rm, rp), eids) in enumerate(sorted(groups.items())):
         nodes = np.array(sorted(eids, key=lambda v: ref[v]), np.int64)   # element e starts at node e
-        ifaces.append({"r_m": rm, "r_p": rp, "eids": np.array(eids, np.int64), "nodes": nodes})
+        zs = 0j if mesh.impedance is None else complex(mesh.impedance[eids[0]])   # one law per interface
+        ifaces.append({"r_m": rm, "r_p": rp, "eids": np.array(eids, np.int64), "nodes": nodes, "zs": zs})
         for side, rid in (("minus", rm), ("plus", rp)):
             if rid >= 0:
                 region_ifaces.setdefault(rid, []).append(mi)
@@ -638,9 +639,27 @@ def region_mass_coefficients(plan: RegionPlan, rid: int) -> np.ndarray:
     return coef
 
 
+def region_alpha(plan: RegionPlan, ifc: dict, pol: str, k_of) -> complex:
+    """Robin coefficient of a conductor interface with a surface impedance Zs, seen from the region
+    it borders (twod/geometry.py:2488-2521 with that region's eta = eta0 mu / n and k;
+    twod/formulations/regions.py:80-89: negated where the region is on the interface's plus side of
+    a minus-side conductor); 0 on PEC and transmission interfaces."""
+    z = complex(ifc.get("zs", 0j))
+    if abs(z) <= EPS or (ifc["r_m"] >= 0 and ifc["r_p"] >= 0):
+        return 0j
+    rid = ifc["r_p"] if ifc["r_m"] < 0 else ifc["r_m"]
+    eps, mu = plan.media[rid]
+    eta = ETA0 * complex(mu) / causal_index(eps, mu)
+    k = k_of(rid)
+    a = 1j * k * eta / z if pol == "TM" else 1j * k * z / eta
+    return -a if ifc["r_m"] < 0 else a
+
+
 def region_routes(plan: RegionPlan, k_of) -> list:
     """The routes that compose both polarizations' regional systems (regions.py:239-333 base rows,
-    combined_regions.py:43-62,116-146 D and W routes; ``k_of(rid)`` is the region's wavenumber)."""
+    combined_regions.py:43-62,116-146 D and W routes; ``k_of(rid)`` is the region's wavenumber).
+    A conductor interface with a surface impedance (one law over the interface) carries the flux
+    (Robin) row in both polarizations: K' + c M + alpha S, then -gamma W + gamma alpha (D - c M)."""
     routes = []
 
     def cols(mj, rid):
@@ -663,10 +682,15 @@ def region_routes(plan: RegionPlan, k_of) -> list:
                 rid = rp if rm < 0 else rm
                 local = plan.subs[rid]["local"]
                 rows = rows_of(mi, "plus" if rm < 0 else "minus")
+                alpha = region_alpha(plan, ifc, pol, k_of)
                 for mj in plan.region_ifaces[rid]:
                     src_nodes, cc = cols(mj, rid)
-                    routes.append(Route(i, rows, cc, (rid, 0 if pol == "TM" else 1), local[nodes], local[src_nodes],
-                                        1.0))
+                    if alpha == 0:
+                        routes.append(Route(i, rows, cc, (rid, 0 if pol == "TM" else 1), local[nodes],
+                                            local[src_nodes], 1.0))
+                    else:
+                        routes.append(Route(i, rows, cc, (rid, 1), local[nodes], local[src_nodes], 1.0))
+                        routes.append(Route(i, rows, cc, (rid, 0), local[nodes], local[src_nodes], alpha))
             else:
                 rs, rt = rows_of(mi, "minus"), rows_of(mi, "plus")
                 for rid, sign, flux in ((rm, 1.0, 1.0), (rp, -1.0, inv_beta(ifc, pol))):
@@ -686,7 +710,11 @@ def region_routes(plan: RegionPlan, k_of) -> list:
                     o_rows = local[tgt["nodes"]]
                     if tgt["r_m"] < 0 or tgt["r_p"] < 0:
                         rows = rows_of(obs, "plus" if tgt["r_m"] < 0 else "minus")
-                        if pol == "TM":
+                        alpha = region_alpha(plan, tgt, pol, k_of)
+                        if alpha != 0:
+                            routes.append(Route(i, rows, cc, (rid, 3), o_rows, local[src_nodes], -gamma))
+                            routes.append(Route(i, rows, cc, (rid, 2), o_rows, local[src_nodes], gamma * alpha))
+                        elif pol == "TM":
                             routes.append(Route(i, rows, cc, (rid, 2), o_rows, local[src_nodes], gamma))
                         else:
                             routes.append(Route(i, rows, cc, (rid, 3), o_rows, local[src_nodes], -gamma))
@@ -699,10 +727,11 @@ def region_routes(plan: RegionPlan, k_of) -> list:
     return routes
 
 
-def region_load_routes(plan: RegionPlan) -> list:
+def region_load_routes(plan: RegionPlan, k_of=None) -> list:
     """Routes of both polarizations' right-hand sides (regions.py:557-590): on interfaces that
-    border air, -b_dn (TE) or -b_u (TM) on conductor rows; transmission rows with air on their minus
-    side get -b_dn (flux) and -b_u (trace), with air on their plus side +(1/beta) b_dn and +b_u."""
+    border air, -b_dn (TE) or -b_u (TM) on conductor rows (-(b_dn + alpha b_u) on the Robin rows of
+    an impedance conductor, which need ``k_of``); transmission rows with air on their minus side get
+    -b_dn (flux) and -b_u (trace), with air on their plus side +(1/beta) b_dn and +b_u."""
     routes = []
     for i, pol in enumerate(("TE", "TM")):
         key = 1 if pol == "TM" else 0
@@ -711,8 +740,14 @@ def region_load_routes(plan: RegionPlan) -> list:
             if rm < 0 or rp < 0:
                 if (rp if rm < 0 else rm) == 0:
                     start, count = plan.dofs[mi, "plus" if rm < 0 else "minus"]
-                    routes.append(Route(i, np.arange(start, start + count), None, "bu" if pol == "TM" else "bdn",
-                                        nodes, None, -1.0))
+                    rows = np.arange(start, start + count)
+                    if abs(complex(ifc.get("zs", 0j))) > EPS:
+                        if k_of is None:
+                            raise ValueError("The loads of an impedance conductor need the region wavenumbers.")
+                        routes.append(Route(i, rows, None, "bdn", nodes, None, -1.0))
+                        routes.append(Route(i, rows, None, "bu", nodes, None, -region_alpha(plan, ifc, pol, k_of)))
+                    else:
+                        routes.append(Route(i, rows, None, "bu" if pol == "TM" else "bdn", nodes, None, -1.0))
                 continue
             fl, tr = plan.dofs[mi, "minus"][0], plan.dofs[mi, "plus"][0]
             rs, rt = np.arange(fl, fl + len(nodes)), np.arange(tr, tr + len(nodes))
@@ -787,15 +822,17 @@ def region_systems(contour: DeviceContour, k0: float, workspace: torch.Tensor, p
     return pair
 
 
-def region_rhs(plan: RegionPlan, bu: torch.Tensor, bdn: torch.Tensor) -> torch.Tensor:
-    """Right-hand sides (2, A, DOFs) [TE, TM] of region_systems (routes of region_load_routes)."""
+def region_rhs(plan: RegionPlan, bu: torch.Tensor, bdn: torch.Tensor, k0: float | None = None) -> torch.Tensor:
+    """Right-hand sides (2, A, DOFs) [TE, TM] of region_systems (routes of region_load_routes; k0, the
+    air wavenumber, for the Robin loads of impedance conductors)."""
     dev = bu.device
     na, n = bu.shape
     out = torch.zeros((2, na, plan.n_dof), dtype=torch.complex128, device=dev)
     route = _Router(dev)
     acols = np.arange(na)
     loads = {"bu": bu, "bdn": bdn}
-    for r in region_load_routes(plan):
+    k_of = None if k0 is None else (lambda rid: _region_wavenumber(plan.media, rid, k0))
+    for r in region_load_routes(plan, k_of):
         route(out[r.pol], plan.n_dof, r.rows_dst, acols, loads[r.src], n, r.rows_src, acols, r.weight)
     return out
 

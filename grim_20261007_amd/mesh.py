@@ -944,8 +944,17 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
         # several regions (twod/solver.py:1850-1865): coated conductors, conductors beside
         # dielectrics, several media
         if has_ibc:
-            raise ValueError("Surface impedances in multi-region layouts (coated impedance bodies, impedance "
-                             "contours beside dielectrics) are not supported by the GPU path.")
+            # one law per conductor interface (all panels of one (minus, plus) region pair); PEC and
+            # impedance zones, tapers or several laws on one interface are the reference's
+            # element-weighted Robin rows, not built here
+            laws_by_iface: dict = {}
+            for e, key in enumerate(keys_e):
+                laws_by_iface.setdefault(key, set()).add(complex(impedance[e]))
+            if any(len(v) > 1 for v in laws_by_iface.values()):
+                raise ValueError("Surface impedances in multi-region layouts are supported by the GPU path as one "
+                                 "constant law per conductor interface (every conductor panel bordering the same "
+                                 "region); impedance zones, tapers, or PEC and impedance conductors in one region "
+                                 "are not.")
         regions = np.array(keys_e, np.int64)
         used = sorted({int(r) for r in regions.ravel() if r > 0})
         if not np.any(regions == 0):

@@ -434,7 +434,7 @@ def execute(req: Request, device, ws: dict, cached: Optional[dict] = None, store
     angles = torch.tensor(np.asarray(req.angles_deg, float), dtype=torch.float64, device=dev)
     bu, bdn = engine.loads(contour, k, angles)
     if plan is not None:
-        rhs = engine.region_rhs(plan, bu, bdn)
+        rhs = engine.region_rhs(plan, bu, bdn, k)
     elif mesh.medium is not None:
         rhs = engine.dielectric_rhs(bu, bdn)
     elif mesh.impedance is not None:

@@ -331,7 +331,7 @@ def _systems(contour, k, angles, workspace):
     mesh = contour.mesh
     if mesh.regions is not None:
         plan = engine.region_plan(mesh)
-        return (engine.region_systems(contour, k, workspace, plan), engine.region_rhs(plan, bu, bdn), bu, bdn,
+        return (engine.region_systems(contour, k, workspace, plan), engine.region_rhs(plan, bu, bdn, k), bu, bdn,
                 lambda x: engine.region_exterior_density(plan, mesh.n, x))
     if mesh.medium is not None:
         return (engine.dielectric_systems(contour, k, workspace), engine.dielectric_rhs(bu, bdn), bu, bdn,

@@ -17,7 +17,8 @@ reference's outputs.  Nothing from the reference is stored except numbers.
 
 Usage:  python tests/golden/make_golden.py [--certified | --meshes | --grim | --bistatic | --ibc |
                                                --ibc-zones | --dielectric | --regions |
-                                               --condition | --poly | --sheets | --lossy]
+                                               --condition | --poly | --sheets | --lossy |
+                                               --region-ibc]
 """
 from __future__ import annotations
 
@@ -605,6 +606,35 @@ def main_lossy():
                   0.3, [0.0, 90.0, 180.0])
 
 
+def main_region_ibc():
+    """Surface impedances on conductors of regional layouts: an impedance core under a dielectric
+    coating (the reference's coated impedance cylinder, validation/cylinder.py:375-448), the same
+    under a lossy coating, and an impedance cylinder in air beside a dielectric one."""
+    f = workloads.frequency_for_ka(1.0, 0.1)
+
+    def snap(contours, media, ibcs):
+        out = _region_snapshot([c[:5] for c in contours], media)
+        for seg, c in zip(out["segments"], contours):
+            seg["properties"][2] = str(c[5])
+        out["ibcs"] = ibcs
+        return out
+
+    law = [["1", "constant", "50", "20", "0", "0"]]
+    run_zone_case("region_ibc_coated", snap([(workloads.circle_points(0.07, 24), 1, 4, 1, 0, 1),
+                                             (workloads.circle_points(0.1, 40), 1, 3, 1, 0, 0)],
+                                            {1: (3.0 + 0j, 1.0 + 0j)}, law), f, [0.0, 90.0, 200.0], keep_matrix=True)
+    run_zone_case("region_ibc_lossy_coating", snap([(workloads.circle_points(0.07, 32), 1, 4, 1, 0, 1),
+                                                    (workloads.circle_points(0.1, 48), 1, 3, 1, 0, 0)],
+                                                   {1: (2.5 - 0.5j, 1.3 - 0.1j)}, [["1", "constant", "120", "-60",
+                                                                                      "0", "0"]]),
+                  f, [0.0, 135.0, 270.0])
+    shift = np.array([0.3, 0.05])
+    run_zone_case("region_ibc_beside_dielectric",
+                  snap([(workloads.circle_points(0.1, 32), 1, 2, 0, 0, 1),
+                        (workloads.circle_points(0.08, 32) + shift, 1, 3, 1, 0, 0)],
+                       {1: (2.5 + 0j, 1.2 + 0j)}, law), f, [0.0, 45.0, 180.0, 300.0], keep_matrix=True)
+
+
 def run_condition_case(name, points, frequency_ghz, angles):
     """The reference's single-polarization entry (twod/solver.py:1984-2712) with
     compute_condition_number=True in both LU precisions: per channel the condition estimate and
@@ -790,7 +820,9 @@ def main_sheets():
 
 
 if __name__ == "__main__":
-    if "--lossy" in sys.argv:
+    if "--region-ibc" in sys.argv:
+        main_region_ibc()
+    elif "--lossy" in sys.argv:
         main_lossy()
     elif "--sheets" in sys.argv:
         main_sheets()

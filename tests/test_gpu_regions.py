@@ -42,7 +42,7 @@ def test_region_systems_match_reference(name):
     plan = engine.region_plan(mesh)
     pair = engine.region_systems(contour, k, ws, plan)
     bu, bdn = engine.loads(contour, k, torch.tensor(angles, dtype=torch.float64, device=dev))
-    rhs = engine.region_rhs(plan, bu, bdn)
+    rhs = engine.region_rhs(plan, bu, bdn, k)
     torch.cuda.synchronize()
     systems = pair.transpose(1, 2).cpu().numpy()
     loads = rhs.transpose(1, 2).cpu().numpy()
@@ -118,3 +118,50 @@ def test_region_serving_and_bistatic():
         grid = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows]).reshape(m, m)
         ref = z[f"amp_{pol}"]
         assert np.max(np.abs(np.diagonal(grid) - ref) / np.abs(ref)) < 1e-10
+
+
+IBC_CASES = ["region_ibc_coated", "region_ibc_lossy_coating", "region_ibc_beside_dielectric"]
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("name", IBC_CASES)
+def test_region_impedance_conductors_match_reference(name, precision):
+    """Conductors with a surface impedance inside regional layouts (an impedance core under a
+    lossless or lossy coating, an impedance cylinder beside a dielectric one): the systems entry by
+    entry where the fixture holds them, loads, exterior densities and the public entry's amplitudes."""
+    _need_gpu()
+    from grim_20261007_amd import engine
+    from grim_20261007_amd.mesh import build_mesh
+    from grim_20261007_amd.solver import solve_frequency, solve_monostatic_rcs_2d, wavenumber
+    snap, f, angles, z = _case(name)
+    mesh = build_mesh(snap, "meters", f)
+    dev = torch.device("cuda:0")
+    k = wavenumber(f)
+    if precision == "double":
+        contour = engine.DeviceContour(mesh, dev)
+        plan = engine.region_plan(mesh)
+        pair = engine.region_systems(contour, k, engine.assembly_workspace(mesh.n, dev), plan)
+        bu, bdn = engine.loads(contour, k, torch.tensor(angles, dtype=torch.float64, device=dev))
+        rhs = engine.region_rhs(plan, bu, bdn, k)
+        systems = pair.transpose(1, 2).cpu().numpy()
+        loads = rhs.transpose(1, 2).cpu().numpy()
+        for i, pol in enumerate(("TE", "TM")):
+            if f"Z_{pol}" in z:
+                err = np.max(np.abs(systems[i] - z[f"Z_{pol}"])) / np.max(np.abs(z[f"Z_{pol}"]))
+                print(f"{name} {pol}: system {err:.2e}", flush=True)
+                assert err < 1e-12
+            err = np.max(np.abs(loads[i] - z[f"rhs_{pol}"])) / np.max(np.abs(z[f"rhs_{pol}"]))
+            assert err < 1e-13
+    out = solve_frequency(mesh, f, angles, ("TE", "TM"), precision, return_density=True)
+    res = solve_monostatic_rcs_2d(snap, [f], angles, geometry_units="meters", max_panels=100000,
+                                  solver_method="direct", precision=precision)
+    tol = 1e-10 if precision == "double" else 1e-8
+    ref = np.arange(mesh.n) if mesh.ref_node is None else mesh.ref_node
+    for export, pol in (("VV", "TE"), ("HH", "TM")):
+        want = z[f"density_{pol}"][ref]
+        err = np.max(np.abs(out[pol]["density"] - want)) / np.max(np.abs(want))
+        rows = res["co_solved_samples"][export]
+        amp = np.array([complex(r["rcs_amp_real"], r["rcs_amp_imag"]) for r in rows])
+        aerr = np.max(np.abs(amp - z[f"sample_amp_{pol}"])) / np.max(np.abs(z[f"sample_amp_{pol}"]))
+        print(f"{name} {precision} {pol}: density {err:.2e}, amplitude {aerr:.2e}", flush=True)
+        assert err < tol and aerr < tol

@@ -16,6 +16,8 @@ from oracle import bem2d
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 CASES = ["region_coated_pec", "region_pec_beside_dielectric", "region_two_media", "region_two_layer"]
+# surface impedances on conductors of regional layouts (make_golden.py --region-ibc)
+IBC_CASES = ["region_ibc_coated", "region_ibc_lossy_coating", "region_ibc_beside_dielectric"]
 
 
 def _case(name):
@@ -23,7 +25,7 @@ def _case(name):
     return json.loads(str(z["snapshot_json"])), float(z["frequency_ghz"]), z["angles_deg"], z
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + IBC_CASES)
 def test_region_meshes_match_reference(name):
     snap, f, _, z = _case(name)
     mesh = build_mesh(snap, "meters", f)
@@ -52,7 +54,7 @@ def test_oracle_region_system_matches_reference(name):
         assert np.max(np.abs(amp - z[f"amp_{pol}"]) / np.abs(z[f"amp_{pol}"])) < 1e-12
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + IBC_CASES)
 def test_route_plan_restates_reference(name):
     """The product's routes, executed with the oracle's S, K' + c M, D - c M, W of each region's
     contours, give the reference's regional matrices and loads; the air-side unknowns give its
@@ -77,7 +79,7 @@ def test_route_plan_restates_reference(name):
     bu, bdn = bem2d.incident_loads(full, k0, angles)
     loads = {"bu": bu, "bdn": bdn}
     rhs = np.zeros((2, plan.n_dof, len(angles)), complex)
-    for r in engine.region_load_routes(plan):
+    for r in engine.region_load_routes(plan, ks.__getitem__):
         rhs[r.pol][r.rows_dst] += r.weight * loads[r.src][r.rows_src]
     ref = np.arange(mesh.n) if mesh.ref_node is None else mesh.ref_node
     for i, pol in enumerate(("TE", "TM")):
@@ -114,9 +116,12 @@ def test_region_refusals():
     with pytest.raises(ValueError, match="orientation"):
         build_mesh(_coated(["4", "1", "0", "1", "0"], ["3", "1", "0", "1", "0"],
                            inner=workloads.circle_points(0.07, 24)[::-1]), "meters", f)
-    with pytest.raises(ValueError, match="Surface impedances in multi-region layouts"):
-        snap = _coated(["4", "1", "1", "1", "0"], ["3", "1", "0", "1", "0"])
-        snap["ibcs"] = [["1", "constant", "50", "0", "0", "0"]]
+    snap = _coated(["4", "1", "1", "1", "0"], ["3", "1", "0", "1", "0"])
+    snap["ibcs"] = [["1", "constant", "50", "0", "0", "0"]]
+    mesh = build_mesh(snap, "meters", f)          # one law on the core: built
+    assert mesh.regions is not None and mesh.impedance is not None
+    with pytest.raises(ValueError, match="one constant law per conductor interface"):
+        snap["ibcs"] = [["1", "linear", "20", "0", "300", "-100"]]      # a taper along the core
         build_mesh(snap, "meters", f)
     with pytest.raises(ValueError, match="Mixed TYPE 1 sheet"):
         snap = _coated(["1", "1", "1", "0", "0"], ["3", "1", "0", "1", "0"])
