@@ -31,6 +31,18 @@ gives a_n = 0 exactly (no scatterer); eps -> infinity in TM (zeta = mu fixed)
 tends to the PEC Dirichlet series; lossless media give |1 + 2 a_n| = 1; and the
 reference-written dielectric golden amplitudes agree with the series to the
 discretisation error.
+
+``impedance_backscatter_amplitude`` (validation/cylinder.py:93-131) is the
+Leontovich cylinder: each harmonic of the total field satisfies du/dr + beta u = 0
+at the surface, beta = -j k eta0 / Zs (TM) or -j k Zs / eta0 (TE).
+``coated_impedance_backscatter_amplitude`` (validation/cylinder.py:375-448) puts
+that surface at the inner radius of an annular medium: the annulus field
+J_n(k1 r) + c Y_n(k1 r) takes c from the Robin condition (with the medium's k1
+and eta1 = eta0 mu / n), then matches the exterior as the dielectric series does.
+Pinned in tests/test_lossy_cpu.py: Zs -> 0 tends to the PEC series in both
+polarizations; a coating of eps = mu = 1 gives the bare impedance cylinder of the
+inner radius; and the coated impedance core's solves converge to it at second
+order (tests/test_gpu_regions.py).
 """
 import math
 
@@ -38,6 +50,7 @@ import numpy as np
 from scipy import special as sp
 
 C0 = 299_792_458.0
+ETA0 = 376.730313668
 
 
 def terms_for_ka(ka: float, pad: int = 10) -> int:
@@ -84,4 +97,43 @@ def dielectric_coefficients(ka: float, eps_r: complex, mu_r: complex, pol: str):
 
 def dielectric_backscatter_amplitude(radius: float, k: float, eps_r: complex, mu_r: complex, pol: str) -> complex:
     n, a = dielectric_coefficients(k * radius, eps_r, mu_r, pol)
+    return complex(-4.0j * np.sum(a * (-1.0) ** n))
+
+
+def impedance_backscatter_amplitude(radius: float, k: float, zs: complex, pol: str) -> complex:
+    """Leontovich cylinder of surface impedance zs in air."""
+    ka = k * radius
+    m = terms_for_ka(ka)
+    n = np.arange(-m, m + 1)
+    if pol.upper() == "TM":
+        beta = -1j * k * ETA0 / complex(zs)
+    elif pol.upper() == "TE":
+        beta = -1j * k * complex(zs) / ETA0
+    else:
+        raise ValueError(pol)
+    a = -(k * sp.jvp(n, ka, 1) + beta * sp.jv(n, ka)) / (k * sp.h2vp(n, ka, 1) + beta * sp.hankel2(n, ka))
+    return complex(-4.0j * np.sum(a * (-1.0) ** n))
+
+
+def coated_impedance_backscatter_amplitude(a_in: float, a_out: float, k0: float, eps_r: complex, mu_r: complex,
+                                           zs: complex, pol: str) -> complex:
+    """Impedance surface of radius a_in under an annulus of (eps_r, mu_r) out to a_out, in air."""
+    index = np.sqrt(complex(eps_r) * complex(mu_r))
+    k1 = k0 * index
+    eta1 = ETA0 * complex(mu_r) / index
+    if pol.upper() == "TM":
+        zeta, beta = complex(mu_r), -1j * k1 * eta1 / complex(zs)
+    elif pol.upper() == "TE":
+        zeta, beta = complex(eps_r), -1j * k1 * complex(zs) / eta1
+    else:
+        raise ValueError(pol)
+    xi, xo1, xo = k1 * a_in, k1 * a_out, k0 * a_out
+    m = terms_for_ka(max(abs(xi), abs(xo1), abs(xo)))
+    n = np.arange(-m, m + 1)
+    c = -(k1 * sp.jvp(n, xi, 1) + beta * sp.jv(n, xi)) / (k1 * sp.yvp(n, xi, 1) + beta * sp.yv(n, xi))
+    phi = sp.jv(n, xo1) + c * sp.yv(n, xo1)
+    dphi = sp.jvp(n, xo1, 1) + c * sp.yvp(n, xo1, 1)
+    p, q = k1 / zeta, k0
+    a = -(p * dphi * sp.jv(n, xo) - q * phi * sp.jvp(n, xo, 1)) / (p * dphi * sp.hankel2(n, xo) -
+                                                                    q * phi * sp.h2vp(n, xo, 1))
     return complex(-4.0j * np.sum(a * (-1.0) ** n))

@@ -103,3 +103,18 @@ def test_oracle_lossy_regions_match_reference():
             assert np.max(np.abs(rho - z[f"density_{pol}"])) < 1e-12 * np.max(np.abs(rho))
             amp = bem2d.farfield(om, k0, angles, rho)
             assert np.max(np.abs(amp - z[f"amp_{pol}"]) / np.abs(z[f"amp_{pol}"])) < 1e-12, name
+
+
+def test_impedance_mie_pins():
+    """The impedance cylinder series tends to the PEC series as Zs -> 0; a coating of eps = mu = 1
+    over an impedance core is the bare impedance cylinder of the core's radius."""
+    from oracle import mie
+    k = 20.0
+    for pol in ("TM", "TE"):
+        pec = mie.pec_backscatter_amplitude(0.1, k, pol)
+        near = mie.impedance_backscatter_amplitude(0.1, k, 1e-9 + 1e-9j, pol)
+        assert abs(near - pec) <= 1e-8 * abs(pec), pol
+        for zs in (50 + 20j, 120 - 60j, 5j):
+            bare = mie.impedance_backscatter_amplitude(0.07, k, zs, pol)
+            coated = mie.coated_impedance_backscatter_amplitude(0.07, 0.1, k, 1.0, 1.0, zs, pol)
+            assert abs(coated - bare) <= 1e-12 * abs(bare), (pol, zs)
