@@ -591,7 +591,8 @@ def region_plan(mesh: ContourMesh) -> RegionPlan:
     ifaces, region_ifaces, dofs, n_dof = [], {}, {}, 0
     for mi, ((rm, rp), eids) in enumerate(sorted(groups.items())):
         nodes = np.array(sorted(eids, key=lambda v: ref[v]), np.int64)   # element e starts at node e
-        zs = 0j if mesh.impedance is None else complex(mesh.impedance[eids[0]])   # one law per interface
+        # one law per contour: node p's elements (p and prev[p]) share it
+        zs = np.zeros(len(nodes), complex) if mesh.impedance is None else np.asarray(mesh.impedance)[nodes]
         ifaces.append({"r_m": rm, "r_p": rp, "eids": np.array(eids, np.int64), "nodes": nodes, "zs": zs})
         for side, rid in (("minus", rm), ("plus", rp)):
             if rid >= 0:
@@ -639,20 +640,32 @@ def region_mass_coefficients(plan: RegionPlan, rid: int) -> np.ndarray:
     return coef
 
 
-def region_alpha(plan: RegionPlan, ifc: dict, pol: str, k_of) -> complex:
-    """Robin coefficient of a conductor interface with a surface impedance Zs, seen from the region
-    it borders (twod/geometry.py:2488-2521 with that region's eta = eta0 mu / n and k;
-    twod/formulations/regions.py:80-89: negated where the region is on the interface's plus side of
-    a minus-side conductor); 0 on PEC and transmission interfaces."""
-    z = complex(ifc.get("zs", 0j))
-    if abs(z) <= EPS or (ifc["r_m"] >= 0 and ifc["r_p"] >= 0):
-        return 0j
+def region_alpha(plan: RegionPlan, ifc: dict, pol: str, k_of) -> np.ndarray:
+    """Robin coefficient of each node of a conductor interface, from its contour's surface impedance
+    Zs and the region the interface borders (twod/geometry.py:2488-2521 with that region's
+    eta = eta0 mu / n and k; twod/formulations/regions.py:80-89: negated where the region is on the
+    plus side of a minus-side conductor); 0 on PEC nodes and on transmission interfaces."""
+    zs = np.asarray(ifc.get("zs", np.zeros(len(ifc["nodes"]), complex)), complex)
+    out = np.zeros(len(zs), complex)
+    if ifc["r_m"] >= 0 and ifc["r_p"] >= 0:
+        return out
     rid = ifc["r_p"] if ifc["r_m"] < 0 else ifc["r_m"]
     eps, mu = plan.media[rid]
     eta = ETA0 * complex(mu) / causal_index(eps, mu)
     k = k_of(rid)
-    a = 1j * k * eta / z if pol == "TM" else 1j * k * z / eta
-    return -a if ifc["r_m"] < 0 else a
+    sign = -1.0 if ifc["r_m"] < 0 else 1.0
+    for j, z in enumerate(zs):
+        if abs(z) > EPS:
+            out[j] = sign * (1j * k * eta / z if pol == "TM" else 1j * k * z / eta)
+    return out
+
+
+def _alpha_groups(alpha: np.ndarray):
+    """(alpha value, node positions) per distinct value, in first-appearance order."""
+    groups: dict = {}
+    for j, a in enumerate(alpha):
+        groups.setdefault(complex(a), []).append(j)
+    return [(a, np.array(idx, np.int64)) for a, idx in groups.items()]
 
 
 def region_routes(plan: RegionPlan, k_of) -> list:
@@ -681,16 +694,17 @@ def region_routes(plan: RegionPlan, k_of) -> list:
             if rm < 0 or rp < 0:
                 rid = rp if rm < 0 else rm
                 local = plan.subs[rid]["local"]
-                rows = rows_of(mi, "plus" if rm < 0 else "minus")
-                alpha = region_alpha(plan, ifc, pol, k_of)
-                for mj in plan.region_ifaces[rid]:
-                    src_nodes, cc = cols(mj, rid)
-                    if alpha == 0:
-                        routes.append(Route(i, rows, cc, (rid, 0 if pol == "TM" else 1), local[nodes],
-                                            local[src_nodes], 1.0))
-                    else:
-                        routes.append(Route(i, rows, cc, (rid, 1), local[nodes], local[src_nodes], 1.0))
-                        routes.append(Route(i, rows, cc, (rid, 0), local[nodes], local[src_nodes], alpha))
+                rows_all = rows_of(mi, "plus" if rm < 0 else "minus")
+                for alpha, idx in _alpha_groups(region_alpha(plan, ifc, pol, k_of)):
+                    rows, obs_nodes = rows_all[idx], local[nodes[idx]]
+                    for mj in plan.region_ifaces[rid]:
+                        src_nodes, cc = cols(mj, rid)
+                        if alpha == 0:
+                            routes.append(Route(i, rows, cc, (rid, 0 if pol == "TM" else 1), obs_nodes,
+                                                local[src_nodes], 1.0))
+                        else:
+                            routes.append(Route(i, rows, cc, (rid, 1), obs_nodes, local[src_nodes], 1.0))
+                            routes.append(Route(i, rows, cc, (rid, 0), obs_nodes, local[src_nodes], alpha))
             else:
                 rs, rt = rows_of(mi, "minus"), rows_of(mi, "plus")
                 for rid, sign, flux in ((rm, 1.0, 1.0), (rp, -1.0, inv_beta(ifc, pol))):
@@ -709,15 +723,16 @@ def region_routes(plan: RegionPlan, k_of) -> list:
                     tgt = plan.ifaces[obs]
                     o_rows = local[tgt["nodes"]]
                     if tgt["r_m"] < 0 or tgt["r_p"] < 0:
-                        rows = rows_of(obs, "plus" if tgt["r_m"] < 0 else "minus")
-                        alpha = region_alpha(plan, tgt, pol, k_of)
-                        if alpha != 0:
-                            routes.append(Route(i, rows, cc, (rid, 3), o_rows, local[src_nodes], -gamma))
-                            routes.append(Route(i, rows, cc, (rid, 2), o_rows, local[src_nodes], gamma * alpha))
-                        elif pol == "TM":
-                            routes.append(Route(i, rows, cc, (rid, 2), o_rows, local[src_nodes], gamma))
-                        else:
-                            routes.append(Route(i, rows, cc, (rid, 3), o_rows, local[src_nodes], -gamma))
+                        rows_all = rows_of(obs, "plus" if tgt["r_m"] < 0 else "minus")
+                        for alpha, idx in _alpha_groups(region_alpha(plan, tgt, pol, k_of)):
+                            rows, orows = rows_all[idx], o_rows[idx]
+                            if alpha != 0:
+                                routes.append(Route(i, rows, cc, (rid, 3), orows, local[src_nodes], -gamma))
+                                routes.append(Route(i, rows, cc, (rid, 2), orows, local[src_nodes], gamma * alpha))
+                            elif pol == "TM":
+                                routes.append(Route(i, rows, cc, (rid, 2), orows, local[src_nodes], gamma))
+                            else:
+                                routes.append(Route(i, rows, cc, (rid, 3), orows, local[src_nodes], -gamma))
                     else:
                         own = rid == tgt["r_m"]
                         routes.append(Route(i, rows_of(obs, "plus"), cc, (rid, 2), o_rows, local[src_nodes],
@@ -741,11 +756,17 @@ def region_load_routes(plan: RegionPlan, k_of=None) -> list:
                 if (rp if rm < 0 else rm) == 0:
                     start, count = plan.dofs[mi, "plus" if rm < 0 else "minus"]
                     rows = np.arange(start, start + count)
-                    if abs(complex(ifc.get("zs", 0j))) > EPS:
+                    zs = np.asarray(ifc.get("zs", np.zeros(count, complex)), complex)
+                    if np.any(np.abs(zs) > EPS):
                         if k_of is None:
                             raise ValueError("The loads of an impedance conductor need the region wavenumbers.")
-                        routes.append(Route(i, rows, None, "bdn", nodes, None, -1.0))
-                        routes.append(Route(i, rows, None, "bu", nodes, None, -region_alpha(plan, ifc, pol, k_of)))
+                        for alpha, idx in _alpha_groups(region_alpha(plan, ifc, pol, k_of)):
+                            if alpha == 0:
+                                routes.append(Route(i, rows[idx], None, "bu" if pol == "TM" else "bdn", nodes[idx],
+                                                    None, -1.0))
+                            else:
+                                routes.append(Route(i, rows[idx], None, "bdn", nodes[idx], None, -1.0))
+                                routes.append(Route(i, rows[idx], None, "bu", nodes[idx], None, -alpha))
                     else:
                         routes.append(Route(i, rows, None, "bu" if pol == "TM" else "bdn", nodes, None, -1.0))
                 continue

@@ -943,18 +943,6 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
     elif kinds != {2}:
         # several regions (twod/solver.py:1850-1865): coated conductors, conductors beside
         # dielectrics, several media
-        if has_ibc:
-            # one law per conductor interface (all panels of one (minus, plus) region pair); PEC and
-            # impedance zones, tapers or several laws on one interface are the reference's
-            # element-weighted Robin rows, not built here
-            laws_by_iface: dict = {}
-            for e, key in enumerate(keys_e):
-                laws_by_iface.setdefault(key, set()).add(complex(impedance[e]))
-            if any(len(v) > 1 for v in laws_by_iface.values()):
-                raise ValueError("Surface impedances in multi-region layouts are supported by the GPU path as one "
-                                 "constant law per conductor interface (every conductor panel bordering the same "
-                                 "region); impedance zones, tapers, or PEC and impedance conductors in one region "
-                                 "are not.")
         regions = np.array(keys_e, np.int64)
         used = sorted({int(r) for r in regions.ravel() if r > 0})
         if not np.any(regions == 0):
@@ -966,6 +954,15 @@ def build_mesh(snapshot: dict, units: str, frequency_ghz: float, max_panels: int
                        None if np.array_equal(first, np.arange(n)) else first.copy(), medium, regions, region_media)
     if regions is not None:
         _check_region_contours(mesh)
+        if has_ibc:
+            # one law per closed contour (PEC contours and impedance contours of several laws may share
+            # a region); zones and tapers along a contour are the reference's element-weighted Robin
+            # rows, not built here
+            for cycle in _contours(mesh):
+                if len({complex(impedance[e]) for e in cycle}) > 1:
+                    raise ValueError("Surface impedances in multi-region layouts are supported by the GPU path as "
+                                     "one constant law per closed contour; impedance zones and tapers along a "
+                                     "contour are not.")
     _check_orientation(mesh)
     return mesh
 

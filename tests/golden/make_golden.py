@@ -633,6 +633,14 @@ def main_region_ibc():
                   snap([(workloads.circle_points(0.1, 32), 1, 2, 0, 0, 1),
                         (workloads.circle_points(0.08, 32) + shift, 1, 3, 1, 0, 0)],
                        {1: (2.5 + 0j, 1.2 + 0j)}, law), f, [0.0, 45.0, 180.0, 300.0], keep_matrix=True)
+    # a PEC contour and two impedance contours of different laws, all in air, beside a dielectric
+    laws = [["1", "constant", "50", "20", "0", "0"], ["2", "constant", "200", "-100", "0", "0"]]
+    run_zone_case("region_ibc_mixed_contours",
+                  snap([(workloads.circle_points(0.06, 24), 1, 2, 0, 0, 0),
+                        (workloads.circle_points(0.05, 24) + np.array([0.0, 0.2]), 1, 2, 0, 0, 1),
+                        (workloads.circle_points(0.05, 24) + np.array([0.2, 0.2]), 1, 2, 0, 0, 2),
+                        (workloads.circle_points(0.08, 32) + shift, 1, 3, 1, 0, 0)],
+                       {1: (2.5 + 0j, 1.2 + 0j)}, laws), f, [0.0, 90.0, 225.0], keep_matrix=True)
 
 
 def run_condition_case(name, points, frequency_ghz, angles):

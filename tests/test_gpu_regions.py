@@ -120,7 +120,7 @@ def test_region_serving_and_bistatic():
         assert np.max(np.abs(np.diagonal(grid) - ref) / np.abs(ref)) < 1e-10
 
 
-IBC_CASES = ["region_ibc_coated", "region_ibc_lossy_coating", "region_ibc_beside_dielectric"]
+IBC_CASES = ["region_ibc_coated", "region_ibc_lossy_coating", "region_ibc_beside_dielectric", "region_ibc_mixed_contours"]
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
